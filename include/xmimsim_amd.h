@@ -1,0 +1,257 @@
+/*
+ * xmimsim_amd.h -- C ABI of the MI355X photon-history engine.
+ *
+ * This is a drop-in for the xmi_main_msim path of XMI-MSIM 8.1: the data
+ * structures below have the same member order and types as the reference's
+ * public C structs (include/xmi_data_structs.h:36-495,
+ * include/xmi_solid_angle.h:27-35), and the entry points take the same
+ * arguments, return 1 on success / 0 on failure, and allocate their outputs
+ * with malloc (release with xmi_free).  Nothing in this header depends on GLib,
+ * torch or HIP.
+ */
+#ifndef XMIMSIM_AMD_H
+#define XMIMSIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- data model: include/xmi_data_structs.h ---------------------------- */
+
+typedef struct _xmi_general {
+  float version;
+  char *outputfile;
+  long n_photons_interval;
+  long n_photons_line;
+  int n_interactions_trajectory;
+  char *comments;
+} xmi_general;
+
+typedef struct _xmi_layer {
+  int n_elements;
+  int *Z;
+  double *weight;
+  double density;
+  double thickness;
+} xmi_layer;
+
+typedef struct _xmi_composition {
+  int n_layers;
+  xmi_layer *layers;
+  int reference_layer; /* 1-based */
+} xmi_composition;
+
+typedef struct _xmi_geometry {
+  double d_sample_source;
+  double n_sample_orientation[3];
+  double p_detector_window[3];
+  double n_detector_orientation[3];
+  double area_detector;
+  double collimator_height;
+  double collimator_diameter;
+  double d_source_slit;
+  double slit_size_x;
+  double slit_size_y;
+} xmi_geometry;
+
+typedef enum {
+  XMI_ENERGY_DISCRETE_DISTRIBUTION_MONOCHROMATIC,
+  XMI_ENERGY_DISCRETE_DISTRIBUTION_GAUSSIAN,
+  XMI_ENERGY_DISCRETE_DISTRIBUTION_LORENTZIAN
+} XmiEnergyDiscreteDistribution;
+
+typedef struct _xmi_energy_discrete {
+  double energy;
+  double horizontal_intensity;
+  double vertical_intensity;
+  double sigma_x;
+  double sigma_xp;
+  double sigma_y;
+  double sigma_yp;
+  XmiEnergyDiscreteDistribution distribution_type;
+  double scale_parameter;
+} xmi_energy_discrete;
+
+typedef struct _xmi_energy_continuous {
+  double energy;
+  double horizontal_intensity;
+  double vertical_intensity;
+  double sigma_x;
+  double sigma_xp;
+  double sigma_y;
+  double sigma_yp;
+} xmi_energy_continuous;
+
+typedef struct _xmi_excitation {
+  int n_discrete;
+  xmi_energy_discrete *discrete;
+  int n_continuous;
+  xmi_energy_continuous *continuous;
+} xmi_excitation;
+
+typedef struct _xmi_absorbers {
+  int n_exc_layers;
+  xmi_layer *exc_layers;
+  int n_det_layers;
+  xmi_layer *det_layers;
+} xmi_absorbers;
+
+typedef enum {
+  XMI_DETECTOR_CONVOLUTION_PROFILE_SILI,
+  XMI_DETECTOR_CONVOLUTION_PROFILE_GE,
+  XMI_DETECTOR_CONVOLUTION_PROFILE_SI_SDD
+} XmiDetectorConvolutionProfile;
+
+typedef struct _xmi_detector {
+  XmiDetectorConvolutionProfile detector_type;
+  double live_time;
+  double pulse_width;
+  double gain;
+  double zero;
+  double fano;
+  double noise;
+  int nchannels;
+  int n_crystal_layers;
+  xmi_layer *crystal_layers;
+} xmi_detector;
+
+typedef struct _xmi_input {
+  xmi_general *general;
+  xmi_composition *composition;
+  xmi_geometry *geometry;
+  xmi_excitation *excitation;
+  xmi_absorbers *absorbers;
+  xmi_detector *detector;
+} xmi_input;
+
+/* include/xmi_data_structs.h:479-495 */
+typedef struct _xmi_main_options {
+  int use_M_lines;
+  int use_cascade_auger;
+  int use_cascade_radiative;
+  int use_variance_reduction;
+  int use_sum_peaks;
+  int use_escape_peaks;
+  int escape_ratios_mode;
+  int verbose;
+  int use_poisson;
+  int use_gpu;
+  int omp_num_threads;
+  int extra_verbose;
+  char *custom_detector_response;
+  int use_advanced_compton;
+  int use_default_seeds;
+} xmi_main_options;
+
+/* include/xmi_solid_angle.h:27-35; solid_angles is double[theta_n][r_n] */
+typedef struct _xmi_solid_angle {
+  double *solid_angles;
+  long int grid_dims_r_n;
+  long int grid_dims_theta_n;
+  double *grid_dims_r_vals;
+  double *grid_dims_theta_vals;
+  char *xmi_input_string;
+} xmi_solid_angle;
+
+/* opaque handles: include/xmi_data.h:71 and src/xmi_aux_f.F90 (xmi_inputFPtr) */
+typedef void *xmi_inputFPtr;
+typedef void *xmi_hdf5FPtr;
+
+/* ---- entry points ------------------------------------------------------ */
+
+/* Default options; replaces xmi_main_options_new (src/xmi_data_structs.c:2531-2566).
+ * The returned struct is malloc'ed. */
+xmi_main_options *xmi_main_options_new(void);
+
+/* Deep copy of an xmi_input with weight fractions normalised per layer and the
+ * derived geometry left for xmi_init_input.  Replaces xmi_input_C2F
+ * (src/xmi_aux_f.F90:782-1025).  Returns 1 on success. */
+int xmi_input_C2F(xmi_input *xmi_inputC, xmi_inputFPtr *xmi_inputFPtr_out);
+
+/* Geometry set-up: detector radius and frames, collimator cone, layer
+ * coordinates.  Replaces xmi_init_input (src/xmi_main.F90:1741-1918). */
+int xmi_init_input(xmi_inputFPtr *inputFPtr);
+
+/* Releases a handle made by xmi_input_C2F (src/xmi_aux_f.F90:1027). */
+void xmi_free_input_F(xmi_inputFPtr *inputFPtr);
+
+/* Writes the interaction tables for the listed elements in the xmimsimdata.h5
+ * layout.  Replaces xmi_db (include/xmi_data.h:85, src/xmi_data.c:149-486);
+ * the physics comes from the built-in analytic provider. */
+int xmi_db(char *filename, int *Zs, int nZ);
+
+/* Reads the tables of the input's elements.  Replaces xmi_init_from_hdf5
+ * (include/xmi_data.h:88, src/xmi_data_f.F90:164-848). */
+int xmi_init_from_hdf5(char *hdf5_file, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *hdf5FPtr,
+                       xmi_main_options *options);
+
+/* Same tables without a file round trip (generated in memory). */
+int xmi_init_from_provider(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *hdf5FPtr, xmi_main_options *options);
+
+/* Binds the input's layers to the loaded tables.  Replaces
+ * xmi_update_input_from_hdf5 (include/xmi_data.h:93, src/xmi_data_f.F90:98-162). */
+int xmi_update_input_from_hdf5(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr);
+
+/* Releases a table handle (include/xmi_data.h:91). */
+void xmi_free_hdf5_F(xmi_hdf5FPtr *hdf5FPtr);
+
+/* Solid-angle grid of the detector, computed on the GPU (1024 x 1024 cells,
+ * xmi_hits_per_single trials per cell).  Drop-in for the OpenCL plugin entry
+ * xmi_solid_angle_calculation_cl (src/xmi_solid_angle_cl.c:120-439), which
+ * itself replaces xmi_solid_angle_calculation_f (src/xmi_solid_angle_f.F90:303).
+ * *solid_angle is malloc'ed; release with xmi_free_solid_angle. */
+int xmi_solid_angle_calculation_cl(xmi_inputFPtr inputFPtr, xmi_solid_angle **solid_angle,
+                                   char *input_string, xmi_main_options *xmo);
+void xmi_free_solid_angle(xmi_solid_angle *solid_angle);
+
+/* Trials per solid-angle cell; the reference's global hits_per_single
+ * (src/xmi_solid_angle_f.F90:43-44), default 5000. */
+extern long xmi_hits_per_single;
+
+/* The photon-history engine on the GPU.  Replaces xmi_main_msim
+ * (include/xmi_main.h:29, src/xmi_main.F90:66-954).
+ * Outputs (malloc'ed, owned by the caller, release with xmi_free):
+ *   *channels        double[n_interactions+1][nchannels], cumulative over
+ *                    interaction order, scaled by live_time;
+ *   *brute_history   double[100][385][n_interactions] (zero: variance
+ *                    reduction never produces detector hits);
+ *   *var_red_history double[100][385][n_interactions], or NULL when variance
+ *                    reduction is off.
+ * Random numbers: Philox4x32-10, one stream per photon history, keyed by the
+ * seed (options->use_default_seeds selects seed 1, otherwise
+ * xmi_set_seed / time-based).  Returns 1 on success, 0 on failure. */
+int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channels,
+                  xmi_main_options *options, double **brute_history, double **var_red_history,
+                  xmi_solid_angle *solid_angles);
+
+/* Releases memory returned by this library (src/xmi_aux.c:217). */
+void xmi_free(void *ptr);
+
+/* Seed used by the next xmi_main_msim / xmi_solid_angle_calculation_cl call
+ * when options->use_default_seeds is 0 (stands in for the /dev/urandom seed
+ * acquisition of src/xmi_random.c:37-120). */
+void xmi_set_seed(uint64_t seed);
+
+/* Timing and counters of the last xmi_main_msim call on this thread. */
+typedef struct _xmi_msim_stats {
+  double transport_ms;        /* device time of the transport kernels */
+  double total_ms;            /* wall time of xmi_main_msim */
+  int64_t histories;          /* primary photons simulated */
+  int64_t interactions;       /* forced interactions processed */
+  double vr_kernel_ms;        /* device time in the detection-weight kernel */
+  double vr_launches;         /* launches of the detection-weight kernel */
+  double step_kernel_ms;
+  double interaction_kernel_ms;
+  int64_t solid_angle_fallbacks;
+} xmi_msim_stats;
+void xmi_msim_get_stats(xmi_msim_stats *out);
+
+/* Last error message of this thread, or "" */
+const char *xmi_msim_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

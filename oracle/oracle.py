@@ -1,0 +1,110 @@
+"""ctypes binding of the oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.  It runs the CPU restatement of the
+reference engine on the handles made by the product library (same process).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+RNG_PER_HISTORY = 0
+RNG_PER_THREAD = 1
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("histories", C.c_int64), ("interactions", C.c_int64), ("vr_calls", C.c_int64),
+                ("solid_angle_fallbacks", C.c_int64), ("seconds", C.c_double)]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    from xmimsim_amd import api
+    lib = C.CDLL(LIB)
+    P = C.POINTER
+    lib.xmi_oracle_main_msim.argtypes = [C.c_void_p, C.c_void_p, C.c_int, P(P(C.c_double)),
+                                         P(api.xmi_main_options), P(P(C.c_double)), P(P(C.c_double)),
+                                         P(api.xmi_solid_angle), C.c_int, C.c_uint64, C.c_int]
+    lib.xmi_oracle_main_msim.restype = C.c_int
+    lib.xmi_oracle_get_stats.argtypes = [P(OracleStats)]
+    lib.xmi_oracle_solid_angle_calculation.argtypes = [C.c_void_p, P(P(api.xmi_solid_angle)), C.c_long, C.c_uint64,
+                                                       C.c_int]
+    lib.xmi_oracle_solid_angle_calculation.restype = C.c_int
+    lib.xmi_oracle_free_solid_angle.argtypes = [P(api.xmi_solid_angle)]
+    lib.xmi_oracle_solid_angle_extents.argtypes = [C.c_void_p] + [P(C.c_double)] * 4
+    lib.xmi_oracle_single_solid_angle.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_long, C.c_uint64,
+                                                  C.c_uint64]
+    lib.xmi_oracle_single_solid_angle.restype = C.c_double
+    lib.xmi_oracle_fpm_first_order.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    lib.xmi_oracle_fpm_first_order.restype = C.c_double
+    lib.xmi_oracle_poly_solve_quadratic.argtypes = [C.c_double] * 3 + [P(C.c_double)] * 2
+    lib.xmi_oracle_poly_solve_quadratic.restype = C.c_int
+    lib.xmi_oracle_energy_from_q.argtypes = [C.c_double] * 3
+    lib.xmi_oracle_energy_from_q.restype = C.c_double
+    lib.xmi_oracle_q_from_energy.argtypes = [C.c_double] * 3
+    lib.xmi_oracle_q_from_energy.restype = C.c_double
+    lib.xmi_oracle_ran_trap.argtypes = [C.c_double] * 5
+    lib.xmi_oracle_ran_trap.restype = C.c_double
+    lib.xmi_oracle_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.xmi_oracle_set_hits_per_single.argtypes = [C.c_long]
+    lib.xmi_oracle_free.argtypes = [C.c_void_p]
+    _lib = lib
+    return lib
+
+
+def run(prepared, seed: int, rng_mode: int = RNG_PER_HISTORY, n_threads: int = 0, n_mpi_hosts: int = 1):
+    """xmi_main_msim restated on the CPU; returns (channels, var_red, brute, stats)."""
+    lib = load()
+    if n_threads <= 0:
+        n_threads = os.cpu_count() or 1
+    ch = C.POINTER(C.c_double)()
+    bh = C.POINTER(C.c_double)()
+    vr = C.POINTER(C.c_double)()
+    ok = lib.xmi_oracle_main_msim(prepared.inputF, prepared.hdf5F, n_mpi_hosts, C.byref(ch),
+                                  C.byref(prepared.options), C.byref(bh), C.byref(vr), prepared.solid_angle,
+                                  rng_mode, seed, n_threads)
+    if not ok:
+        raise RuntimeError("oracle run failed")
+    st = OracleStats()
+    lib.xmi_oracle_get_stats(C.byref(st))
+    n_int = prepared.inp.n_interactions_trajectory
+    nch = prepared.inp.nchannels
+    channels = np.ctypeslib.as_array(ch, shape=(n_int + 1, nch)).copy()
+    var_red = np.ctypeslib.as_array(vr, shape=(100, 385, n_int)).copy()
+    brute = np.ctypeslib.as_array(bh, shape=(100, 385, n_int)).copy()
+    for p in (ch, bh, vr):
+        lib.xmi_oracle_free(p)
+    return channels, var_red, brute, st
+
+
+def solid_angle(prepared, hits: int, seed: int, n_threads: int = 0):
+    """Oracle grid; installs it on `prepared` (owner: oracle) and returns arrays."""
+    lib = load()
+    from xmimsim_amd import api
+    sa = C.POINTER(api.xmi_solid_angle)()
+    if not lib.xmi_oracle_solid_angle_calculation(prepared.inputF, C.byref(sa), hits, seed,
+                                                  n_threads or (os.cpu_count() or 1)):
+        raise RuntimeError("oracle solid angle failed")
+    prepared.set_solid_angle(sa, owner="oracle")
+    prepared._oracle_sa = sa
+    return prepared.solid_angle_array()
+
+
+def fpm_first_order(prepared, Z: int, line: int, cascade: int = 4) -> float:
+    return load().xmi_oracle_fpm_first_order(prepared.inputF, Z, line, cascade)

@@ -1,0 +1,1316 @@
+/*
+ * xmi_oracle.c -- CPU/OpenMP restatement of the reference photon-history
+ * engine (variance-reduction path), used as the correctness oracle and the CPU
+ * baseline of the GPU engine.
+ *
+ * TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.
+ *
+ * Function-by-function correspondence with the reference:
+ *   orc_main_msim            src/xmi_main.F90:66-954   (photon split :253-317,
+ *                            continuous loop :319-572, discrete loop :579-858,
+ *                            reshape and live-time scaling :923-950)
+ *   orc_coords_dir           src/xmi_main.F90:957-1136
+ *   orc_shift_first_layer    src/xmi_main.F90:1140-1186
+ *   orc_simulate_photon      src/xmi_main.F90:1188-1685 (forced step :1414-1518)
+ *   orc_rayleigh             src/xmi_main.F90:1986-2101
+ *   orc_compton              src/xmi_main.F90:2103-2229, energy :4985-5069
+ *   orc_fluorescence         src/xmi_main.F90:2231-2411, CK :5184-5323,
+ *                            line :5352-5437, FY weight
+ *                            src/xmi_variance_reduction.F90:728-750
+ *   orc_update_dirv/elecv    src/xmi_main.F90:5071-5182
+ *   orc_variance_reduction   src/xmi_variance_reduction.F90:29-726,
+ *                            Compton part :949-1101
+ *   orc_get_solid_angle      src/xmi_solid_angle_f.F90:712-801
+ *   orc_solid_angle_inputs   src/xmi_solid_angle_f.F90:62-301
+ *   orc_single_solid_angle   src/xmi_solid_angle_f.F90:432-710
+ *   numerics                 src/xmi_aux_f.F90:1109-1428,1841-2073
+ *
+ * The xraylib calls of the reference are answered by the built-in analytic
+ * provider (xrl_model.h), the same one the GPU engine uses, so the oracle and
+ * the engine sample the same physics from the same tables.
+ *
+ * Documented deviations (numerically negligible):
+ *   - the interaction-probability interval is found by bisection; the
+ *     reference's findpos_fast starts from an approximate guess and can pick a
+ *     neighbouring interval of the (smooth) table;
+ *   - atom selection falls back to the last element when rounding leaves the
+ *     cumulative threshold just below the random number (the reference keeps
+ *     the previous interaction's element);
+ *   - Gaussian variates use Box-Muller instead of GSL's ziggurat.
+ */
+#include "xmi_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../xmimsim_amd/csrc/xmi_internal.h"
+#include "../xmimsim_amd/csrc/xrl_model.h"
+#include "phx_oracle.h"
+
+#define MAX_INT_TRAJ 64
+
+/* constants of src/xmi_main.F90:48-60; Fortran default-real literals are kept
+ * at single precision where the reference writes them without a kind */
+static const double ORC_MEC2 = 9.10938188e-31 * 2.99792458e8 * 2.99792458e8 / 1.602176487e-19 / 1000.0;
+static const double ORC_PI = 3.14159265358979323846;
+
+static xmi_oracle_stats g_stats;
+static long g_hits_per_single = 5000;
+
+void xmi_oracle_set_hits_per_single(long hits) { g_hits_per_single = hits; }
+
+void xmi_oracle_get_stats(xmi_oracle_stats *s) { *s = g_stats; }
+void xmi_oracle_free(void *p) { free(p); }
+
+void xmi_oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  orc_philox(c, key[0], key[1]);
+  memcpy(out, c, sizeof(c));
+}
+
+/* ------------------------------------------------------------------------ */
+/* small numerics */
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double norm3(const double *a) { return sqrt(dot3(a, a)); }
+static void normalize3(double *a) {
+  double n = norm3(a);
+  a[0] /= n;
+  a[1] /= n;
+  a[2] /= n;
+}
+static void cross3(const double *a, const double *b, double *r) {
+  double t0 = a[1] * b[2] - a[2] * b[1], t1 = a[2] * b[0] - a[0] * b[2], t2 = a[0] * b[1] - a[1] * b[0];
+  r[0] = t0;
+  r[1] = t1;
+  r[2] = t2;
+}
+static void matvec3(const double m[3][3], const double *v, double *r) {
+  for (int i = 0; i < 3; i++) r[i] = m[i][0] * v[0] + m[i][1] * v[1] + m[i][2] * v[2];
+}
+static double distance3(const double *a, const double *b) {
+  return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+/* src/xmi_aux_f.F90:1143-1170 */
+static int intersection_plane_line(const double *pp, const double *pn, const double *lp, const double *ld, double *out) {
+  double ItimesN = dot3(ld, pn);
+  if (ItimesN == 0.0) return 0;
+  double diff[3] = {pp[0] - lp[0], pp[1] - lp[1], pp[2] - lp[2]};
+  double d = dot3(diff, pn) / ItimesN;
+  for (int i = 0; i < 3; i++) out[i] = d * ld[i] + lp[i];
+  return 1;
+}
+
+static double interpolate_simple(double x0, double y0, double x1, double y1, double x) {
+  return y0 + ((y1 - y0) * (x - x0) / (x1 - x0));
+}
+
+/* findpos (src/xmi_aux_f.F90:1305-1335), 0-based: first i>=1 with x <= a[i]
+ * gives i-1; -1 when x > a[n-1] */
+static int findpos(const double *a, int n, double x) {
+  if (fabs(x - a[0]) < 1e-10) return 0;
+  if (x > a[n - 1]) return -1;
+  int lo = 1, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) / 2;
+    if (x <= a[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo - 1;
+}
+
+/* bilinear_interpolation (src/xmi_aux_f.F90:1337-1428); array is [n1][n2] */
+static double bilinear(const double *arr, const double *a1, int n1, const double *a2, int n2, double x1, double x2) {
+  int p1 = findpos(a1, n1, x1);
+  int p2 = findpos(a2, n2, x2);
+  if (p1 < 0) p1 = n1 - 2;
+  if (p2 < 0) p2 = n2 - 2;
+  double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
+  double c1 = (a1[p1 + 1] - x1) * (a2[p2 + 1] - x2) / denom;
+  double c2 = (x1 - a1[p1]) * (a2[p2 + 1] - x2) / denom;
+  double c3 = (a1[p1 + 1] - x1) * (x2 - a2[p2]) / denom;
+  double c4 = (x1 - a1[p1]) * (x2 - a2[p2]) / denom;
+  return c1 * arr[(size_t)p1 * n2 + p2] + c2 * arr[(size_t)(p1 + 1) * n2 + p2] + c3 * arr[(size_t)p1 * n2 + p2 + 1] +
+         c4 * arr[(size_t)(p1 + 1) * n2 + p2 + 1];
+}
+
+int xmi_oracle_poly_solve_quadratic(double a, double b, double c, double *r1, double *r2) {
+  if (a == 0.0) {
+    if (b == 0.0) return 0;
+    *r1 = -1.0 * c / b;
+    return 1;
+  }
+  double delta = b * b - 4.0 * a * c;
+  if (delta < 0.0) return 0;
+  if (delta == 0.0) {
+    *r1 = -b / 2.0 / a;
+    *r2 = *r1;
+    return 2;
+  }
+  double sd = sqrt(delta);
+  double t1 = (-b + sd) / 2.0 / a, t2 = (-b - sd) / 2.0 / a;
+  *r1 = t1 < t2 ? t1 : t2;
+  *r2 = t1 < t2 ? t2 : t1;
+  return 2;
+}
+
+/* trapezoid sampler, src/xmi_aux_f.F90:1836-1941; returns -1 on failure */
+double xmi_oracle_ran_trap(double x1, double x2, double y1, double y2, double u) {
+  double m = (y2 - y1) / (x2 - x1);
+  double denom = (x2 - x1) * (y1 - x1 * m) + m * (x2 * x2 - x1 * x1) / 2.0;
+  double a = m / 2.0, b = y1 - x1 * m, c = -x1 * y1 + m * x1 * x1 / 2.0 - denom * u;
+  double r1 = 0.0, r2 = 0.0;
+  if (xmi_oracle_poly_solve_quadratic(a, b, c, &r1, &r2) == 0) return -1.0;
+  if (x1 <= r1 && r1 <= x2) return r1;
+  if (x1 <= r2 && r2 <= x2) return r2;
+  return -1.0;
+}
+
+double xmi_oracle_q_from_energy(double old_energy, double new_energy, double theta) {
+  double ct = cos(theta);
+  double Q = 137.0 * (new_energy - old_energy + (1.0 - ct) * old_energy * new_energy / ORC_MEC2);
+  return Q / sqrt(new_energy * new_energy + old_energy * old_energy - 2.0 * old_energy * new_energy * ct);
+}
+
+double xmi_oracle_energy_from_q(double init_energy, double Q, double theta) {
+  double a = init_energy, b = ORC_MEC2, c = cos(theta);
+  if (fabs(c - 1.0) < 1e-8) return 0.0;
+  if (fabs(Q) < 1e-4) return init_energy / (1.0 + init_energy * (1.0 - c) / ORC_MEC2);
+  double d = 1.0 + a / b - a * c / b;
+  double aq = 137.0 * 137.0 * d * d - Q * Q;
+  double bq = -2.0 * 137.0 * 137.0 * a * d + 2.0 * a * c * Q * Q;
+  double cq = 137.0 * 137.0 * a * a - a * a * Q * Q;
+  double E1 = 0.0, E2 = 0.0;
+  int rv = xmi_oracle_poly_solve_quadratic(aq, bq, cq, &E1, &E2);
+  if (rv == 0) return 0.0;
+  if (rv == 1) E2 = E1;
+  double Q1 = xmi_oracle_q_from_energy(init_energy, E1, theta);
+  double Q2 = xmi_oracle_q_from_energy(init_energy, E2, theta);
+  if (Q * Q1 > 0.0) return E1;
+  if (Q * Q2 > 0.0) return E2;
+  if (fabs(E1 - E2) < 1e-10 || fabs(Q1 - Q2) < 1e-10) return E1;
+  return -1.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* run context */
+
+typedef struct {
+  const xmi_inputF *in;
+  const xmi_hdf5F *h;
+  const xmi_main_options *opt;
+  const xmi_solid_angle *sa;
+  const xrl_atom **atoms;     /* per hdf5 index */
+  double *line_energies;      /* [nZ][219] by hdf5 index */
+  double *precalc_mu;         /* [n_layers][nZ][219] */
+  int nZ;
+  int n_int;
+  int nch;
+  int line_last;
+  int cascade_type;
+  long hits_per_single;
+  uint64_t seed;
+  int rng_mode;
+} orc_ctx;
+
+typedef struct {
+  double *channels;           /* [(n_int+1)][nch] */
+  double *var_red;            /* [100][385][n_int] Fortran-equivalent indexed as [Z-1][row-1][n-1] */
+  int64_t interactions, vr_calls, fallbacks;
+} orc_tally;
+
+typedef struct {
+  double coords[3], dirv[3], elecv[3];
+  double theta, phi;
+  double energy, weight;
+  double mus[64];
+  int current_layer;          /* 1-based */
+  int n_interactions;
+  int last_interaction;
+  int current_element;        /* Z */
+  int current_element_index;  /* 0-based in layer */
+  int hist[MAX_INT_TRAJ + 1][3];
+  int energy_changed;
+  orc_rng *tr;                /* transport stream */
+  orc_rng *vr;                /* detection-weight stream */
+} orc_photon;
+
+static double layer_mu(const orc_ctx *c, const xmi_layerF *L, double E) {
+  double rv = 0.0;
+  for (int i = 0; i < L->n_elements; i++) rv += xrl_cs_total(xrl_atom_get(L->Z[i]), E) * L->weight[i];
+  return rv;
+}
+
+static void mu_calc_all(const orc_ctx *c, double E, double *mus) {
+  for (int i = 0; i < c->in->n_layers; i++) mus[i] = layer_mu(c, &c->in->layers[i], E);
+}
+
+static inline int channel_of(const orc_ctx *c, double E) {
+  if (E >= XMI_ENERGY_THRESHOLD) return (int)((E - c->in->detector.zero) / c->in->detector.gain);
+  return -1;
+}
+
+static inline void tally_channel(const orc_ctx *c, orc_tally *t, int n, double E, double w) {
+  int ch = channel_of(c, E);
+  if (ch >= 0 && ch <= c->nch - 1)
+    for (int k = n; k <= c->n_int; k++) t->channels[(size_t)k * c->nch + ch] += w;
+}
+
+static inline void tally_history(const orc_ctx *c, orc_tally *t, int Z, int row, int n, double w) {
+  t->var_red[((size_t)(Z - 1) * XMI_HIST_ROWS + (row - 1)) * c->n_int + (n - 1)] += w;
+}
+
+/* ------------------------------------------------------------------------ */
+/* solid angle */
+
+typedef struct {
+  int collimator_present;
+  double detector_radius, collimator_radius, collimator_height;
+} orc_det;
+
+static double orc_single_solid_angle(const orc_det *d, double r1, double theta1, long hits, orc_rng *rng) {
+  double r, theta, full_cone_base_radius;
+  int outside_collimator = 0;
+  if (!d->collimator_present) {
+    r = r1;
+    theta = theta1;
+    full_cone_base_radius = d->detector_radius;
+  } else if (fabs(d->collimator_radius - d->detector_radius) < 0.000001) {
+    if (r1 * cos(theta1) <= d->detector_radius) {
+      r = r1;
+      theta = theta1;
+      full_cone_base_radius = d->detector_radius;
+    } else {
+      r = sqrt(r1 * r1 - 2.0 * r1 * sin(theta1) * d->collimator_height + d->collimator_height * d->collimator_height);
+      theta = acos(r1 * cos(theta1) / r);
+      full_cone_base_radius = d->collimator_radius;
+    }
+    outside_collimator = r1 * sin(theta1) > d->collimator_height;
+  } else {
+    if (r1 * cos(theta1) <= d->detector_radius &&
+        r1 * sin(theta1) <= d->collimator_height * (r1 * cos(theta1) - d->detector_radius) /
+                                (d->collimator_radius - d->detector_radius)) {
+      r = r1;
+      theta = theta1;
+      full_cone_base_radius = d->detector_radius;
+    } else if (r1 * sin(theta1) <= d->collimator_height) {
+      return 0.0;
+    } else {
+      r = sqrt(r1 * r1 - 2.0 * r1 * sin(theta1) * d->collimator_height + d->collimator_height * d->collimator_height);
+      theta = acos(r1 * cos(theta1) / r);
+      full_cone_base_radius = d->collimator_radius;
+    }
+    outside_collimator = r1 * sin(theta1) > d->collimator_height;
+  }
+  double beta = atan(full_cone_base_radius / r);
+  double alpha1 = atan(full_cone_base_radius * sin(theta) / (r - full_cone_base_radius * cos(theta)));
+  if (alpha1 <= 0.0) alpha1 += ORC_PI;
+  double full_cone_apex = beta > alpha1 ? beta : alpha1;
+  double cos_apex = cos(full_cone_apex);
+  double full_cone_solid_angle = 2.0 * ORC_PI * (1.0 - cos_apex);
+  /* rotation matrix columns: (1,0,0), (0,-sin,cos), (0,-cos,-sin) */
+  double R[3][3] = {{1.0, 0.0, 0.0}, {0.0, -sin(theta), -cos(theta)}, {0.0, cos(theta), -sin(theta)}};
+  const double det_point[3] = {0.0, 0.0, 0.0}, det_normal[3] = {0.0, 0.0, 1.0};
+  const double coll_point[3] = {0.0, 0.0, d->collimator_height};
+  double line_point[3] = {0.0, r1 * cos(theta1), r1 * sin(theta1)};
+  long hitsc = 0;
+  for (long i = 0; i < hits; i++) {
+    double theta_rng = acos(1.0 - orc_uniform(rng) * (1.0 - cos_apex));
+    double phi_rng = orc_uniform(rng) * 2.0 * ORC_PI;
+    double cone[3] = {sin(theta_rng) * cos(phi_rng), sin(theta_rng) * sin(phi_rng), cos(theta_rng)};
+    double dv[3];
+    matvec3(R, cone, dv);
+    if (dot3(det_normal, dv) >= 0.0) continue;
+    double ip[3];
+    if (outside_collimator) {
+      if (!intersection_plane_line(coll_point, det_normal, line_point, dv, ip)) continue;
+      ip[2] = 0.0;
+      if (norm3(ip) > d->collimator_radius) continue;
+    }
+    if (!intersection_plane_line(det_point, det_normal, line_point, dv, ip)) continue;
+    if (norm3(ip) <= d->detector_radius) hitsc++;
+  }
+  return full_cone_solid_angle * (double)hitsc / (double)hits;
+}
+
+static orc_det make_det(const xmi_inputF *in) {
+  orc_det d;
+  d.collimator_present = in->collimator_present;
+  d.detector_radius = in->detector_radius;
+  d.collimator_radius = in->collimator_radius;
+  d.collimator_height = in->geometry.collimator_height;
+  return d;
+}
+
+/* returns 1 when the Monte Carlo fallback was used */
+static int orc_get_solid_angle(const orc_ctx *c, const double *coords, orc_rng *rng, double *omega) {
+  const xmi_inputF *in = c->in;
+  const xmi_solid_angle *sa = c->sa;
+  double r = distance3(in->geometry.p_detector_window, coords);
+  double dv[3] = {coords[0] - in->geometry.p_detector_window[0], coords[1] - in->geometry.p_detector_window[1],
+                  coords[2] - in->geometry.p_detector_window[2]};
+  normalize3(dv);
+  double temp_theta = acos(dot3(dv, in->geometry.n_detector_orientation));
+  if (temp_theta > ORC_PI / 2.0) temp_theta = ORC_PI - temp_theta;
+  double theta = ORC_PI / 2.0 - temp_theta;
+  if (theta < sa->grid_dims_theta_vals[0]) {
+    *omega = 0.0;
+    return 0;
+  }
+  int p1 = findpos(sa->grid_dims_r_vals, (int)sa->grid_dims_r_n, r);
+  int p2 = findpos(sa->grid_dims_theta_vals, (int)sa->grid_dims_theta_n, theta);
+  if (p1 < 0 || p2 < 0) {
+    orc_det d = make_det(in);
+    *omega = orc_single_solid_angle(&d, r, theta, c->hits_per_single, rng);
+    return 1;
+  }
+  /* grid is [theta][r] in C order, i.e. Fortran (r, theta) */
+  const double *a1 = sa->grid_dims_r_vals, *a2 = sa->grid_dims_theta_vals;
+  int n_r = (int)sa->grid_dims_r_n;
+  double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
+  double c1 = (a1[p1 + 1] - r) * (a2[p2 + 1] - theta) / denom;
+  double c2 = (r - a1[p1]) * (a2[p2 + 1] - theta) / denom;
+  double c3 = (a1[p1 + 1] - r) * (theta - a2[p2]) / denom;
+  double c4 = (r - a1[p1]) * (theta - a2[p2]) / denom;
+  const double *g = sa->solid_angles;
+  *omega = c1 * g[(size_t)p2 * n_r + p1] + c2 * g[(size_t)p2 * n_r + p1 + 1] + c3 * g[(size_t)(p2 + 1) * n_r + p1] +
+           c4 * g[(size_t)(p2 + 1) * n_r + p1 + 1];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* photon kinematics */
+
+static void orc_update_dirv(orc_photon *p, double theta_i, double phi_i) {
+  double phi_new = phi_i;
+  if (phi_i > 2.0 * ORC_PI) phi_new = phi_i - 2.0 * ORC_PI;
+  else if (phi_i < 0.0) phi_new = phi_i + 2.0 * ORC_PI;
+  double cp = cos(p->phi), sp = sin(p->phi), ct = cos(p->theta), st = sin(p->theta);
+  double T[3][3] = {{ct * cp, -sp, st * cp}, {ct * sp, cp, st * sp}, {-st, 0.0, ct}};
+  double ts = sin(theta_i);
+  double v[3] = {ts * cos(phi_new), ts * sin(phi_new), cos(theta_i)};
+  matvec3(T, v, p->dirv);
+  normalize3(p->dirv);
+  p->theta = acos(p->dirv[2]);
+  p->phi = atan2(p->dirv[1], p->dirv[0]);
+  if (p->phi > 2.0 * ORC_PI) p->phi -= 2.0 * ORC_PI;
+  else if (p->phi < 0.0) p->phi += 2.0 * ORC_PI;
+}
+
+static void orc_update_elecv(orc_photon *p) {
+  double cosalfa = dot3(p->dirv, p->elecv);
+  double c_alfa = acos(cosalfa);
+  double sinalfa = sin(c_alfa);
+  double c_ae = 1.0 / sinalfa;
+  double c_be = -c_ae * cosalfa;
+  for (int i = 0; i < 3; i++) p->elecv[i] = c_ae * p->elecv[i] + c_be * p->dirv[i];
+  normalize3(p->elecv);
+}
+
+static double orc_phi0(const orc_photon *p) {
+  double cp = cos(p->phi), sp = sin(p->phi), ct = cos(p->theta), st = sin(p->theta);
+  double v1[3] = {cp * ct, ct * sp, -st}, v2[3] = {sp, -cp, 0.0};
+  double cosphi0 = dot3(p->elecv, v1), sinphi0 = dot3(p->elecv, v2);
+  if (fabs(cosphi0) > 1.0) cosphi0 = cosphi0 > 0 ? 1.0 : -1.0;
+  double phi0 = acos(cosphi0);
+  if (sinphi0 > 0.0) phi0 = -phi0;
+  return phi0;
+}
+
+static const xmi_hdf5_ZF *photon_hdf5(const orc_ctx *c, const orc_photon *p) {
+  const xmi_layerF *L = &c->in->layers[p->current_layer - 1];
+  return &c->h->Zs[L->hdf5_index[p->current_element_index]];
+}
+
+static int orc_rayleigh(const orc_ctx *c, orc_photon *p) {
+  const xmi_hdf5_ZF *z = photon_hdf5(c, p);
+  double r = orc_uniform(p->tr);
+  double theta_i = bilinear(z->RayleighTheta_ICDF, z->Energies, z->n_e, z->RandomNumbers, z->n_r, p->energy, r);
+  double tt = sin(theta_i);
+  tt = tt * tt;
+  tt = tt / (4.0 - 2.0 * tt);
+  double r2 = orc_uniform(p->tr);
+  double phi_i = bilinear(c->h->Phi_ICDF, c->h->Thetas, c->h->phi_n_theta, c->h->RandomNumbers, c->h->phi_n_r, tt, r2);
+  double phi0 = orc_phi0(p);
+  orc_update_dirv(p, theta_i, phi0 + phi_i);
+  orc_update_elecv(p);
+  p->hist[p->n_interactions][0] = XMI_RAYLEIGH_INTERACTION;
+  p->hist[p->n_interactions][1] = p->current_element;
+  return 1;
+}
+
+static double doppler_pz(const xmi_hdf5_ZF *z, double r) {
+  const double *rn = z->cp_random_numbers;
+  int pos = (int)(r / (rn[1] - rn[0]));
+  return interpolate_simple(rn[pos], z->cp_total_icdf[pos], rn[pos + 1], z->cp_total_icdf[pos + 1], r);
+}
+
+static int orc_compton(const orc_ctx *c, orc_photon *p) {
+  const xmi_hdf5_ZF *z = photon_hdf5(c, p);
+  double theta_i = bilinear(z->ComptonTheta_ICDF, z->Energies, z->n_e, z->RandomNumbers, z->n_r, p->energy,
+                            orc_uniform(p->tr));
+  double K0K = 1.0 + p->energy * (1.0 - cos(theta_i)) / ORC_MEC2;
+  double tt = sin(theta_i);
+  tt = tt * tt;
+  tt = tt / (K0K + (1.0 / K0K) - tt) / 2.0;
+  double phi_i = bilinear(c->h->Phi_ICDF, c->h->Thetas, c->h->phi_n_theta, c->h->RandomNumbers, c->h->phi_n_r, tt,
+                          orc_uniform(p->tr));
+  double phi0 = orc_phi0(p);
+  /* xmi_update_photon_energy_compton2 */
+  {
+    const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
+    double energy = p->energy * 1000.0;
+    double c_lamb0 = cc / energy;
+    double sth2 = sin(theta_i / 2.0);
+    for (;;) {
+      double r = orc_uniform(p->tr);
+      double pz = doppler_pz(z, r);
+      if (orc_uniform(p->tr) < 0.5) pz = -pz;
+      double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
+      double c_lamb = c_lamb0 + dlamb;
+      energy = cc / c_lamb / 1000.0;
+      if (energy <= p->energy) break;
+    }
+    p->energy = energy;
+    p->energy_changed = 0;
+    mu_calc_all(c, p->energy, p->mus);
+  }
+  if (p->energy == 0.0) return 1;
+  orc_update_dirv(p, theta_i, phi_i + phi0);
+  orc_update_elecv(p);
+  double ct = cos(theta_i), cph = cos(phi_i), sph = sin(phi_i);
+  double pp = 2.0 * ((ct * cph) * (ct * cph) + sph * sph);
+  double rat = 1.0 / (1.0 + (1.0 - ct) * p->energy / (double)510.998910f);
+  double rk = rat - 2.0 + 1.0 / rat;
+  pp = pp / (rk + pp);
+  double r = orc_uniform(p->tr);
+  double w_h = (1.0 + pp) / 2.0;
+  if (r > w_h) cross3(p->dirv, p->elecv, p->elecv);
+  p->hist[p->n_interactions][0] = XMI_COMPTON_INTERACTION;
+  p->hist[p->n_interactions][1] = p->current_element;
+  return 1;
+}
+
+/* src/xmi_main.F90:5184-5323 */
+static void orc_coster_kronig(const xrl_atom *a, orc_rng *rng, int *shell) {
+  const double *f = a->ck;
+  if (*shell == 1 || *shell == 2) {
+    for (;;) {
+      double r = orc_uniform(rng);
+      if (*shell == 2) {
+        if (r < f[XRL_FL23]) *shell = 3;
+        return;
+      }
+      /* L1 */
+      if (r < f[XRL_FL12]) {
+        *shell = 2;
+        continue;
+      }
+      if (r < f[XRL_FL12] + f[XRL_FL13]) *shell = 3;
+      return;
+    }
+  } else if (*shell >= 4 && *shell <= 7) {
+    for (;;) {
+      double r = orc_uniform(rng);
+      if (*shell == 7) {
+        if (r < f[XRL_FM45]) *shell = 8;
+        return;
+      } else if (*shell == 6) {
+        if (r < f[XRL_FM34]) {
+          *shell = 7;
+          continue;
+        }
+        if (r < f[XRL_FM34] + f[XRL_FM35]) *shell = 8;
+        return;
+      } else if (*shell == 5) {
+        double s = f[XRL_FM23];
+        if (r < s) {
+          *shell = 6;
+          continue;
+        }
+        s += f[XRL_FM24];
+        if (r < s) {
+          *shell = 7;
+          continue;
+        }
+        s += f[XRL_FM25];
+        if (r < s) *shell = 8;
+        return;
+      } else { /* M1 */
+        double s = f[XRL_FM12];
+        if (r < s) {
+          *shell = 5;
+          continue;
+        }
+        s += f[XRL_FM13];
+        if (r < s) {
+          *shell = 6;
+          continue;
+        }
+        s += f[XRL_FM14];
+        if (r < s) {
+          *shell = 7;
+          continue;
+        }
+        s += f[XRL_FM15];
+        if (r < s) *shell = 8;
+        return;
+      }
+    }
+  }
+}
+
+static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
+  const xmi_layerF *L = &c->in->layers[p->current_layer - 1];
+  int hidx = L->hdf5_index[p->current_element_index];
+  const xrl_atom *a = c->atoms[hidx];
+  const xmi_hdf5_ZF *z = &c->h->Zs[hidx];
+  double photo_total = xrl_cs_photo_total(a, p->energy);
+  double sumz = 0.0;
+  double r = orc_uniform(p->tr);
+  int max_shell = c->opt->use_M_lines ? XRL_M5_SHELL : XRL_L3_SHELL;
+  int shell, found = 0;
+  for (shell = 0; shell <= max_shell; shell++) {
+    sumz += xrl_cs_photo_partial(a, shell, p->energy) / photo_total;
+    if (r < sumz) {
+      found = 1;
+      break;
+    }
+  }
+  if (!found) {
+    p->energy = 0.0;
+    return 1;
+  }
+  /* xmi_fluorescence_yield_check_varred_optim: draws a number it never uses */
+  (void)orc_uniform(p->tr);
+  p->weight *= z->FluorYieldsCorr[shell];
+  orc_coster_kronig(a, p->tr, &shell);
+  /* xmi_fluorescence_line_check */
+  r = orc_uniform(p->tr);
+  sumz = 0.0;
+  int first, last, line = 0;
+  xrl_shell_lines(shell, &first, &last);
+  found = 0;
+  for (int l = first; l <= last; l++) {
+    sumz += a->rad_rate[l];
+    if (r < sumz) {
+      found = 1;
+      line = l;
+      break;
+    }
+  }
+  if (!found) {
+    p->energy = 0.0;
+    return 1;
+  }
+  p->energy = c->line_energies[(size_t)hidx * XMI_NLINES_VR + (line - 1)];
+  p->energy_changed = 0;
+  for (int i = 0; i < c->in->n_layers; i++)
+    p->mus[i] = c->precalc_mu[((size_t)i * c->nZ + hidx) * XMI_NLINES_VR + (line - 1)];
+  double theta_i = acos(-2.0 * orc_uniform(p->tr) + 1.0);
+  double phi_i = 2.0 * ORC_PI * orc_uniform(p->tr);
+  orc_update_dirv(p, theta_i, phi_i);
+  orc_update_elecv(p);
+  p->hist[p->n_interactions][0] = -line;
+  p->hist[p->n_interactions][1] = p->current_element;
+  p->hist[p->n_interactions][2] = c->h->uniqZ[p->current_element];
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* detection-weight estimator, src/xmi_variance_reduction.F90:29-726 */
+
+static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t) {
+  const xmi_inputF *in = c->in;
+  if (p->energy <= XMI_ENERGY_THRESHOLD) return;
+  t->vr_calls++;
+  orc_rng *rng = p->vr;
+  double radius = sqrt(orc_uniform(rng)) * in->detector_radius;
+  double theta = 2.0 * ORC_PI * orc_uniform(rng);
+  double detector_point[3] = {0.0, cos(theta) * radius, sin(theta) * radius};
+  double rel[3] = {p->coords[0] - in->geometry.p_detector_window[0], p->coords[1] - in->geometry.p_detector_window[1],
+                   p->coords[2] - in->geometry.p_detector_window[2]};
+  double lpoint[3], dirv[3], ldirv[3];
+  matvec3(in->n_detector_orientation_inverse, rel, lpoint);
+  matvec3(in->n_detector_orientation_inverse, p->dirv, dirv);
+  for (int i = 0; i < 3; i++) ldirv[i] = detector_point[i] - lpoint[i];
+  if (ldirv[0] >= 0.0) return;
+  double total_distance = distance3(detector_point, lpoint);
+  normalize3(dirv);
+  normalize3(ldirv);
+  double new_dirv_coords[3];
+  matvec3(in->n_detector_orientation_new, ldirv, new_dirv_coords);
+  double dp = dot3(dirv, ldirv);
+  if (dp > 1.0) dp = 1.0;
+  else if (dp < -1.0) dp = -1.0;
+  theta = acos(dp);
+  double proj[3], nd = dot3(new_dirv_coords, p->dirv);
+  for (int i = 0; i < 3; i++) proj[i] = new_dirv_coords[i] - nd * p->dirv[i];
+  normalize3(proj);
+  double en = norm3(p->elecv);
+  double elecv_norm[3] = {p->elecv[0] / en, p->elecv[1] / en, p->elecv[2] / en};
+  dp = dot3(proj, elecv_norm);
+  if (dp > 1.0) dp = 1.0;
+  else if (dp < -1.0) dp = -1.0;
+  double phi = acos(dp);
+
+  int step_do_max, step_do_dir;
+  if (dot3(new_dirv_coords, in->geometry.n_sample_orientation) > 0.0) {
+    step_do_max = in->n_layers;
+    step_do_dir = 1;
+  } else {
+    step_do_max = 1;
+    step_do_dir = -1;
+  }
+  double distances[64];
+  for (int i = 0; i < in->n_layers; i++) distances[i] = 0.0;
+  double temp_coords[3] = {p->coords[0], p->coords[1], p->coords[2]};
+  for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
+    double plane_point[3] = {0.0, 0.0,
+                             step_do_dir == 1 ? in->layers[i - 1].Z_coord_end : in->layers[i - 1].Z_coord_begin};
+    double ip[3];
+    if (!intersection_plane_line(plane_point, in->geometry.n_sample_orientation, temp_coords, new_dirv_coords, ip))
+      return;
+    distances[i - 1] = distance3(temp_coords, ip);
+    if (distances[i - 1] > total_distance) {
+      distances[i - 1] = total_distance;
+      break;
+    }
+    temp_coords[0] = ip[0];
+    temp_coords[1] = ip[1];
+    temp_coords[2] = ip[2];
+    total_distance -= distances[i - 1];
+  }
+  double murhod = 0.0;
+  for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
+    murhod += p->mus[j - 1] * in->layers[j - 1].density * distances[j - 1];
+  double Pesc_rayl = exp(-murhod);
+
+  double omega;
+  t->fallbacks += orc_get_solid_angle(c, p->coords, rng, &omega);
+  double Pdir_fluo = omega / 4.0 / ORC_PI;
+
+  const xmi_layerF *L = &in->layers[p->current_layer - 1];
+  int n_ia = p->n_interactions;
+  double mu_cur = p->mus[p->current_layer - 1];
+  for (int i = 0; i < L->n_elements; i++) {
+    int Zi = L->Z[i];
+    int hidx = L->hdf5_index[i];
+    const xrl_atom *a = c->atoms[hidx];
+    const xmi_hdf5_ZF *z = &c->h->Zs[hidx];
+    /* Rayleigh */
+    double Pconv = L->weight[i] / mu_cur;
+    double Pdir = omega * xrl_dcsp_rayl(a, p->energy, theta, phi);
+    double tw = Pconv * Pdir * Pesc_rayl * p->weight;
+    tally_history(c, t, Zi, 384, n_ia, tw);
+    tally_channel(c, t, n_ia, p->energy, tw);
+    /* Compton, xmi_compton_varred2 with N_COMPTON_VARRED = 1 */
+    {
+      const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
+      double energy = p->energy * 1000.0;
+      double c_lamb0 = cc / energy;
+      double sth2 = sin(theta / 2.0);
+      int it = 0;
+      int nrn = z->cp_len;
+      for (;;) {
+        double r = orc_uniform(rng);
+        int pos = (int)(r / (z->cp_random_numbers[1] - z->cp_random_numbers[0]));
+        if (pos == nrn - 2) continue; /* last interval excluded */
+        double pz = interpolate_simple(z->cp_random_numbers[pos], z->cp_total_icdf[pos], z->cp_random_numbers[pos + 1],
+                                       z->cp_total_icdf[pos + 1], r);
+        if (orc_uniform(rng) < 0.5) pz = -pz;
+        double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
+        energy = cc / (c_lamb0 + dlamb) / 1000.0;
+        if (energy <= p->energy) break;
+        if (it == 100) {
+          energy = -1.0;
+          break;
+        }
+        it++;
+      }
+      if (energy > 0.0) {
+        double m2 = 0.0;
+        for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
+          m2 += layer_mu(c, &in->layers[j - 1], energy) * in->layers[j - 1].density * distances[j - 1];
+        double Pesc_comp = exp(-m2);
+        double Pdir_c = omega * xrl_dcsp_compt(a, p->energy, theta, phi);
+        double twc = Pconv * Pdir_c * Pesc_comp * p->weight / 1.0;
+        tally_history(c, t, Zi, 385, n_ia, twc);
+        tally_channel(c, t, n_ia, energy, twc);
+      }
+    }
+    /* fluorescence */
+    double P[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (n_ia > 1 && p->hist[n_ia - 1][0] < 0) {
+      int prev_line = -p->hist[n_ia - 1][0];
+      int prev_u = p->hist[n_ia - 1][2] - 1;
+      int smax = c->opt->use_M_lines ? 8 : 3;
+      for (int s = 0; s <= smax; s++)
+        P[s] = z->precalc_xrf_cs[((size_t)s * c->nZ + prev_u) * XMI_NLINES_VR + (prev_line - 1)];
+    } else {
+      xrl_shell_cross_sections(a, c->cascade_type, p->energy, P);
+      if (!c->opt->use_M_lines)
+        for (int s = 4; s < 9; s++) P[s] = 0.0;
+    }
+    for (int line = 1; line <= c->line_last; line++) {
+      double ef = c->line_energies[(size_t)hidx * XMI_NLINES_VR + (line - 1)];
+      if (ef < XMI_ENERGY_THRESHOLD) continue;
+      int s = xrl_line_shell(line);
+      if (P[s] == 0.0) continue;
+      double Pc = L->weight[i] * P[s] * a->fluor_yield[s] * a->rad_rate[line] / mu_cur;
+      double m2 = 0.0;
+      for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
+        m2 += c->precalc_mu[((size_t)(j - 1) * c->nZ + hidx) * XMI_NLINES_VR + (line - 1)] * in->layers[j - 1].density *
+              distances[j - 1];
+      double Pesc = exp(-m2);
+      double twf = Pc * Pdir_fluo * Pesc * p->weight;
+      if (twf == 0.0) continue;
+      tally_history(c, t, Zi, line, n_ia, twf);
+      tally_channel(c, t, n_ia, ef, twf);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* transport of one photon, VR branch of src/xmi_main.F90:1188-1685 */
+
+static void reseed_interaction(const orc_ctx *c, orc_photon *p, orc_rng *tr, orc_rng *vr, uint64_t hid, int k) {
+  if (c->rng_mode != XMI_ORACLE_RNG_PER_HISTORY) return;
+  orc_rng_init(tr, c->seed, ((uint32_t)(k & 0xFF) << 4) | 1u, hid);
+  orc_rng_init(vr, c->seed, ((uint32_t)(k & 0xFF) << 4) | 2u, hid);
+  p->tr = tr;
+  p->vr = vr;
+}
+
+static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, uint64_t hid, orc_rng *tr, orc_rng *vr) {
+  const xmi_inputF *in = c->in;
+  for (;;) {
+    if (p->energy < XMI_ENERGY_THRESHOLD) break;
+    if (p->energy_changed) {
+      mu_calc_all(c, p->energy, p->mus);
+      p->energy_changed = 0;
+    }
+    int step_do_max, step_do_dir;
+    if (dot3(p->dirv, in->geometry.n_sample_orientation) > 0.0) {
+      step_do_max = in->n_layers;
+      step_do_dir = 1;
+    } else {
+      step_do_max = 1;
+      step_do_dir = -1;
+    }
+    reseed_interaction(c, p, tr, vr, hid, p->n_interactions + 1);
+    double interactionR = orc_uniform(p->tr);
+    if (p->n_interactions == c->n_int) break;
+    double distances[64];
+    double lp[3] = {p->coords[0], p->coords[1], p->coords[2]};
+    for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
+      double pp[3] = {0.0, 0.0, step_do_dir == 1 ? in->layers[i - 1].Z_coord_end : in->layers[i - 1].Z_coord_begin};
+      double ip[3];
+      if (!intersection_plane_line(pp, in->geometry.n_sample_orientation, lp, p->dirv, ip)) return 0;
+      distances[i - 1] = distance3(lp, ip);
+      lp[0] = ip[0];
+      lp[1] = ip[1];
+      lp[2] = ip[2];
+    }
+    double Pabs = 0.0;
+    for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir)
+      Pabs += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
+    double Pabs2 = -1.0 * expm1(-1.0 * Pabs);
+    p->weight *= Pabs2;
+    double negln = -1.0 * log1p(-1.0 * interactionR * Pabs2);
+    int my_index = p->current_layer;
+    double my_sum = 0.0;
+    for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
+      my_sum += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
+      if (my_sum > negln) {
+        my_index = i;
+        break;
+      }
+    }
+    double mrm = p->mus[my_index - 1] * in->layers[my_index - 1].density;
+    double temp_sum = 0.0;
+    for (int i = p->current_layer; step_do_dir > 0 ? i <= my_index : i >= my_index; i += step_do_dir)
+      temp_sum += (1.0 - (p->mus[i - 1] * in->layers[i - 1].density / mrm)) * distances[i - 1];
+    temp_sum = temp_sum - 1.0 * log1p(-1.0 * interactionR * Pabs2) / mrm;
+    for (int i = 0; i < 3; i++) p->coords[i] += temp_sum * p->dirv[i];
+    p->current_layer = my_index;
+
+    if (p->n_interactions == c->n_int) break;
+    p->n_interactions++;
+    t->interactions++;
+    orc_variance_reduction(c, p, t);
+
+    /* atom selection */
+    const xmi_layerF *L = &in->layers[p->current_layer - 1];
+    interactionR = orc_uniform(p->tr);
+    double thr = 0.0;
+    int sel = L->n_elements - 1;
+    for (int i = 0; i < L->n_elements; i++) {
+      thr += L->weight[i] * xrl_cs_total(c->atoms[L->hdf5_index[i]], p->energy) / p->mus[p->current_layer - 1];
+      if (interactionR < thr) {
+        sel = i;
+        break;
+      }
+    }
+    p->current_element = L->Z[sel];
+    p->current_element_index = sel;
+    /* interaction type */
+    interactionR = orc_uniform(p->tr);
+    const xmi_hdf5_ZF *z = &c->h->Zs[L->hdf5_index[sel]];
+    int pos = findpos(z->ip_energies, z->ip_len, p->energy);
+    if (pos < 0) return 0;
+    double pr = interpolate_simple(z->ip_energies[pos], z->ip_rayl[pos], z->ip_energies[pos + 1], z->ip_rayl[pos + 1],
+                                   p->energy);
+    double pc = interpolate_simple(z->ip_energies[pos], z->ip_rayl_compt[pos], z->ip_energies[pos + 1],
+                                   z->ip_rayl_compt[pos + 1], p->energy);
+    int rv;
+    if (interactionR < pr) {
+      p->last_interaction = XMI_RAYLEIGH_INTERACTION;
+      rv = orc_rayleigh(c, p);
+    } else if (interactionR < pc) {
+      p->last_interaction = XMI_COMPTON_INTERACTION;
+      rv = orc_compton(c, p);
+    } else {
+      p->last_interaction = XMI_PHOTOELECTRIC_INTERACTION;
+      rv = orc_fluorescence(c, p);
+    }
+    if (rv != 1) break;
+  }
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* source, src/xmi_main.F90:957-1186 */
+
+static double orc_gauss(orc_rng *r, double sigma) {
+  double u1 = orc_uniform(r), u2 = orc_uniform(r);
+  return sigma * sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * ORC_PI * u2);
+}
+
+static void orc_coords_dir(const orc_ctx *c, orc_rng *rng, double sx, double sxp, double sy, double syp, orc_photon *p) {
+  const xmi_geometry *g = &c->in->geometry;
+  double x1, y1;
+  if (fabs(sx * sy) < 1.0e-20) {
+    double x1_max = atan(g->slit_size_x / g->d_source_slit / 2.0);
+    double y1_max = atan(g->slit_size_y / g->d_source_slit / 2.0);
+    x1 = x1_max * (-1.0 + 2.0 * orc_uniform(rng));
+    y1 = y1_max * (-1.0 + 2.0 * orc_uniform(rng));
+    p->coords[0] = p->coords[1] = p->coords[2] = 0.0;
+  } else {
+    x1 = orc_gauss(rng, sxp);
+    y1 = orc_gauss(rng, syp);
+    p->coords[0] = orc_gauss(rng, sx) - g->d_source_slit * sin(x1);
+    p->coords[1] = orc_gauss(rng, sy) - g->d_source_slit * sin(y1);
+    p->coords[2] = 0.0;
+  }
+  p->dirv[0] = tan(x1);
+  p->dirv[1] = tan(y1);
+  p->dirv[2] = 1.0;
+  normalize3(p->dirv);
+  p->theta = acos(p->dirv[2]);
+  p->phi = atan2(p->dirv[1], p->dirv[0]);
+}
+
+static int orc_shift_first_layer(const orc_ctx *c, orc_photon *p) {
+  const xmi_inputF *in = c->in;
+  p->current_layer = 0;
+  if (p->coords[2] >= in->layers[0].Z_coord_begin) {
+    for (int i = 0; i < in->n_layers; i++)
+      if (p->coords[2] < in->layers[i].Z_coord_end) {
+        p->current_layer = i + 1;
+        return 1;
+      }
+    return 0;
+  }
+  double pp[3] = {0.0, 0.0, in->layers[0].Z_coord_begin};
+  double ip[3];
+  if (!intersection_plane_line(pp, in->geometry.n_sample_orientation, p->coords, p->dirv, ip)) return 0;
+  p->coords[0] = ip[0];
+  p->coords[1] = ip[1];
+  p->coords[2] = ip[2];
+  p->current_layer = 1;
+  return 1;
+}
+
+static int orc_polarise(orc_photon *p, int horizontal) {
+  if (horizontal) {
+    p->elecv[0] = 0.0;
+    p->elecv[1] = 1.0;
+    p->elecv[2] = 0.0;
+  } else {
+    p->elecv[0] = 1.0;
+    p->elecv[1] = 0.0;
+    p->elecv[2] = 0.0;
+  }
+  double cosalfa = dot3(p->elecv, p->dirv);
+  if (fabs(cosalfa) > 1.0) return 0;
+  double c_alfa = acos(cosalfa);
+  double c_ae = 1.0 / sin(c_alfa);
+  double c_be = -c_ae * cosalfa;
+  for (int i = 0; i < 3; i++) p->elecv[i] = c_ae * p->elecv[i] + c_be * p->dirv[i];
+  return 1;
+}
+
+static double exc_correction(const orc_ctx *c, double E) {
+  double corr = 1.0;
+  for (int k = 0; k < c->in->n_exc_layers; k++) {
+    const xmi_layerF *l = &c->in->exc_layers[k];
+    corr *= exp(-1.0 * l->density * l->thickness * layer_mu(c, l, E));
+  }
+  return corr;
+}
+
+/* ------------------------------------------------------------------------ */
+
+int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
+                         xmi_main_options *options, double **brute_historyPtr, double **var_red_historyPtr,
+                         xmi_solid_angle *solid_angles, int rng_mode, uint64_t seed, int n_threads) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const xmi_hdf5F *h = (const xmi_hdf5F *)hdf5FPtr;
+  if (!in || !h || !options || !channelsPtr || !brute_historyPtr || !var_red_historyPtr) return 0;
+  if (!options->use_variance_reduction || !solid_angles) {
+    fprintf(stderr, "xmi_oracle_main_msim: only the variance-reduction path is restated\n");
+    return 0;
+  }
+  if (options->use_advanced_compton) {
+    fprintf(stderr, "xmi_oracle_main_msim: advanced Compton is not restated\n");
+    return 0;
+  }
+  if (n_mpi_hosts < 1) n_mpi_hosts = 1;
+  if (n_threads < 1) n_threads = 1;
+  double t0 = omp_get_wtime();
+  orc_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.in = in;
+  c.h = h;
+  c.opt = options;
+  c.sa = solid_angles;
+  c.nZ = h->n_Z;
+  c.n_int = in->general.n_interactions_trajectory;
+  c.nch = in->detector.nchannels;
+  c.line_last = options->use_M_lines ? XRL_M5P5 : XRL_L3Q1;
+  c.hits_per_single = g_hits_per_single;
+  c.seed = seed;
+  c.rng_mode = rng_mode;
+  if (!options->use_cascade_auger && !options->use_cascade_radiative) c.cascade_type = XRL_CASCADE_NONE;
+  else if (options->use_cascade_auger && !options->use_cascade_radiative) c.cascade_type = XRL_CASCADE_NONRADIATIVE;
+  else if (!options->use_cascade_auger && options->use_cascade_radiative) c.cascade_type = XRL_CASCADE_RADIATIVE;
+  else c.cascade_type = XRL_CASCADE_FULL;
+  if (c.n_int > MAX_INT_TRAJ || in->n_layers > 64) return 0;
+  c.atoms = (const xrl_atom **)malloc(sizeof(xrl_atom *) * c.nZ);
+  c.line_energies = (double *)calloc((size_t)c.nZ * XMI_NLINES_VR, sizeof(double));
+  for (int i = 0; i < c.nZ; i++) {
+    c.atoms[i] = xrl_atom_get(h->Zs[i].Z);
+    for (int l = 1; l <= XMI_NLINES_VR; l++) c.line_energies[(size_t)i * XMI_NLINES_VR + l - 1] = c.atoms[i]->line_energy[l];
+  }
+  c.precalc_mu = (double *)calloc((size_t)in->n_layers * c.nZ * XMI_NLINES_VR, sizeof(double));
+  for (int k = 0; k < in->n_layers; k++)
+    for (int i = 0; i < c.nZ; i++)
+      for (int l = 0; l < XMI_NLINES_VR; l++) {
+        double E = c.line_energies[(size_t)i * XMI_NLINES_VR + l];
+        c.precalc_mu[((size_t)k * c.nZ + i) * XMI_NLINES_VR + l] = E > 0.0 ? layer_mu(&c, &in->layers[k], E) : 0.0;
+      }
+
+  size_t nchan_tot = (size_t)(c.n_int + 1) * c.nch;
+  size_t nhist = (size_t)100 * XMI_HIST_ROWS * c.n_int;
+  double *channels = (double *)calloc(nchan_tot, sizeof(double));
+  double *var_red = (double *)calloc(nhist, sizeof(double));
+  int64_t histories = 0, interactions = 0, vr_calls = 0, fallbacks = 0;
+  int ok = 1;
+
+  /* valid continuous intervals and history numbering: continuous intervals
+   * first, then discrete lines */
+  int n_cont = in->n_continuous;
+  long nint_ph = in->general.n_photons_interval, nline_ph = in->general.n_photons_line;
+
+#pragma omp parallel num_threads(n_threads) reduction(+ : histories, interactions, vr_calls, fallbacks)
+  {
+    int tid = omp_get_thread_num();
+    int nth = omp_get_num_threads();
+    orc_tally t;
+    memset(&t, 0, sizeof(t));
+    t.channels = (double *)calloc(nchan_tot, sizeof(double));
+    t.var_red = (double *)calloc(nhist, sizeof(double));
+    orc_rng thread_rng, src_rng, tr_rng, vr_rng;
+    orc_rng_init(&thread_rng, seed, 0xF0000000u | (uint32_t)tid, 0);
+    orc_photon p;
+
+    /* photon ranges: per-thread mode follows the reference split (thread 0
+     * takes the remainder); per-history mode hands out contiguous chunks */
+    long cont_n = nint_ph / nth / n_mpi_hosts;
+    if (tid == 0) cont_n += nint_ph % nth;
+    long line_n = nline_ph / nth / n_mpi_hosts;
+    if (tid == 0) line_n += nline_ph % nth;
+    long cont_start = 0, line_start = 0;
+    if (rng_mode == XMI_ORACLE_RNG_PER_HISTORY) {
+      long base = nint_ph / n_mpi_hosts;
+      cont_n = base / nth + (tid < base % nth ? 1 : 0);
+      cont_start = tid * (base / nth) + (tid < base % nth ? tid : base % nth);
+      base = nline_ph / n_mpi_hosts;
+      line_n = base / nth + (tid < base % nth ? 1 : 0);
+      line_start = tid * (base / nth) + (tid < base % nth ? tid : base % nth);
+    }
+
+    for (int i = 0; i + 1 < n_cont && ok; i++) {
+      const xmi_energy_continuous *e0 = &in->continuous[i], *e1 = &in->continuous[i + 1];
+      double x1 = e0->energy, x2 = e1->energy;
+      double y1 = e0->vertical_intensity + e0->horizontal_intensity;
+      double y2 = e1->vertical_intensity + e1->horizontal_intensity;
+      double total_intensity = (y1 + y2) * (x2 - x1) / 2.0;
+      if (total_intensity == 0.0) continue;
+      for (long j = 0; j < cont_n; j++) {
+        uint64_t hid = (uint64_t)i * (uint64_t)nint_ph + (uint64_t)(cont_start + j);
+        orc_rng *src = &thread_rng;
+        if (rng_mode == XMI_ORACLE_RNG_PER_HISTORY) {
+          orc_rng_init(&src_rng, seed, 0u, hid);
+          src = &src_rng;
+        }
+        memset(&p, 0, sizeof(p));
+        p.tr = p.vr = &thread_rng;
+        p.energy = xmi_oracle_ran_trap(x1, x2, y1, y2, orc_uniform(src));
+        if (p.energy < 0.0) {
+          ok = 0;
+          break;
+        }
+        double hor = interpolate_simple(x1, e0->horizontal_intensity, x2, e1->horizontal_intensity, p.energy) /
+                     interpolate_simple(x1, y1, x2, y2, p.energy);
+        double exc_corr = exc_correction(&c, p.energy);
+        mu_calc_all(&c, p.energy, p.mus);
+        double weight = total_intensity * exc_corr / (double)nint_ph;
+        orc_coords_dir(&c, src, e0->sigma_x, e0->sigma_xp, e0->sigma_y, e0->sigma_yp, &p);
+        int horizontal = orc_uniform(src) <= hor;
+        p.weight = weight;
+        if (!orc_polarise(&p, horizontal) || !orc_shift_first_layer(&c, &p)) {
+          ok = 0;
+          break;
+        }
+        if (!orc_simulate_photon(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
+        histories++;
+      }
+    }
+
+    for (int i = 0; i < in->n_discrete && ok; i++) {
+      const xmi_energy_discrete *d = &in->discrete[i];
+      double total_intensity = d->vertical_intensity + d->horizontal_intensity;
+      long n_here = (rng_mode == XMI_ORACLE_RNG_PER_HISTORY) ? nline_ph : line_n;
+      double hor_ver_ratio = d->horizontal_intensity * (double)n_here / total_intensity;
+      double exc_corr = exc_correction(&c, d->energy);
+      double initial_mus[64];
+      mu_calc_all(&c, d->energy, initial_mus);
+      double weight = total_intensity * exc_corr / (double)nline_ph;
+      for (long j = 0; j < line_n; j++) {
+        long jj = line_start + j; /* 0-based photon index within the line */
+        uint64_t hid = (uint64_t)(n_cont > 1 ? n_cont - 1 : 0) * (uint64_t)nint_ph + (uint64_t)i * (uint64_t)nline_ph +
+                       (uint64_t)jj;
+        orc_rng *src = &thread_rng;
+        if (rng_mode == XMI_ORACLE_RNG_PER_HISTORY) {
+          orc_rng_init(&src_rng, seed, 0u, hid);
+          src = &src_rng;
+        }
+        memset(&p, 0, sizeof(p));
+        p.tr = p.vr = &thread_rng;
+        if (d->distribution_type == XMI_ENERGY_DISCRETE_DISTRIBUTION_GAUSSIAN) {
+          p.energy = orc_gauss(src, d->scale_parameter) + d->energy;
+          if (p.energy <= XMI_ENERGY_THRESHOLD || p.energy > XMI_ENERGY_MAX) continue;
+          mu_calc_all(&c, p.energy, p.mus);
+        } else if (d->distribution_type == XMI_ENERGY_DISCRETE_DISTRIBUTION_LORENTZIAN) {
+          p.energy = d->scale_parameter * tan(ORC_PI * (orc_uniform(src) - 0.5)) + d->energy;
+          if (p.energy <= XMI_ENERGY_THRESHOLD || p.energy > XMI_ENERGY_MAX) continue;
+          mu_calc_all(&c, p.energy, p.mus);
+        } else {
+          p.energy = d->energy;
+          memcpy(p.mus, initial_mus, sizeof(double) * in->n_layers);
+        }
+        orc_coords_dir(&c, src, d->sigma_x, d->sigma_xp, d->sigma_y, d->sigma_yp, &p);
+        int horizontal = (double)(jj + 1) <= hor_ver_ratio;
+        p.weight = weight;
+        if (!orc_polarise(&p, horizontal) || !orc_shift_first_layer(&c, &p)) {
+          ok = 0;
+          break;
+        }
+        if (!orc_simulate_photon(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
+        histories++;
+      }
+    }
+    interactions += t.interactions;
+    vr_calls += t.vr_calls;
+    fallbacks += t.fallbacks;
+#pragma omp critical
+    {
+      for (size_t k = 0; k < nchan_tot; k++) channels[k] += t.channels[k];
+      for (size_t k = 0; k < nhist; k++) var_red[k] += t.var_red[k];
+    }
+    free(t.channels);
+    free(t.var_red);
+  }
+
+  double live = in->detector.live_time;
+  /* [Z][row][n] already C order; scale */
+  double *vr_out = (double *)malloc(sizeof(double) * nhist);
+  double *bf_out = (double *)calloc(nhist, sizeof(double));
+  for (size_t k = 0; k < nhist; k++) vr_out[k] = var_red[k] * live * n_mpi_hosts;
+  double *ch_out = (double *)malloc(sizeof(double) * nchan_tot);
+  for (size_t k = 0; k < nchan_tot; k++) ch_out[k] = channels[k] * live;
+  free(channels);
+  free(var_red);
+  free(c.atoms);
+  free(c.line_energies);
+  free(c.precalc_mu);
+  g_stats.histories = histories;
+  g_stats.interactions = interactions;
+  g_stats.vr_calls = vr_calls;
+  g_stats.solid_angle_fallbacks = fallbacks;
+  g_stats.seconds = omp_get_wtime() - t0;
+  if (!ok) {
+    free(vr_out);
+    free(bf_out);
+    free(ch_out);
+    return 0;
+  }
+  *channelsPtr = ch_out;
+  *brute_historyPtr = bf_out;
+  *var_red_historyPtr = vr_out;
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* solid-angle grid, src/xmi_solid_angle_f.F90:62-301,303-429 */
+
+static double depth_quantile(const xmi_inputF *in, double energy, double R) {
+  int nl = in->n_layers;
+  double mu[64], my_sum = 0.0;
+  for (int i = 0; i < nl; i++) {
+    mu[i] = 0.0;
+    for (int j = 0; j < in->layers[i].n_elements; j++)
+      mu[i] += xrl_cs_total(xrl_atom_get(in->layers[i].Z[j]), energy) * in->layers[i].weight[j];
+    my_sum += mu[i] * in->layers[i].density * in->layers[i].thickness_along_Z;
+  }
+  double Pabs = -1.0 * expm1(-1.0 * my_sum);
+  double myln = -1.0 * log1p(-1.0 * R * Pabs);
+  int m = nl - 1;
+  my_sum = 0.0;
+  for (int i = 0; i < nl; i++) {
+    my_sum += mu[i] * in->layers[i].density * in->layers[i].thickness_along_Z;
+    if (my_sum > myln) {
+      m = i;
+      break;
+    }
+  }
+  my_sum = 0.0;
+  for (int i = 0; i < m; i++)
+    my_sum += (1.0 - (mu[i] * in->layers[i].density) / (mu[m] * in->layers[m].density)) * in->layers[i].thickness_along_Z;
+  return my_sum + myln / (mu[m] * in->layers[m].density) + in->layers[0].Z_coord_begin;
+}
+
+int xmi_oracle_solid_angle_extents(xmi_inputFPtr inputFPtr, double *r_min, double *r_max, double *theta_min,
+                                   double *theta_max) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  double e1, e2;
+  if (in->n_continuous > 1 && in->n_discrete > 0) {
+    e1 = fmin(in->continuous[0].energy, in->discrete[0].energy);
+    e2 = fmax(in->continuous[in->n_continuous - 1].energy, in->discrete[in->n_discrete - 1].energy);
+  } else if (in->n_continuous > 1) {
+    e1 = in->continuous[0].energy;
+    e2 = in->continuous[in->n_continuous - 1].energy;
+  } else {
+    e1 = in->discrete[0].energy;
+    e2 = in->discrete[in->n_discrete - 1].energy;
+  }
+  double S1 = depth_quantile(in, e1, 0.00001), S2 = depth_quantile(in, e2, 0.99999);
+  double p1[3] = {0.0, 0.0, S1}, p2[3] = {0.0, 0.0, S2};
+  double d1 = distance3(p1, in->geometry.p_detector_window), d2 = distance3(p2, in->geometry.p_detector_window);
+  double rmax = (d1 > d2 ? d1 : d2) * 1.25;
+  *r_max = rmax;
+  *r_min = rmax / 1024.0;
+  *theta_min = 0.00001;
+  *theta_max = ORC_PI / 2.0;
+  return 1;
+}
+
+double xmi_oracle_single_solid_angle(xmi_inputFPtr inputFPtr, double r1, double theta1, long hits, uint64_t seed,
+                                     uint64_t cell) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  orc_det d = make_det(in);
+  orc_rng rng;
+  orc_rng_init(&rng, seed, 3u, cell);
+  return orc_single_solid_angle(&d, r1, theta1, hits, &rng);
+}
+
+int xmi_oracle_solid_angle_calculation(xmi_inputFPtr inputFPtr, xmi_solid_angle **out, long hits, uint64_t seed,
+                                       int n_threads) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const long n = 1024;
+  double rmin, rmax, tmin, tmax;
+  xmi_oracle_solid_angle_extents(inputFPtr, &rmin, &rmax, &tmin, &tmax);
+  xmi_solid_angle *sa = (xmi_solid_angle *)calloc(1, sizeof(xmi_solid_angle));
+  sa->grid_dims_r_n = n;
+  sa->grid_dims_theta_n = n;
+  sa->grid_dims_r_vals = (double *)malloc(sizeof(double) * n);
+  sa->grid_dims_theta_vals = (double *)malloc(sizeof(double) * n);
+  sa->solid_angles = (double *)malloc(sizeof(double) * n * n);
+  for (long i = 0; i < n; i++) {
+    sa->grid_dims_r_vals[i] = rmin + (rmax - rmin) * (double)i / (double)(n - 1);
+    sa->grid_dims_theta_vals[i] = tmin + (tmax - tmin) * (double)i / (double)(n - 1);
+  }
+  orc_det d = make_det(in);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads > 0 ? n_threads : 1)
+  for (long ir = 0; ir < n; ir++) {
+    for (long jt = 0; jt < n; jt++) {
+      orc_rng rng;
+      orc_rng_init(&rng, seed, 3u, (uint64_t)(jt * n + ir));
+      sa->solid_angles[jt * n + ir] =
+          orc_single_solid_angle(&d, sa->grid_dims_r_vals[ir], sa->grid_dims_theta_vals[jt], hits, &rng);
+    }
+  }
+  *out = sa;
+  return 1;
+}
+
+void xmi_oracle_free_solid_angle(xmi_solid_angle *sa) {
+  if (!sa) return;
+  free(sa->solid_angles);
+  free(sa->grid_dims_r_vals);
+  free(sa->grid_dims_theta_vals);
+  free(sa);
+}
+
+/* first-order fundamental-parameter intensity (tests/libxmimsim-test.c:153-180):
+ * I0 * G * w * sigma_line(E0) * (1 - exp(-chi rho d)) / chi, single layer,
+ * alpha/beta from the sample normal and the detector position */
+double xmi_oracle_fpm_first_order(xmi_inputFPtr inputFPtr, int Z, int line, int cascade_type) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const xmi_layerF *L = &in->layers[0];
+  double w = 0.0;
+  for (int i = 0; i < L->n_elements; i++)
+    if (L->Z[i] == Z) w = L->weight[i];
+  const xrl_atom *a = xrl_atom_get(Z);
+  double E0 = in->discrete[0].energy;
+  double I0 = in->discrete[0].horizontal_intensity + in->discrete[0].vertical_intensity;
+  const double *n = in->geometry.n_sample_orientation;
+  const double *pd = in->geometry.p_detector_window;
+  double beam[3] = {0.0, 0.0, 1.0};
+  double to_det[3] = {pd[0], pd[1], pd[2] - in->geometry.d_sample_source};
+  double D = norm3(to_det);
+  normalize3(to_det);
+  double sin_alpha = fabs(dot3(beam, n));
+  double sin_beta = fabs(dot3(to_det, n));
+  double th = atan(in->detector_radius / D);
+  double Omega = 2.0 * ORC_PI * (1.0 - cos(th));
+  double G = Omega / 4.0 / ORC_PI / sin_alpha;
+  double P[9];
+  xrl_shell_cross_sections(a, cascade_type, E0, P);
+  int s = xrl_line_shell(line);
+  double sigma_line = P[s] * a->fluor_yield[s] * a->rad_rate[line];
+  double mu0 = 0.0, mu1 = 0.0;
+  for (int i = 0; i < L->n_elements; i++) {
+    mu0 += L->weight[i] * xrl_cs_total(xrl_atom_get(L->Z[i]), E0);
+    mu1 += L->weight[i] * xrl_cs_total(xrl_atom_get(L->Z[i]), a->line_energy[line]);
+  }
+  double chi = mu0 / sin_alpha + mu1 / sin_beta;
+  return I0 * G * w * sigma_line * (1.0 - exp(-chi * L->density * L->thickness)) / chi;
+}

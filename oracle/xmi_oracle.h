@@ -1,0 +1,71 @@
+/*
+ * xmi_oracle.h -- CPU restatement of the reference's photon-history engine.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library; the product never does.
+ */
+#ifndef XMI_ORACLE_H
+#define XMI_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/xmimsim_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* rng_mode */
+#define XMI_ORACLE_RNG_PER_HISTORY 0 /* Philox substreams per history, as on the GPU */
+#define XMI_ORACLE_RNG_PER_THREAD 1  /* one sequential stream per thread, reference structure */
+
+/* xmi_main_msim restated (src/xmi_main.F90:66-954, VR path).  Same outputs and
+ * layouts as the product's xmi_main_msim.  n_threads OpenMP threads. */
+int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channels,
+                         xmi_main_options *options, double **brute_history, double **var_red_history,
+                         xmi_solid_angle *solid_angles, int rng_mode, uint64_t seed, int n_threads);
+
+/* per-run counters of the last xmi_oracle_main_msim call */
+typedef struct {
+  int64_t histories;
+  int64_t interactions;
+  int64_t vr_calls;
+  int64_t solid_angle_fallbacks;
+  double seconds;
+} xmi_oracle_stats;
+void xmi_oracle_get_stats(xmi_oracle_stats *s);
+
+/* solid-angle grid restated (src/xmi_solid_angle_f.F90:62-710), one Philox
+ * stream per cell; *out malloc'ed (free with xmi_oracle_free_solid_angle). */
+int xmi_oracle_solid_angle_calculation(xmi_inputFPtr inputFPtr, xmi_solid_angle **out, long hits_per_single,
+                                       uint64_t seed, int n_threads);
+void xmi_oracle_free_solid_angle(xmi_solid_angle *sa);
+
+/* grid extents only (xmi_solid_angle_inputs_f), for checking the GPU's */
+int xmi_oracle_solid_angle_extents(xmi_inputFPtr inputFPtr, double *r_min, double *r_max, double *theta_min,
+                                   double *theta_max);
+
+/* one cell, reference algorithm (xmi_single_solid_angle_calculation) */
+double xmi_oracle_single_solid_angle(xmi_inputFPtr inputFPtr, double r1, double theta1, long hits_per_single,
+                                     uint64_t seed, uint64_t cell);
+
+/* closed-form first-order intensity, tests/libxmimsim-test.c:153-180 (fpm) */
+double xmi_oracle_fpm_first_order(xmi_inputFPtr inputFPtr, int Z, int line, int cascade_type);
+
+/* small numerical helpers exposed for unit tests */
+int xmi_oracle_poly_solve_quadratic(double a, double b, double c, double *r1, double *r2);
+double xmi_oracle_energy_from_q(double init_energy, double Q, double theta);
+double xmi_oracle_q_from_energy(double old_energy, double new_energy, double theta);
+double xmi_oracle_ran_trap(double x1, double x2, double y1, double y2, double u);
+void xmi_oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+void xmi_oracle_free(void *p);
+
+/* trials of the Monte Carlo solid-angle fallback inside the estimator
+ * (the reference's hits_per_single), default 5000 */
+void xmi_oracle_set_hits_per_single(long hits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+/*
+ * engine_device.h -- device-side data model and helpers of the photon-history
+ * engine (gfx950).
+ *
+ * Layout in HBM (all arrays structure-of-arrays, double precision):
+ *   photon state       PhotonSoA, one slot per history of the current batch
+ *   angle tables       ray/comp ICDF [nZ][400][2000] (r contiguous: one
+ *                      bilinear gather = two 16-byte pairs)
+ *   azimuth table      Phi ICDF [200][2000]
+ *   solid-angle grid   [theta][r] 1024 x 1024
+ *   per-element data   xrl_atom[nZ] (edges, yields, rates, cross-section
+ *                      tables), interaction-probability tables with a uniform
+ *                      index accelerator, Compton profile ICDF [nZ][10001]
+ *   detection tables   active fluorescence lines per element (line, shell,
+ *                      yield*rate, channel) and mu*rho per layer
+ *   tallies            128-bit fixed point (two u64 words), order independent
+ */
+#ifndef XMIMSIM_AMD_ENGINE_DEVICE_H
+#define XMIMSIM_AMD_ENGINE_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.h"
+#include "xrl_model.h"
+
+#define ENG_MAX_LAYERS 16
+#define ENG_MAX_ACT 96          /* active fluorescence lines per element */
+#define ENG_IP_ACC 4096         /* interaction-probability accelerator cells */
+#define ENG_PI 3.14159265358979323846
+
+struct PhotonSoA {
+  double *x, *y, *z;
+  double *dx, *dy, *dz;
+  double *ex, *ey, *ez;
+  double *theta, *phi;
+  double *energy, *weight;
+  double *mus;                  /* [n_layers][cap] */
+  int *layer;                   /* 1-based, 0 = dead */
+  int *elem;                    /* selected element index within layer */
+  int *prev_line;               /* line of the previous interaction if fluorescence, else 0 */
+  int *prev_u;                  /* hdf5 index of the element that emitted it */
+  uint64_t *hid;                /* history id */
+};
+
+struct EngineParams {
+  /* geometry */
+  double n_sample[3];
+  double p_det[3];
+  double n_det[3];
+  double det_new[3][3];
+  double det_inv[3][3];
+  double det_radius, coll_radius, coll_height;
+  int coll_present;
+  double slit_x, slit_y, d_source_slit;
+  int n_layers;
+  int n_int;
+  int nch;
+  double gain, zero;
+  int line_last;
+  int cascade;
+  int use_M_lines;
+  long hits_per_single;
+  uint64_t seed;
+  uint32_t sample;
+  int nZ;
+  /* layers */
+  double layer_density[ENG_MAX_LAYERS];
+  double layer_zb[ENG_MAX_LAYERS];
+  double layer_ze[ENG_MAX_LAYERS];
+  int layer_nel[ENG_MAX_LAYERS];
+  int layer_off[ENG_MAX_LAYERS];
+  const int *el_Z;              /* flattened per layer */
+  const double *el_w;
+  const int *el_u;              /* hdf5 index */
+  /* per element */
+  const xrl_atom *atoms;        /* [nZ] */
+  const double *ray_icdf;       /* [nZ][n_e][n_r] */
+  const double *comp_icdf;
+  const double *tab_E;          /* [n_e] shared energy grid */
+  const double *tab_R;          /* [n_r] */
+  int n_e, n_r;
+  double tab_E0, tab_Einv, tab_R0, tab_Rinv;
+  const double *fycorr;         /* [nZ][9] */
+  const double *ip_E;           /* concatenated */
+  const double *ip_r;
+  const double *ip_rc;
+  const int *ip_off;            /* [nZ+1] */
+  const int *ip_acc;            /* [nZ][ENG_IP_ACC] */
+  double ip_acc_E0, ip_acc_inv;
+  const double *cp_icdf;        /* [nZ][cp_len] */
+  const double *cp_rn;          /* [cp_len] shared */
+  int cp_len;
+  const double *precalc_xrf;    /* [nZ][9][nZ][219] */
+  const double *line_E;         /* [nZ][219] */
+  const double *precalc_mu;     /* [n_layers][nZ][219] */
+  /* detection tables */
+  const int *act_n;             /* [nZ] */
+  const int *act_line;          /* [nZ][ENG_MAX_ACT] */
+  const int *act_shell;
+  const double *act_factor;     /* fluor_yield(shell) * rad_rate(line) */
+  const double *act_E;
+  const int *act_ch;
+  const double *act_murho;      /* [n_layers][nZ][ENG_MAX_ACT] */
+  /* azimuth and solid angle */
+  const double *phi_icdf;       /* [n_t][n_pr] */
+  const double *phi_T;
+  const double *phi_R;
+  int phi_nt, phi_nr;
+  double phi_T0, phi_Tinv, phi_R0, phi_Rinv;
+  const double *sa_grid;        /* [n_theta][n_r] */
+  const double *sa_r;
+  const double *sa_t;
+  int sa_nr, sa_nt;
+  double sa_r0, sa_rinv, sa_t0, sa_tinv;
+  /* tallies */
+  unsigned long long *t_ch;     /* [n_int][nch][2] */
+  unsigned long long *t_line;   /* [n_int][nZ][ENG_MAX_ACT][2] */
+  unsigned long long *t_rc;     /* [n_int][nZ][2][2] */
+  int tally_shift;              /* fixed point: value * 2^tally_shift */
+  unsigned int *counters;       /* [0..2] queue sizes, [3] vr calls, [4] fallbacks, [5] errors */
+  int *queue;                   /* [3][cap] */
+  int cap;
+};
+
+/* ------------------------------------------------------------------------ */
+/* 128-bit fixed-point accumulation: order independent, hence bit-reproducible */
+
+__device__ __forceinline__ void fx_split(double v, int shift, unsigned long long *lo, unsigned long long *hi) {
+  if (!(v > 0.0)) {
+    *lo = 0ull;
+    *hi = 0ull;
+    return;
+  }
+  double t = ldexp(v, shift - 64);
+  double th = trunc(t);
+  *hi = (unsigned long long)th;
+  *lo = (unsigned long long)ldexp(t - th, 64);
+}
+
+__device__ __forceinline__ void fx_add_global(unsigned long long *bin, double v, int shift) {
+  unsigned long long lo, hi;
+  fx_split(v, shift, &lo, &hi);
+  if (lo) {
+    unsigned long long old = atomicAdd(bin, lo);
+    if (old + lo < old) hi += 1ull;
+  }
+  if (hi) atomicAdd(bin + 1, hi);
+}
+
+__device__ __forceinline__ void fx_add_shared(unsigned long long *bin, double v, int shift) {
+  unsigned long long lo, hi;
+  fx_split(v, shift, &lo, &hi);
+  if (lo) {
+    unsigned long long old = atomicAdd(bin, lo);
+    if (old + lo < old) hi += 1ull;
+  }
+  if (hi) atomicAdd(bin + 1, hi);
+}
+
+/* adds a 128-bit (lo,hi) pair held elsewhere into a global bin */
+__device__ __forceinline__ void fx_merge_global(unsigned long long *bin, unsigned long long lo, unsigned long long hi) {
+  if (lo) {
+    unsigned long long old = atomicAdd(bin, lo);
+    if (old + lo < old) hi += 1ull;
+  }
+  if (hi) atomicAdd(bin + 1, hi);
+}
+
+/* ------------------------------------------------------------------------ */
+/* vector helpers */
+
+__device__ __forceinline__ double d_dot(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ double d_norm(const double *a) { return sqrt(d_dot(a, a)); }
+__device__ __forceinline__ void d_normalize(double *a) {
+  double n = d_norm(a);
+  a[0] /= n;
+  a[1] /= n;
+  a[2] /= n;
+}
+__device__ __forceinline__ void d_matvec(const double m[3][3], const double *v, double *r) {
+  for (int i = 0; i < 3; i++) r[i] = m[i][0] * v[0] + m[i][1] * v[1] + m[i][2] * v[2];
+}
+__device__ __forceinline__ double d_distance(const double *a, const double *b) {
+  return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+__device__ __forceinline__ int d_intersect(const double *pp, const double *pn, const double *lp, const double *ld,
+                                           double *out) {
+  double itn = d_dot(ld, pn);
+  if (itn == 0.0) return 0;
+  double diff[3] = {pp[0] - lp[0], pp[1] - lp[1], pp[2] - lp[2]};
+  double d = d_dot(diff, pn) / itn;
+  for (int i = 0; i < 3; i++) out[i] = d * ld[i] + lp[i];
+  return 1;
+}
+__device__ __forceinline__ double d_interp(double x0, double y0, double x1, double y1, double x) {
+  return y0 + ((y1 - y0) * (x - x0) / (x1 - x0));
+}
+
+/* index on a (nearly) uniform grid with the semantics of the reference's
+ * findpos: a[p] < x <= a[p+1]; -1 when x > a[n-1] */
+__device__ __forceinline__ int d_findpos_uniform(const double *a, int n, double a0, double inv_h, double x) {
+  if (fabs(x - a[0]) < 1e-10) return 0;
+  if (x > a[n - 1]) return -1;
+  int p = (int)floor((x - a0) * inv_h);
+  if (p < 0) p = 0;
+  if (p > n - 2) p = n - 2;
+  while (p > 0 && x <= a[p]) p--;
+  while (p < n - 2 && x > a[p + 1]) p++;
+  return p;
+}
+
+__device__ __forceinline__ double d_bilinear(const double *arr, const double *a1, int n1, double a10, double a1inv,
+                                             const double *a2, int n2, double a20, double a2inv, double x1, double x2) {
+  int p1 = d_findpos_uniform(a1, n1, a10, a1inv, x1);
+  int p2 = d_findpos_uniform(a2, n2, a20, a2inv, x2);
+  if (p1 < 0) p1 = n1 - 2;
+  if (p2 < 0) p2 = n2 - 2;
+  double a1l = a1[p1], a1h = a1[p1 + 1], a2l = a2[p2], a2h = a2[p2 + 1];
+  double denom = (a1h - a1l) * (a2h - a2l);
+  double c1 = (a1h - x1) * (a2h - x2) / denom;
+  double c2 = (x1 - a1l) * (a2h - x2) / denom;
+  double c3 = (a1h - x1) * (x2 - a2l) / denom;
+  double c4 = (x1 - a1l) * (x2 - a2l) / denom;
+  const double *r0 = arr + (size_t)p1 * n2 + p2;
+  const double *r1 = arr + (size_t)(p1 + 1) * n2 + p2;
+  return c1 * r0[0] + c2 * r1[0] + c3 * r0[1] + c4 * r1[1];
+}
+
+/* phx stream positioned at draw index k */
+__device__ __forceinline__ void d_stream_seek(phx_stream *s, uint32_t k) {
+  s->ctr[0] = k >> 1;
+  s->avail = 0;
+  if (k & 1u) {
+    uint32_t o[4];
+    phx_4x32_10(s->ctr, s->key, o);
+    s->ctr[0]++;
+    s->buf[0] = phx_u01(o[0], o[1]);
+    s->buf[1] = phx_u01(o[2], o[3]);
+    s->avail = 1;
+  }
+}
+
+#endif

@@ -1,0 +1,861 @@
+/*
+ * engine_host.cpp -- host driver of the photon-history engine: the C ABI
+ * entry points xmi_main_msim and xmi_solid_angle_calculation_cl.
+ *
+ * xmi_main_msim (src/xmi_main.F90:66-954) becomes:
+ *   1. device-resident copies of the loaded tables, cached per table handle;
+ *   2. a per-call plan: layers, per-layer attenuation at every fluorescence
+ *      line (precalc_mu_cs, src/xmi_main.F90:227-237), the active-line lists
+ *      of the estimator, the photon sources (discrete lines, continuous
+ *      intervals) with their weights (:253-317, :319-378, :580-601);
+ *   3. batches of histories through the event kernels;
+ *   4. the 128-bit fixed-point tallies read back once, summed exactly, made
+ *      cumulative over interaction order and scaled by live_time (:923-950).
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <chrono>
+#include <mutex>
+#include <vector>
+
+#include "engine_kernels.h"
+#include "xmi_internal.h"
+
+long xmi_hits_per_single = 5000;
+
+static uint64_t g_seed = 0;
+static int g_seed_set = 0;
+static thread_local xmi_msim_stats g_stats;
+
+extern "C" void xmi_set_seed(uint64_t seed) {
+  g_seed = seed;
+  g_seed_set = 1;
+}
+
+extern "C" void xmi_msim_get_stats(xmi_msim_stats *out) {
+  if (out) *out = g_stats;
+}
+
+#define HIP_TRY(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) {                                                                 \
+      xmi_set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      goto fail;                                                                            \
+    }                                                                                       \
+  } while (0)
+
+/* ------------------------------------------------------------------------ */
+/* device allocations */
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+template <typename T>
+static hipError_t upload(DevBuf &b, const T *src, size_t count) {
+  hipError_t e = b.ensure(sizeof(T) * count);
+  if (e != hipSuccess) return e;
+  if (count) return hipMemcpy(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice);
+  return hipSuccess;
+}
+
+/* tables of one xmi_hdf5F handle on the device */
+struct DeviceTables {
+  const xmi_hdf5F *owner = nullptr;
+  int nZ = 0;
+  int Zs[XMI_MAXZ + 1];
+  DevBuf atoms, ray, comp, tabE, tabR, fycorr, ipE, ipr, iprc, ipoff, ipacc, cp_icdf, cp_rn, precalc_xrf, lineE;
+  DevBuf phi, phiT, phiR;
+  int n_e = 0, n_r = 0, cp_len = 0, phi_nt = 0, phi_nr = 0;
+  double tab_E0 = 0, tab_Einv = 0, tab_R0 = 0, tab_Rinv = 0, phi_T0 = 0, phi_Tinv = 0, phi_R0 = 0, phi_Rinv = 0;
+  double ip_acc_E0 = 0, ip_acc_inv = 0;
+};
+
+/* per-process engine state: one device, one stream */
+struct Engine {
+  std::mutex lock;
+  bool ready = false;
+  hipStream_t stream = nullptr;
+  std::vector<DeviceTables *> tables;
+  /* solid-angle grid cache */
+  const void *sa_key = nullptr;
+  uint64_t sa_hash = 0;
+  DevBuf sa_grid, sa_r, sa_t;
+  int sa_nr = 0, sa_nt = 0;
+  /* per-call buffers (grow only) */
+  DevBuf el_Z, el_w, el_u, precalc_mu, act_n, act_line, act_shell, act_factor, act_E, act_ch, act_murho;
+  DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
+  DevBuf state, tallies, counters, queue;
+};
+
+static Engine g_engine;
+
+static int engine_init() {
+  if (g_engine.ready) return 1;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    xmi_set_error("no HIP device available");
+    return 0;
+  }
+  if (hipStreamCreateWithFlags(&g_engine.stream, hipStreamNonBlocking) != hipSuccess) {
+    xmi_set_error("hipStreamCreate failed");
+    return 0;
+  }
+  g_engine.ready = true;
+  return 1;
+}
+
+static double uniform_inv(const double *a, int n) { return (double)(n - 1) / (a[n - 1] - a[0]); }
+
+static DeviceTables *get_tables(const xmi_hdf5F *h) {
+  for (DeviceTables *t : g_engine.tables)
+    if (t->owner == h) return t;
+  DeviceTables *t = new DeviceTables();
+  t->owner = h;
+  t->nZ = h->n_Z;
+  const int nZ = h->n_Z;
+  std::vector<xrl_atom> atoms(nZ);
+  for (int i = 0; i < nZ; i++) {
+    t->Zs[i] = h->Zs[i].Z;
+    const xrl_atom *a = xrl_atom_get(h->Zs[i].Z);
+    if (!a) {
+      delete t;
+      return nullptr;
+    }
+    atoms[i] = *a;
+  }
+  const xmi_hdf5_ZF *z0 = &h->Zs[0];
+  t->n_e = z0->n_e;
+  t->n_r = z0->n_r;
+  t->cp_len = z0->cp_len;
+  size_t plane = (size_t)t->n_e * t->n_r;
+  std::vector<double> ray(plane * nZ), comp(plane * nZ), fy(9 * (size_t)nZ), cp((size_t)t->cp_len * nZ);
+  std::vector<double> ipE, ipr, iprc;
+  std::vector<int> ipoff(nZ + 1), ipacc((size_t)nZ * ENG_IP_ACC);
+  std::vector<double> pxrf((size_t)nZ * 9 * nZ * XMI_NLINES_VR, 0.0), lineE((size_t)nZ * XMI_NLINES_VR, 0.0);
+  t->ip_acc_E0 = z0->ip_energies[0];
+  double Emax = z0->ip_energies[z0->ip_len - 1];
+  t->ip_acc_inv = (double)ENG_IP_ACC / (Emax - t->ip_acc_E0);
+  for (int i = 0; i < nZ; i++) {
+    const xmi_hdf5_ZF *z = &h->Zs[i];
+    if (z->n_e != t->n_e || z->n_r != t->n_r || z->cp_len != t->cp_len || !z->precalc_xrf_cs) {
+      xmi_set_error("inconsistent table dimensions for Z=%d", z->Z);
+      delete t;
+      return nullptr;
+    }
+    memcpy(&ray[plane * i], z->RayleighTheta_ICDF, sizeof(double) * plane);
+    memcpy(&comp[plane * i], z->ComptonTheta_ICDF, sizeof(double) * plane);
+    memcpy(&fy[9 * (size_t)i], z->FluorYieldsCorr, sizeof(double) * 9);
+    memcpy(&cp[(size_t)t->cp_len * i], z->cp_total_icdf, sizeof(double) * t->cp_len);
+    ipoff[i] = (int)ipE.size();
+    ipE.insert(ipE.end(), z->ip_energies, z->ip_energies + z->ip_len);
+    ipr.insert(ipr.end(), z->ip_rayl, z->ip_rayl + z->ip_len);
+    iprc.insert(iprc.end(), z->ip_rayl_compt, z->ip_rayl_compt + z->ip_len);
+    /* accelerator: largest p with e[p] < cell start */
+    int p = 0;
+    for (int c = 0; c < ENG_IP_ACC; c++) {
+      double E = t->ip_acc_E0 + c / t->ip_acc_inv;
+      while (p + 2 < z->ip_len && z->ip_energies[p + 1] < E) p++;
+      ipacc[(size_t)i * ENG_IP_ACC + c] = p;
+    }
+    memcpy(&pxrf[(size_t)i * 9 * nZ * XMI_NLINES_VR], z->precalc_xrf_cs, sizeof(double) * 9 * nZ * XMI_NLINES_VR);
+    for (int l = 1; l <= XMI_NLINES_VR; l++) lineE[(size_t)i * XMI_NLINES_VR + l - 1] = atoms[i].line_energy[l];
+  }
+  ipoff[nZ] = (int)ipE.size();
+  t->tab_E0 = z0->Energies[0];
+  t->tab_Einv = uniform_inv(z0->Energies, t->n_e);
+  t->tab_R0 = z0->RandomNumbers[0];
+  t->tab_Rinv = uniform_inv(z0->RandomNumbers, t->n_r);
+  t->phi_nt = h->phi_n_theta;
+  t->phi_nr = h->phi_n_r;
+  t->phi_T0 = h->Thetas[0];
+  t->phi_Tinv = uniform_inv(h->Thetas, h->phi_n_theta);
+  t->phi_R0 = h->RandomNumbers[0];
+  t->phi_Rinv = uniform_inv(h->RandomNumbers, h->phi_n_r);
+  hipError_t e = hipSuccess;
+  e = upload(t->atoms, atoms.data(), atoms.size());
+  if (e == hipSuccess) e = upload(t->ray, ray.data(), ray.size());
+  if (e == hipSuccess) e = upload(t->comp, comp.data(), comp.size());
+  if (e == hipSuccess) e = upload(t->tabE, z0->Energies, (size_t)t->n_e);
+  if (e == hipSuccess) e = upload(t->tabR, z0->RandomNumbers, (size_t)t->n_r);
+  if (e == hipSuccess) e = upload(t->fycorr, fy.data(), fy.size());
+  if (e == hipSuccess) e = upload(t->ipE, ipE.data(), ipE.size());
+  if (e == hipSuccess) e = upload(t->ipr, ipr.data(), ipr.size());
+  if (e == hipSuccess) e = upload(t->iprc, iprc.data(), iprc.size());
+  if (e == hipSuccess) e = upload(t->ipoff, ipoff.data(), ipoff.size());
+  if (e == hipSuccess) e = upload(t->ipacc, ipacc.data(), ipacc.size());
+  if (e == hipSuccess) e = upload(t->cp_icdf, cp.data(), cp.size());
+  if (e == hipSuccess) e = upload(t->cp_rn, z0->cp_random_numbers, (size_t)t->cp_len);
+  if (e == hipSuccess) e = upload(t->precalc_xrf, pxrf.data(), pxrf.size());
+  if (e == hipSuccess) e = upload(t->lineE, lineE.data(), lineE.size());
+  if (e == hipSuccess) e = upload(t->phi, h->Phi_ICDF, (size_t)h->phi_n_theta * h->phi_n_r);
+  if (e == hipSuccess) e = upload(t->phiT, h->Thetas, (size_t)h->phi_n_theta);
+  if (e == hipSuccess) e = upload(t->phiR, h->RandomNumbers, (size_t)h->phi_n_r);
+  if (e != hipSuccess) {
+    xmi_set_error("table upload failed: %s", hipGetErrorString(e));
+    delete t;
+    return nullptr;
+  }
+  g_engine.tables.push_back(t);
+  return t;
+}
+
+/* released through xmi_hdf5F's destructor hook */
+extern "C" void xmi_engine_forget_tables(const void *h) {
+  std::lock_guard<std::mutex> g(g_engine.lock);
+  for (size_t k = 0; k < g_engine.tables.size(); k++)
+    if (g_engine.tables[k]->owner == h) {
+      delete g_engine.tables[k];
+      g_engine.tables.erase(g_engine.tables.begin() + k);
+      return;
+    }
+}
+
+static uint64_t sampled_hash(const xmi_solid_angle *sa) {
+  uint64_t hsh = 1469598103934665603ull;
+  auto mix = [&](const void *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    for (size_t i = 0; i < n; i++) hsh = (hsh ^ c[i]) * 1099511628211ull;
+  };
+  size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
+  mix(sa->grid_dims_r_vals, sizeof(double) * sa->grid_dims_r_n);
+  mix(sa->grid_dims_theta_vals, sizeof(double) * sa->grid_dims_theta_n);
+  size_t step = tot / 8192 + 1;
+  for (size_t i = 0; i < tot; i += step) mix(&sa->solid_angles[i], sizeof(double));
+  mix(&sa->solid_angles[tot - 1], sizeof(double));
+  return hsh;
+}
+
+static int ensure_grid(const xmi_solid_angle *sa) {
+  uint64_t hsh = sampled_hash(sa);
+  if (g_engine.sa_key == sa && g_engine.sa_hash == hsh) return 1;
+  size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
+  if (upload(g_engine.sa_grid, sa->solid_angles, tot) != hipSuccess ||
+      upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n) != hipSuccess ||
+      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n) != hipSuccess) {
+    xmi_set_error("solid-angle grid upload failed");
+    return 0;
+  }
+  g_engine.sa_key = sa;
+  g_engine.sa_hash = hsh;
+  g_engine.sa_nr = (int)sa->grid_dims_r_n;
+  g_engine.sa_nt = (int)sa->grid_dims_theta_n;
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+
+static double layer_mu_host(const xmi_layerF *L, double E) {
+  double rv = 0.0;
+  for (int i = 0; i < L->n_elements; i++) rv += xrl_cs_total(xrl_atom_get(L->Z[i]), E) * L->weight[i];
+  return rv;
+}
+
+static void fill_geometry(EngineParams &P, const xmi_inputF *in) {
+  for (int k = 0; k < 3; k++) {
+    P.n_sample[k] = in->geometry.n_sample_orientation[k];
+    P.p_det[k] = in->geometry.p_detector_window[k];
+    P.n_det[k] = in->geometry.n_detector_orientation[k];
+    for (int j = 0; j < 3; j++) {
+      P.det_new[k][j] = in->n_detector_orientation_new[k][j];
+      P.det_inv[k][j] = in->n_detector_orientation_inverse[k][j];
+    }
+  }
+  P.det_radius = in->detector_radius;
+  P.coll_radius = in->collimator_radius;
+  P.coll_height = in->geometry.collimator_height;
+  P.coll_present = in->collimator_present;
+  P.slit_x = in->geometry.slit_size_x;
+  P.slit_y = in->geometry.slit_size_y;
+  P.d_source_slit = in->geometry.d_source_slit;
+}
+
+static int cascade_of(const xmi_main_options *o) {
+  if (!o->use_cascade_auger && !o->use_cascade_radiative) return XRL_CASCADE_NONE;
+  if (o->use_cascade_auger && !o->use_cascade_radiative) return XRL_CASCADE_NONRADIATIVE;
+  if (!o->use_cascade_auger && o->use_cascade_radiative) return XRL_CASCADE_RADIATIVE;
+  return XRL_CASCADE_FULL;
+}
+
+typedef unsigned __int128 u128;
+
+static inline u128 fx_get(const unsigned long long *b) { return ((u128)b[1] << 64) | (u128)b[0]; }
+static inline double fx_to_double(u128 v, int shift) { return ldexp((double)v, -shift); }
+
+extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
+                             xmi_main_options *options, double **brute_historyPtr, double **var_red_historyPtr,
+                             xmi_solid_angle *solid_angles) {
+  auto t_wall0 = std::chrono::steady_clock::now();
+  memset(&g_stats, 0, sizeof(g_stats));
+  xmi_inputF *in = (xmi_inputF *)inputFPtr;
+  xmi_hdf5F *h = (xmi_hdf5F *)hdf5FPtr;
+  if (!in || !h || !options || !channelsPtr || !brute_historyPtr || !var_red_historyPtr) {
+    xmi_set_error("xmi_main_msim: NULL argument");
+    return 0;
+  }
+  if (!in->initialised) {
+    xmi_set_error("xmi_main_msim: xmi_init_input was not called");
+    return 0;
+  }
+  if (!options->use_variance_reduction) {
+    xmi_set_error("xmi_main_msim: the brute-force path (use_variance_reduction=0) is not built on the GPU");
+    return 0;
+  }
+  if (options->use_advanced_compton) {
+    xmi_set_error("xmi_main_msim: use_advanced_compton is not supported");
+    return 0;
+  }
+  if (!solid_angles || !solid_angles->solid_angles) {
+    xmi_set_error("xmi_main_msim: variance reduction needs a solid-angle grid");
+    return 0;
+  }
+  if (in->n_layers > ENG_MAX_LAYERS) {
+    xmi_set_error("xmi_main_msim: at most %d layers", ENG_MAX_LAYERS);
+    return 0;
+  }
+  if (n_mpi_hosts < 1) n_mpi_hosts = 1;
+  std::lock_guard<std::mutex> guard(g_engine.lock);
+  if (!engine_init()) return 0;
+  DeviceTables *T = get_tables(h);
+  if (!T) return 0;
+  if (!ensure_grid(solid_angles)) return 0;
+  hipStream_t st = g_engine.stream;
+  const int nZ = T->nZ;
+  const int n_int = in->general.n_interactions_trajectory;
+  const int nch = in->detector.nchannels;
+  const int line_last = options->use_M_lines ? XRL_M5P5 : XRL_L3Q1;
+  const int NL = in->n_layers;
+  double *ch_out = nullptr, *vr_out = nullptr, *bf_out = nullptr;
+  hipEvent_t evs[2] = {nullptr, nullptr};
+  int ok = 0;
+
+  EngineParams P;
+  memset(&P, 0, sizeof(P));
+  fill_geometry(P, in);
+  P.n_layers = NL;
+  P.n_int = n_int;
+  P.nch = nch;
+  P.gain = in->detector.gain;
+  P.zero = in->detector.zero;
+  P.line_last = line_last;
+  P.cascade = cascade_of(options);
+  P.use_M_lines = options->use_M_lines;
+  P.hits_per_single = xmi_hits_per_single;
+  if (options->use_default_seeds) P.seed = 1;
+  else {
+    if (!g_seed_set) {
+      g_seed = (uint64_t)time(NULL) * 6364136223846793005ull + (uint64_t)clock();
+      g_seed_set = 1;
+    }
+    P.seed = g_seed;
+  }
+  P.sample = 0;
+  P.nZ = nZ;
+
+  /* layers */
+  std::vector<int> elZ, elU;
+  std::vector<double> elW;
+  for (int l = 0; l < NL; l++) {
+    const xmi_layerF *L = &in->layers[l];
+    P.layer_density[l] = L->density;
+    P.layer_zb[l] = L->Z_coord_begin;
+    P.layer_ze[l] = L->Z_coord_end;
+    P.layer_nel[l] = L->n_elements;
+    P.layer_off[l] = (int)elZ.size();
+    for (int k = 0; k < L->n_elements; k++) {
+      int u = h->uniqZ[L->Z[k]] - 1;
+      if (u < 0) {
+        xmi_set_error("xmi_main_msim: Z=%d has no tables (xmi_update_input_from_hdf5 not called?)", L->Z[k]);
+        return 0;
+      }
+      elZ.push_back(L->Z[k]);
+      elW.push_back(L->weight[k]);
+      elU.push_back(u);
+    }
+  }
+  /* attenuation at the fluorescence lines, and the estimator's active lines */
+  std::vector<double> pmu((size_t)NL * nZ * XMI_NLINES_VR, 0.0);
+  for (int l = 0; l < NL; l++)
+    for (int u = 0; u < nZ; u++) {
+      const xrl_atom *a = xrl_atom_get(T->Zs[u]);
+      for (int line = 1; line <= XMI_NLINES_VR; line++) {
+        double E = a->line_energy[line];
+        pmu[((size_t)l * nZ + u) * XMI_NLINES_VR + line - 1] = E > 0.0 ? layer_mu_host(&in->layers[l], E) : 0.0;
+      }
+    }
+  std::vector<int> actn(nZ, 0), actl((size_t)nZ * ENG_MAX_ACT, 0), acts((size_t)nZ * ENG_MAX_ACT, 0),
+      actc((size_t)nZ * ENG_MAX_ACT, -1);
+  std::vector<double> actf((size_t)nZ * ENG_MAX_ACT, 0.0), actE((size_t)nZ * ENG_MAX_ACT, 0.0),
+      actm((size_t)NL * nZ * ENG_MAX_ACT, 0.0);
+  for (int u = 0; u < nZ; u++) {
+    const xrl_atom *a = xrl_atom_get(T->Zs[u]);
+    int k = 0;
+    for (int line = 1; line <= line_last; line++) {
+      double E = a->line_energy[line];
+      if (E < XMI_ENERGY_THRESHOLD) continue;
+      int s = xrl_line_shell(line);
+      double f = a->fluor_yield[s] * a->rad_rate[line];
+      if (f == 0.0) continue;
+      if (k >= ENG_MAX_ACT) {
+        xmi_set_error("xmi_main_msim: more than %d active lines for Z=%d", ENG_MAX_ACT, T->Zs[u]);
+        return 0;
+      }
+      size_t o = (size_t)u * ENG_MAX_ACT + k;
+      actl[o] = line;
+      acts[o] = s;
+      actf[o] = f;
+      actE[o] = E;
+      int ch = (int)((E - in->detector.zero) / in->detector.gain);
+      actc[o] = (ch >= 0 && ch <= nch - 1) ? ch : -1;
+      for (int l = 0; l < NL; l++)
+        actm[((size_t)l * nZ + u) * ENG_MAX_ACT + k] =
+            pmu[((size_t)l * nZ + u) * XMI_NLINES_VR + line - 1] * in->layers[l].density;
+      k++;
+    }
+    actn[u] = k;
+  }
+
+  /* sources: continuous intervals, then discrete lines */
+  std::vector<SourceDesc> src;
+  std::vector<double> src_mus;
+  double total_weight = 0.0;
+  uint64_t hid = 0;
+  const long nint_ph = in->general.n_photons_interval, nline_ph = in->general.n_photons_line;
+  for (int i = 0; i + 1 < in->n_continuous; i++) {
+    const xmi_energy_continuous *e0 = &in->continuous[i], *e1 = &in->continuous[i + 1];
+    SourceDesc d;
+    memset(&d, 0, sizeof(d));
+    d.kind = 0;
+    d.x1 = e0->energy;
+    d.x2 = e1->energy;
+    d.y1 = e0->vertical_intensity + e0->horizontal_intensity;
+    d.y2 = e1->vertical_intensity + e1->horizontal_intensity;
+    d.h1 = e0->horizontal_intensity;
+    d.h2 = e1->horizontal_intensity;
+    double tot = (d.y1 + d.y2) * (d.x2 - d.x1) / 2.0;
+    d.valid = tot != 0.0;
+    if (d.valid && d.x1 >= d.x2) {
+      xmi_set_error("xmi_main_msim: continuous energies must increase");
+      return 0;
+    }
+    d.weight = tot / (double)nint_ph;
+    d.count = nint_ph / n_mpi_hosts;
+    d.hid0 = hid;
+    d.sx = e0->sigma_x;
+    d.sxp = e0->sigma_xp;
+    d.sy = e0->sigma_y;
+    d.syp = e0->sigma_yp;
+    hid += (uint64_t)nint_ph;
+    if (d.valid) total_weight += tot;
+    src.push_back(d);
+    for (int l = 0; l < NL; l++) src_mus.push_back(0.0);
+  }
+  for (int i = 0; i < in->n_discrete; i++) {
+    const xmi_energy_discrete *e = &in->discrete[i];
+    SourceDesc d;
+    memset(&d, 0, sizeof(d));
+    d.kind = 1;
+    d.valid = 1;
+    d.dist = (int)e->distribution_type;
+    d.energy = e->energy;
+    d.scale = e->scale_parameter;
+    double tot = e->vertical_intensity + e->horizontal_intensity;
+    d.count = nline_ph / n_mpi_hosts;
+    d.hor_ver_ratio = e->horizontal_intensity * (double)d.count / tot;
+    double exc = 1.0;
+    for (int k = 0; k < in->n_exc_layers; k++) {
+      const xmi_layerF *L = &in->exc_layers[k];
+      exc *= exp(-1.0 * L->density * L->thickness * layer_mu_host(L, e->energy));
+    }
+    d.weight = tot * exc / (double)nline_ph;
+    d.hid0 = hid;
+    d.sx = e->sigma_x;
+    d.sxp = e->sigma_xp;
+    d.sy = e->sigma_y;
+    d.syp = e->sigma_yp;
+    hid += (uint64_t)nline_ph;
+    total_weight += tot;
+    src.push_back(d);
+    for (int l = 0; l < NL; l++) src_mus.push_back(layer_mu_host(&in->layers[l], e->energy));
+  }
+  /* history ids are laid out with the full per-source budget; batches only run
+   * the first `count` of each source */
+  std::vector<double> exc_flat;
+  for (int k = 0; k < in->n_exc_layers; k++) {
+    const xmi_layerF *L = &in->exc_layers[k];
+    exc_flat.push_back(L->density);
+    exc_flat.push_back(L->thickness);
+    exc_flat.push_back((double)L->n_elements);
+    for (int j = 0; j < L->n_elements; j++) exc_flat.push_back((double)L->Z[j]);
+    for (int j = 0; j < L->n_elements; j++) exc_flat.push_back(L->weight[j]);
+  }
+  std::vector<xrl_atom> exc_atoms;
+  if (in->n_exc_layers > 0) {
+    exc_atoms.resize(XMI_MAXZ + 1);
+    for (int k = 0; k < in->n_exc_layers; k++)
+      for (int j = 0; j < in->exc_layers[k].n_elements; j++) exc_atoms[in->exc_layers[k].Z[j]] = *xrl_atom_get(in->exc_layers[k].Z[j]);
+  }
+  /* fixed-point scale: the total source intensity maps to 2^100, leaving a
+   * factor 2^27 of headroom below 2^127 and a resolution of 2^-100 of it */
+  int shift = 100;
+  if (total_weight > 0.0) shift = 100 - (int)ceil(log2(total_weight));
+  if (shift > 900) shift = 900;
+  if (shift < -900) shift = -900;
+  P.tally_shift = shift;
+
+  {
+    Engine &G = g_engine;
+    /* histories: run each source's first `count` ids as contiguous ranges */
+    int64_t total_hist = 0;
+    for (auto &d : src) total_hist += d.valid ? d.count : 0;
+    const int cap_max = 1 << 21;
+    int cap = (int)(total_hist < cap_max ? (total_hist > 0 ? total_hist : 1) : cap_max);
+    P.cap = cap;
+    size_t per = (size_t)cap;
+    size_t state_bytes = per * (15 * sizeof(double) + (size_t)NL * sizeof(double) + 4 * sizeof(int) + sizeof(uint64_t));
+    size_t n_ch2 = (size_t)n_int * nch * 2, n_line2 = (size_t)n_int * nZ * ENG_MAX_ACT * 2, n_rc2 = (size_t)n_int * nZ * 4;
+    size_t tally_words = n_ch2 + n_line2 + n_rc2;
+    HIP_TRY(G.state.ensure(state_bytes));
+    HIP_TRY(G.tallies.ensure(tally_words * sizeof(unsigned long long)));
+    HIP_TRY(G.counters.ensure(ENG_NCOUNTERS * sizeof(unsigned int)));
+    HIP_TRY(G.queue.ensure(3 * per * sizeof(int)));
+    HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size()));
+    HIP_TRY(upload(G.el_w, elW.data(), elW.size()));
+    HIP_TRY(upload(G.el_u, elU.data(), elU.size()));
+    HIP_TRY(upload(G.precalc_mu, pmu.data(), pmu.size()));
+    HIP_TRY(upload(G.act_n, actn.data(), actn.size()));
+    HIP_TRY(upload(G.act_line, actl.data(), actl.size()));
+    HIP_TRY(upload(G.act_shell, acts.data(), acts.size()));
+    HIP_TRY(upload(G.act_factor, actf.data(), actf.size()));
+    HIP_TRY(upload(G.act_E, actE.data(), actE.size()));
+    HIP_TRY(upload(G.act_ch, actc.data(), actc.size()));
+    HIP_TRY(upload(G.act_murho, actm.data(), actm.size()));
+    HIP_TRY(upload(G.src_desc, src.data(), src.size()));
+    HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size()));
+    HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size()));
+    HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size()));
+    HIP_TRY(hipMemsetAsync(G.tallies.p, 0, tally_words * sizeof(unsigned long long), st));
+    HIP_TRY(hipMemsetAsync(G.counters.p, 0, ENG_NCOUNTERS * sizeof(unsigned int), st));
+
+    P.el_Z = G.el_Z.as<int>();
+    P.el_w = G.el_w.as<double>();
+    P.el_u = G.el_u.as<int>();
+    P.atoms = T->atoms.as<xrl_atom>();
+    P.ray_icdf = T->ray.as<double>();
+    P.comp_icdf = T->comp.as<double>();
+    P.tab_E = T->tabE.as<double>();
+    P.tab_R = T->tabR.as<double>();
+    P.n_e = T->n_e;
+    P.n_r = T->n_r;
+    P.tab_E0 = T->tab_E0;
+    P.tab_Einv = T->tab_Einv;
+    P.tab_R0 = T->tab_R0;
+    P.tab_Rinv = T->tab_Rinv;
+    P.fycorr = T->fycorr.as<double>();
+    P.ip_E = T->ipE.as<double>();
+    P.ip_r = T->ipr.as<double>();
+    P.ip_rc = T->iprc.as<double>();
+    P.ip_off = T->ipoff.as<int>();
+    P.ip_acc = T->ipacc.as<int>();
+    P.ip_acc_E0 = T->ip_acc_E0;
+    P.ip_acc_inv = T->ip_acc_inv;
+    P.cp_icdf = T->cp_icdf.as<double>();
+    P.cp_rn = T->cp_rn.as<double>();
+    P.cp_len = T->cp_len;
+    P.precalc_xrf = T->precalc_xrf.as<double>();
+    P.line_E = T->lineE.as<double>();
+    P.precalc_mu = G.precalc_mu.as<double>();
+    P.act_n = G.act_n.as<int>();
+    P.act_line = G.act_line.as<int>();
+    P.act_shell = G.act_shell.as<int>();
+    P.act_factor = G.act_factor.as<double>();
+    P.act_E = G.act_E.as<double>();
+    P.act_ch = G.act_ch.as<int>();
+    P.act_murho = G.act_murho.as<double>();
+    P.phi_icdf = T->phi.as<double>();
+    P.phi_T = T->phiT.as<double>();
+    P.phi_R = T->phiR.as<double>();
+    P.phi_nt = T->phi_nt;
+    P.phi_nr = T->phi_nr;
+    P.phi_T0 = T->phi_T0;
+    P.phi_Tinv = T->phi_Tinv;
+    P.phi_R0 = T->phi_R0;
+    P.phi_Rinv = T->phi_Rinv;
+    P.sa_grid = G.sa_grid.as<double>();
+    P.sa_r = G.sa_r.as<double>();
+    P.sa_t = G.sa_t.as<double>();
+    P.sa_nr = G.sa_nr;
+    P.sa_nt = G.sa_nt;
+    P.sa_r0 = solid_angles->grid_dims_r_vals[0];
+    P.sa_rinv = uniform_inv(solid_angles->grid_dims_r_vals, (int)solid_angles->grid_dims_r_n);
+    P.sa_t0 = solid_angles->grid_dims_theta_vals[0];
+    P.sa_tinv = uniform_inv(solid_angles->grid_dims_theta_vals, (int)solid_angles->grid_dims_theta_n);
+    unsigned long long *tw = G.tallies.as<unsigned long long>();
+    P.t_ch = tw;
+    P.t_line = tw + n_ch2;
+    P.t_rc = tw + n_ch2 + n_line2;
+    P.counters = G.counters.as<unsigned int>();
+    P.queue = G.queue.as<int>();
+
+    PhotonSoA S;
+    char *sp = G.state.as<char>();
+    double **dfields[] = {&S.x, &S.y, &S.z, &S.dx, &S.dy, &S.dz, &S.ex, &S.ey, &S.ez, &S.theta, &S.phi, &S.energy, &S.weight};
+    for (double **f : dfields) {
+      *f = (double *)sp;
+      sp += per * sizeof(double);
+    }
+    sp += 2 * per * sizeof(double);
+    S.mus = (double *)sp;
+    sp += per * NL * sizeof(double);
+    S.hid = (uint64_t *)sp;
+    sp += per * sizeof(uint64_t);
+    S.layer = (int *)sp;
+    sp += per * sizeof(int);
+    S.elem = (int *)sp;
+    sp += per * sizeof(int);
+    S.prev_line = (int *)sp;
+    sp += per * sizeof(int);
+    S.prev_u = (int *)sp;
+
+    SourceTable ST;
+    ST.d = G.src_desc.as<SourceDesc>();
+    ST.n = (int)src.size();
+    ST.line_mus = G.src_line_mus.as<double>();
+    ST.exc_layers = G.exc_layers.as<double>();
+    ST.n_exc = in->n_exc_layers;
+    ST.exc_atoms = G.exc_atoms.as<xrl_atom>();
+
+    int dev = 0, n_cu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    int vr_blocks = n_cu * 8;
+
+    HIP_TRY(hipEventCreate(&evs[0]));
+    HIP_TRY(hipEventCreate(&evs[1]));
+    HIP_TRY(hipEventRecord(evs[0], st));
+    /* walk the sources, batching consecutive history ids of each source */
+    for (size_t si = 0; si < src.size(); si++) {
+      const SourceDesc &d = src[si];
+      if (!d.valid || d.count <= 0) continue;
+      for (long done = 0; done < d.count; done += cap) {
+        int count = (int)((d.count - done) < cap ? (d.count - done) : cap);
+        HIP_TRY(eng_launch_source(&P, &S, &ST, d.hid0 + (uint64_t)done, count, st));
+        for (int n = 1; n <= n_int; n++) HIP_TRY(eng_launch_order(&P, &S, n, count, vr_blocks, st, nullptr));
+      }
+    }
+    HIP_TRY(hipEventRecord(evs[1], st));
+    std::vector<unsigned long long> th(tally_words);
+    unsigned int cnt[ENG_NCOUNTERS];
+    HIP_TRY(hipMemcpyAsync(th.data(), G.tallies.p, tally_words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cnt, G.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, evs[0], evs[1]));
+    g_stats.transport_ms = ms;
+    g_stats.histories = cnt[ENG_CNT_HISTORIES];
+    g_stats.vr_launches = (double)cnt[ENG_CNT_VR];
+    g_stats.solid_angle_fallbacks = cnt[ENG_CNT_FALLBACKS];
+    if (cnt[ENG_CNT_ERRORS]) {
+      xmi_set_error("xmi_main_msim: %u photons hit an invalid state on the device", cnt[ENG_CNT_ERRORS]);
+      goto fail;
+    }
+
+    /* exact sums: lines into their channels, prefix over orders */
+    const unsigned long long *tch = th.data(), *tln = th.data() + n_ch2, *trc = th.data() + n_ch2 + n_line2;
+    std::vector<u128> order_ch((size_t)n_int * nch);
+    for (int n = 0; n < n_int; n++) {
+      for (int c = 0; c < nch; c++) order_ch[(size_t)n * nch + c] = fx_get(&tch[((size_t)n * nch + c) * 2]);
+      for (int u = 0; u < nZ; u++)
+        for (int k = 0; k < actn[u]; k++) {
+          int c = actc[(size_t)u * ENG_MAX_ACT + k];
+          if (c < 0) continue;
+          order_ch[(size_t)n * nch + c] += fx_get(&tln[(((size_t)n * nZ + u) * ENG_MAX_ACT + k) * 2]);
+        }
+    }
+    const double live = in->detector.live_time;
+    size_t nhist = (size_t)100 * XMI_HIST_ROWS * n_int;
+    ch_out = (double *)calloc((size_t)(n_int + 1) * nch, sizeof(double));
+    vr_out = (double *)calloc(nhist, sizeof(double));
+    bf_out = (double *)calloc(nhist, sizeof(double));
+    if (!ch_out || !vr_out || !bf_out) {
+      xmi_set_error("out of host memory");
+      goto fail;
+    }
+    for (int c = 0; c < nch; c++) {
+      u128 acc = 0;
+      for (int k = 1; k <= n_int; k++) {
+        acc += order_ch[(size_t)(k - 1) * nch + c];
+        ch_out[(size_t)k * nch + c] = fx_to_double(acc, shift) * live;
+      }
+    }
+    for (int u = 0; u < nZ; u++) {
+      int Z = T->Zs[u];
+      for (int n = 0; n < n_int; n++) {
+        for (int k = 0; k < actn[u]; k++) {
+          int line = actl[(size_t)u * ENG_MAX_ACT + k];
+          vr_out[((size_t)(Z - 1) * XMI_HIST_ROWS + (line - 1)) * n_int + n] =
+              fx_to_double(fx_get(&tln[(((size_t)n * nZ + u) * ENG_MAX_ACT + k) * 2]), shift) * live * n_mpi_hosts;
+        }
+        for (int rc = 0; rc < 2; rc++)
+          vr_out[((size_t)(Z - 1) * XMI_HIST_ROWS + (383 + rc)) * n_int + n] =
+              fx_to_double(fx_get(&trc[(((size_t)n * nZ + u) * 2 + rc) * 2]), shift) * live * n_mpi_hosts;
+      }
+    }
+    ok = 1;
+  }
+fail:
+  for (int k = 0; k < 2; k++)
+    if (evs[k]) (void)hipEventDestroy(evs[k]);
+  if (!ok) {
+    free(ch_out);
+    free(vr_out);
+    free(bf_out);
+    return 0;
+  }
+  *channelsPtr = ch_out;
+  *var_red_historyPtr = vr_out;
+  *brute_historyPtr = bf_out;
+  g_stats.total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall0).count();
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* solid-angle grid (xmi_solid_angle_inputs_f, src/xmi_solid_angle_f.F90:62-301,
+ * then the per-cell estimate of :432-710 on the GPU) */
+
+static double depth_quantile(const xmi_inputF *in, double energy, double R) {
+  int nl = in->n_layers;
+  std::vector<double> mu(nl);
+  double my_sum = 0.0;
+  for (int i = 0; i < nl; i++) {
+    mu[i] = layer_mu_host(&in->layers[i], energy);
+    my_sum += mu[i] * in->layers[i].density * in->layers[i].thickness_along_Z;
+  }
+  double Pabs = -1.0 * expm1(-1.0 * my_sum);
+  double myln = -1.0 * log1p(-1.0 * R * Pabs);
+  int m = nl - 1;
+  my_sum = 0.0;
+  for (int i = 0; i < nl; i++) {
+    my_sum += mu[i] * in->layers[i].density * in->layers[i].thickness_along_Z;
+    if (my_sum > myln) {
+      m = i;
+      break;
+    }
+  }
+  my_sum = 0.0;
+  for (int i = 0; i < m; i++)
+    my_sum += (1.0 - (mu[i] * in->layers[i].density) / (mu[m] * in->layers[m].density)) * in->layers[i].thickness_along_Z;
+  return my_sum + myln / (mu[m] * in->layers[m].density) + in->layers[0].Z_coord_begin;
+}
+
+extern "C" void xmi_free_solid_angle(xmi_solid_angle *sa) {
+  if (!sa) return;
+  free(sa->solid_angles);
+  free(sa->grid_dims_r_vals);
+  free(sa->grid_dims_theta_vals);
+  free(sa);
+}
+
+extern "C" int xmi_solid_angle_calculation_cl(xmi_inputFPtr inputFPtr, xmi_solid_angle **out, char *input_string,
+                                              xmi_main_options *xmo) {
+  xmi_inputF *in = (xmi_inputF *)inputFPtr;
+  if (!in || !out || !in->initialised) {
+    xmi_set_error("xmi_solid_angle_calculation_cl: invalid input");
+    return 0;
+  }
+  (void)xmo;
+  const long n = 1024;
+  double e1, e2;
+  if (in->n_continuous > 1 && in->n_discrete > 0) {
+    e1 = fmin(in->continuous[0].energy, in->discrete[0].energy);
+    e2 = fmax(in->continuous[in->n_continuous - 1].energy, in->discrete[in->n_discrete - 1].energy);
+  } else if (in->n_continuous > 1) {
+    e1 = in->continuous[0].energy;
+    e2 = in->continuous[in->n_continuous - 1].energy;
+  } else {
+    e1 = in->discrete[0].energy;
+    e2 = in->discrete[in->n_discrete - 1].energy;
+  }
+  double S1 = depth_quantile(in, e1, 0.00001), S2 = depth_quantile(in, e2, 0.99999);
+  const double *pd = in->geometry.p_detector_window;
+  double d1 = sqrt(pd[0] * pd[0] + pd[1] * pd[1] + (pd[2] - S1) * (pd[2] - S1));
+  double d2 = sqrt(pd[0] * pd[0] + pd[1] * pd[1] + (pd[2] - S2) * (pd[2] - S2));
+  double rmax = (d1 > d2 ? d1 : d2) * 1.25, rmin = rmax / (double)n;
+  double tmin = 0.00001, tmax = M_PI / 2.0;
+  xmi_solid_angle *sa = (xmi_solid_angle *)calloc(1, sizeof(xmi_solid_angle));
+  sa->grid_dims_r_n = n;
+  sa->grid_dims_theta_n = n;
+  sa->grid_dims_r_vals = (double *)malloc(sizeof(double) * n);
+  sa->grid_dims_theta_vals = (double *)malloc(sizeof(double) * n);
+  sa->solid_angles = (double *)malloc(sizeof(double) * n * n);
+  sa->xmi_input_string = input_string;
+  for (long i = 0; i < n; i++) {
+    sa->grid_dims_r_vals[i] = rmin + (rmax - rmin) * (double)i / (double)(n - 1);
+    sa->grid_dims_theta_vals[i] = tmin + (tmax - tmin) * (double)i / (double)(n - 1);
+  }
+  std::lock_guard<std::mutex> guard(g_engine.lock);
+  if (!engine_init()) {
+    xmi_free_solid_angle(sa);
+    return 0;
+  }
+  EngineParams P;
+  memset(&P, 0, sizeof(P));
+  fill_geometry(P, in);
+  uint64_t seed;
+  if (xmo && xmo->use_default_seeds) seed = 1;
+  else {
+    if (!g_seed_set) {
+      g_seed = (uint64_t)time(NULL) * 6364136223846793005ull + (uint64_t)clock();
+      g_seed_set = 1;
+    }
+    seed = g_seed;
+  }
+  hipStream_t st = g_engine.stream;
+  int ok = 0;
+  if (upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)n) == hipSuccess &&
+      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)n) == hipSuccess &&
+      g_engine.sa_grid.ensure(sizeof(double) * n * n) == hipSuccess &&
+      eng_launch_grid(&P, g_engine.sa_grid.as<double>(), g_engine.sa_r.as<double>(), g_engine.sa_t.as<double>(), (int)n,
+                      (int)n, xmi_hits_per_single, seed, st) == hipSuccess &&
+      hipMemcpyAsync(sa->solid_angles, g_engine.sa_grid.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, st) ==
+          hipSuccess &&
+      hipStreamSynchronize(st) == hipSuccess)
+    ok = 1;
+  if (!ok) {
+    xmi_set_error("xmi_solid_angle_calculation_cl: device failure");
+    g_engine.sa_key = nullptr;
+    xmi_free_solid_angle(sa);
+    return 0;
+  }
+  /* the device copy is the grid just returned: remember it */
+  g_engine.sa_key = sa;
+  g_engine.sa_hash = sampled_hash(sa);
+  g_engine.sa_nr = (int)n;
+  g_engine.sa_nt = (int)n;
+  *out = sa;
+  return 1;
+}

@@ -1,0 +1,63 @@
+/*
+ * engine_kernels.h -- interface between the host driver (engine_host.cpp) and
+ * the gfx950 kernels (engine_kernels.hip) of the photon-history engine.
+ */
+#ifndef XMIMSIM_AMD_ENGINE_KERNELS_H
+#define XMIMSIM_AMD_ENGINE_KERNELS_H
+
+#include "engine_device.h"
+
+#define ENG_BLOCK 256
+#define ENG_LDS_BYTES 65536        /* LDS budget of the estimator's tallies */
+#define XMI_NLINES_VR_DEV 219
+
+/* counters[] slots */
+#define ENG_CNT_Q_RAYL 0
+#define ENG_CNT_Q_COMPT 1
+#define ENG_CNT_Q_PHOTO 2
+#define ENG_CNT_VR 3
+#define ENG_CNT_FALLBACKS 4
+#define ENG_CNT_ERRORS 5
+#define ENG_CNT_HISTORIES 6
+#define ENG_NCOUNTERS 8
+
+/* m_e c^2 in keV as the reference computes it (src/xmi_main.F90:48-55) */
+#define ENG_MEC2 (9.10938188e-31 * 2.99792458e8 * 2.99792458e8 / 1.602176487e-19 / 1000.0)
+
+struct SourceDesc {
+  int kind;               /* 0 = continuous interval, 1 = discrete line */
+  int valid;
+  int dist;               /* XmiEnergyDiscreteDistribution for lines */
+  int pad;
+  long count;             /* histories of this source */
+  uint64_t hid0;          /* first history id */
+  double x1, x2, y1, y2;  /* interval: energies and total intensities at the ends */
+  double h1, h2;          /* interval: horizontal intensities at the ends */
+  double energy, scale;   /* line */
+  double hor_ver_ratio;   /* line: photons with 1-based index <= this are horizontal */
+  double weight;          /* interval: before excitation-absorber correction; line: final */
+  double sx, sxp, sy, syp;
+};
+
+struct SourceTable {
+  const SourceDesc *d;
+  int n;
+  const double *line_mus;     /* [n_sources][n_layers] mus at the line energy */
+  const double *exc_layers;   /* per absorber: density, thickness, n_el, Z[n_el], w[n_el] */
+  int n_exc;
+  const xrl_atom *exc_atoms;  /* [95] indexed by Z */
+};
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv, const double *tv, int n_r, int n_t,
+                           long hits, uint64_t seed, hipStream_t st);
+hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,
+                             int count, hipStream_t st);
+hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, int n, int count, int vr_blocks,
+                            hipStream_t st, hipEvent_t *ev);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,331 @@
+/*
+ * xrl_model.h -- self-contained atomic-physics provider for the photon-history
+ * engine.
+ *
+ * The reference engine evaluates xraylib at run time (CS_Total_Kissel,
+ * CS_Photo_Partial, DCSP_Rayl, DCSP_Compt, FluorYield, RadRate, LineEnergy,
+ * EdgeEnergy, CosKronTransProb and the P{L,M}x_*_cascade_kissel family; call
+ * sites listed in src/xmi_variance_reduction.F90:344-649 and
+ * src/xmi_main.F90:1564,2257,2272,5200-5291,5406).  xraylib and its data files
+ * are not available offline, so this provider defines a closed, smooth,
+ * physically shaped model of the same quantities:
+ *
+ *   - binding (edge) energies: power laws a*(Z-z0)^p per subshell, fitted to a
+ *     handful of tabulated edges; a subshell exists when Z > z0;
+ *   - line energy = E_edge(vacancy shell) - E_edge(donor shell);
+ *   - photo-ionisation: every subshell s has tau_s(E) = B_s * E^-2.75 above its
+ *     edge, with the jump ratios J_s fixing the split (nested construction, so
+ *     that the total is continuous in log-log between edges);
+ *   - Rayleigh / Compton: Thomson and Klein-Nishina differential cross sections
+ *     times a model form factor F(x,Z) and incoherent scattering function
+ *     S(x,Z); the integrated cross sections are tabulated per element on a
+ *     uniform ln(E) grid and interpolated log-log;
+ *   - fluorescence yields, Coster-Kronig probabilities, radiative rates
+ *     (dipole rule weighted by donor occupancy), Auger vacancy transfer;
+ *   - Compton profiles: sum over occupied subshells of hydrogenic 1s-shaped
+ *     profiles with the subshell's characteristic momentum.
+ *
+ * All per-element parameters live in one POD struct (xrl_atom) so that the host
+ * (oracle, table generator) and the device kernels evaluate exactly the same
+ * formulas on exactly the same numbers.  The functions below are the only
+ * place where the model is evaluated.
+ */
+#ifndef XMIMSIM_AMD_XRL_MODEL_H
+#define XMIMSIM_AMD_XRL_MODEL_H
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define XRL_FN __host__ __device__ static inline
+#else
+#define XRL_FN static inline
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* xraylib shell numbering: K=0, L1..L3=1..3, M1..M5=4..8, N1..N7=9..15,
+ * O1..O7=16..22, P1..P5=23..27, Q1..Q3=28..30 */
+#define XRL_NSHELLS 31
+#define XRL_K_SHELL 0
+#define XRL_L1_SHELL 1
+#define XRL_L2_SHELL 2
+#define XRL_L3_SHELL 3
+#define XRL_M1_SHELL 4
+#define XRL_M5_SHELL 8
+
+/* xraylib line numbering (absolute values of the *_LINE macros):
+ * KL1=1 ... KP5=29, L1L2=30 ... L3Q1=113, M1M2=114 ... M5P5=219 ... P3P5=383 */
+#define XRL_NLINES 383
+#define XRL_KL1 1
+#define XRL_KP5 29
+#define XRL_L1_FIRST 30
+#define XRL_L1_LAST 58
+#define XRL_L2_FIRST 59
+#define XRL_L2_LAST 85
+#define XRL_L3_FIRST 86
+#define XRL_L3_LAST 113
+#define XRL_M1_FIRST 114
+#define XRL_M1_LAST 136
+#define XRL_M2_FIRST 137
+#define XRL_M2_LAST 158
+#define XRL_M3_FIRST 159
+#define XRL_M3_LAST 180
+#define XRL_M4_FIRST 181
+#define XRL_M4_LAST 200
+#define XRL_M5_FIRST 201
+#define XRL_M5_LAST 219
+#define XRL_L3Q1 113
+#define XRL_M5P5 219
+
+/* Coster-Kronig transitions, xraylib order: FL12, FL13, FLP13, FL23, FM12,
+ * FM13, FM14, FM15, FM23, FM24, FM25, FM34, FM35, FM45 */
+#define XRL_NCK 14
+enum { XRL_FL12 = 0, XRL_FL13, XRL_FLP13, XRL_FL23, XRL_FM12, XRL_FM13, XRL_FM14,
+       XRL_FM15, XRL_FM23, XRL_FM24, XRL_FM25, XRL_FM34, XRL_FM35, XRL_FM45 };
+
+/* cascade types, numbering of src/xmi_aux_f.F90:662-665 */
+#define XRL_CASCADE_NONE 1
+#define XRL_CASCADE_NONRADIATIVE 2
+#define XRL_CASCADE_RADIATIVE 3
+#define XRL_CASCADE_FULL 4
+
+/* integrated Rayleigh/Compton cross sections: uniform grid in ln(E) */
+#define XRL_NCS 1024
+#define XRL_CS_LNE0 (-4.605170185988091)  /* ln(0.01 keV) */
+#define XRL_CS_LNE1 (6.907755278982137)   /* ln(1000 keV) */
+
+#define XRL_PHOTO_EXPONENT 2.75
+#define XRL_MEC2 510.998910              /* keV, value used in xmi_main.F90:2198 */
+#define XRL_RE2 0.079407877              /* classical electron radius^2, barn */
+#define XRL_AVOG_BARN 0.602214129        /* N_A * 1e-24 */
+#define XRL_KEV_ANGSTROM 12.39842        /* hc in keV*Angstrom */
+
+typedef struct xrl_atom {
+  int Z;
+  int n_occupied;                        /* number of occupied subshells */
+  double A;                              /* g/mol */
+  double edge[XRL_NSHELLS];              /* keV, 0 when absent */
+  double occupancy[XRL_NSHELLS];         /* electrons */
+  double fluor_yield[9];                 /* K..M5 */
+  double ck[XRL_NCK];
+  double rad_rate[XRL_NLINES + 1];       /* index |line|, [0] unused */
+  double line_energy[XRL_NLINES + 1];    /* keV, 0 when the line is absent */
+  double photo_B[9];                     /* tau_s(E) = photo_B[s]*E^-p, E>=edge */
+  double photo_B_outer;                  /* all shells beyond M5 */
+  double auger_KL[3];                    /* vacancies produced in L1..L3 per K Auger decay */
+  double auger_KM[5];                    /* ... in M1..M5 per K Auger decay */
+  double auger_LM[3][5];                 /* ... in M1..M5 per L1..L3 Auger decay */
+  double ff_x0;                          /* Rayleigh form factor scale, 1/Angstrom */
+  double sf_x0;                          /* incoherent scattering function scale */
+  double profile_p[XRL_NSHELLS];         /* Compton profile momentum, atomic units */
+  double ln_cs_rayl[XRL_NCS];            /* ln(cm2/g) on the ln(E) grid */
+  double ln_cs_compt[XRL_NCS];
+} xrl_atom;
+
+/* --------------------------------------------------------------------------
+ * Evaluation (host and device)
+ * ------------------------------------------------------------------------*/
+
+XRL_FN double xrl_cs_table(const double *tab, double E) {
+  const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
+  double x = (log(E) - XRL_CS_LNE0) / h;
+  int i = (int)floor(x);
+  if (i < 0) i = 0;
+  if (i > XRL_NCS - 2) i = XRL_NCS - 2;
+  double f = x - (double)i;
+  return exp(tab[i] + f * (tab[i + 1] - tab[i]));
+}
+
+XRL_FN double xrl_cs_rayl(const xrl_atom *a, double E) { return xrl_cs_table(a->ln_cs_rayl, E); }
+XRL_FN double xrl_cs_compt(const xrl_atom *a, double E) { return xrl_cs_table(a->ln_cs_compt, E); }
+
+/* photo-ionisation partial cross section of shell s (0..8), cm2/g */
+XRL_FN double xrl_cs_photo_partial(const xrl_atom *a, int s, double E) {
+  if (s < 0 || s > 8) return 0.0;
+  double e = a->edge[s];
+  if (e <= 0.0 || E < e) return 0.0;
+  return a->photo_B[s] * pow(E, -XRL_PHOTO_EXPONENT);
+}
+
+/* total photo-ionisation cross section, cm2/g */
+XRL_FN double xrl_cs_photo_total(const xrl_atom *a, double E) {
+  double B = a->photo_B_outer;
+  for (int s = 0; s < 9; s++)
+    if (a->edge[s] > 0.0 && E >= a->edge[s]) B += a->photo_B[s];
+  return B * pow(E, -XRL_PHOTO_EXPONENT);
+}
+
+/* CS_Total_Kissel equivalent: photo + Rayleigh + Compton, cm2/g */
+XRL_FN double xrl_cs_total(const xrl_atom *a, double E) {
+  return xrl_cs_photo_total(a, E) + xrl_cs_rayl(a, E) + xrl_cs_compt(a, E);
+}
+
+/* momentum transfer x = sin(theta/2)/lambda, 1/Angstrom */
+XRL_FN double xrl_moment_transf(double E, double theta) {
+  return E / XRL_KEV_ANGSTROM * sin(theta * 0.5);
+}
+
+XRL_FN double xrl_ff_rayl(const xrl_atom *a, double x) {
+  double u = x / a->ff_x0;
+  double d = 1.0 + 3.0 * u * u;
+  return (double)a->Z / (d * d);
+}
+
+XRL_FN double xrl_sf_compt(const xrl_atom *a, double x) {
+  double u = x / a->sf_x0;
+  double d = 1.0 + u * u;
+  return (double)a->Z * (1.0 - 1.0 / (d * d));
+}
+
+/* unpolarised differential cross sections, cm2/g/sr */
+XRL_FN double xrl_dcs_rayl(const xrl_atom *a, double E, double theta) {
+  double c = cos(theta);
+  double F = xrl_ff_rayl(a, xrl_moment_transf(E, theta));
+  return XRL_RE2 * 0.5 * (1.0 + c * c) * F * F * XRL_AVOG_BARN / a->A;
+}
+
+XRL_FN double xrl_dcs_compt(const xrl_atom *a, double E, double theta) {
+  double c = cos(theta);
+  double k = 1.0 / (1.0 + E / XRL_MEC2 * (1.0 - c));
+  double kn = XRL_RE2 * 0.5 * k * k * (k + 1.0 / k - (1.0 - c * c));
+  return kn * xrl_sf_compt(a, xrl_moment_transf(E, theta)) * XRL_AVOG_BARN / a->A;
+}
+
+/* linearly polarised differential cross sections (DCSP_*), cm2/g/sr */
+XRL_FN double xrl_dcsp_rayl(const xrl_atom *a, double E, double theta, double phi) {
+  double st = sin(theta), cp = cos(phi);
+  double F = xrl_ff_rayl(a, xrl_moment_transf(E, theta));
+  return XRL_RE2 * (1.0 - st * st * cp * cp) * F * F * XRL_AVOG_BARN / a->A;
+}
+
+XRL_FN double xrl_dcsp_compt(const xrl_atom *a, double E, double theta, double phi) {
+  double c = cos(theta), st = sin(theta), cp = cos(phi);
+  double k = 1.0 / (1.0 + E / XRL_MEC2 * (1.0 - c));
+  double kn = XRL_RE2 * 0.5 * k * k * (k + 1.0 / k - 2.0 * st * st * cp * cp);
+  return kn * xrl_sf_compt(a, xrl_moment_transf(E, theta)) * XRL_AVOG_BARN / a->A;
+}
+
+/* total Compton profile (electrons per atomic unit of momentum) */
+XRL_FN double xrl_compton_profile_partial(const xrl_atom *a, int s, double pz) {
+  if (a->occupancy[s] <= 0.0) return 0.0;
+  double p = a->profile_p[s];
+  double u = pz / p;
+  double d = 1.0 + u * u;
+  return a->occupancy[s] * 8.0 / (3.0 * 3.14159265358979323846 * p) / (d * d * d);
+}
+
+XRL_FN double xrl_compton_profile(const xrl_atom *a, double pz) {
+  double sum = 0.0;
+  for (int s = 0; s < XRL_NSHELLS; s++) sum += xrl_compton_profile_partial(a, s, pz);
+  return sum;
+}
+
+/* vacancy-shell -> first/last line, lines are 1-based |xraylib line| */
+XRL_FN void xrl_shell_lines(int shell, int *first, int *last) {
+  const int f[9] = {XRL_KL1, XRL_L1_FIRST, XRL_L2_FIRST, XRL_L3_FIRST, XRL_M1_FIRST,
+                    XRL_M2_FIRST, XRL_M3_FIRST, XRL_M4_FIRST, XRL_M5_FIRST};
+  const int l[9] = {XRL_KP5, XRL_L1_LAST, XRL_L2_LAST, XRL_L3_LAST, XRL_M1_LAST,
+                    XRL_M2_LAST, XRL_M3_LAST, XRL_M4_LAST, XRL_M5_LAST};
+  if (shell < 0 || shell > 8) { *first = 1; *last = 0; return; }
+  *first = f[shell];
+  *last = l[shell];
+}
+
+/* vacancy shell (0..8) of a line 1..219, -1 otherwise */
+XRL_FN int xrl_line_shell(int line) {
+  if (line >= XRL_KL1 && line <= XRL_KP5) return 0;
+  if (line >= XRL_L1_FIRST && line <= XRL_L1_LAST) return 1;
+  if (line >= XRL_L2_FIRST && line <= XRL_L2_LAST) return 2;
+  if (line >= XRL_L3_FIRST && line <= XRL_L3_LAST) return 3;
+  if (line >= XRL_M1_FIRST && line <= XRL_M1_LAST) return 4;
+  if (line >= XRL_M2_FIRST && line <= XRL_M2_LAST) return 5;
+  if (line >= XRL_M3_FIRST && line <= XRL_M3_LAST) return 6;
+  if (line >= XRL_M4_FIRST && line <= XRL_M4_LAST) return 7;
+  if (line >= XRL_M5_FIRST && line <= XRL_M5_LAST) return 8;
+  return -1;
+}
+
+/* xraylib line numbers of the K->L_i, K->M_j, L_i->M_j radiative transitions */
+XRL_FN int xrl_line_KL(int i) { return 1 + i; }          /* KL1..KL3 */
+XRL_FN int xrl_line_KM(int j) { return 4 + j; }          /* KM1..KM5 */
+XRL_FN int xrl_line_LM(int i, int j) {                   /* L1M1.., L2M1.., L3M1.. */
+  const int base[3] = {32, 60, 86};
+  return base[i] + j;
+}
+
+/*
+ * Cascade-corrected shell ionisation cross sections (P_K .. P_M5) at energy E,
+ * the equivalent of the xraylib family P{L1..M5}_{pure,auger_cascade,
+ * rad_cascade,full_cascade}_kissel used in src/xmi_variance_reduction.F90:443-572
+ * and src/xmi_data_f.F90:1311-1474.  P[0] is only filled when E >= K edge.
+ */
+XRL_FN void xrl_shell_cross_sections(const xrl_atom *a, int cascade_type, double E, double P[9]) {
+  const int rad = (cascade_type == XRL_CASCADE_RADIATIVE || cascade_type == XRL_CASCADE_FULL);
+  const int aug = (cascade_type == XRL_CASCADE_NONRADIATIVE || cascade_type == XRL_CASCADE_FULL);
+  double pe = pow(E, -XRL_PHOTO_EXPONENT);
+  double tau[9];
+  for (int s = 0; s < 9; s++)
+    tau[s] = (a->edge[s] > 0.0 && E >= a->edge[s]) ? a->photo_B[s] * pe : 0.0;
+  double PK = tau[0];
+  P[0] = PK;
+  /* L shells: feeding from K vacancies, then L Coster-Kronig */
+  double fromK[3];
+  for (int i = 0; i < 3; i++) {
+    double t = 0.0;
+    if (rad) t += a->fluor_yield[0] * a->rad_rate[xrl_line_KL(i)];
+    if (aug) t += (1.0 - a->fluor_yield[0]) * a->auger_KL[i];
+    fromK[i] = PK * t;
+  }
+  double PL1 = tau[1] + fromK[0];
+  double PL2 = tau[2] + fromK[1] + a->ck[XRL_FL12] * PL1;
+  double PL3 = tau[3] + fromK[2] + a->ck[XRL_FL23] * PL2 + (a->ck[XRL_FL13] + a->ck[XRL_FLP13]) * PL1;
+  P[1] = PL1;
+  P[2] = PL2;
+  P[3] = PL3;
+  double PL[3] = {PL1, PL2, PL3};
+  double feed[5];
+  for (int j = 0; j < 5; j++) {
+    double t = 0.0;
+    if (rad) {
+      t += PK * a->fluor_yield[0] * a->rad_rate[xrl_line_KM(j)];
+      for (int i = 0; i < 3; i++) t += PL[i] * a->fluor_yield[1 + i] * a->rad_rate[xrl_line_LM(i, j)];
+    }
+    if (aug) {
+      t += PK * (1.0 - a->fluor_yield[0]) * a->auger_KM[j];
+      for (int i = 0; i < 3; i++) t += PL[i] * (1.0 - a->fluor_yield[1 + i]) * a->auger_LM[i][j];
+    }
+    feed[j] = t;
+  }
+  double PM1 = tau[4] + feed[0];
+  double PM2 = tau[5] + feed[1] + a->ck[XRL_FM12] * PM1;
+  double PM3 = tau[6] + feed[2] + a->ck[XRL_FM23] * PM2 + a->ck[XRL_FM13] * PM1;
+  double PM4 = tau[7] + feed[3] + a->ck[XRL_FM34] * PM3 + a->ck[XRL_FM24] * PM2 + a->ck[XRL_FM14] * PM1;
+  double PM5 = tau[8] + feed[4] + a->ck[XRL_FM45] * PM4 + a->ck[XRL_FM35] * PM3 + a->ck[XRL_FM25] * PM2 +
+               a->ck[XRL_FM15] * PM1;
+  P[4] = PM1;
+  P[5] = PM2;
+  P[6] = PM3;
+  P[7] = PM4;
+  P[8] = PM5;
+}
+
+/* --------------------------------------------------------------------------
+ * Construction (host only, xrl_model.c)
+ * ------------------------------------------------------------------------*/
+/* fill *a for element Z (1..94); returns 1 on success */
+int xrl_atom_init(xrl_atom *a, int Z);
+/* process-wide cache of all 94 elements; never freed */
+const xrl_atom *xrl_atom_get(int Z);
+/* xraylib line names, index 1..383 (0 = ""), 384 = "Rayleigh", 385 = "Compton" */
+const char *xrl_line_name(int line);
+/* donor/vacancy shells of a line (-1 for composite lines such as KO, L1N67) */
+int xrl_line_shells(int line, int *vacancy, int *donor);
+/* fluorescence yields corrected for Coster-Kronig transfer (src/xmi_data_f.F90:1244-1305) */
+void xrl_fluor_yields_corr(const xrl_atom *a, double out[9]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
