@@ -236,17 +236,40 @@ void xmi_set_seed(uint64_t seed);
 
 /* Timing and counters of the last xmi_main_msim call on this thread. */
 typedef struct _xmi_msim_stats {
-  double transport_ms;        /* device time of the transport kernels */
+  double transport_ms;        /* device time of all transport kernels (events) */
   double total_ms;            /* wall time of xmi_main_msim */
   int64_t histories;          /* primary photons simulated */
-  int64_t interactions;       /* forced interactions processed */
-  double vr_kernel_ms;        /* device time in the detection-weight kernel */
-  double vr_launches;         /* launches of the detection-weight kernel */
-  double step_kernel_ms;
-  double interaction_kernel_ms;
+  int64_t interactions;       /* with profiling on: launches of each kernel class */
+  double vr_kernel_ms;        /* with profiling on: device time of the detection-weight kernel */
+  double vr_launches;         /* detection-weight evaluations (photon-interactions) */
+  double step_kernel_ms;      /* with profiling on: forced-step kernel (incl. queue reset) */
+  double interaction_kernel_ms; /* with profiling on: Rayleigh + Compton + photo kernels */
   int64_t solid_angle_fallbacks;
 } xmi_msim_stats;
 void xmi_msim_get_stats(xmi_msim_stats *out);
+
+/* Times every kernel class with HIP events (synchronising once per batch)
+ * when on; used by the benchmark to measure the detection-weight kernel. */
+void xmi_msim_set_profiling(int on);
+
+/* Detection-weight evaluations per layer of the last xmi_main_msim call on
+ * this thread; writes min(n, 16) values, returns how many. */
+int xmi_msim_get_layer_events(int64_t *out, int n);
+
+/* ---- physics provider queries ----------------------------------------
+ * No counterpart in the reference, which calls xraylib directly; these expose
+ * the built-in provider that replaces xraylib (line names of
+ * src/xmi_lines.c's numbering, cross sections in cm2/g, energies in keV). */
+const char *xmi_line_name(int line);              /* 1..383, 384 Rayleigh, 385 Compton */
+int xmi_provider_atom(int Z, void *xrl_atom_out);  /* copies the provider's per-element record */
+double xmi_provider_cs(int kind, int Z, double E); /* 0 total, 1 photo, 2 Rayleigh, 3 Compton */
+double xmi_provider_dcs(int kind, int Z, double E, double theta); /* 0 Rayleigh, 1 Compton, cm2/g/sr */
+int xmi_provider_shell_cross_sections(int Z, int cascade_type, double E, double *P9);
+
+/* Number of fluorescence lines (1..line_last, line_last <= 219) of element Z
+ * that the detection-weight estimator evaluates: energy >= 1 keV and a
+ * non-zero yield x radiative rate in the built-in physics provider. */
+int xmi_provider_active_lines(int Z, int line_last);
 
 /* Last error message of this thread, or "" */
 const char *xmi_msim_last_error(void);

@@ -73,6 +73,10 @@ def run(prepared, seed: int, rng_mode: int = RNG_PER_HISTORY, n_threads: int = 0
     lib = load()
     if n_threads <= 0:
         n_threads = os.cpu_count() or 1
+    # the reference shares hits_per_single between the grid and the in-estimator
+    # Monte Carlo fallback: use the engine's current value
+    from xmimsim_amd import api
+    lib.xmi_oracle_set_hits_per_single(C.c_long.in_dll(api.load_library(), "xmi_hits_per_single").value)
     ch = C.POINTER(C.c_double)()
     bh = C.POINTER(C.c_double)()
     vr = C.POINTER(C.c_double)()
@@ -98,6 +102,7 @@ def solid_angle(prepared, hits: int, seed: int, n_threads: int = 0):
     lib = load()
     from xmimsim_amd import api
     sa = C.POINTER(api.xmi_solid_angle)()
+    C.c_long.in_dll(api.load_library(), "xmi_hits_per_single").value = hits
     if not lib.xmi_oracle_solid_angle_calculation(prepared.inputF, C.byref(sa), hits, seed,
                                                   n_threads or (os.cpu_count() or 1)):
         raise RuntimeError("oracle solid angle failed")

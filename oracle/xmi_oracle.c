@@ -715,7 +715,7 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
       double energy = p->energy * 1000.0;
       double c_lamb0 = cc / energy;
       double sth2 = sin(theta / 2.0);
-      int it = 0;
+      int it = 0, failed = 0;
       int nrn = z->cp_len;
       for (;;) {
         double r = orc_uniform(rng);
@@ -726,14 +726,14 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
         if (orc_uniform(rng) < 0.5) pz = -pz;
         double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
         energy = cc / (c_lamb0 + dlamb) / 1000.0;
-        if (energy <= p->energy) break;
+        if (energy <= p->energy) break; /* also accepts negative energies, as the reference does */
         if (it == 100) {
-          energy = -1.0;
+          failed = 1; /* the reference stops with an error here */
           break;
         }
         it++;
       }
-      if (energy > 0.0) {
+      if (!failed) {
         double m2 = 0.0;
         for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
           m2 += layer_mu(c, &in->layers[j - 1], energy) * in->layers[j - 1].density * distances[j - 1];

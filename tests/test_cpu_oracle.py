@@ -1,0 +1,152 @@
+"""CPU tests: the oracle's numerics against closed forms and the reference's
+own known answers, and the oracle itself against the closed-form first-order
+fluorescence intensity (tests/libxmimsim-test.c:153-180)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from xmimsim_amd import api, workloads
+
+
+# tests/test-poly-solve-quadratic.F90:10-12 (Fortran default-real literals)
+QA = np.float32([0.0, 0.0, 0.0, 0.0, 1.0, -5.00, 5.00, 3.00, -2.56, 1e-3]).astype(np.float64)
+QB = np.float32([0.0, 0.0, 3.0, 3.0, 2.0, 8.50, -8.50, 0.00, 7.25, -2.00]).astype(np.float64)
+QC = np.float32([0.0, 1.0, 0.0, -5.0, 1.0, -27.0, 27.0, 6.00, -6e-3, -2.00]).astype(np.float64)
+
+
+def gsl_quadratic(a, b, c):
+    """Root count and sorted roots with the conventions of gsl_poly_solve_quadratic."""
+    if a == 0.0:
+        if b == 0.0:
+            return 0, []
+        return 1, [-c / b]
+    disc = b * b - 4 * a * c
+    if disc < 0:
+        return 0, []
+    if disc == 0:
+        r = -0.5 * b / a
+        return 2, [r, r]
+    r = np.sort(np.roots([a, b, c]).real)
+    return 2, list(r)
+
+
+def test_quadratic_solver_matches_reference_cases(orc):
+    lib = orc.load()
+    for a, b, c in zip(QA, QB, QC):
+        r1, r2 = C.c_double(), C.c_double()
+        n = lib.xmi_oracle_poly_solve_quadratic(a, b, c, C.byref(r1), C.byref(r2))
+        n_ref, roots = gsl_quadratic(a, b, c)
+        assert n == n_ref, (a, b, c)
+        if n >= 1:
+            assert abs(r1.value - roots[0]) < 1e-7
+        if n == 2:
+            assert abs(r2.value - roots[1]) < 1e-7
+
+
+def test_philox_known_answers(orc):
+    """Random123 known-answer vectors of Philox4x32-10 (Salmon et al., SC'11)."""
+    lib = orc.load()
+    cases = [
+        ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+        ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+        ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+         (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+    ]
+    for ctr, key, want in cases:
+        out = (C.c_uint32 * 4)()
+        lib.xmi_oracle_philox((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)
+        assert tuple(out) == want
+
+
+def test_compton_energy_q_roundtrip(orc):
+    lib = orc.load()
+    mec2 = 9.10938188e-31 * 2.99792458e8 ** 2 / 1.602176487e-19 / 1000.0
+    for E in (5.0, 17.4, 60.0, 150.0):
+        for th in (0.3, 1.2, 2.5, 3.0):
+            # Q = 0: textbook Compton formula
+            e0 = lib.xmi_oracle_energy_from_q(E, 0.0, th)
+            assert e0 == pytest.approx(E / (1 + E / mec2 * (1 - math.cos(th))), rel=1e-12)
+            for Q in (-20.0, -1.5, 0.7, 12.0):
+                e1 = lib.xmi_oracle_energy_from_q(E, Q, th)
+                if e1 <= 0:
+                    continue
+                assert lib.xmi_oracle_q_from_energy(E, e1, th) == pytest.approx(Q, rel=1e-9, abs=1e-9)
+
+
+def test_trapezoid_sampler_cdf(orc):
+    """Inverse-CDF sampler of the linear density on [x1,x2]: F(x) is closed form."""
+    lib = orc.load()
+    x1, x2, y1, y2 = 10.0, 10.1, 3.0, 1.0
+    u = (np.arange(2000) + 0.5) / 2000
+    xs = np.array([lib.xmi_oracle_ran_trap(x1, x2, y1, y2, float(v)) for v in u])
+    m = (y2 - y1) / (x2 - x1)
+    F = ((y1 - m * x1) * (xs - x1) + m * (xs ** 2 - x1 ** 2) / 2) / ((y1 + y2) * (x2 - x1) / 2)
+    assert np.max(np.abs(F - u)) < 1e-9
+    # rising density as well
+    xs = np.array([lib.xmi_oracle_ran_trap(1.0, 1.1, 0.5, 4.0, float(v)) for v in u])
+    assert np.all(np.diff(xs) > 0) and xs[0] >= 1.0 and xs[-1] <= 1.1
+
+
+def _prepared(inp, hits=64, seed=3):
+    import oracle
+    prep = api.Prepared(inp)
+    oracle.solid_angle(prep, hits, seed, n_threads=0)
+    return prep
+
+
+def test_oracle_first_order_matches_closed_form(orc):
+    """The reference's CaSO4 known-answer set-up: first-order Ca-KL3 of the
+    variance-reduced Monte Carlo equals the fundamental-parameter formula.
+    The reference asserts 1 % against its own tables; here the provider's
+    tables are used for both sides and the tolerance covers 2e5 photons and a
+    64-trial solid-angle grid."""
+    prep = _prepared(workloads.caso4_kat(n_photons_line=200_000), hits=64)
+    ch, vr, bf, st = orc.run(prep, seed=9, rng_mode=orc.RNG_PER_HISTORY)
+    for Z, line in ((20, 3), (20, 2), (16, 3)):
+        mc = vr[Z - 1, line - 1, 0]
+        fpm = orc.fpm_first_order(prep, Z, line, 4)
+        assert mc == pytest.approx(fpm, rel=0.025), (Z, line, mc, fpm)
+    assert np.all(bf == 0.0)
+
+
+def test_oracle_rng_modes_agree_statistically(orc):
+    """Per-history Philox streams and the reference's one-stream-per-thread
+    structure estimate the same intensities (3 sigma over 6 seeds each)."""
+    prep = _prepared(workloads.single_layer_steel(n_photons_line=4000), hits=32)
+    a = np.array([orc.run(prep, seed=s, rng_mode=orc.RNG_PER_HISTORY)[1][[25, 23, 27], 2].sum(axis=-1)
+                  for s in range(6)])
+    b = np.array([orc.run(prep, seed=100 + s, rng_mode=orc.RNG_PER_THREAD, n_threads=4)[1][[25, 23, 27], 2]
+                  .sum(axis=-1) for s in range(6)])
+    sigma = np.sqrt(a.var(axis=0, ddof=1) / 6 + b.var(axis=0, ddof=1) / 6)
+    assert np.all(np.abs(a.mean(0) - b.mean(0)) < 4 * sigma + 1e-12)
+
+
+def test_oracle_per_history_mode_independent_of_threads(orc):
+    prep = _prepared(workloads.single_layer_steel(n_photons_line=3000), hits=16)
+    a = orc.run(prep, seed=4, rng_mode=orc.RNG_PER_HISTORY, n_threads=1)
+    b = orc.run(prep, seed=4, rng_mode=orc.RNG_PER_HISTORY, n_threads=5)
+    assert np.allclose(a[1], b[1], rtol=1e-12, atol=0)
+    assert np.allclose(a[0], b[0], rtol=1e-12, atol=0)
+
+
+def test_oracle_channels_are_cumulative(orc):
+    prep = _prepared(workloads.single_layer_steel(n_photons_line=2000), hits=16)
+    ch, vr, _, _ = orc.run(prep, seed=1)
+    assert np.all(ch[0] == 0)
+    assert np.all(np.diff(ch, axis=0) >= -1e-9 * ch.max())
+    # every history row contributes to exactly one channel row per order
+    assert ch[-1].sum() <= vr.sum() * (1 + 1e-12)
+
+
+def test_oracle_grid_on_axis_closed_form(orc):
+    """Far from a detector without collimator, the cone estimator of a point on
+    the detector axis is the disk solid angle 2 pi (1 - cos atan(R/D))."""
+    prep = api.Prepared(workloads.single_layer_steel(n_photons_line=10))
+    lib = orc.load()
+    R = math.sqrt(0.3 / math.pi)
+    for D in (1.0, 3.0, 10.0):
+        om = lib.xmi_oracle_single_solid_angle(prep.inputF, D, math.pi / 2, 200000, 7, 1)
+        exact = 2 * math.pi * (1 - math.cos(math.atan(R / D)))
+        assert om == pytest.approx(exact, rel=0.01)

@@ -1,0 +1,161 @@
+"""CPU tests of the physics provider, the interaction tables and the C ABI
+library (loads and exports every declared symbol; no device calls)."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from xmimsim_amd import api, workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(lib):
+    header = open(os.path.join(ROOT, "include", "xmimsim_amd.h")).read()
+    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)                 # comments
+    header = re.sub(r"typedef\s+(struct|enum)[^{]*\{.*?\}[^;]*;", "", header, flags=re.S)  # types
+    declared = set(re.findall(r"\b(xmi_\w+)\s*\(", header))
+    declared |= set(re.findall(r"^\s*extern\s+[\w\s\*]+?\b(xmi_\w+)\s*;", header, re.M))
+    declared -= {"xmi_inputFPtr", "xmi_hdf5FPtr"}
+    assert declared, "no declarations parsed"
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(api.EXPORTED_SYMBOLS) <= declared
+
+
+def test_default_options(lib):
+    o = api.default_options()
+    assert (o.use_M_lines, o.use_cascade_auger, o.use_cascade_radiative, o.use_variance_reduction) == (1, 1, 1, 1)
+    assert (o.use_escape_peaks, o.use_advanced_compton, o.use_default_seeds) == (1, 0, 0)
+
+
+def test_struct_layouts_match_reference_c_types():
+    # sizes of the reference's public structs on x86-64 (same member order/types)
+    assert C.sizeof(api.xmi_layer) == 40
+    assert C.sizeof(api.xmi_geometry) == 16 * 8
+    assert C.sizeof(api.xmi_energy_discrete) == 9 * 8
+    assert C.sizeof(api.xmi_main_options) == 12 * 4 + 8 + 2 * 4
+    assert C.sizeof(api.xmi_solid_angle) == 6 * 8
+
+
+def test_prepare_and_geometry(lib):
+    prep = api.Prepared(workloads.single_layer_steel(n_photons_line=10))
+    # detector radius sqrt(A/pi), on-axis solid angle 2 pi (1 - cos atan(R/D))
+    assert prep.inputF
+    prep.close()
+
+
+def test_line_numbering(lib):
+    """xraylib line numbering used for the history rows (KL1=1, KL3=3,
+    KP5=29, L3M5=90, M5P5=219, P3P5=383, 384 Rayleigh, 385 Compton)."""
+    import xmimsim_amd.lines as L
+    names = L.line_names()
+    assert len(names) == 386
+    assert names[1] == "KL1" and names[3] == "KL3" and names[29] == "KP5"
+    assert names[90] == "L3M5" and names[219] == "M5P5" and names[383] == "P3P5"
+    assert names[384] == "Rayleigh" and names[385] == "Compton"
+    assert len(set(names[1:384])) == 383
+
+
+def test_provider_invariants():
+    import xmimsim_amd.lines as L
+    for Z in (6, 14, 26, 42, 56, 79, 92):
+        a = L.atom(Z)
+        edges = a["edge"][:9]
+        present = edges[edges > 0]
+        assert np.all(np.diff(present) < 0), (Z, edges)
+        fy = a["fluor_yield"]
+        assert np.all((fy >= 0) & (fy <= 1))
+        for shell, (first, last) in enumerate(L.SHELL_LINES):
+            s = a["rad_rate"][first:last + 1].sum()
+            assert s == pytest.approx(1.0, abs=1e-12) or s == 0.0, (Z, shell, s)
+        # photo cross section: K jump ratio is the model's 125/Z + 3.5
+        eK = edges[0]
+        if eK > 0.2:
+            hi = L.cs_photo_total(Z, eK * (1 + 1e-9))
+            lo = L.cs_photo_total(Z, eK * (1 - 1e-9))
+            assert hi / lo == pytest.approx(125.0 / Z + 3.5, rel=1e-6)
+
+
+def test_rayleigh_compton_tables_integrate_dcs():
+    """The tabulated integrated cross sections equal a fresh numerical
+    integration of the differential cross sections (independent quadrature)."""
+    import xmimsim_amd.lines as L
+    th = np.linspace(0, math.pi, 20001)
+    for Z in (8, 26, 82):
+        for E in (1.5, 10.0, 33.0, 120.0):
+            for kind in ("rayl", "compt"):
+                f = np.array([L.dcs(kind, Z, E, t) for t in th[::10]]) * np.sin(th[::10])
+                integ = 2 * math.pi * np.trapezoid(f, th[::10])
+                assert L.cs(kind, Z, E) == pytest.approx(integ, rel=2e-4), (Z, E, kind)
+
+
+def test_generated_tables_invert_their_cdfs(lib):
+    """Angle ICDFs (Rayleigh), azimuth ICDF and Compton-profile ICDF are the
+    inverses of independently evaluated CDFs."""
+    import xmimsim_amd.lines as L
+    t = L.tables_for([26])
+    E = t["Energies"]
+    rs = t["RandomNumbers"]
+    th = np.linspace(0, math.pi, 40001)
+    for ei in (20, 150, 390):
+        f = np.array([L.dcs("rayl", 26, E[ei], x) for x in th]) * np.sin(th)
+        cdf = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(th))])
+        cdf /= cdf[-1]
+        for ri in (100, 1000, 1900):
+            theta = t["RayleighTheta_ICDF"][ei, ri]
+            assert np.interp(theta, th, cdf) == pytest.approx(rs[ri], abs=2e-3)
+    # azimuth: CDF(phi) = (phi - t sin 2 phi) / (2 pi)
+    phi = t["Phi_ICDF"]
+    T = t["Thetas"]
+    for ti in (0, 77, 199):
+        for ri in (1, 500, 1999):
+            p = phi[ti, ri]
+            assert (p - T[ti] * math.sin(2 * p)) / (2 * math.pi) == pytest.approx(rs[ri], abs=1e-9)
+    # interaction probabilities
+    e = t["ip_energies"]
+    k = len(e) // 3
+    assert t["ip_rayl"][k] == pytest.approx(L.cs("rayl", 26, e[k]) / L.cs_total(26, e[k]), rel=1e-12)
+
+
+def test_hdf5_roundtrip(lib, tmp_path):
+    """xmi_db writes the xmimsimdata.h5 layout; xmi_init_from_hdf5 reads back
+    exactly the tables the in-memory provider path builds."""
+    inp = workloads.single_layer_steel(n_photons_line=10)
+    fn = str(tmp_path / "xmimsimdata-test.h5")
+    Zs = (C.c_int * 4)(*inp.unique_Z())
+    assert lib.xmi_db(fn.encode(), Zs, 4) == 1
+    a = api.Prepared(inp, hdf5_file=fn)
+    b = api.Prepared(inp)
+    import xmimsim_amd.lines as L
+    ta, tb = L.tables_of(a), L.tables_of(b)
+    for k in ta:
+        assert np.array_equal(ta[k], tb[k]), k
+    a.close()
+    b.close()
+
+
+def test_missing_element_is_an_error(lib, tmp_path):
+    fn = str(tmp_path / "db.h5")
+    Zs = (C.c_int * 1)(26)
+    assert lib.xmi_db(fn.encode(), Zs, 1) == 1
+    with pytest.raises(RuntimeError):
+        api.Prepared(workloads.single_layer_steel(n_photons_line=10), hdf5_file=fn)
+
+
+def test_no_gpu_is_a_loud_error(lib):
+    """Without a HIP device the engine refuses instead of falling back."""
+    import ctypes as C2
+    try:
+        n = C2.c_int()
+        hip = C2.CDLL("libamdhip64.so")
+        if hip.hipGetDeviceCount(C2.byref(n)) == 0 and n.value > 0:
+            pytest.skip("a device is present")
+    except OSError:
+        pass
+    prep = api.Prepared(workloads.single_layer_steel(n_photons_line=10))
+    with pytest.raises(RuntimeError):
+        prep.compute_solid_angle(hits_per_single=10)

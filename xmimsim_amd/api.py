@@ -108,7 +108,8 @@ EXPORTED_SYMBOLS = [
     "xmi_main_options_new", "xmi_input_C2F", "xmi_init_input", "xmi_free_input_F", "xmi_db",
     "xmi_init_from_hdf5", "xmi_init_from_provider", "xmi_update_input_from_hdf5", "xmi_free_hdf5_F",
     "xmi_solid_angle_calculation_cl", "xmi_free_solid_angle", "xmi_hits_per_single", "xmi_main_msim",
-    "xmi_free", "xmi_set_seed", "xmi_msim_get_stats", "xmi_msim_last_error",
+    "xmi_free", "xmi_set_seed", "xmi_msim_get_stats", "xmi_msim_last_error", "xmi_msim_set_profiling",
+    "xmi_msim_get_layer_events", "xmi_provider_active_lines",
 ]
 
 _lib = None
@@ -149,6 +150,11 @@ def load_library(path: str = LIB_PATH):
     lib.xmi_set_seed.argtypes = [C.c_uint64]
     lib.xmi_msim_get_stats.argtypes = [P(xmi_msim_stats)]
     lib.xmi_msim_last_error.restype = C.c_char_p
+    lib.xmi_msim_set_profiling.argtypes = [C.c_int]
+    lib.xmi_msim_get_layer_events.argtypes = [P(C.c_int64), C.c_int]
+    lib.xmi_msim_get_layer_events.restype = C.c_int
+    lib.xmi_provider_active_lines.argtypes = [C.c_int, C.c_int]
+    lib.xmi_provider_active_lines.restype = C.c_int
     _lib = lib
     return lib
 
@@ -366,6 +372,9 @@ class Prepared:
             raise RuntimeError("xmi_main_msim failed: " + last_error())
         stats = xmi_msim_stats()
         lib.xmi_msim_get_stats(C.byref(stats))
+        ev = (C.c_int64 * 16)()
+        lib.xmi_msim_get_layer_events(ev, 16)
+        self.layer_events = list(ev)[: len(self.inp.layers)]
         return _collect(lib, self.inp, ch, bh, vr) + (stats,)
 
     def close(self):

@@ -30,7 +30,17 @@ long xmi_hits_per_single = 5000;
 
 static uint64_t g_seed = 0;
 static int g_seed_set = 0;
+static int g_profile = 0;
+
+extern "C" void xmi_msim_set_profiling(int on) { g_profile = on; }
 static thread_local xmi_msim_stats g_stats;
+static thread_local int64_t g_layer_events[ENG_MAX_LAYERS];
+
+extern "C" int xmi_msim_get_layer_events(int64_t *out, int n) {
+  int m = n < ENG_MAX_LAYERS ? n : ENG_MAX_LAYERS;
+  for (int l = 0; l < m; l++) out[l] = g_layer_events[l];
+  return m;
+}
 
 extern "C" void xmi_set_seed(uint64_t seed) {
   g_seed = seed;
@@ -110,6 +120,9 @@ struct Engine {
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, tallies, counters, queue;
 };
+
+#define ENG_MAX_ORDERS_PROFILED 16
+static hipEvent_t prof_ev[4 * ENG_MAX_ORDERS_PROFILED];
 
 static Engine g_engine;
 
@@ -655,6 +668,9 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
 
     HIP_TRY(hipEventCreate(&evs[0]));
     HIP_TRY(hipEventCreate(&evs[1]));
+    if (g_profile)
+      for (int k = 0; k < 4 * ENG_MAX_ORDERS_PROFILED; k++)
+        if (!prof_ev[k]) HIP_TRY(hipEventCreate(&prof_ev[k]));
     HIP_TRY(hipEventRecord(evs[0], st));
     /* walk the sources, batching consecutive history ids of each source */
     for (size_t si = 0; si < src.size(); si++) {
@@ -663,7 +679,22 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
       for (long done = 0; done < d.count; done += cap) {
         int count = (int)((d.count - done) < cap ? (d.count - done) : cap);
         HIP_TRY(eng_launch_source(&P, &S, &ST, d.hid0 + (uint64_t)done, count, st));
-        for (int n = 1; n <= n_int; n++) HIP_TRY(eng_launch_order(&P, &S, n, count, vr_blocks, st, nullptr));
+        int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
+        for (int n = 1; n <= n_int; n++)
+          HIP_TRY(eng_launch_order(&P, &S, n, count, vr_blocks, st, prof ? &prof_ev[4 * (n - 1)] : nullptr));
+        if (prof) {
+          HIP_TRY(hipStreamSynchronize(st));
+          for (int n = 0; n < n_int; n++) {
+            float a = 0.f, b = 0.f, c = 0.f;
+            HIP_TRY(hipEventElapsedTime(&a, prof_ev[4 * n], prof_ev[4 * n + 1]));
+            HIP_TRY(hipEventElapsedTime(&b, prof_ev[4 * n + 1], prof_ev[4 * n + 2]));
+            HIP_TRY(hipEventElapsedTime(&c, prof_ev[4 * n + 2], prof_ev[4 * n + 3]));
+            g_stats.step_kernel_ms += a;
+            g_stats.vr_kernel_ms += b;
+            g_stats.interaction_kernel_ms += c;
+          }
+          g_stats.interactions += n_int; /* launches of each kernel class */
+        }
       }
     }
     HIP_TRY(hipEventRecord(evs[1], st));
@@ -678,8 +709,11 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     g_stats.histories = cnt[ENG_CNT_HISTORIES];
     g_stats.vr_launches = (double)cnt[ENG_CNT_VR];
     g_stats.solid_angle_fallbacks = cnt[ENG_CNT_FALLBACKS];
+    for (int l = 0; l < ENG_MAX_LAYERS; l++) g_layer_events[l] = cnt[ENG_CNT_LAYER0 + l];
     if (cnt[ENG_CNT_ERRORS]) {
-      xmi_set_error("xmi_main_msim: %u photons hit an invalid state on the device", cnt[ENG_CNT_ERRORS]);
+      xmi_set_error("xmi_main_msim: %u photons hit an invalid state on the device (source %u, estimator Doppler %u, "
+                    "transport Doppler %u)",
+                    cnt[ENG_CNT_ERRORS], cnt[ENG_CNT_ERR_SOURCE], cnt[ENG_CNT_ERR_VR_DOPPLER], cnt[ENG_CNT_ERR_DOPPLER]);
       goto fail;
     }
 

@@ -19,7 +19,11 @@
 #define ENG_CNT_FALLBACKS 4
 #define ENG_CNT_ERRORS 5
 #define ENG_CNT_HISTORIES 6
-#define ENG_NCOUNTERS 8
+#define ENG_CNT_LAYER0 8            /* detection-weight events per layer */
+#define ENG_CNT_ERR_SOURCE (ENG_CNT_LAYER0 + ENG_MAX_LAYERS)
+#define ENG_CNT_ERR_VR_DOPPLER (ENG_CNT_ERR_SOURCE + 1)
+#define ENG_CNT_ERR_DOPPLER (ENG_CNT_ERR_SOURCE + 2)
+#define ENG_NCOUNTERS (ENG_CNT_ERR_SOURCE + 4)
 
 /* m_e c^2 in keV as the reference computes it (src/xmi_main.F90:48-55) */
 #define ENG_MEC2 (9.10938188e-31 * 2.99792458e8 * 2.99792458e8 / 1.602176487e-19 / 1000.0)

@@ -245,6 +245,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
   if (d.kind == 0) {
     if (!d_ran_trap(d.x1, d.x2, d.y1, d.y2, phx_uniform(&s), &E)) {
       atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
       return;
     }
     hor = d_interp(d.x1, d.h1, d.x2, d.h2, E) / d_interp(d.x1, d.y1, d.x2, d.y2, E);
@@ -294,6 +295,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
   double cosalfa = d_dot(ev, dirv);
   if (fabs(cosalfa) > 1.0) {
     atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
     return;
   }
   double c_alfa = acos(cosalfa);
@@ -311,12 +313,14 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
       }
     if (layer == 0) {
       atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
       return;
     }
   } else {
     double pp[3] = {0.0, 0.0, P.layer_zb[0]}, ip[3];
     if (!d_intersect(pp, P.n_sample, coords, dirv, ip)) {
       atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
       return;
     }
     coords[0] = ip[0];
@@ -593,6 +597,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
         }
         Pdir_fluo = omega / 4.0 / ENG_PI;
         atomicAdd(&P.counters[ENG_CNT_VR], 1u);
+        atomicAdd(&P.counters[ENG_CNT_LAYER0 + cur - 1], 1u);
       }
     }
 
@@ -629,7 +634,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           const double *rn = P.cp_rn;
           const double *icdf = P.cp_icdf + (size_t)u * P.cp_len;
           const double drn = rn[1] - rn[0];
-          int it = 0;
+          int it = 0, failed = 0;
           for (;;) {
             double r = phx_uniform(&s);
             int pos = (int)(r / drn);
@@ -638,14 +643,16 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
             if (phx_uniform(&s) < 0.5) pz = -pz;
             double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
             energy = cc / (c_lamb0 + dlamb) / 1000.0;
+            /* a large positive pz can give a negative energy: accepted, as in
+             * the reference, where it then only reaches the history row */
             if (energy <= E) break;
             if (it == 100) {
-              energy = -1.0;
+              failed = 1; /* the reference stops with an error here */
               break;
             }
             it++;
           }
-          if (energy > 0.0) {
+          if (!failed) {
             double m2 = 0.0;
             for (int l = cur; up ? l <= last : l >= last; l += dir)
               m2 += d_layer_mu(P, l - 1, energy) * P.layer_density[l - 1] * dist[l - 1];
@@ -657,6 +664,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
             }
           } else {
             atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
+            atomicAdd(&P.counters[ENG_CNT_ERR_VR_DOPPLER], 1u);
           }
         }
         d_tally_wave(&trc[((size_t)u * 2 + 1) * 2], twc, shift);
@@ -793,6 +801,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_compton(EngineParams P, PhotonSoA
   }
   if (!accepted) {
     atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
+    atomicAdd(&P.counters[ENG_CNT_ERR_DOPPLER], 1u);
     S.energy[i] = 0.0;
     return;
   }

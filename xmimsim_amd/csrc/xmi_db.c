@@ -662,3 +662,54 @@ int xmi_init_from_hdf5(char *hdf5_file, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *h
   *hdf5FPtr = h;
   return 1;
 }
+
+int xmi_provider_active_lines(int Z, int line_last) {
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a) return 0;
+  int n = 0;
+  for (int line = 1; line <= line_last && line <= XMI_NLINES_VR; line++) {
+    if (a->line_energy[line] < XMI_ENERGY_THRESHOLD) continue;
+    int s = xrl_line_shell(line);
+    if (a->fluor_yield[s] * a->rad_rate[line] == 0.0) continue;
+    n++;
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* physics provider queries */
+
+const char *xmi_line_name(int line) { return xrl_line_name(line); }
+
+int xmi_provider_atom(int Z, void *out_) {
+  xrl_atom *out = (xrl_atom *)out_;
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a || !out) return 0;
+  *out = *a;
+  return 1;
+}
+
+double xmi_provider_cs(int kind, int Z, double E) {
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a) return -1.0;
+  switch (kind) {
+    case 0: return xrl_cs_total(a, E);
+    case 1: return xrl_cs_photo_total(a, E);
+    case 2: return xrl_cs_rayl(a, E);
+    case 3: return xrl_cs_compt(a, E);
+    default: return -1.0;
+  }
+}
+
+double xmi_provider_dcs(int kind, int Z, double E, double theta) {
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a) return -1.0;
+  return kind == 0 ? xrl_dcs_rayl(a, E, theta) : xrl_dcs_compt(a, E, theta);
+}
+
+int xmi_provider_shell_cross_sections(int Z, int cascade_type, double E, double *P9) {
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a) return 0;
+  xrl_shell_cross_sections(a, cascade_type, E, P9);
+  return 1;
+}

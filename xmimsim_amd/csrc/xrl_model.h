@@ -129,6 +129,7 @@ typedef struct xrl_atom {
  * ------------------------------------------------------------------------*/
 
 XRL_FN double xrl_cs_table(const double *tab, double E) {
+  if (!(E > 0.0)) return 0.0; /* xraylib returns 0 for non-positive energies */
   const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
   double x = (log(E) - XRL_CS_LNE0) / h;
   int i = (int)floor(x);
@@ -151,6 +152,7 @@ XRL_FN double xrl_cs_photo_partial(const xrl_atom *a, int s, double E) {
 
 /* total photo-ionisation cross section, cm2/g */
 XRL_FN double xrl_cs_photo_total(const xrl_atom *a, double E) {
+  if (!(E > 0.0)) return 0.0;
   double B = a->photo_B_outer;
   for (int s = 0; s < 9; s++)
     if (a->edge[s] > 0.0 && E >= a->edge[s]) B += a->photo_B[s];
