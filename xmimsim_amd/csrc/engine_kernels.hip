@@ -29,15 +29,22 @@
 /* ------------------------------------------------------------------------ */
 /* physics helpers on the device */
 
-__device__ __forceinline__ double d_layer_mu(const EngineParams &P, int layer0, double E) {
+__device__ __forceinline__ double d_layer_mu_c(const EngineParams &P, int layer0, const xrl_ectx *c) {
   double rv = 0.0;
   int off = P.layer_off[layer0], n = P.layer_nel[layer0];
-  for (int i = 0; i < n; i++) rv += xrl_cs_total(&P.atoms[P.el_u[off + i]], E) * P.el_w[off + i];
+  for (int i = 0; i < n; i++) rv += xrl_cs_total_c(&P.atoms[P.el_u[off + i]], c) * P.el_w[off + i];
   return rv;
 }
 
 __device__ __forceinline__ void d_mu_all(const EngineParams &P, double E, double *mus, size_t stride, size_t idx) {
-  for (int l = 0; l < P.n_layers; l++) mus[(size_t)l * stride + idx] = d_layer_mu(P, l, E);
+  xrl_ectx c = xrl_energy(E);
+  for (int l = 0; l < P.n_layers; l++) mus[(size_t)l * stride + idx] = d_layer_mu_c(P, l, &c);
+}
+
+/* wavefront-aggregated counter: call from converged code only */
+__device__ __forceinline__ void d_wave_count(unsigned int *ctr, int pred) {
+  unsigned long long m = __ballot(pred);
+  if (m != 0ull && __lane_id() == 0) atomicAdd(ctr, (unsigned int)__popcll(m));
 }
 
 __device__ __forceinline__ void d_update_dirv(double *dirv, double *theta, double *phi, double theta_i, double phi_i) {
@@ -83,7 +90,8 @@ __device__ __forceinline__ double d_phi0(double theta, double phi, const double 
 
 /* cone-sampling estimate of the detector solid angle seen from (r1, theta1),
  * src/xmi_solid_angle_f.F90:432-710 (same estimator as src/xmi_kernels.cl:219-452) */
-__device__ double d_single_solid_angle(const EngineParams &P, double r1, double theta1, long hits, phx_stream *rng) {
+__device__ __forceinline__ double d_single_solid_angle_impl(const EngineParams &P, double r1, double theta1, long hits,
+                                                           phx_stream *rng) {
   double r, theta, fcbr;
   int outside = 0;
   const double R = P.det_radius, Rc = P.coll_radius, H = P.coll_height;
@@ -147,6 +155,12 @@ __device__ double d_single_solid_angle(const EngineParams &P, double r1, double 
   return cone_sa * (double)nhit / (double)hits;
 }
 
+/* out-of-line copy for the rare off-grid evaluations inside the estimator */
+__device__ __noinline__ double d_single_solid_angle(const EngineParams &P, double r1, double theta1, long hits,
+                                                    phx_stream *rng) {
+  return d_single_solid_angle_impl(P, r1, theta1, hits, rng);
+}
+
 /* ------------------------------------------------------------------------ */
 /* solid-angle grid: one thread per (r, theta) cell, Philox stream per cell */
 
@@ -158,7 +172,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_solid_angle_grid(EngineParams P, 
     int jt = (int)(cell / n_r), ir = (int)(cell % n_r);
     phx_stream s;
     phx_stream_init(&s, seed, 0u, 0u, PHX_SUB_GRID, (uint64_t)cell);
-    grid[cell] = d_single_solid_angle(P, rv[ir], tv[jt], hits, &s);
+    grid[cell] = d_single_solid_angle_impl(P, rv[ir], tv[jt], hits, &s);
   }
 }
 
@@ -219,10 +233,9 @@ __device__ int d_ran_trap(double x1, double x2, double y1, double y2, double u, 
   return 0;
 }
 
-__global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA S, SourceTable T,
-                                                       uint64_t batch_start, int count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+/* one primary photon; returns 1 when it enters the sample */
+__device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const SourceTable &T, uint64_t batch_start,
+                            int i) {
   uint64_t hid = batch_start + (uint64_t)i;
   int lo = 0, hi = T.n - 1;
   while (lo < hi) { /* last descriptor with hid0 <= hid */
@@ -236,7 +249,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
   S.hid[i] = hid;
   S.prev_line[i] = 0;
   S.prev_u[i] = -1;
-  if (!d.valid) return;
+  if (!d.valid) return 0;
   const long j = (long)(hid - d.hid0);
   phx_stream s;
   phx_stream_init(&s, P.seed, P.sample, 0u, PHX_SUB_SOURCE, hid);
@@ -246,7 +259,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
     if (!d_ran_trap(d.x1, d.x2, d.y1, d.y2, phx_uniform(&s), &E)) {
       atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
       atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
-      return;
+      return 0;
     }
     hor = d_interp(d.x1, d.h1, d.x2, d.h2, E) / d_interp(d.x1, d.y1, d.x2, d.y2, E);
     w = d.weight * d_exc_corr(T, E);
@@ -254,11 +267,11 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
   } else {
     if (d.dist == 1) {
       E = d_gauss(&s, d.scale) + d.energy;
-      if (E <= 1.0 || E > 200.0) return;
+      if (E <= 1.0 || E > 200.0) return 0;
       d_mu_all(P, E, S.mus, cap, i);
     } else if (d.dist == 2) {
       E = d.scale * tan(ENG_PI * (phx_uniform(&s) - 0.5)) + d.energy;
-      if (E <= 1.0 || E > 200.0) return;
+      if (E <= 1.0 || E > 200.0) return 0;
       d_mu_all(P, E, S.mus, cap, i);
     } else {
       E = d.energy;
@@ -296,7 +309,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
   if (fabs(cosalfa) > 1.0) {
     atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
       atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
-    return;
+    return 0;
   }
   double c_alfa = acos(cosalfa);
   double c_ae = 1.0 / sin(c_alfa);
@@ -314,14 +327,14 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
     if (layer == 0) {
       atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
       atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
-      return;
+      return 0;
     }
   } else {
     double pp[3] = {0.0, 0.0, P.layer_zb[0]}, ip[3];
     if (!d_intersect(pp, P.n_sample, coords, dirv, ip)) {
       atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
       atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
-      return;
+      return 0;
     }
     coords[0] = ip[0];
     coords[1] = ip[1];
@@ -342,7 +355,14 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
   S.energy[i] = E;
   S.weight[i] = w;
   S.layer[i] = layer;
-  atomicAdd(&P.counters[ENG_CNT_HISTORIES], 1u);
+  return 1;
+}
+
+__global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA S, SourceTable T,
+                                                       uint64_t batch_start, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int alive = (i < count) ? d_source_one(P, S, T, batch_start, i) : 0;
+  d_wave_count(&P.counters[ENG_CNT_HISTORIES], alive);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -432,8 +452,9 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
       double mu_cur = S.mus[(size_t)(my - 1) * cap + i];
       double thr = 0.0;
       int sel = nel - 1;
+      xrl_ectx ec = xrl_energy(E);
       for (int k = 0; k < nel; k++) {
-        thr += P.el_w[off + k] * xrl_cs_total(&P.atoms[P.el_u[off + k]], E) / mu_cur;
+        thr += P.el_w[off + k] * xrl_cs_total_c(&P.atoms[P.el_u[off + k]], &ec) / mu_cur;
         if (R2 < thr) {
           sel = k;
           break;
@@ -596,10 +617,10 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           }
         }
         Pdir_fluo = omega / 4.0 / ENG_PI;
-        atomicAdd(&P.counters[ENG_CNT_VR], 1u);
-        atomicAdd(&P.counters[ENG_CNT_LAYER0 + cur - 1], 1u);
       }
     }
+    d_wave_count(&P.counters[ENG_CNT_VR], active);
+    for (int L = 1; L <= P.n_layers; L++) d_wave_count(&P.counters[ENG_CNT_LAYER0 + L - 1], active && cur == L);
 
     /* element loop: uniform over the wave per layer */
     for (int L = 1; L <= P.n_layers; L++) {
@@ -654,8 +675,9 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           }
           if (!failed) {
             double m2 = 0.0;
+            xrl_ectx ce = xrl_energy(energy);
             for (int l = cur; up ? l <= last : l >= last; l += dir)
-              m2 += d_layer_mu(P, l - 1, energy) * P.layer_density[l - 1] * dist[l - 1];
+              m2 += d_layer_mu_c(P, l - 1, &ce) * P.layer_density[l - 1] * dist[l - 1];
             double Pesc_comp = exp(-m2);
             twc = Pconv * (omega * xrl_dcsp_compt(a, E, theta, phi)) * Pesc_comp * w / 1.0;
             int ch = (energy >= 1.0) ? (int)((energy - P.zero) / P.gain) : -1;
@@ -676,7 +698,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
             for (int sh = 0; sh <= smax; sh++)
               Psh[sh] = P.precalc_xrf[(((size_t)u * 9 + sh) * P.nZ + prev_u) * XMI_NLINES_VR_DEV + (prev_line - 1)];
           } else {
-            xrl_shell_cross_sections(a, P.cascade, E, Psh);
+            xrl_ectx ce = xrl_energy(E);
+            xrl_shell_cross_sections_c(a, P.cascade, &ce, Psh);
             if (!P.use_M_lines)
               for (int sh = 4; sh < 9; sh++) Psh[sh] = 0.0;
           }
@@ -919,13 +942,14 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S
   phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_TRANSPORT, S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
-  double photo_total = xrl_cs_photo_total(a, E);
+  xrl_ectx ec = xrl_energy(E);
+  double photo_total = xrl_cs_photo_total_c(a, &ec);
   double sumz = 0.0;
   double r = phx_uniform(&s);
   int max_shell = P.use_M_lines ? 8 : 3;
   int shell = -1;
   for (int sh = 0; sh <= max_shell; sh++) {
-    sumz += xrl_cs_photo_partial(a, sh, E) / photo_total;
+    sumz += xrl_cs_photo_partial_c(a, sh, &ec) / photo_total;
     if (r < sumz) {
       shell = sh;
       break;

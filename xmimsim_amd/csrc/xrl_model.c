@@ -351,8 +351,21 @@ int xrl_atom_init(xrl_atom *a, int Z) {
   const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
   for (int i = 0; i < XRL_NCS; i++) {
     double E = exp(XRL_CS_LNE0 + i * h);
-    a->ln_cs_rayl[i] = log(integrate_dcs(a, E, 0));
-    a->ln_cs_compt[i] = log(integrate_dcs(a, E, 1));
+    a->cs_rayl[i] = integrate_dcs(a, E, 0);
+    a->cs_compt[i] = integrate_dcs(a, E, 1);
+    a->cs_rc[i] = a->cs_rayl[i] + a->cs_compt[i];
+  }
+
+  /* photo cross section and cascade-corrected shell cross sections as
+   * functions of the number k of K..M5 edges above the energy */
+  for (int k = 0; k <= 9; k++) {
+    double tau[9], sum = a->photo_B_outer;
+    for (int s2 = 0; s2 < 9; s2++) {
+      tau[s2] = (s2 >= k && a->edge[s2] > 0.0) ? a->photo_B[s2] : 0.0;
+      sum += tau[s2];
+    }
+    a->photo_Bsum[k] = sum;
+    for (int c = 0; c < 4; c++) xrl_shell_cross_sections_build(a, c + 1, tau, a->shell_C[c][k]);
   }
   return 1;
 }

@@ -120,48 +120,102 @@ typedef struct xrl_atom {
   double ff_x0;                          /* Rayleigh form factor scale, 1/Angstrom */
   double sf_x0;                          /* incoherent scattering function scale */
   double profile_p[XRL_NSHELLS];         /* Compton profile momentum, atomic units */
-  double ln_cs_rayl[XRL_NCS];            /* ln(cm2/g) on the ln(E) grid */
-  double ln_cs_compt[XRL_NCS];
+  /* derived, for evaluation: with k = number of K..M5 edges above E,
+   *   photo total  = E^-p * photo_Bsum[k]
+   *   P_shell[s]   = E^-p * shell_C[cascade-1][k][s]   (P_K .. P_M5) */
+  double photo_Bsum[10];
+  double shell_C[4][10][9];
+  double cs_rayl[XRL_NCS];               /* cm2/g on the ln(E) grid, linear interpolation */
+  double cs_compt[XRL_NCS];
+  double cs_rc[XRL_NCS];                 /* cs_rayl + cs_compt */
 } xrl_atom;
+
+/* per-energy quantities shared by every element evaluated at that energy */
+typedef struct xrl_ectx {
+  double E;
+  double pe;   /* E^-p */
+  double cf;   /* fraction within the ln(E) cell */
+  int ci;      /* ln(E) cell */
+} xrl_ectx;
 
 /* --------------------------------------------------------------------------
  * Evaluation (host and device)
  * ------------------------------------------------------------------------*/
 
-XRL_FN double xrl_cs_table(const double *tab, double E) {
-  if (!(E > 0.0)) return 0.0; /* xraylib returns 0 for non-positive energies */
+XRL_FN xrl_ectx xrl_energy(double E) {
+  xrl_ectx c;
+  c.E = E;
+  if (!(E > 0.0)) { /* xraylib returns 0 for non-positive energies */
+    c.pe = 0.0;
+    c.cf = 0.0;
+    c.ci = 0;
+    return c;
+  }
   const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
   double x = (log(E) - XRL_CS_LNE0) / h;
   int i = (int)floor(x);
   if (i < 0) i = 0;
   if (i > XRL_NCS - 2) i = XRL_NCS - 2;
-  double f = x - (double)i;
-  return exp(tab[i] + f * (tab[i + 1] - tab[i]));
+  c.ci = i;
+  c.cf = x - (double)i;
+  c.pe = pow(E, -XRL_PHOTO_EXPONENT);
+  return c;
 }
 
-XRL_FN double xrl_cs_rayl(const xrl_atom *a, double E) { return xrl_cs_table(a->ln_cs_rayl, E); }
-XRL_FN double xrl_cs_compt(const xrl_atom *a, double E) { return xrl_cs_table(a->ln_cs_compt, E); }
-
-/* photo-ionisation partial cross section of shell s (0..8), cm2/g */
-XRL_FN double xrl_cs_photo_partial(const xrl_atom *a, int s, double E) {
-  if (s < 0 || s > 8) return 0.0;
-  double e = a->edge[s];
-  if (e <= 0.0 || E < e) return 0.0;
-  return a->photo_B[s] * pow(E, -XRL_PHOTO_EXPONENT);
+/* number of K..M5 edges above E (edges decrease along K..M5; absent shells are 0) */
+XRL_FN int xrl_edges_above(const xrl_atom *a, double E) {
+  int k = 0;
+  for (int s = 0; s < 9; s++) k += (a->edge[s] > E) ? 1 : 0;
+  return k;
 }
+
+XRL_FN double xrl_table(const double *tab, const xrl_ectx *c) {
+  if (!(c->E > 0.0)) return 0.0;
+  return tab[c->ci] + c->cf * (tab[c->ci + 1] - tab[c->ci]);
+}
+
+XRL_FN double xrl_cs_rayl_c(const xrl_atom *a, const xrl_ectx *c) { return xrl_table(a->cs_rayl, c); }
+XRL_FN double xrl_cs_compt_c(const xrl_atom *a, const xrl_ectx *c) { return xrl_table(a->cs_compt, c); }
 
 /* total photo-ionisation cross section, cm2/g */
-XRL_FN double xrl_cs_photo_total(const xrl_atom *a, double E) {
-  if (!(E > 0.0)) return 0.0;
-  double B = a->photo_B_outer;
-  for (int s = 0; s < 9; s++)
-    if (a->edge[s] > 0.0 && E >= a->edge[s]) B += a->photo_B[s];
-  return B * pow(E, -XRL_PHOTO_EXPONENT);
+XRL_FN double xrl_cs_photo_total_c(const xrl_atom *a, const xrl_ectx *c) {
+  if (!(c->E > 0.0)) return 0.0;
+  return c->pe * a->photo_Bsum[xrl_edges_above(a, c->E)];
 }
 
 /* CS_Total_Kissel equivalent: photo + Rayleigh + Compton, cm2/g */
+XRL_FN double xrl_cs_total_c(const xrl_atom *a, const xrl_ectx *c) {
+  if (!(c->E > 0.0)) return 0.0;
+  return c->pe * a->photo_Bsum[xrl_edges_above(a, c->E)] + xrl_table(a->cs_rc, c);
+}
+
+/* photo-ionisation partial cross section of shell s (0..8), cm2/g */
+XRL_FN double xrl_cs_photo_partial_c(const xrl_atom *a, int s, const xrl_ectx *c) {
+  if (s < 0 || s > 8) return 0.0;
+  double e = a->edge[s];
+  if (e <= 0.0 || !(c->E >= e)) return 0.0;
+  return a->photo_B[s] * c->pe;
+}
+
+XRL_FN double xrl_cs_rayl(const xrl_atom *a, double E) {
+  xrl_ectx c = xrl_energy(E);
+  return xrl_cs_rayl_c(a, &c);
+}
+XRL_FN double xrl_cs_compt(const xrl_atom *a, double E) {
+  xrl_ectx c = xrl_energy(E);
+  return xrl_cs_compt_c(a, &c);
+}
+XRL_FN double xrl_cs_photo_total(const xrl_atom *a, double E) {
+  xrl_ectx c = xrl_energy(E);
+  return xrl_cs_photo_total_c(a, &c);
+}
 XRL_FN double xrl_cs_total(const xrl_atom *a, double E) {
-  return xrl_cs_photo_total(a, E) + xrl_cs_rayl(a, E) + xrl_cs_compt(a, E);
+  xrl_ectx c = xrl_energy(E);
+  return xrl_cs_total_c(a, &c);
+}
+XRL_FN double xrl_cs_photo_partial(const xrl_atom *a, int s, double E) {
+  xrl_ectx c = xrl_energy(E);
+  return xrl_cs_photo_partial_c(a, s, &c);
 }
 
 /* momentum transfer x = sin(theta/2)/lambda, 1/Angstrom */
@@ -261,15 +315,12 @@ XRL_FN int xrl_line_LM(int i, int j) {                   /* L1M1.., L2M1.., L3M1
  * Cascade-corrected shell ionisation cross sections (P_K .. P_M5) at energy E,
  * the equivalent of the xraylib family P{L1..M5}_{pure,auger_cascade,
  * rad_cascade,full_cascade}_kissel used in src/xmi_variance_reduction.F90:443-572
- * and src/xmi_data_f.F90:1311-1474.  P[0] is only filled when E >= K edge.
+ * and src/xmi_data_f.F90:1311-1474, as a linear map of the photo-ionisation
+ * cross sections tau[K..M5].  Used once per element to fill shell_C.
  */
-XRL_FN void xrl_shell_cross_sections(const xrl_atom *a, int cascade_type, double E, double P[9]) {
+XRL_FN void xrl_shell_cross_sections_build(const xrl_atom *a, int cascade_type, const double tau[9], double P[9]) {
   const int rad = (cascade_type == XRL_CASCADE_RADIATIVE || cascade_type == XRL_CASCADE_FULL);
   const int aug = (cascade_type == XRL_CASCADE_NONRADIATIVE || cascade_type == XRL_CASCADE_FULL);
-  double pe = pow(E, -XRL_PHOTO_EXPONENT);
-  double tau[9];
-  for (int s = 0; s < 9; s++)
-    tau[s] = (a->edge[s] > 0.0 && E >= a->edge[s]) ? a->photo_B[s] * pe : 0.0;
   double PK = tau[0];
   P[0] = PK;
   /* L shells: feeding from K vacancies, then L Coster-Kronig */
@@ -311,6 +362,21 @@ XRL_FN void xrl_shell_cross_sections(const xrl_atom *a, int cascade_type, double
   P[6] = PM3;
   P[7] = PM4;
   P[8] = PM5;
+}
+
+/* P_K .. P_M5 at energy E for a cascade type (1..4): E^-p * shell_C[k] */
+XRL_FN void xrl_shell_cross_sections_c(const xrl_atom *a, int cascade_type, const xrl_ectx *c, double P[9]) {
+  if (!(c->E > 0.0)) {
+    for (int s = 0; s < 9; s++) P[s] = 0.0;
+    return;
+  }
+  const double *C = a->shell_C[cascade_type - 1][xrl_edges_above(a, c->E)];
+  for (int s = 0; s < 9; s++) P[s] = c->pe * C[s];
+}
+
+XRL_FN void xrl_shell_cross_sections(const xrl_atom *a, int cascade_type, double E, double P[9]) {
+  xrl_ectx c = xrl_energy(E);
+  xrl_shell_cross_sections_c(a, cascade_type, &c, P);
 }
 
 /* --------------------------------------------------------------------------

@@ -28,8 +28,9 @@ class XrlAtom(C.Structure):
                 ("rad_rate", C.c_double * (NLINES + 1)), ("line_energy", C.c_double * (NLINES + 1)),
                 ("photo_B", C.c_double * 9), ("photo_B_outer", C.c_double), ("auger_KL", C.c_double * 3),
                 ("auger_KM", C.c_double * 5), ("auger_LM", C.c_double * 15), ("ff_x0", C.c_double),
-                ("sf_x0", C.c_double), ("profile_p", C.c_double * NSHELLS), ("ln_cs_rayl", C.c_double * NCS),
-                ("ln_cs_compt", C.c_double * NCS)]
+                ("sf_x0", C.c_double), ("profile_p", C.c_double * NSHELLS), ("photo_Bsum", C.c_double * 10),
+                ("shell_C", C.c_double * 360), ("cs_rayl", C.c_double * NCS), ("cs_compt", C.c_double * NCS),
+                ("cs_rc", C.c_double * NCS)]
 
 
 class HDF5Z(C.Structure):
