@@ -134,8 +134,8 @@ def main():
         lib.xmi_msim_set_profiling(1 if w == 0 else 0)
         _, _, _, st = prep.run(seed=args.seed + 1000 + w)
         if w == 0:
-            prof = (st.vr_kernel_ms, st.step_kernel_ms, st.interaction_kernel_ms, st.interactions,
-                    list(prep.layer_events), st.histories)
+            prof = (st.vr_kernel_ms, st.step_kernel_ms, st.interaction_kernel_ms, st.launches,
+                    list(prep.layer_events), st.histories, st.solid_angle_fallbacks)
         print(f"[bench] warmup {w}: {st.histories} histories, {st.transport_ms:.1f} ms", file=sys.stderr, flush=True)
     lib.xmi_msim_set_profiling(0)
 
@@ -153,14 +153,15 @@ def main():
 
     roofline = None
     if prof is not None and prof[0] > 0:
-        vr_ms, step_ms, int_ms, launches, layer_events, _ = prof
+        vr_ms, step_ms, int_ms, launches, layer_events, _, fallbacks = prof
         tot_bytes, per_event = vr_bytes_per_event(inp, layer_events)
         achieved = tot_bytes / (vr_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "kernel": "k_varred (detection-weight estimator)",
                     "bytes_per_event": round(per_event, 1), "events": int(sum(layer_events)),
-                    "avg_launch_ms": round(vr_ms / max(launches, 1), 4),
+                    "launches": int(launches), "avg_launch_ms": round(vr_ms / max(launches, 1), 4),
+                    "solid_angle_fallbacks": int(fallbacks),
                     "kernel_ms": {"k_varred": round(vr_ms, 2), "k_step": round(step_ms, 2),
                                   "interactions": round(int_ms, 2)}}
 

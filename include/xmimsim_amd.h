@@ -239,9 +239,9 @@ typedef struct _xmi_msim_stats {
   double transport_ms;        /* device time of all transport kernels (events) */
   double total_ms;            /* wall time of xmi_main_msim */
   int64_t histories;          /* primary photons simulated */
-  int64_t interactions;       /* with profiling on: launches of each kernel class */
+  int64_t launches;           /* with profiling on: launches of each kernel class */
   double vr_kernel_ms;        /* with profiling on: device time of the detection-weight kernel */
-  double vr_launches;         /* detection-weight evaluations (photon-interactions) */
+  int64_t vr_events;          /* detection-weight evaluations (photon-interactions) */
   double step_kernel_ms;      /* with profiling on: forced-step kernel (incl. queue reset) */
   double interaction_kernel_ms; /* with profiling on: Rayleigh + Compton + photo kernels */
   int64_t solid_angle_fallbacks;

@@ -57,7 +57,7 @@ def test_same_streams_single_layer(orc):
     gpu = prep.run(seed=42)
     ref = orc.run(prep, seed=42, rng_mode=orc.RNG_PER_HISTORY)
     assert gpu[3].histories == ref[3].histories == 20000
-    assert gpu[3].vr_launches == ref[3].vr_calls
+    assert gpu[3].vr_events == ref[3].vr_calls
     _assert_same(gpu, ref, "steel")
 
 

@@ -98,7 +98,7 @@ class xmi_solid_angle(C.Structure):
 
 class xmi_msim_stats(C.Structure):
     _fields_ = [("transport_ms", C.c_double), ("total_ms", C.c_double), ("histories", C.c_int64),
-                ("interactions", C.c_int64), ("vr_kernel_ms", C.c_double), ("vr_launches", C.c_double),
+                ("launches", C.c_int64), ("vr_kernel_ms", C.c_double), ("vr_events", C.c_int64),
                 ("step_kernel_ms", C.c_double), ("interaction_kernel_ms", C.c_double),
                 ("solid_angle_fallbacks", C.c_int64)]
 

@@ -102,6 +102,14 @@ struct EngineParams {
   const double *act_E;
   const int *act_ch;
   const double *act_murho;      /* [n_layers][nZ][ENG_MAX_ACT] */
+  /* the same lines flattened per layer: entry t of layer l (fl_off[l] <= t <
+   * fl_off[l+1]) is active line q of element u of that layer */
+  int fl_off[ENG_MAX_LAYERS + 1];
+  const int4 *fl_info;          /* {u, q, shell, compact bin} */
+  const double2 *fl_wf;         /* {weight fraction of u in the layer, yield*rate} */
+  const double *fl_mr;          /* [entry][n_layers] mu*rho of the line in each layer */
+  const int *lbin_g;            /* compact bin -> u * ENG_MAX_ACT + q */
+  int n_lbins;
   /* azimuth and solid angle */
   const double *phi_icdf;       /* [n_t][n_pr] */
   const double *phi_T;
@@ -118,7 +126,7 @@ struct EngineParams {
   unsigned long long *t_line;   /* [n_int][nZ][ENG_MAX_ACT][2] */
   unsigned long long *t_rc;     /* [n_int][nZ][2][2] */
   int tally_shift;              /* fixed point: value * 2^tally_shift */
-  unsigned int *counters;       /* [0..2] queue sizes, [3] vr calls, [4] fallbacks, [5] errors */
+  unsigned long long *counters; /* slots ENG_CNT_*: queue sizes, events, fallbacks, errors */
   int *queue;                   /* [3][cap] */
   int cap;
 };

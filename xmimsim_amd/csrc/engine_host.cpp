@@ -117,6 +117,7 @@ struct Engine {
   int sa_nr = 0, sa_nt = 0;
   /* per-call buffers (grow only) */
   DevBuf el_Z, el_w, el_u, precalc_mu, act_n, act_line, act_shell, act_factor, act_E, act_ch, act_murho;
+  DevBuf fl_info, fl_wf, fl_mr, lbin_g;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, tallies, counters, queue;
 };
@@ -450,6 +451,29 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     }
     actn[u] = k;
   }
+  /* compact line bins (estimator LDS) and the per-layer flat (element, line)
+   * entries of the lane-per-line phase */
+  std::vector<int> act_base(nZ + 1, 0), lbin_g;
+  for (int u = 0; u < nZ; u++) act_base[u + 1] = act_base[u] + actn[u];
+  for (int u = 0; u < nZ; u++)
+    for (int q = 0; q < actn[u]; q++) lbin_g.push_back(u * ENG_MAX_ACT + q);
+  P.n_lbins = act_base[nZ];
+  std::vector<int4> fl_info;
+  std::vector<double2> fl_wf;
+  std::vector<double> fl_mr;
+  for (int l = 0; l < NL; l++) {
+    P.fl_off[l] = (int)fl_info.size();
+    for (int k = 0; k < P.layer_nel[l]; k++) {
+      int u = elU[(size_t)P.layer_off[l] + k];
+      for (int q = 0; q < actn[u]; q++) {
+        size_t o = (size_t)u * ENG_MAX_ACT + q;
+        fl_info.push_back(make_int4(u, q, acts[o], act_base[u] + q));
+        fl_wf.push_back(make_double2(elW[(size_t)P.layer_off[l] + k], actf[o]));
+        for (int l2 = 0; l2 < NL; l2++) fl_mr.push_back(actm[((size_t)l2 * nZ + u) * ENG_MAX_ACT + q]);
+      }
+    }
+  }
+  P.fl_off[NL] = (int)fl_info.size();
 
   /* sources: continuous intervals, then discrete lines */
   std::vector<SourceDesc> src;
@@ -541,9 +565,13 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
 
   {
     Engine &G = g_engine;
-    /* histories: run each source's first `count` ids as contiguous ranges */
+    /* histories: each source's first `count` ids, all sources back to back in
+     * one run order, so that every batch is full whatever the source sizes */
     int64_t total_hist = 0;
-    for (auto &d : src) total_hist += d.valid ? d.count : 0;
+    for (auto &d : src) {
+      d.run0 = (uint64_t)total_hist;
+      total_hist += (d.valid && d.count > 0) ? d.count : 0;
+    }
     const int cap_max = 1 << 21;
     int cap = (int)(total_hist < cap_max ? (total_hist > 0 ? total_hist : 1) : cap_max);
     P.cap = cap;
@@ -553,7 +581,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     size_t tally_words = n_ch2 + n_line2 + n_rc2;
     HIP_TRY(G.state.ensure(state_bytes));
     HIP_TRY(G.tallies.ensure(tally_words * sizeof(unsigned long long)));
-    HIP_TRY(G.counters.ensure(ENG_NCOUNTERS * sizeof(unsigned int)));
+    HIP_TRY(G.counters.ensure(ENG_NCOUNTERS * sizeof(unsigned long long)));
     HIP_TRY(G.queue.ensure(3 * per * sizeof(int)));
     HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size()));
     HIP_TRY(upload(G.el_w, elW.data(), elW.size()));
@@ -566,12 +594,16 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     HIP_TRY(upload(G.act_E, actE.data(), actE.size()));
     HIP_TRY(upload(G.act_ch, actc.data(), actc.size()));
     HIP_TRY(upload(G.act_murho, actm.data(), actm.size()));
+    HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size()));
+    HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size()));
+    HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size()));
+    HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size()));
     HIP_TRY(upload(G.src_desc, src.data(), src.size()));
     HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size()));
     HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size()));
     HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size()));
     HIP_TRY(hipMemsetAsync(G.tallies.p, 0, tally_words * sizeof(unsigned long long), st));
-    HIP_TRY(hipMemsetAsync(G.counters.p, 0, ENG_NCOUNTERS * sizeof(unsigned int), st));
+    HIP_TRY(hipMemsetAsync(G.counters.p, 0, ENG_NCOUNTERS * sizeof(unsigned long long), st));
 
     P.el_Z = G.el_Z.as<int>();
     P.el_w = G.el_w.as<double>();
@@ -608,6 +640,10 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     P.act_E = G.act_E.as<double>();
     P.act_ch = G.act_ch.as<int>();
     P.act_murho = G.act_murho.as<double>();
+    P.fl_info = G.fl_info.as<int4>();
+    P.fl_wf = G.fl_wf.as<double2>();
+    P.fl_mr = G.fl_mr.as<double>();
+    P.lbin_g = G.lbin_g.as<int>();
     P.phi_icdf = T->phi.as<double>();
     P.phi_T = T->phiT.as<double>();
     P.phi_R = T->phiR.as<double>();
@@ -630,7 +666,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     P.t_ch = tw;
     P.t_line = tw + n_ch2;
     P.t_rc = tw + n_ch2 + n_line2;
-    P.counters = G.counters.as<unsigned int>();
+    P.counters = G.counters.as<unsigned long long>();
     P.queue = G.queue.as<int>();
 
     PhotonSoA S;
@@ -672,13 +708,11 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
       for (int k = 0; k < 4 * ENG_MAX_ORDERS_PROFILED; k++)
         if (!prof_ev[k]) HIP_TRY(hipEventCreate(&prof_ev[k]));
     HIP_TRY(hipEventRecord(evs[0], st));
-    /* walk the sources, batching consecutive history ids of each source */
-    for (size_t si = 0; si < src.size(); si++) {
-      const SourceDesc &d = src[si];
-      if (!d.valid || d.count <= 0) continue;
-      for (long done = 0; done < d.count; done += cap) {
-        int count = (int)((d.count - done) < cap ? (d.count - done) : cap);
-        HIP_TRY(eng_launch_source(&P, &S, &ST, d.hid0 + (uint64_t)done, count, st));
+    /* batches of `cap` consecutive positions of the run order */
+    {
+      for (int64_t done = 0; done < total_hist; done += cap) {
+        int count = (int)((total_hist - done) < cap ? (total_hist - done) : cap);
+        HIP_TRY(eng_launch_source(&P, &S, &ST, (uint64_t)done, count, st));
         int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
         for (int n = 1; n <= n_int; n++)
           HIP_TRY(eng_launch_order(&P, &S, n, count, vr_blocks, st, prof ? &prof_ev[4 * (n - 1)] : nullptr));
@@ -693,13 +727,13 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
             g_stats.vr_kernel_ms += b;
             g_stats.interaction_kernel_ms += c;
           }
-          g_stats.interactions += n_int; /* launches of each kernel class */
+          g_stats.launches += n_int; /* launches of each kernel class */
         }
       }
     }
     HIP_TRY(hipEventRecord(evs[1], st));
     std::vector<unsigned long long> th(tally_words);
-    unsigned int cnt[ENG_NCOUNTERS];
+    unsigned long long cnt[ENG_NCOUNTERS];
     HIP_TRY(hipMemcpyAsync(th.data(), G.tallies.p, tally_words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(cnt, G.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -707,12 +741,12 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     HIP_TRY(hipEventElapsedTime(&ms, evs[0], evs[1]));
     g_stats.transport_ms = ms;
     g_stats.histories = cnt[ENG_CNT_HISTORIES];
-    g_stats.vr_launches = (double)cnt[ENG_CNT_VR];
+    g_stats.vr_events = (int64_t)cnt[ENG_CNT_VR];
     g_stats.solid_angle_fallbacks = cnt[ENG_CNT_FALLBACKS];
     for (int l = 0; l < ENG_MAX_LAYERS; l++) g_layer_events[l] = cnt[ENG_CNT_LAYER0 + l];
     if (cnt[ENG_CNT_ERRORS]) {
-      xmi_set_error("xmi_main_msim: %u photons hit an invalid state on the device (source %u, estimator Doppler %u, "
-                    "transport Doppler %u)",
+      xmi_set_error("xmi_main_msim: %llu photons hit an invalid state on the device (source %llu, estimator Doppler %llu, "
+                    "transport Doppler %llu)",
                     cnt[ENG_CNT_ERRORS], cnt[ENG_CNT_ERR_SOURCE], cnt[ENG_CNT_ERR_VR_DOPPLER], cnt[ENG_CNT_ERR_DOPPLER]);
       goto fail;
     }

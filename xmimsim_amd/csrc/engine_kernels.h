@@ -35,6 +35,8 @@ struct SourceDesc {
   int pad;
   long count;             /* histories of this source */
   uint64_t hid0;          /* first history id */
+  uint64_t run0;          /* position of its first history in the run order
+                             (all sources back to back, empty ones skipped) */
   double x1, x2, y1, y2;  /* interval: energies and total intensities at the ends */
   double h1, h2;          /* interval: horizontal intensities at the ends */
   double energy, scale;   /* line */

@@ -42,9 +42,9 @@ __device__ __forceinline__ void d_mu_all(const EngineParams &P, double E, double
 }
 
 /* wavefront-aggregated counter: call from converged code only */
-__device__ __forceinline__ void d_wave_count(unsigned int *ctr, int pred) {
+__device__ __forceinline__ void d_wave_count(unsigned long long *ctr, int pred) {
   unsigned long long m = __ballot(pred);
-  if (m != 0ull && __lane_id() == 0) atomicAdd(ctr, (unsigned int)__popcll(m));
+  if (m != 0ull && __lane_id() == 0) atomicAdd(ctr, (unsigned long long)__popcll(m));
 }
 
 __device__ __forceinline__ void d_update_dirv(double *dirv, double *theta, double *phi, double theta_i, double phi_i) {
@@ -89,13 +89,37 @@ __device__ __forceinline__ double d_phi0(double theta, double phi, const double 
 }
 
 /* cone-sampling estimate of the detector solid angle seen from (r1, theta1),
- * src/xmi_solid_angle_f.F90:432-710 (same estimator as src/xmi_kernels.cl:219-452) */
-__device__ __forceinline__ double d_single_solid_angle_impl(const EngineParams &P, double r1, double theta1, long hits,
-                                                           phx_stream *rng) {
+ * src/xmi_solid_angle_f.F90:432-710 (same estimator as src/xmi_kernels.cl:219-452).
+ * Split in a set-up and a per-trial test: trial i consumes exactly the
+ * uniforms 2i and 2i+1 of the stream, so a wavefront can run the trials of one
+ * estimate on all its lanes (trial i = Philox block first_block + i) and count
+ * exactly the hits of the sequential loop. */
+struct SaCone {
+  double cone_sa, cos_apex, st, ct, lp1, lp2, R, Rc, H;
+  int outside;
+};
+
+/* detector window radius and collimator (radius, height, present) */
+struct SaGeom {
+  double R, Rc, H;
+  int coll;
+};
+
+__device__ __forceinline__ SaGeom d_sa_geom(const EngineParams &P) {
+  SaGeom g;
+  g.R = P.det_radius;
+  g.Rc = P.coll_radius;
+  g.H = P.coll_height;
+  g.coll = P.coll_present;
+  return g;
+}
+
+/* returns 0 when the collimator hides the detector (estimate 0, no draws) */
+__device__ __forceinline__ int d_sa_setup(const SaGeom g, double r1, double theta1, SaCone *c) {
   double r, theta, fcbr;
   int outside = 0;
-  const double R = P.det_radius, Rc = P.coll_radius, H = P.coll_height;
-  if (!P.coll_present) {
+  const double R = g.R, Rc = g.Rc, H = g.H;
+  if (!g.coll) {
     r = r1;
     theta = theta1;
     fcbr = R;
@@ -116,7 +140,7 @@ __device__ __forceinline__ double d_single_solid_angle_impl(const EngineParams &
       theta = theta1;
       fcbr = R;
     } else if (r1 * sin(theta1) <= H) {
-      return 0.0;
+      return 0;
     } else {
       r = sqrt(r1 * r1 - 2.0 * r1 * sin(theta1) * H + H * H);
       theta = acos(r1 * cos(theta1) / r);
@@ -128,37 +152,92 @@ __device__ __forceinline__ double d_single_solid_angle_impl(const EngineParams &
   double alpha1 = atan(fcbr * sin(theta) / (r - fcbr * cos(theta)));
   if (alpha1 <= 0.0) alpha1 += ENG_PI;
   double apex = beta > alpha1 ? beta : alpha1;
-  double cos_apex = cos(apex);
-  double cone_sa = 2.0 * ENG_PI * (1.0 - cos_apex);
-  double st = sin(theta), ct = cos(theta);
-  double lp1 = r1 * cos(theta1), lp2 = r1 * sin(theta1);
-  long nhit = 0;
-  for (long i = 0; i < hits; i++) {
-    double th = acos(1.0 - phx_uniform(rng) * (1.0 - cos_apex));
-    double ph = phx_uniform(rng) * 2.0 * ENG_PI;
-    double sth = sin(th);
-    double c0 = sth * cos(ph), c1 = sth * sin(ph), c2 = cos(th);
-    /* rotation matrix rows (1,0,0), (0,-sin,-cos), (0,cos,-sin) */
-    double v0 = 1.0 * c0 + 0.0 * c1 + 0.0 * c2;
-    double v1 = 0.0 * c0 + (-st) * c1 + (-ct) * c2;
-    double v2 = 0.0 * c0 + ct * c1 + (-st) * c2;
-    if (0.0 * v0 + 0.0 * v1 + 1.0 * v2 >= 0.0) continue;
-    if (outside) {
-      double d = (H - lp2) / v2;
-      double ix = d * v0 + 0.0, iy = d * v1 + lp1;
-      if (sqrt(ix * ix + iy * iy + 0.0) > Rc) continue;
-    }
-    double d = (0.0 - lp2) / v2;
-    double ix = d * v0 + 0.0, iy = d * v1 + lp1;
-    if (sqrt(ix * ix + iy * iy + 0.0) <= R) nhit++;
-  }
-  return cone_sa * (double)nhit / (double)hits;
+  c->cos_apex = cos(apex);
+  c->cone_sa = 2.0 * ENG_PI * (1.0 - c->cos_apex);
+  c->st = sin(theta);
+  c->ct = cos(theta);
+  c->lp1 = r1 * cos(theta1);
+  c->lp2 = r1 * sin(theta1);
+  c->R = R;
+  c->Rc = Rc;
+  c->H = H;
+  c->outside = outside;
+  return 1;
 }
 
-/* out-of-line copy for the rare off-grid evaluations inside the estimator */
-__device__ __noinline__ double d_single_solid_angle(const EngineParams &P, double r1, double theta1, long hits,
-                                                    phx_stream *rng) {
-  return d_single_solid_angle_impl(P, r1, theta1, hits, rng);
+/* one cone direction from two uniforms; 1 if it reaches the detector window */
+__device__ __forceinline__ int d_sa_trial(const SaCone &c, double u1, double u2) {
+  double th = acos(1.0 - u1 * (1.0 - c.cos_apex));
+  double ph = u2 * 2.0 * ENG_PI;
+  double sth = sin(th);
+  double c0 = sth * cos(ph), c1 = sth * sin(ph), c2 = cos(th);
+  /* rotation matrix rows (1,0,0), (0,-sin,-cos), (0,cos,-sin) */
+  double v0 = 1.0 * c0 + 0.0 * c1 + 0.0 * c2;
+  double v1 = 0.0 * c0 + (-c.st) * c1 + (-c.ct) * c2;
+  double v2 = 0.0 * c0 + c.ct * c1 + (-c.st) * c2;
+  if (0.0 * v0 + 0.0 * v1 + 1.0 * v2 >= 0.0) return 0;
+  if (c.outside) {
+    double d = (c.H - c.lp2) / v2;
+    double ix = d * v0 + 0.0, iy = d * v1 + c.lp1;
+    if (sqrt(ix * ix + iy * iy + 0.0) > c.Rc) return 0;
+  }
+  double d = (0.0 - c.lp2) / v2;
+  double ix = d * v0 + 0.0, iy = d * v1 + c.lp1;
+  return sqrt(ix * ix + iy * iy + 0.0) <= c.R;
+}
+
+/* sequential estimate (one thread per grid cell) */
+__device__ __forceinline__ double d_single_solid_angle_impl(const SaGeom g, double r1, double theta1, long hits,
+                                                           phx_stream *rng) {
+  SaCone c;
+  if (!d_sa_setup(g, r1, theta1, &c)) return 0.0;
+  long nhit = 0;
+  for (long i = 0; i < hits; i++) {
+    double u1 = phx_uniform(rng);
+    double u2 = phx_uniform(rng);
+    nhit += d_sa_trial(c, u1, u2);
+  }
+  return c.cone_sa * (double)nhit / (double)hits;
+}
+
+/* Off-grid estimates inside the estimator, run by the whole wavefront: each
+ * lane with need != 0 gets omega for (r1, theta1) from its own stream *s,
+ * which must sit on a block boundary; afterwards *s has advanced by the
+ * 2*hits uniforms the sequential loop would have drawn. */
+__device__ __forceinline__ void d_wave_solid_angle(const SaGeom g, int need, double r1, double theta1, long hits,
+                                                   phx_stream *s, double *omega) {
+  unsigned long long pending = __ballot(need);
+  const int lane = __lane_id();
+  while (pending) {
+    const int src = __ffsll((long long)pending) - 1;
+    pending &= pending - 1ull;
+    double r_l = __shfl(r1, src), t_l = __shfl(theta1, src);
+    phx_stream q;
+    q.key[0] = (uint32_t)__shfl((int)s->key[0], src);
+    q.key[1] = (uint32_t)__shfl((int)s->key[1], src);
+    for (int k = 0; k < 4; k++) q.ctr[k] = (uint32_t)__shfl((int)s->ctr[k], src);
+    const uint32_t first = q.ctr[0];
+    SaCone c;
+    int open = d_sa_setup(g, r_l, t_l, &c);
+    long nhit = 0;
+    if (open) {
+      for (long t = lane; t < hits; t += 64) {
+        uint32_t o[4];
+        uint32_t ctr[4] = {first + (uint32_t)t, q.ctr[1], q.ctr[2], q.ctr[3]};
+        phx_4x32_10(ctr, q.key, o);
+        nhit += d_sa_trial(c, phx_u01(o[0], o[1]), phx_u01(o[2], o[3]));
+      }
+    }
+    /* exact integer total over the wave */
+    for (int off = 32; off > 0; off >>= 1) nhit += __shfl_xor(nhit, off);
+    if (lane == src) {
+      *omega = open ? c.cone_sa * (double)nhit / (double)hits : 0.0;
+      if (open) {
+        s->ctr[0] = first + (uint32_t)hits;
+        s->avail = 0;
+      }
+    }
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -170,9 +249,9 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_solid_angle_grid(EngineParams P, 
   long total = (long)n_r * n_t;
   for (long cell = blockIdx.x * (long)blockDim.x + threadIdx.x; cell < total; cell += (long)gridDim.x * blockDim.x) {
     int jt = (int)(cell / n_r), ir = (int)(cell % n_r);
-    phx_stream s;
+    phx_stream s = {};
     phx_stream_init(&s, seed, 0u, 0u, PHX_SUB_GRID, (uint64_t)cell);
-    grid[cell] = d_single_solid_angle_impl(P, rv[ir], tv[jt], hits, &s);
+    grid[cell] = d_single_solid_angle_impl(d_sa_geom(P), rv[ir], tv[jt], hits, &s);
   }
 }
 
@@ -236,14 +315,16 @@ __device__ int d_ran_trap(double x1, double x2, double y1, double y2, double u, 
 /* one primary photon; returns 1 when it enters the sample */
 __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const SourceTable &T, uint64_t batch_start,
                             int i) {
-  uint64_t hid = batch_start + (uint64_t)i;
+  const uint64_t run = batch_start + (uint64_t)i;
   int lo = 0, hi = T.n - 1;
-  while (lo < hi) { /* last descriptor with hid0 <= hid */
+  while (lo < hi) { /* last descriptor with run0 <= run: empty sources share
+                       their successor's run0 and are skipped */
     int mid = (lo + hi + 1) >> 1;
-    if (T.d[mid].hid0 <= hid) lo = mid;
+    if (T.d[mid].run0 <= run) lo = mid;
     else hi = mid - 1;
   }
   const SourceDesc &d = T.d[lo];
+  const uint64_t hid = d.hid0 + (run - d.run0);
   const size_t cap = (size_t)P.cap;
   S.layer[i] = 0;
   S.hid[i] = hid;
@@ -257,8 +338,8 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   double E, w, hor = 0.0;
   if (d.kind == 0) {
     if (!d_ran_trap(d.x1, d.x2, d.y1, d.y2, phx_uniform(&s), &E)) {
-      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
       return 0;
     }
     hor = d_interp(d.x1, d.h1, d.x2, d.h2, E) / d_interp(d.x1, d.y1, d.x2, d.y2, E);
@@ -307,8 +388,8 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   double ev[3] = {horizontal ? 0.0 : 1.0, horizontal ? 1.0 : 0.0, 0.0};
   double cosalfa = d_dot(ev, dirv);
   if (fabs(cosalfa) > 1.0) {
-    atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
+    atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
     return 0;
   }
   double c_alfa = acos(cosalfa);
@@ -325,15 +406,15 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
         break;
       }
     if (layer == 0) {
-      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
       return 0;
     }
   } else {
     double pp[3] = {0.0, 0.0, P.layer_zb[0]}, ip[3];
     if (!d_intersect(pp, P.n_sample, coords, dirv, ip)) {
-      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1u);
+      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
+      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
       return 0;
     }
     coords[0] = ip[0];
@@ -369,7 +450,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
 /* queues */
 
 __global__ void k_reset_queues(EngineParams P) {
-  if (threadIdx.x < 3) P.counters[threadIdx.x] = 0u;
+  if (threadIdx.x < 3) P.counters[threadIdx.x] = 0ull;
 }
 
 /* wavefront-aggregated append: one atomic per wave and queue */
@@ -380,7 +461,7 @@ __device__ __forceinline__ void d_enqueue(const EngineParams &P, int type, int i
     int lane = __lane_id();
     int leader = __ffsll((long long)mask) - 1;
     unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(&P.counters[t], (unsigned int)__popcll(mask));
+    if (lane == leader) base = (unsigned int)atomicAdd(&P.counters[t], (unsigned long long)__popcll(mask));
     base = __shfl(base, leader);
     if (type == t) {
       unsigned long long below = mask & ((1ull << lane) - 1ull);
@@ -485,10 +566,22 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
 }
 
 /* ------------------------------------------------------------------------ */
-/* detection-weight estimator, one thread per photon at order n.
- * Tallies: Rayleigh/Compton rows and fluorescence-line bins are summed over
- * the wavefront in a fixed butterfly order (deterministic), then added as
- * 128-bit fixed point; scattered-energy channels go through LDS. */
+/* detection-weight estimator of every photon at order n
+ * (src/xmi_variance_reduction.F90:29-726, xmi_compton_varred2 :949-1101).
+ *
+ * Phase A, one lane per photon: detector point, angles, escape path through
+ * the layers, solid angle, then the Rayleigh and Compton terms of every
+ * element of the photon's layer.  These draw from the photon's estimator
+ * stream in element order, so they stay sequential per photon.
+ * Phase B, one lane per (element, fluorescence line) entry of a layer: the
+ * wavefront walks its photons one at a time and reads each photon's scalars
+ * from the lane that owns it (readlane), so the escape attenuation and tally
+ * add of every line is a lane of its own and needs no cross-lane sum.
+ *
+ * Register arrays are indexed only by compile-time indices (NL = layer-count
+ * bound); per-photon path lengths are kept in crossing order.  Tallies are
+ * 128-bit fixed point, privatised in LDS when they fit; integer adds commute,
+ * so the totals do not depend on the order of the adds. */
 
 __device__ __forceinline__ double d_wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -500,15 +593,22 @@ __device__ __forceinline__ void d_tally_wave(unsigned long long *bin, double v, 
   if (__lane_id() == 0 && s > 0.0) fx_add_global(bin, s, shift);
 }
 
-template <bool use_lds>
+__device__ __forceinline__ double d_readlane(double v, int lane) {
+  long long b = __double_as_longlong(v);
+  unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int NL, bool use_lds>
 __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA S, int n, int count) {
-  /* LDS tallies (when they fit): channels [nch][2], line bins [nZ][ENG_MAX_ACT][2],
-   * Rayleigh/Compton rows [nZ][2][2]; flushed once per workgroup */
+  /* LDS tallies (when they fit): channels [nch][2], compact line bins
+   * [n_lbins][2], Rayleigh/Compton rows [nZ][2][2]; flushed once per workgroup */
   extern __shared__ unsigned long long lds[];
-  const size_t n_ch2 = (size_t)P.nch * 2, n_line2 = (size_t)P.nZ * ENG_MAX_ACT * 2, n_rc2 = (size_t)P.nZ * 4;
+  const size_t n_ch2 = (size_t)P.nch * 2, n_line2 = (size_t)P.n_lbins * 2, n_rc2 = (size_t)P.nZ * 4;
   const size_t cap = (size_t)P.cap;
   unsigned long long *gch = P.t_ch + (size_t)(n - 1) * n_ch2;
-  unsigned long long *gline = P.t_line + (size_t)(n - 1) * n_line2;
+  unsigned long long *gline = P.t_line + (size_t)(n - 1) * P.nZ * ENG_MAX_ACT * 2;
   unsigned long long *grc = P.t_rc + (size_t)(n - 1) * n_rc2;
   unsigned long long *tch = gch, *tline = gline, *trc = grc;
   if (use_lds) {
@@ -519,19 +619,22 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
     __syncthreads();
   }
   const int shift = P.tally_shift;
+  const int lane = __lane_id();
+  const int smax = P.use_M_lines ? 8 : 3;
 
-  /* grid-stride over photons; every lane runs the same trip count so that the
-   * wavefront sums below see the whole wave */
+  /* grid-stride over photons; every lane runs the same trip count */
   const int stride = gridDim.x * blockDim.x;
-  const int base0 = blockIdx.x * blockDim.x;
-  for (int base = base0; base < count; base += stride) {
-    int i = base + threadIdx.x;
+  for (int base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    const int i = base + threadIdx.x;
     int active = (i < count) && S.layer[i] > 0 && S.energy[i] > 1.0;
-    /* per-lane state */
-    double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, Pdir_fluo = 0.0, Pesc_rayl = 0.0;
-    double dist[ENG_MAX_LAYERS];
-    int cur = 0, up = 1, last = 1, dir = 1;
-    phx_stream s;
+    double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, Pesc_rayl = 0.0, mu_cur = 1.0, pe = 0.0;
+    double dist[NL]; /* escape path length in the k-th layer crossed (k = 0: own layer) */
+#pragma unroll
+    for (int k = 0; k < NL; k++) dist[k] = 0.0;
+    int cur = 0, dir = 1, nc = 0, prev_line = 0, prev_u = 0;
+    int need_sa = 0;
+    double sa_r = 0.0, sa_t = 0.0;
+    phx_stream s = {};
     if (active) {
       phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
       E = S.energy[i];
@@ -567,27 +670,41 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
         c = d_dot(proj, evn);
         c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
         phi = acos(c);
-        up = d_dot(nd, P.n_sample) > 0.0;
-        last = up ? P.n_layers : 1;
+        const int up = d_dot(nd, P.n_sample) > 0.0;
         dir = up ? 1 : -1;
-        for (int l = 0; l < P.n_layers; l++) dist[l] = 0.0;
+        const int ncross = up ? P.n_layers - cur + 1 : cur;
+        /* path to the detector through the layers, in crossing order; the
+         * layer holding the detector point ends the walk */
         double tc[3] = {coords[0], coords[1], coords[2]};
-        for (int l = cur; up ? l <= last : l >= last; l += dir) {
-          double pp[3] = {0.0, 0.0, up ? P.layer_ze[l - 1] : P.layer_zb[l - 1]}, ip[3];
-          d_intersect(pp, P.n_sample, tc, nd, ip);
-          dist[l - 1] = d_distance(tc, ip);
-          if (dist[l - 1] > total_distance) {
-            dist[l - 1] = total_distance;
-            break;
+        int done = 0;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+          if (k < ncross && !done) {
+            const int l0 = cur - 1 + k * dir;
+            double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]}, ip[3];
+            d_intersect(pp, P.n_sample, tc, nd, ip);
+            double dd = d_distance(tc, ip);
+            if (dd > total_distance) {
+              dd = total_distance;
+              done = 1;
+            } else {
+              tc[0] = ip[0];
+              tc[1] = ip[1];
+              tc[2] = ip[2];
+              total_distance -= dd;
+            }
+            dist[k] = dd;
+            nc = k + 1;
           }
-          tc[0] = ip[0];
-          tc[1] = ip[1];
-          tc[2] = ip[2];
-          total_distance -= dist[l - 1];
         }
         double murhod = 0.0;
-        for (int l = cur; up ? l <= last : l >= last; l += dir)
-          murhod += S.mus[(size_t)(l - 1) * cap + i] * P.layer_density[l - 1] * dist[l - 1];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+          if (k < nc) {
+            const int l0 = cur - 1 + k * dir;
+            murhod += S.mus[(size_t)l0 * cap + i] * P.layer_density[l0] * dist[k];
+          }
+        }
         Pesc_rayl = exp(-murhod);
         /* solid angle, src/xmi_solid_angle_f.F90:712-801 */
         double r = d_distance(P.p_det, coords);
@@ -602,8 +719,9 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           int p1 = d_findpos_uniform(P.sa_r, P.sa_nr, P.sa_r0, P.sa_rinv, r);
           int p2 = d_findpos_uniform(P.sa_t, P.sa_nt, P.sa_t0, P.sa_tinv, sth);
           if (p1 < 0 || p2 < 0) {
-            omega = d_single_solid_angle(P, r, sth, P.hits_per_single, &s);
-            atomicAdd(&P.counters[ENG_CNT_FALLBACKS], 1u);
+            need_sa = 1;
+            sa_r = r;
+            sa_t = sth;
           } else {
             const double *a1 = P.sa_r, *a2 = P.sa_t;
             double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
@@ -616,33 +734,37 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
             omega = c1 * g0[0] + c2 * g0[1] + c3 * g1[0] + c4 * g1[1];
           }
         }
-        Pdir_fluo = omega / 4.0 / ENG_PI;
+        mu_cur = S.mus[(size_t)(cur - 1) * cap + i];
+        prev_line = S.prev_line[i];
+        prev_u = S.prev_u[i];
+        pe = xrl_energy(E).pe;
       }
     }
+    /* off-grid points: the whole wavefront runs each cone estimate */
+    if (__ballot(need_sa)) {
+      d_wave_solid_angle(d_sa_geom(P), need_sa, sa_r, sa_t, P.hits_per_single, &s, &omega);
+      d_wave_count(&P.counters[ENG_CNT_FALLBACKS], need_sa);
+    }
+    const double Pdir_fluo = omega / 4.0 / ENG_PI;
     d_wave_count(&P.counters[ENG_CNT_VR], active);
     for (int L = 1; L <= P.n_layers; L++) d_wave_count(&P.counters[ENG_CNT_LAYER0 + L - 1], active && cur == L);
 
-    /* element loop: uniform over the wave per layer */
+    /* phase A: scattering terms, element loop uniform over the wave per layer */
     for (int L = 1; L <= P.n_layers; L++) {
-      int mine = active && cur == L;
+      const int mine = active && cur == L;
       if (__ballot(mine) == 0ull) continue;
-      int off = P.layer_off[L - 1], nel = P.layer_nel[L - 1];
-      double mu_cur = mine ? S.mus[(size_t)(L - 1) * cap + i] : 1.0;
-      int prev_line = mine ? S.prev_line[i] : 0;
-      int prev_u = mine ? S.prev_u[i] : 0;
+      const int off = P.layer_off[L - 1], nel = P.layer_nel[L - 1];
       for (int k = 0; k < nel; k++) {
-        int u = P.el_u[off + k];
+        const int u = P.el_u[off + k];
         const xrl_atom *a = &P.atoms[u];
-        double wi = P.el_w[off + k];
-        double Pconv = wi / mu_cur;
+        const double wi = P.el_w[off + k];
+        const double Pconv = wi / mu_cur;
         /* Rayleigh */
         double twr = 0.0;
         if (mine) {
           twr = Pconv * (omega * xrl_dcsp_rayl(a, E, theta, phi)) * Pesc_rayl * w;
           int ch = (E >= 1.0) ? (int)((E - P.zero) / P.gain) : -1;
-          if (ch >= 0 && ch <= P.nch - 1 && twr > 0.0) {
-            fx_add_global(&tch[2 * ch], twr, shift);
-          }
+          if (ch >= 0 && ch <= P.nch - 1 && twr > 0.0) fx_add_global(&tch[2 * ch], twr, shift);
         }
         d_tally_wave(&trc[((size_t)u * 2 + 0) * 2], twr, shift);
         /* Compton, xmi_compton_varred2 */
@@ -676,50 +798,82 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           if (!failed) {
             double m2 = 0.0;
             xrl_ectx ce = xrl_energy(energy);
-            for (int l = cur; up ? l <= last : l >= last; l += dir)
-              m2 += d_layer_mu_c(P, l - 1, &ce) * P.layer_density[l - 1] * dist[l - 1];
+#pragma unroll
+            for (int kk = 0; kk < NL; kk++) {
+              if (kk < nc) {
+                const int l0 = cur - 1 + kk * dir;
+                m2 += d_layer_mu_c(P, l0, &ce) * P.layer_density[l0] * dist[kk];
+              }
+            }
             double Pesc_comp = exp(-m2);
             twc = Pconv * (omega * xrl_dcsp_compt(a, E, theta, phi)) * Pesc_comp * w / 1.0;
             int ch = (energy >= 1.0) ? (int)((energy - P.zero) / P.gain) : -1;
-            if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) {
-              fx_add_global(&tch[2 * ch], twc, shift);
-            }
+            if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) fx_add_global(&tch[2 * ch], twc, shift);
           } else {
-            atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
-            atomicAdd(&P.counters[ENG_CNT_ERR_VR_DOPPLER], 1u);
+            atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
+            atomicAdd(&P.counters[ENG_CNT_ERR_VR_DOPPLER], 1ull);
           }
         }
         d_tally_wave(&trc[((size_t)u * 2 + 1) * 2], twc, shift);
-        /* fluorescence */
-        double Psh[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (mine) {
-          if (n > 1 && prev_line > 0) {
-            int smax = P.use_M_lines ? 8 : 3;
-            for (int sh = 0; sh <= smax; sh++)
-              Psh[sh] = P.precalc_xrf[(((size_t)u * 9 + sh) * P.nZ + prev_u) * XMI_NLINES_VR_DEV + (prev_line - 1)];
-          } else {
-            xrl_ectx ce = xrl_energy(E);
-            xrl_shell_cross_sections_c(a, P.cascade, &ce, Psh);
-            if (!P.use_M_lines)
-              for (int sh = 4; sh < 9; sh++) Psh[sh] = 0.0;
-          }
+      }
+    }
+
+    /* phase B: fluorescence lines, one lane per (element, line) entry */
+    for (int L = 1; L <= P.n_layers; L++) {
+      const unsigned long long m = __ballot(active && cur == L);
+      if (m == 0ull) continue;
+      const int t0 = P.fl_off[L - 1], T = P.fl_off[L] - t0;
+      for (int tb = 0; tb < T; tb += 64) {
+        const int ok = tb + lane < T;
+        const int t = ok ? t0 + tb + lane : t0;
+        const int4 info = P.fl_info[t];
+        const double2 wf = P.fl_wf[t];
+        const xrl_atom *a = &P.atoms[info.x];
+        double edge[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) edge[q] = a->edge[q];
+        const double *shC = &a->shell_C[P.cascade - 1][0][0];
+        /* mu*rho of this line in the k-th layer crossed, upwards and downwards */
+        double mr_up[NL], mr_dn[NL];
+#pragma unroll
+        for (int kk = 0; kk < NL; kk++) {
+          const int lu = L - 1 + kk, ld = L - 1 - kk;
+          mr_up[kk] = lu < P.n_layers ? P.fl_mr[(size_t)t * P.n_layers + lu] : 0.0;
+          mr_dn[kk] = ld >= 0 ? P.fl_mr[(size_t)t * P.n_layers + ld] : 0.0;
         }
-        const int nact = P.act_n[u];
-        const size_t abase = (size_t)u * ENG_MAX_ACT;
-        for (int q = 0; q < nact; q++) {
-          double twf = 0.0;
-          if (mine) {
-            int sh = P.act_shell[abase + q];
-            double Ps = Psh[sh];
-            if (Ps != 0.0) {
-              double Pc = wi * Ps * P.act_factor[abase + q] / mu_cur;
-              double m2 = 0.0;
-              for (int l = cur; up ? l <= last : l >= last; l += dir)
-                m2 += P.act_murho[((size_t)(l - 1) * P.nZ + u) * ENG_MAX_ACT + q] * dist[l - 1];
-              twf = Pc * Pdir_fluo * exp(-m2) * w;
-            }
+        unsigned long long *bin =
+            use_lds ? &tline[(size_t)info.w * 2] : &tline[((size_t)info.x * ENG_MAX_ACT + info.y) * 2];
+        unsigned long long mm = m;
+        while (mm) {
+          const int p = __ffsll((long long)mm) - 1;
+          mm &= mm - 1ull;
+          const double Ep = d_readlane(E, p), pep = d_readlane(pe, p), mup = d_readlane(mu_cur, p);
+          const double pdp = d_readlane(Pdir_fluo, p), wp = d_readlane(w, p);
+          const int dirp = __builtin_amdgcn_readlane(dir, p);
+          const int plp = __builtin_amdgcn_readlane(prev_line, p), pup = __builtin_amdgcn_readlane(prev_u, p);
+          double dk[NL];
+#pragma unroll
+          for (int kk = 0; kk < NL; kk++) dk[kk] = d_readlane(dist[kk], p);
+          if (!ok) continue;
+          double Ps;
+          if (n > 1 && plp > 0) {
+            Ps = info.z <= smax
+                     ? P.precalc_xrf[(((size_t)info.x * 9 + info.z) * P.nZ + pup) * XMI_NLINES_VR_DEV + (plp - 1)]
+                     : 0.0;
+          } else {
+            int kE = 0;
+#pragma unroll
+            for (int q = 0; q < 9; q++) kE += edge[q] > Ep ? 1 : 0;
+            Ps = (!P.use_M_lines && info.z >= 4) ? 0.0 : pep * shC[kE * 9 + info.z];
           }
-          d_tally_wave(&tline[(abase + q) * 2], twf, shift);
+          if (Ps != 0.0) {
+            const double Pc = wf.x * Ps * wf.y / mup;
+            double m2 = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < NL; kk++) m2 += (dirp > 0 ? mr_up[kk] : mr_dn[kk]) * dk[kk];
+            const double twf = Pc * pdp * exp(-m2) * wp;
+            fx_add_global(bin, twf, shift);
+          }
         }
       }
     }
@@ -729,7 +883,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
     for (size_t k = threadIdx.x; k < n_ch2 / 2; k += blockDim.x)
       if (tch[2 * k] | tch[2 * k + 1]) fx_merge_global(&gch[2 * k], tch[2 * k], tch[2 * k + 1]);
     for (size_t k = threadIdx.x; k < n_line2 / 2; k += blockDim.x)
-      if (tline[2 * k] | tline[2 * k + 1]) fx_merge_global(&gline[2 * k], tline[2 * k], tline[2 * k + 1]);
+      if (tline[2 * k] | tline[2 * k + 1])
+        fx_merge_global(&gline[(size_t)P.lbin_g[k] * 2], tline[2 * k], tline[2 * k + 1]);
     for (size_t k = threadIdx.x; k < n_rc2 / 2; k += blockDim.x)
       if (trc[2 * k] | trc[2 * k + 1]) fx_merge_global(&grc[2 * k], trc[2 * k], trc[2 * k + 1]);
   }
@@ -823,8 +978,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_compton(EngineParams P, PhotonSoA
     }
   }
   if (!accepted) {
-    atomicAdd(&P.counters[ENG_CNT_ERRORS], 1u);
-    atomicAdd(&P.counters[ENG_CNT_ERR_DOPPLER], 1u);
+    atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
+    atomicAdd(&P.counters[ENG_CNT_ERR_DOPPLER], 1ull);
     S.energy[i] = 0.0;
     return;
   }
@@ -1004,6 +1159,28 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S
 /* ------------------------------------------------------------------------ */
 /* host-side launch wrappers */
 
+size_t eng_varred_lds_bytes(const EngineParams *P) {
+  size_t need = ((size_t)P->nch * 2 + (size_t)P->n_lbins * 2 + (size_t)P->nZ * 4) * sizeof(unsigned long long);
+  return need <= ENG_LDS_BYTES ? need : 0;
+}
+
+template <int NL>
+static void launch_varred_nl(const EngineParams *P, const PhotonSoA *S, int n, int count, int vb, hipStream_t st) {
+  size_t lds = eng_varred_lds_bytes(P);
+  if (lds) hipLaunchKernelGGL((k_varred<NL, true>), dim3(vb), dim3(ENG_BLOCK), lds, st, *P, *S, n, count);
+  else hipLaunchKernelGGL((k_varred<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+}
+
+/* the layer-count bound of the estimator: the smallest of 1, 2, 4, 8, 16 */
+static void launch_varred(const EngineParams *P, const PhotonSoA *S, int n, int count, int vb, hipStream_t st) {
+  const int nl = P->n_layers;
+  if (nl <= 1) launch_varred_nl<1>(P, S, n, count, vb, st);
+  else if (nl <= 2) launch_varred_nl<2>(P, S, n, count, vb, st);
+  else if (nl <= 4) launch_varred_nl<4>(P, S, n, count, vb, st);
+  else if (nl <= 8) launch_varred_nl<8>(P, S, n, count, vb, st);
+  else launch_varred_nl<16>(P, S, n, count, vb, st);
+}
+
 extern "C" {
 
 hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv, const double *tv, int n_r, int n_t,
@@ -1028,12 +1205,9 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, int n, in
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, st, *P);
   hipLaunchKernelGGL(k_step, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
   if (ev) (void)hipEventRecord(ev[1], st);
-  size_t need = ((size_t)P->nch * 2 + (size_t)P->nZ * ENG_MAX_ACT * 2 + (size_t)P->nZ * 4) * sizeof(unsigned long long);
-  size_t lds = need <= ENG_LDS_BYTES ? need : 0;
   int vb = vr_blocks < blocks ? vr_blocks : blocks;
   if (vb < 1) vb = 1;
-  if (lds) hipLaunchKernelGGL(k_varred<true>, dim3(vb), dim3(ENG_BLOCK), lds, st, *P, *S, n, count);
-  else hipLaunchKernelGGL(k_varred<false>, dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+  launch_varred(P, S, n, count, vb, st);
   if (ev) (void)hipEventRecord(ev[2], st);
   hipLaunchKernelGGL(k_rayleigh, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
   hipLaunchKernelGGL(k_compton, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
