@@ -87,6 +87,8 @@ def cpu_baseline(args, prep_gpu):
     import oracle
     from xmimsim_amd import api
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    # a job's CPU share is what OMP_NUM_THREADS grants it, when set
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores)
     # pilot to size the sample
     pilot_n = 20 if args.workload == "three_layer_kramers" else 2000
     sample = make_input(argparse.Namespace(**{**vars(args), "photons": pilot_n}))

@@ -53,6 +53,8 @@ def load():
     lib.xmi_oracle_single_solid_angle.restype = C.c_double
     lib.xmi_oracle_fpm_first_order.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     lib.xmi_oracle_fpm_first_order.restype = C.c_double
+    lib.xmi_oracle_scatter_first_order.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.xmi_oracle_scatter_first_order.restype = C.c_double
     lib.xmi_oracle_poly_solve_quadratic.argtypes = [C.c_double] * 3 + [P(C.c_double)] * 2
     lib.xmi_oracle_poly_solve_quadratic.restype = C.c_int
     lib.xmi_oracle_energy_from_q.argtypes = [C.c_double] * 3
@@ -113,3 +115,8 @@ def solid_angle(prepared, hits: int, seed: int, n_threads: int = 0):
 
 def fpm_first_order(prepared, Z: int, line: int, cascade: int = 4) -> float:
     return load().xmi_oracle_fpm_first_order(prepared.inputF, Z, line, cascade)
+
+
+def scatter_first_order(prepared, Z: int, kind: str) -> float:
+    """Closed-form first-order Rayleigh ("rayl") or Compton ("compt") intensity."""
+    return load().xmi_oracle_scatter_first_order(prepared.inputF, Z, {"rayl": 0, "compt": 1}[kind])

@@ -1314,3 +1314,47 @@ double xmi_oracle_fpm_first_order(xmi_inputFPtr inputFPtr, int Z, int line, int 
   double chi = mu0 / sin_alpha + mu1 / sin_beta;
   return I0 * G * w * sigma_line * (1.0 - exp(-chi * L->density * L->thickness)) / chi;
 }
+
+/* First-order scatter of a thick single layer seen by a distant point
+ * detector: the Rayleigh (kind 0) or Compton (kind 1) analogue of the
+ * fundamental-parameter formula above.  Each polarisation of the line source
+ * (horizontal: electric vector along y, vertical: along x, as set in
+ * src/xmi_main.F90:396-411) scatters with the polarised differential cross
+ * section at the angles towards the detector centre; the exit attenuation of
+ * Compton photons is taken at the pz = 0 energy E/(1 + E/mc2 (1 - cos theta)). */
+double xmi_oracle_scatter_first_order(xmi_inputFPtr inputFPtr, int Z, int kind) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const xmi_layerF *L = &in->layers[0];
+  double w = 0.0;
+  for (int i = 0; i < L->n_elements; i++)
+    if (L->Z[i] == Z) w = L->weight[i];
+  const xrl_atom *a = xrl_atom_get(Z);
+  const double E0 = in->discrete[0].energy;
+  const double *n = in->geometry.n_sample_orientation;
+  const double *pd = in->geometry.p_detector_window;
+  double beam[3] = {0.0, 0.0, 1.0};
+  double to_det[3] = {pd[0], pd[1], pd[2] - in->geometry.d_sample_source};
+  double D = norm3(to_det);
+  normalize3(to_det);
+  double sin_alpha = fabs(dot3(beam, n));
+  double sin_beta = fabs(dot3(to_det, n));
+  double Omega = 2.0 * ORC_PI * (1.0 - cos(atan(in->detector_radius / D)));
+  double cth = dot3(beam, to_det);
+  double theta = acos(cth);
+  double proj[3];
+  for (int k = 0; k < 3; k++) proj[k] = to_det[k] - cth * beam[k];
+  normalize3(proj);
+  const double e_hor[3] = {0.0, 1.0, 0.0}, e_ver[3] = {1.0, 0.0, 0.0};
+  double phi_h = acos(dot3(proj, e_hor)), phi_v = acos(dot3(proj, e_ver));
+  double E1 = kind == 0 ? E0 : E0 / (1.0 + E0 / ORC_MEC2 * (1.0 - cth));
+  double mu0 = 0.0, mu1 = 0.0;
+  for (int i = 0; i < L->n_elements; i++) {
+    mu0 += L->weight[i] * xrl_cs_total(xrl_atom_get(L->Z[i]), E0);
+    mu1 += L->weight[i] * xrl_cs_total(xrl_atom_get(L->Z[i]), E1);
+  }
+  double chi = mu0 / sin_alpha + mu1 / sin_beta;
+  double Ih = in->discrete[0].horizontal_intensity, Iv = in->discrete[0].vertical_intensity;
+  double dcs_h = kind == 0 ? xrl_dcsp_rayl(a, E0, theta, phi_h) : xrl_dcsp_compt(a, E0, theta, phi_h);
+  double dcs_v = kind == 0 ? xrl_dcsp_rayl(a, E0, theta, phi_v) : xrl_dcsp_compt(a, E0, theta, phi_v);
+  return (Ih * dcs_h + Iv * dcs_v) * Omega / sin_alpha * w * (1.0 - exp(-chi * L->density * L->thickness)) / chi;
+}

@@ -51,6 +51,8 @@ double xmi_oracle_single_solid_angle(xmi_inputFPtr inputFPtr, double r1, double 
 
 /* closed-form first-order intensity, tests/libxmimsim-test.c:153-180 (fpm) */
 double xmi_oracle_fpm_first_order(xmi_inputFPtr inputFPtr, int Z, int line, int cascade_type);
+/* first-order Rayleigh (kind 0) / Compton (kind 1) scatter of element Z, closed form */
+double xmi_oracle_scatter_first_order(xmi_inputFPtr inputFPtr, int Z, int kind);
 
 /* small numerical helpers exposed for unit tests */
 int xmi_oracle_poly_solve_quadratic(double a, double b, double c, double *r1, double *r2);
