@@ -37,7 +37,23 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20240601)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once (a reference kernel of known traffic for the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB: reads and writes exactly that
+    many bytes, so the HBM counters of this kernel give their scale factor."""
+    import torch
+    n = int(mib * 2 ** 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "check": float(b[n - 1].item())}))
 
 
 def make_input(args):
@@ -90,14 +106,14 @@ def cpu_baseline(args, prep_gpu):
     # a job's CPU share is what OMP_NUM_THREADS grants it, when set
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores)
     # pilot to size the sample
-    pilot_n = 20 if args.workload == "three_layer_kramers" else 2000
+    pilot_n = 50 if args.workload == "three_layer_kramers" else 5000
     sample = make_input(argparse.Namespace(**{**vars(args), "photons": pilot_n}))
     prep = api.Prepared(sample)
     prep.solid_angle = prep_gpu.solid_angle
     prep._sa_owner = "borrowed"
-    t = time.time()
     _, _, _, st = oracle.run(prep, seed=args.seed, rng_mode=oracle.RNG_PER_THREAD, n_threads=cores)
-    pilot_s = max(time.time() - t, 1e-3)
+    # size from the pilot's transport time (its set-up time does not scale)
+    pilot_s = max(st.seconds, 1e-3)
     n = max(pilot_n, int(pilot_n * args.cpu_seconds / pilot_s))
     sample = make_input(argparse.Namespace(**{**vars(args), "photons": n}))
     prep2 = api.Prepared(sample)
@@ -118,6 +134,9 @@ def cpu_baseline(args, prep_gpu):
 
 def main():
     args = parse()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not part of this build",
                           "n_gpus": args.gpus}))
