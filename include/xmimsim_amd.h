@@ -240,11 +240,15 @@ typedef struct _xmi_msim_stats {
   double total_ms;            /* wall time of xmi_main_msim */
   int64_t histories;          /* primary photons simulated */
   int64_t launches;           /* with profiling on: launches of each kernel class */
-  double vr_kernel_ms;        /* with profiling on: device time of the detection-weight kernel */
+  double vr_kernel_ms;        /* with profiling on: device time of the detection-weight kernels */
   int64_t vr_events;          /* detection-weight evaluations (photon-interactions) */
   double step_kernel_ms;      /* with profiling on: forced-step kernel (incl. queue reset) */
   double interaction_kernel_ms; /* with profiling on: Rayleigh + Compton + photo kernels */
-  int64_t solid_angle_fallbacks;
+  int64_t solid_angle_fallbacks; /* estimator points off the solid-angle grid */
+  double vr_geom_ms;          /* with profiling on: the estimator's four kernels */
+  double vr_cone_ms;
+  double vr_scatter_ms;
+  double vr_lines_ms;
 } xmi_msim_stats;
 void xmi_msim_get_stats(xmi_msim_stats *out);
 
@@ -273,6 +277,29 @@ int xmi_provider_active_lines(int Z, int line_last);
 
 /* Last error message of this thread, or "" */
 const char *xmi_msim_last_error(void);
+
+/* ---- net-line surrogate ------------------------------------------------
+ * No counterpart in the reference.  An ensemble of multilayer perceptrons
+ * (dims[0] inputs, GELU hidden layers, linear outputs) trained on the GPU
+ * with BF16 matrix-core products, FP32 master weights and Adam on a masked
+ * mean-square loss.  Layer widths dims[1..n_layers] must be multiples of 64,
+ * batches multiples of 64 up to max_batch; inputs are padded internally. */
+typedef struct xmi_surrogate xmi_surrogate;
+xmi_surrogate *xmi_surrogate_create(int n_nets, int n_layers, const int *dims, int max_batch, uint64_t seed);
+void xmi_surrogate_free(xmi_surrogate *s);
+long xmi_surrogate_param_count(const xmi_surrogate *s);      /* per net */
+int xmi_surrogate_padded_width(const xmi_surrogate *s, int layer);
+/* one Adam step of every net on the batch; loss[n_nets] before the step */
+int xmi_surrogate_train_step(xmi_surrogate *s, const float *X, const float *T, const float *mask, int batch,
+                             float lr, float *loss);
+/* loss and parameter gradients without an update; grad[n_nets][param_count] */
+int xmi_surrogate_gradients(xmi_surrogate *s, const float *X, const float *T, const float *mask, int batch,
+                            float *loss, float *grad);
+/* Y[n_nets][batch][dims[n_layers]] */
+int xmi_surrogate_predict(xmi_surrogate *s, const float *X, int batch, float *Y);
+/* f32 parameters per net: for each layer W[N][K padded] then b[N] */
+int xmi_surrogate_get_params(const xmi_surrogate *s, float *out);
+int xmi_surrogate_set_params(xmi_surrogate *s, const float *in);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,63 @@
+"""CPU tests of the surrogate's reference arithmetic (oracle/surrogate_ref.py)
+and of the XRF encoding: the reference backward pass equals central finite
+differences of its own loss, and bf16 rounding is round-to-nearest-even."""
+import numpy as np
+import pytest
+
+
+def _tiny(rng, dims=(5, 7, 6, 3)):
+    layers = []
+    for l in range(len(dims) - 1):
+        K = dims[l]
+        layers.append((rng.normal(0, 0.6, (dims[l + 1], K)), rng.normal(0, 0.1, dims[l + 1])))
+    return layers
+
+
+def test_reference_gradients_equal_finite_differences(orc):
+    import surrogate_ref as R
+    rng = np.random.default_rng(3)
+    layers = _tiny(rng)
+    X = rng.normal(size=(9, 5))
+    T = rng.normal(size=(9, 3))
+    M = (rng.random((9, 3)) > 0.3).astype(float)
+    loss, grads = R.loss_and_grads(layers, X, T, M, use_bf16=False)
+    h = 1e-6
+    for l, (W, b) in enumerate(layers):
+        for (i, j) in [(0, 0), (2, 3), (W.shape[0] - 1, W.shape[1] - 1)]:
+            Wp = [(w.copy(), c.copy()) for w, c in layers]
+            Wm = [(w.copy(), c.copy()) for w, c in layers]
+            Wp[l][0][i, j] += h
+            Wm[l][0][i, j] -= h
+            fd = (R.loss_and_grads(Wp, X, T, M, False)[0] - R.loss_and_grads(Wm, X, T, M, False)[0]) / (2 * h)
+            assert grads[l][0][i, j] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+        for i in (0, b.shape[0] - 1):
+            Wp = [(w.copy(), c.copy()) for w, c in layers]
+            Wm = [(w.copy(), c.copy()) for w, c in layers]
+            Wp[l][1][i] += h
+            Wm[l][1][i] -= h
+            fd = (R.loss_and_grads(Wp, X, T, M, False)[0] - R.loss_and_grads(Wm, X, T, M, False)[0]) / (2 * h)
+            assert grads[l][1][i] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def test_bf16_rounding_is_nearest_even(orc):
+    import surrogate_ref as R
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: rounds to the even mantissa (1)
+    assert R.bf16(np.float32([1.0 + 2 ** -8]))[0] == 1.0
+    assert R.bf16(np.float32([1.0 + 3 * 2 ** -8]))[0] == 1.0 + 2 ** -6
+    assert R.bf16(np.float32([1.0 + 2 ** -8 + 2 ** -20]))[0] == 1.0 + 2 ** -7
+    x = np.random.default_rng(0).normal(size=1000).astype(np.float32)
+    assert np.max(np.abs(R.bf16(x) - x) / np.abs(x)) <= 2 ** -8  # half an ulp of an 8-bit significand
+
+
+def test_xrf_encoding():
+    from xmimsim_amd import surrogate as S
+    assert S.N_IN == 85 and S.N_OUT == 256 and len(set(S.OUT_LINES)) == 256
+    x = S.encode([26, 82], [0.7, 0.3], 0.01, 7.9, 20.0)
+    assert x[26 - 11] == np.float32(0.7) and x[82 - 11] == np.float32(0.3)
+    assert x[-3] == np.float32(-2.0) and x[-1] == np.float32(0.5)
+    vr = np.zeros((100, 385, 4))
+    vr[25, 2, :] = [1e3, 10.0, 1.0, 0.0]   # Fe KL3
+    t, m = S.targets(vr, [26, 82])
+    k = S.OUT_LINES.index((26, 3))
+    assert m[k] == 1.0 and t[k] == pytest.approx(np.log10(1011.0))
+    assert m.sum() == 1.0

@@ -1,0 +1,82 @@
+"""GPU tests of the net-line surrogate kernels against the CPU reference
+(oracle/surrogate_ref.py): matrix-core forward pass, loss and parameter
+gradients, and that Adam training reduces the loss."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (85, 128, 128, 64)
+
+
+def _net(nets=2, seed=1):
+    from xmimsim_amd.surrogate import Surrogate
+    return Surrogate(n_nets=nets, dims=DIMS, max_batch=256, seed=seed)
+
+
+def _data(rng, B=128):
+    X = rng.normal(0, 1, (B, DIMS[0])).astype(np.float32)
+    T = rng.normal(0, 1, (B, DIMS[-1])).astype(np.float32)
+    M = (rng.random((B, DIMS[-1])) > 0.2).astype(np.float32)
+    return X, T, M
+
+
+def test_forward_matches_reference():
+    import surrogate_ref as R
+    rng = np.random.default_rng(7)
+    s = _net()
+    X, _, _ = _data(rng)
+    Y = s.predict(X)
+    P = s.get_params()
+    for e in range(s.n_nets):
+        layers = s.unpack(P[e])
+        ref, _ = R.forward(layers, X, use_bf16=True)
+        exact, _ = R.forward(layers, X, use_bf16=False)
+        scale = np.abs(ref).max()
+        # same bf16 operands, f32 vs f64 accumulation
+        assert np.abs(Y[e] - ref).max() <= 2e-3 * scale, np.abs(Y[e] - ref).max() / scale
+        # against exact arithmetic: the bf16 rounding of operands only
+        assert np.abs(Y[e] - exact).max() <= 5e-2 * scale
+    s.close()
+
+
+def test_gradients_match_reference():
+    import surrogate_ref as R
+    rng = np.random.default_rng(11)
+    s = _net()
+    X, T, M = _data(rng)
+    loss, g = s.gradients(X, T, M)
+    P = s.get_params()
+    for e in range(s.n_nets):
+        layers = s.unpack(P[e])
+        rl, rg = R.loss_and_grads(layers, X, T, M, use_bf16=True)
+        el, eg = R.loss_and_grads(layers, X, T, M, use_bf16=False)
+        assert loss[e] == pytest.approx(rl, rel=1e-3)
+        flat_ref = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in rg])
+        flat_ex = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in eg])
+        err = np.linalg.norm(g[e] - flat_ref) / np.linalg.norm(flat_ref)
+        assert err < 1e-2, err
+        assert np.linalg.norm(g[e] - flat_ex) / np.linalg.norm(flat_ex) < 5e-2
+    s.close()
+
+
+def test_adam_training_reduces_loss():
+    rng = np.random.default_rng(5)
+    s = _net(nets=2, seed=3)
+    X = rng.normal(0, 1, (256, DIMS[0])).astype(np.float32)
+    A = rng.normal(0, 0.2, (DIMS[0], DIMS[-1]))
+    T = np.tanh(X @ A).astype(np.float32)
+    M = np.ones_like(T)
+    first = s.train_step(X, T, M, 1e-3)
+    for _ in range(300):
+        last = s.train_step(X, T, M, 1e-3)
+    assert np.all(last < 0.1 * first), (first, last)
+    s.close()
+
+
+def test_batch_constraints_are_errors():
+    s = _net()
+    X = np.zeros((100, DIMS[0]), np.float32)
+    with pytest.raises(RuntimeError):
+        s.gradients(X, np.zeros((100, DIMS[-1]), np.float32), np.ones((100, DIMS[-1]), np.float32))
+    s.close()

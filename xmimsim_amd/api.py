@@ -100,7 +100,8 @@ class xmi_msim_stats(C.Structure):
     _fields_ = [("transport_ms", C.c_double), ("total_ms", C.c_double), ("histories", C.c_int64),
                 ("launches", C.c_int64), ("vr_kernel_ms", C.c_double), ("vr_events", C.c_int64),
                 ("step_kernel_ms", C.c_double), ("interaction_kernel_ms", C.c_double),
-                ("solid_angle_fallbacks", C.c_int64)]
+                ("solid_angle_fallbacks", C.c_int64), ("vr_geom_ms", C.c_double), ("vr_cone_ms", C.c_double),
+                ("vr_scatter_ms", C.c_double), ("vr_lines_ms", C.c_double)]
 
 
 # exported symbols of include/xmimsim_amd.h

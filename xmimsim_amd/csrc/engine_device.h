@@ -43,6 +43,18 @@ struct PhotonSoA {
   uint64_t *hid;                /* history id */
 };
 
+/* per-photon records of the estimator at the current order, written by the
+ * geometry kernel and read by the cone, scattering and line kernels */
+struct VrSoA {
+  double *theta, *phi;          /* scattering angle to the detector point, azimuth to the polarisation */
+  double *omega;                /* solid angle of the detector */
+  double *pesc;                 /* escape probability at the photon energy */
+  double *dist;                 /* [NL][cap] path in the k-th layer crossed (k = 0: own layer) */
+  double *sar, *sat;            /* off-grid point (r, theta) for the cone estimate */
+  int *info;                    /* bit 0 active, bit 1 upwards, bits 8.. layers crossed */
+  unsigned int *blk;            /* estimator stream position (Philox block) after the solid angle */
+};
+
 struct EngineParams {
   /* geometry */
   double n_sample[3];
@@ -73,6 +85,15 @@ struct EngineParams {
   const int *el_Z;              /* flattened per layer */
   const double *el_w;
   const int *el_u;              /* hdf5 index */
+  /* attenuation of each layer as one lookup: mu_l(E) = E^-p * pb[K] + rc(E),
+   * K = number of the layer's edges above E (edges sorted descending), rc the
+   * weighted sum of the elements' Rayleigh+Compton tables on the shared ln E
+   * grid; kstart[ci] = edges certainly above any E of ln E cell ci */
+  const double *lay_rc;         /* [n_layers][XRL_NCS] */
+  const double *lay_edge;       /* [lay_eoff[l] .. lay_eoff[l+1]) */
+  const double *lay_pb;         /* [lay_eoff[l] + l .. lay_eoff[l+1] + l] */
+  const int *lay_kstart;        /* [n_layers][XRL_NCS] */
+  int lay_eoff[ENG_MAX_LAYERS + 1];
   /* per element */
   const xrl_atom *atoms;        /* [nZ] */
   const double *ray_icdf;       /* [nZ][n_e][n_r] */
@@ -106,7 +127,7 @@ struct EngineParams {
    * fl_off[l+1]) is active line q of element u of that layer */
   int fl_off[ENG_MAX_LAYERS + 1];
   const int4 *fl_info;          /* {u, q, shell, compact bin} */
-  const double2 *fl_wf;         /* {weight fraction of u in the layer, yield*rate} */
+  const double *fl_wf;          /* weight fraction of u in the layer * yield * rate */
   const double *fl_mr;          /* [entry][n_layers] mu*rho of the line in each layer */
   const int *lbin_g;            /* compact bin -> u * ENG_MAX_ACT + q */
   int n_lbins;
@@ -126,8 +147,9 @@ struct EngineParams {
   unsigned long long *t_line;   /* [n_int][nZ][ENG_MAX_ACT][2] */
   unsigned long long *t_rc;     /* [n_int][nZ][2][2] */
   int tally_shift;              /* fixed point: value * 2^tally_shift */
-  unsigned long long *counters; /* slots ENG_CNT_*: queue sizes, events, fallbacks, errors */
-  int *queue;                   /* [3][cap] */
+  unsigned long long *counters; /* [slot ENG_CNT_*][ENG_SPREAD]: queue sizes, events, fallbacks, errors */
+  int *queue;                   /* [ENG_NQUEUES][ENG_SPREAD][qreg] */
+  int qreg;                     /* entries per sub-queue */
   int cap;
 };
 

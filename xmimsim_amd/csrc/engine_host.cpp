@@ -21,6 +21,7 @@
 
 #include <chrono>
 #include <mutex>
+#include <algorithm>
 #include <vector>
 
 #include "engine_kernels.h"
@@ -118,12 +119,13 @@ struct Engine {
   /* per-call buffers (grow only) */
   DevBuf el_Z, el_w, el_u, precalc_mu, act_n, act_line, act_shell, act_factor, act_E, act_ch, act_murho;
   DevBuf fl_info, fl_wf, fl_mr, lbin_g;
+  DevBuf lay_rc, lay_edge, lay_pb, lay_kstart;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
-  DevBuf state, tallies, counters, queue;
+  DevBuf state, vr, tallies, counters, queue;
 };
 
 #define ENG_MAX_ORDERS_PROFILED 16
-static hipEvent_t prof_ev[4 * ENG_MAX_ORDERS_PROFILED];
+static hipEvent_t prof_ev[ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED];
 
 static Engine g_engine;
 
@@ -410,6 +412,43 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
       elU.push_back(u);
     }
   }
+  /* per-layer attenuation tables (d_layer_mu_tab) */
+  std::vector<double> lay_rc((size_t)NL * XRL_NCS, 0.0), lay_edge, lay_pb;
+  std::vector<int> lay_kstart((size_t)NL * XRL_NCS, 0);
+  for (int l = 0; l < NL; l++) {
+    const xmi_layerF *L = &in->layers[l];
+    std::vector<std::pair<double, int>> edges; /* (edge, element index in layer) */
+    for (int k = 0; k < L->n_elements; k++) {
+      const xrl_atom *a = xrl_atom_get(L->Z[k]);
+      for (int c = 0; c < XRL_NCS; c++) lay_rc[(size_t)l * XRL_NCS + c] += L->weight[k] * a->cs_rc[c];
+      for (int sh = 0; sh < 9; sh++)
+        if (a->edge[sh] > 0.0) edges.push_back({a->edge[sh], k});
+    }
+    std::sort(edges.begin(), edges.end(), [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+      return x.first > y.first;
+    });
+    P.lay_eoff[l] = (int)lay_edge.size();
+    std::vector<int> above(L->n_elements, 0);
+    for (size_t K = 0; K <= edges.size(); K++) {
+      /* K edges above E: each element's own count among them */
+      double pb = 0.0;
+      for (int k = 0; k < L->n_elements; k++) pb += L->weight[k] * xrl_atom_get(L->Z[k])->photo_Bsum[above[k]];
+      lay_pb.push_back(pb);
+      if (K < edges.size()) {
+        lay_edge.push_back(edges[K].first);
+        above[edges[K].second]++;
+      }
+    }
+    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
+    for (int c = 0; c < XRL_NCS; c++) {
+      const double e_hi = exp(XRL_CS_LNE0 + (double)(c + 1) * h) * (1.0 + 1e-9);
+      int K = 0;
+      while (K < (int)edges.size() && edges[K].first > e_hi) K++;
+      lay_kstart[(size_t)l * XRL_NCS + c] = K;
+    }
+  }
+  P.lay_eoff[NL] = (int)lay_edge.size();
+
   /* attenuation at the fluorescence lines, and the estimator's active lines */
   std::vector<double> pmu((size_t)NL * nZ * XMI_NLINES_VR, 0.0);
   for (int l = 0; l < NL; l++)
@@ -459,7 +498,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     for (int q = 0; q < actn[u]; q++) lbin_g.push_back(u * ENG_MAX_ACT + q);
   P.n_lbins = act_base[nZ];
   std::vector<int4> fl_info;
-  std::vector<double2> fl_wf;
+  std::vector<double> fl_wf;
   std::vector<double> fl_mr;
   for (int l = 0; l < NL; l++) {
     P.fl_off[l] = (int)fl_info.size();
@@ -468,7 +507,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
       for (int q = 0; q < actn[u]; q++) {
         size_t o = (size_t)u * ENG_MAX_ACT + q;
         fl_info.push_back(make_int4(u, q, acts[o], act_base[u] + q));
-        fl_wf.push_back(make_double2(elW[(size_t)P.layer_off[l] + k], actf[o]));
+        fl_wf.push_back(elW[(size_t)P.layer_off[l] + k] * actf[o]);
         for (int l2 = 0; l2 < NL; l2++) fl_mr.push_back(actm[((size_t)l2 * nZ + u) * ENG_MAX_ACT + q]);
       }
     }
@@ -576,13 +615,21 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     int cap = (int)(total_hist < cap_max ? (total_hist > 0 ? total_hist : 1) : cap_max);
     P.cap = cap;
     size_t per = (size_t)cap;
+    /* the kernels' layer-count bound (estimator records hold nl_bound paths) */
+    const int nl_bound = NL <= 1 ? 1 : NL <= 2 ? 2 : NL <= 4 ? 4 : NL <= 8 ? 8 : 16;
     size_t state_bytes = per * (15 * sizeof(double) + (size_t)NL * sizeof(double) + 4 * sizeof(int) + sizeof(uint64_t));
+    size_t vr_bytes = per * ((6 + (size_t)nl_bound) * sizeof(double) + 2 * sizeof(int));
     size_t n_ch2 = (size_t)n_int * nch * 2, n_line2 = (size_t)n_int * nZ * ENG_MAX_ACT * 2, n_rc2 = (size_t)n_int * nZ * 4;
     size_t tally_words = n_ch2 + n_line2 + n_rc2;
     HIP_TRY(G.state.ensure(state_bytes));
+    HIP_TRY(G.vr.ensure(vr_bytes));
     HIP_TRY(G.tallies.ensure(tally_words * sizeof(unsigned long long)));
-    HIP_TRY(G.counters.ensure(ENG_NCOUNTERS * sizeof(unsigned long long)));
-    HIP_TRY(G.queue.ensure(3 * per * sizeof(int)));
+    /* sub-queue size: k_step runs ceil(cap / ENG_BLOCK) workgroups, and each
+     * sub-queue takes the photons of every ENG_SPREAD-th of their waves */
+    const int step_waves = ((cap + ENG_BLOCK - 1) / ENG_BLOCK) * (ENG_BLOCK / 64);
+    P.qreg = ((step_waves + ENG_SPREAD - 1) / ENG_SPREAD) * 64;
+    HIP_TRY(G.counters.ensure((size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long)));
+    HIP_TRY(G.queue.ensure((size_t)ENG_NQUEUES * ENG_SPREAD * P.qreg * sizeof(int)));
     HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size()));
     HIP_TRY(upload(G.el_w, elW.data(), elW.size()));
     HIP_TRY(upload(G.el_u, elU.data(), elU.size()));
@@ -598,12 +645,16 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size()));
     HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size()));
     HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size()));
+    HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size()));
+    HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size()));
+    HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size()));
+    HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size()));
     HIP_TRY(upload(G.src_desc, src.data(), src.size()));
     HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size()));
     HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size()));
     HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size()));
     HIP_TRY(hipMemsetAsync(G.tallies.p, 0, tally_words * sizeof(unsigned long long), st));
-    HIP_TRY(hipMemsetAsync(G.counters.p, 0, ENG_NCOUNTERS * sizeof(unsigned long long), st));
+    HIP_TRY(hipMemsetAsync(G.counters.p, 0, (size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long), st));
 
     P.el_Z = G.el_Z.as<int>();
     P.el_w = G.el_w.as<double>();
@@ -641,9 +692,13 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     P.act_ch = G.act_ch.as<int>();
     P.act_murho = G.act_murho.as<double>();
     P.fl_info = G.fl_info.as<int4>();
-    P.fl_wf = G.fl_wf.as<double2>();
+    P.fl_wf = G.fl_wf.as<double>();
     P.fl_mr = G.fl_mr.as<double>();
     P.lbin_g = G.lbin_g.as<int>();
+    P.lay_rc = G.lay_rc.as<double>();
+    P.lay_edge = G.lay_edge.as<double>();
+    P.lay_pb = G.lay_pb.as<double>();
+    P.lay_kstart = G.lay_kstart.as<int>();
     P.phi_icdf = T->phi.as<double>();
     P.phi_T = T->phiT.as<double>();
     P.phi_R = T->phiR.as<double>();
@@ -689,6 +744,19 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     sp += per * sizeof(int);
     S.prev_u = (int *)sp;
 
+    VrSoA V;
+    char *vp = G.vr.as<char>();
+    double **vfields[] = {&V.theta, &V.phi, &V.omega, &V.pesc, &V.sar, &V.sat};
+    for (double **f : vfields) {
+      *f = (double *)vp;
+      vp += per * sizeof(double);
+    }
+    V.dist = (double *)vp;
+    vp += per * nl_bound * sizeof(double);
+    V.info = (int *)vp;
+    vp += per * sizeof(int);
+    V.blk = (unsigned int *)vp;
+
     SourceTable ST;
     ST.d = G.src_desc.as<SourceDesc>();
     ST.n = (int)src.size();
@@ -700,12 +768,12 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     int dev = 0, n_cu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-    int vr_blocks = n_cu * 8;
+
 
     HIP_TRY(hipEventCreate(&evs[0]));
     HIP_TRY(hipEventCreate(&evs[1]));
     if (g_profile)
-      for (int k = 0; k < 4 * ENG_MAX_ORDERS_PROFILED; k++)
+      for (int k = 0; k < ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED; k++)
         if (!prof_ev[k]) HIP_TRY(hipEventCreate(&prof_ev[k]));
     HIP_TRY(hipEventRecord(evs[0], st));
     /* batches of `cap` consecutive positions of the run order */
@@ -715,17 +783,21 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
         HIP_TRY(eng_launch_source(&P, &S, &ST, (uint64_t)done, count, st));
         int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
         for (int n = 1; n <= n_int; n++)
-          HIP_TRY(eng_launch_order(&P, &S, n, count, vr_blocks, st, prof ? &prof_ev[4 * (n - 1)] : nullptr));
+          HIP_TRY(eng_launch_order(&P, &S, &V, n, count, n_cu, st,
+                                   prof ? &prof_ev[ENG_ORDER_EVENTS * (n - 1)] : nullptr));
         if (prof) {
           HIP_TRY(hipStreamSynchronize(st));
           for (int n = 0; n < n_int; n++) {
-            float a = 0.f, b = 0.f, c = 0.f;
-            HIP_TRY(hipEventElapsedTime(&a, prof_ev[4 * n], prof_ev[4 * n + 1]));
-            HIP_TRY(hipEventElapsedTime(&b, prof_ev[4 * n + 1], prof_ev[4 * n + 2]));
-            HIP_TRY(hipEventElapsedTime(&c, prof_ev[4 * n + 2], prof_ev[4 * n + 3]));
-            g_stats.step_kernel_ms += a;
-            g_stats.vr_kernel_ms += b;
-            g_stats.interaction_kernel_ms += c;
+            float t[ENG_ORDER_EVENTS - 1];
+            const hipEvent_t *e = &prof_ev[ENG_ORDER_EVENTS * n];
+            for (int k = 0; k + 1 < ENG_ORDER_EVENTS; k++) HIP_TRY(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
+            g_stats.step_kernel_ms += t[0];
+            g_stats.vr_geom_ms += t[1];
+            g_stats.vr_cone_ms += t[2];
+            g_stats.vr_scatter_ms += t[3];
+            g_stats.vr_lines_ms += t[4];
+            g_stats.vr_kernel_ms += t[1] + t[2] + t[3] + t[4];
+            g_stats.interaction_kernel_ms += t[5];
           }
           g_stats.launches += n_int; /* launches of each kernel class */
         }
@@ -733,10 +805,16 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     }
     HIP_TRY(hipEventRecord(evs[1], st));
     std::vector<unsigned long long> th(tally_words);
+    std::vector<unsigned long long> cnt_sub((size_t)ENG_NCOUNTERS * ENG_SPREAD);
     unsigned long long cnt[ENG_NCOUNTERS];
     HIP_TRY(hipMemcpyAsync(th.data(), G.tallies.p, tally_words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(cnt, G.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cnt_sub.data(), G.counters.p, cnt_sub.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    for (int c = 0; c < ENG_NCOUNTERS; c++) {
+      cnt[c] = 0ull;
+      for (int j = 0; j < ENG_SPREAD; j++) cnt[c] += cnt_sub[(size_t)c * ENG_SPREAD + j];
+    }
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, evs[0], evs[1]));
     g_stats.transport_ms = ms;

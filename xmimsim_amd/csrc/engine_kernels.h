@@ -11,19 +11,29 @@
 #define ENG_LDS_BYTES 65536        /* LDS budget of the estimator's tallies */
 #define XMI_NLINES_VR_DEV 219
 
-/* counters[] slots */
+/* counters[] slots; every slot is ENG_SPREAD sub-counters and a wavefront
+ * adds to sub-counter (global wave index mod ENG_SPREAD), so that same-address
+ * atomics from all waves of a launch do not serialise on one location.  The
+ * queue slots (0..ENG_NQUEUES-1) are ENG_SPREAD sub-queues of P.qreg entries each. */
+#define ENG_SPREAD 64
 #define ENG_CNT_Q_RAYL 0
 #define ENG_CNT_Q_COMPT 1
 #define ENG_CNT_Q_PHOTO 2
-#define ENG_CNT_VR 3
-#define ENG_CNT_FALLBACKS 4
-#define ENG_CNT_ERRORS 5
-#define ENG_CNT_HISTORIES 6
+#define ENG_CNT_Q_SA 3              /* off-grid solid angles of the estimator */
+#define ENG_NQUEUES 4
+#define ENG_CNT_VR 4
+#define ENG_CNT_FALLBACKS 5
+#define ENG_CNT_ERRORS 6
+#define ENG_CNT_HISTORIES 7
 #define ENG_CNT_LAYER0 8            /* detection-weight events per layer */
 #define ENG_CNT_ERR_SOURCE (ENG_CNT_LAYER0 + ENG_MAX_LAYERS)
 #define ENG_CNT_ERR_VR_DOPPLER (ENG_CNT_ERR_SOURCE + 1)
 #define ENG_CNT_ERR_DOPPLER (ENG_CNT_ERR_SOURCE + 2)
 #define ENG_NCOUNTERS (ENG_CNT_ERR_SOURCE + 4)
+
+/* events recorded per order when profiling: start, after the step, after
+ * each of the four estimator kernels, after the interactions */
+#define ENG_ORDER_EVENTS 7
 
 /* m_e c^2 in keV as the reference computes it (src/xmi_main.F90:48-55) */
 #define ENG_MEC2 (9.10938188e-31 * 2.99792458e8 * 2.99792458e8 / 1.602176487e-19 / 1000.0)
@@ -61,7 +71,7 @@ hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv
                            long hits, uint64_t seed, hipStream_t st);
 hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,
                              int count, hipStream_t st);
-hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, int n, int count, int vr_blocks,
+hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
                             hipStream_t st, hipEvent_t *ev);
 #ifdef __cplusplus
 }

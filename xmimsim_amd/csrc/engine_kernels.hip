@@ -24,21 +24,34 @@
  * kernels on one code path.  All kernels are FP64 VALU work; nothing here is
  * a dense contraction, so no MFMA is used.
  */
+#include <stdlib.h>
+
 #include "engine_kernels.h"
 
 /* ------------------------------------------------------------------------ */
 /* physics helpers on the device */
 
-__device__ __forceinline__ double d_layer_mu_c(const EngineParams &P, int layer0, const xrl_ectx *c) {
-  double rv = 0.0;
-  int off = P.layer_off[layer0], n = P.layer_nel[layer0];
-  for (int i = 0; i < n; i++) rv += xrl_cs_total_c(&P.atoms[P.el_u[off + i]], c) * P.el_w[off + i];
-  return rv;
+/* mass attenuation of a layer, sum_i w_i CS_Total(Z_i, E) (src/xmi_aux_f.F90:1109-1141),
+ * from the per-layer tables: equal to the element sum up to rounding, since
+ * the layer sums are formed once on the host */
+__device__ __forceinline__ double d_layer_mu_tab(const EngineParams &P, int layer0, const xrl_ectx *c) {
+  if (!(c->E > 0.0)) return 0.0;
+  const double *T = P.lay_rc + (size_t)layer0 * XRL_NCS;
+  const int eo = P.lay_eoff[layer0], ne = P.lay_eoff[layer0 + 1] - eo;
+  int K = P.lay_kstart[(size_t)layer0 * XRL_NCS + c->ci];
+  while (K < ne && P.lay_edge[eo + K] > c->E) K++;
+  return c->pe * P.lay_pb[eo + layer0 + K] + (T[c->ci] + c->cf * (T[c->ci + 1] - T[c->ci]));
 }
 
 __device__ __forceinline__ void d_mu_all(const EngineParams &P, double E, double *mus, size_t stride, size_t idx) {
   xrl_ectx c = xrl_energy(E);
-  for (int l = 0; l < P.n_layers; l++) mus[(size_t)l * stride + idx] = d_layer_mu_c(P, l, &c);
+  for (int l = 0; l < P.n_layers; l++) mus[(size_t)l * stride + idx] = d_layer_mu_tab(P, l, &c);
+}
+
+/* this wavefront's sub-counter of a counter slot (1-D launches) */
+__device__ __forceinline__ unsigned long long *d_ctr(const EngineParams &P, int slot) {
+  const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  return &P.counters[(size_t)slot * ENG_SPREAD + (wave & (ENG_SPREAD - 1))];
 }
 
 /* wavefront-aggregated counter: call from converged code only */
@@ -200,44 +213,27 @@ __device__ __forceinline__ double d_single_solid_angle_impl(const SaGeom g, doub
   return c.cone_sa * (double)nhit / (double)hits;
 }
 
-/* Off-grid estimates inside the estimator, run by the whole wavefront: each
- * lane with need != 0 gets omega for (r1, theta1) from its own stream *s,
- * which must sit on a block boundary; afterwards *s has advanced by the
- * 2*hits uniforms the sequential loop would have drawn. */
-__device__ __forceinline__ void d_wave_solid_angle(const SaGeom g, int need, double r1, double theta1, long hits,
-                                                   phx_stream *s, double *omega) {
-  unsigned long long pending = __ballot(need);
-  const int lane = __lane_id();
-  while (pending) {
-    const int src = __ffsll((long long)pending) - 1;
-    pending &= pending - 1ull;
-    double r_l = __shfl(r1, src), t_l = __shfl(theta1, src);
-    phx_stream q;
-    q.key[0] = (uint32_t)__shfl((int)s->key[0], src);
-    q.key[1] = (uint32_t)__shfl((int)s->key[1], src);
-    for (int k = 0; k < 4; k++) q.ctr[k] = (uint32_t)__shfl((int)s->ctr[k], src);
-    const uint32_t first = q.ctr[0];
-    SaCone c;
-    int open = d_sa_setup(g, r_l, t_l, &c);
-    long nhit = 0;
-    if (open) {
-      for (long t = lane; t < hits; t += 64) {
-        uint32_t o[4];
-        uint32_t ctr[4] = {first + (uint32_t)t, q.ctr[1], q.ctr[2], q.ctr[3]};
-        phx_4x32_10(ctr, q.key, o);
-        nhit += d_sa_trial(c, phx_u01(o[0], o[1]), phx_u01(o[2], o[3]));
-      }
-    }
-    /* exact integer total over the wave */
-    for (int off = 32; off > 0; off >>= 1) nhit += __shfl_xor(nhit, off);
-    if (lane == src) {
-      *omega = open ? c.cone_sa * (double)nhit / (double)hits : 0.0;
-      if (open) {
-        s->ctr[0] = first + (uint32_t)hits;
-        s->avail = 0;
-      }
+/* One off-grid cone estimate run by a whole wavefront (all arguments
+ * wave-uniform): trial t uses Philox block `first + t` of the photon's
+ * estimator stream (key, ctr1..3), i.e. exactly the uniforms 2t and 2t+1 of
+ * the sequential loop that starts at block `first`, and the lanes' hit counts
+ * are summed as integers.  Returns the estimate; *open = 0 when the
+ * collimator hides the window (no draws). */
+__device__ __forceinline__ double d_cone_estimate_wave(const SaGeom g, double r1, double theta1, long hits,
+                                                       const uint32_t key[2], const uint32_t ctr[4], int *open) {
+  SaCone c;
+  *open = d_sa_setup(g, r1, theta1, &c);
+  long nhit = 0;
+  if (*open) {
+    for (long t = __lane_id(); t < hits; t += 64) {
+      uint32_t o[4];
+      uint32_t cc[4] = {ctr[0] + (uint32_t)t, ctr[1], ctr[2], ctr[3]};
+      phx_4x32_10(cc, key, o);
+      nhit += d_sa_trial(c, phx_u01(o[0], o[1]), phx_u01(o[2], o[3]));
     }
   }
+  for (int off = 32; off > 0; off >>= 1) nhit += __shfl_xor(nhit, off);
+  return *open ? c.cone_sa * (double)nhit / (double)hits : 0.0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -338,8 +334,8 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   double E, w, hor = 0.0;
   if (d.kind == 0) {
     if (!d_ran_trap(d.x1, d.x2, d.y1, d.y2, phx_uniform(&s), &E)) {
-      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERR_SOURCE), 1ull);
       return 0;
     }
     hor = d_interp(d.x1, d.h1, d.x2, d.h2, E) / d_interp(d.x1, d.y1, d.x2, d.y2, E);
@@ -388,8 +384,8 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   double ev[3] = {horizontal ? 0.0 : 1.0, horizontal ? 1.0 : 0.0, 0.0};
   double cosalfa = d_dot(ev, dirv);
   if (fabs(cosalfa) > 1.0) {
-    atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
+    atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERR_SOURCE), 1ull);
     return 0;
   }
   double c_alfa = acos(cosalfa);
@@ -406,15 +402,15 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
         break;
       }
     if (layer == 0) {
-      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERR_SOURCE), 1ull);
       return 0;
     }
   } else {
     double pp[3] = {0.0, 0.0, P.layer_zb[0]}, ip[3];
     if (!d_intersect(pp, P.n_sample, coords, dirv, ip)) {
-      atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
-      atomicAdd(&P.counters[ENG_CNT_ERR_SOURCE], 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+      atomicAdd(d_ctr(P, ENG_CNT_ERR_SOURCE), 1ull);
       return 0;
     }
     coords[0] = ip[0];
@@ -443,36 +439,56 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
                                                        uint64_t batch_start, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int alive = (i < count) ? d_source_one(P, S, T, batch_start, i) : 0;
-  d_wave_count(&P.counters[ENG_CNT_HISTORIES], alive);
+  d_wave_count(d_ctr(P, ENG_CNT_HISTORIES), alive);
 }
 
 /* ------------------------------------------------------------------------ */
 /* queues */
 
 __global__ void k_reset_queues(EngineParams P) {
-  if (threadIdx.x < 3) P.counters[threadIdx.x] = 0ull;
+  if (threadIdx.x < ENG_NQUEUES * ENG_SPREAD) P.counters[threadIdx.x] = 0ull;
 }
 
-/* wavefront-aggregated append: one atomic per wave and queue */
-__device__ __forceinline__ void d_enqueue(const EngineParams &P, int type, int idx) {
-  for (int t = 0; t < 3; t++) {
-    unsigned long long mask = __ballot(type == t);
-    if (mask == 0ull) continue;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    unsigned int base = 0;
-    if (lane == leader) base = (unsigned int)atomicAdd(&P.counters[t], (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    if (type == t) {
-      unsigned long long below = mask & ((1ull << lane) - 1ull);
-      P.queue[(size_t)t * P.cap + base + __popcll(below)] = idx;
-    }
+/* wavefront-aggregated append of the lanes with pred to queue t, in this
+ * wave's sub-queue: one atomic per wave, spread over ENG_SPREAD counters.  A
+ * sub-queue receives the photons of every ENG_SPREAD-th wave of a launch with
+ * one thread per photon slot, so qreg = ceil(waves / ENG_SPREAD) * 64 entries
+ * cannot overflow.  Call from converged code. */
+__device__ __forceinline__ void d_append(const EngineParams &P, int t, int pred, int idx) {
+  const unsigned int j = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & (ENG_SPREAD - 1);
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  int lane = __lane_id();
+  int leader = __ffsll((long long)mask) - 1;
+  unsigned int base = 0;
+  if (lane == leader)
+    base = (unsigned int)atomicAdd(&P.counters[(size_t)t * ENG_SPREAD + j], (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (pred) {
+    unsigned long long below = mask & ((1ull << lane) - 1ull);
+    P.queue[((size_t)t * ENG_SPREAD + j) * P.qreg + base + __popcll(below)] = idx;
   }
+}
+
+/* the interaction-type queues (Rayleigh, Compton, photo) */
+__device__ __forceinline__ void d_enqueue(const EngineParams &P, int type, int idx) {
+  for (int t = 0; t < 3; t++) d_append(P, t, type == t, idx);
+}
+
+/* photon of queue entry q (0 <= q < ENG_SPREAD * qreg) of type t, or -1 */
+__device__ __forceinline__ int d_dequeue(const EngineParams &P, int t, unsigned int q) {
+  const unsigned int j = q / (unsigned int)P.qreg, r = q - j * (unsigned int)P.qreg;
+  if (j >= ENG_SPREAD || r >= P.counters[(size_t)t * ENG_SPREAD + j]) return -1;
+  return P.queue[((size_t)t * ENG_SPREAD + j) * P.qreg + r];
 }
 
 /* ------------------------------------------------------------------------ */
 /* forced interaction + atom and interaction-type selection */
 
+/* NL bounds the layer count; path lengths and mu*rho are kept in crossing
+ * order (k = 0: the photon's layer) so that register arrays are only indexed
+ * by compile-time indices */
+template <int NL>
 __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S, int n, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int type = -1;
@@ -488,39 +504,57 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
       double dirv[3] = {S.dx[i], S.dy[i], S.dz[i]};
       double coords[3] = {S.x[i], S.y[i], S.z[i]};
       double w = S.weight[i];
-      int cur = S.layer[i];
-      int up = d_dot(dirv, P.n_sample) > 0.0;
-      int last = up ? P.n_layers : 1, dir = up ? 1 : -1;
-      double dist[ENG_MAX_LAYERS];
+      const int cur = S.layer[i];
+      const int up = d_dot(dirv, P.n_sample) > 0.0;
+      const int dir = up ? 1 : -1;
+      const int nc = up ? P.n_layers - cur + 1 : cur; /* layers up to the stack's edge */
+      double dist[NL], md[NL];
       double lp[3] = {coords[0], coords[1], coords[2]};
-      for (int l = cur; up ? l <= last : l >= last; l += dir) {
-        double pp[3] = {0.0, 0.0, up ? P.layer_ze[l - 1] : P.layer_zb[l - 1]}, ip[3];
-        d_intersect(pp, P.n_sample, lp, dirv, ip);
-        dist[l - 1] = d_distance(lp, ip);
-        lp[0] = ip[0];
-        lp[1] = ip[1];
-        lp[2] = ip[2];
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        dist[k] = 0.0;
+        md[k] = 0.0;
+        if (k < nc) {
+          const int l0 = cur - 1 + k * dir;
+          double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]}, ip[3];
+          d_intersect(pp, P.n_sample, lp, dirv, ip);
+          dist[k] = d_distance(lp, ip);
+          lp[0] = ip[0];
+          lp[1] = ip[1];
+          lp[2] = ip[2];
+          md[k] = S.mus[(size_t)l0 * cap + i] * P.layer_density[l0];
+        }
       }
       double Pabs = 0.0;
-      for (int l = cur; up ? l <= last : l >= last; l += dir)
-        Pabs += S.mus[(size_t)(l - 1) * cap + i] * P.layer_density[l - 1] * dist[l - 1];
+#pragma unroll
+      for (int k = 0; k < NL; k++)
+        if (k < nc) Pabs += md[k] * dist[k];
       double Pabs2 = -1.0 * expm1(-1.0 * Pabs);
       w *= Pabs2;
       double negln = -1.0 * log1p(-1.0 * R * Pabs2);
-      int my = cur;
+      /* layer of the interaction: first where the optical depth exceeds negln */
+      int kmy = 0, found = 0;
       double sum = 0.0;
-      for (int l = cur; up ? l <= last : l >= last; l += dir) {
-        sum += S.mus[(size_t)(l - 1) * cap + i] * P.layer_density[l - 1] * dist[l - 1];
-        if (sum > negln) {
-          my = l;
-          break;
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        if (k < nc && !found) {
+          sum += md[k] * dist[k];
+          if (sum > negln) {
+            kmy = k;
+            found = 1;
+          }
         }
       }
-      double mrm = S.mus[(size_t)(my - 1) * cap + i] * P.layer_density[my - 1];
+      double mrm = md[0];
+#pragma unroll
+      for (int k = 1; k < NL; k++)
+        if (k == kmy) mrm = md[k];
       double tsum = 0.0;
-      for (int l = cur; up ? l <= my : l >= my; l += dir)
-        tsum += (1.0 - (S.mus[(size_t)(l - 1) * cap + i] * P.layer_density[l - 1] / mrm)) * dist[l - 1];
+#pragma unroll
+      for (int k = 0; k < NL; k++)
+        if (k <= kmy) tsum += (1.0 - (md[k] / mrm)) * dist[k];
       tsum = tsum - 1.0 * log1p(-1.0 * R * Pabs2) / mrm;
+      const int my = cur + kmy * dir;
       S.x[i] = coords[0] + tsum * dirv[0];
       S.y[i] = coords[1] + tsum * dirv[1];
       S.z[i] = coords[2] + tsum * dirv[2];
@@ -567,21 +601,26 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
 
 /* ------------------------------------------------------------------------ */
 /* detection-weight estimator of every photon at order n
- * (src/xmi_variance_reduction.F90:29-726, xmi_compton_varred2 :949-1101).
+ * (src/xmi_variance_reduction.F90:29-726, xmi_compton_varred2 :949-1101),
+ * as four kernels with their own register and LDS budgets:
  *
- * Phase A, one lane per photon: detector point, angles, escape path through
- * the layers, solid angle, then the Rayleigh and Compton terms of every
- * element of the photon's layer.  These draw from the photon's estimator
- * stream in element order, so they stay sequential per photon.
- * Phase B, one lane per (element, fluorescence line) entry of a layer: the
- * wavefront walks its photons one at a time and reads each photon's scalars
- * from the lane that owns it (readlane), so the escape attenuation and tally
- * add of every line is a lane of its own and needs no cross-lane sum.
+ *   k_vr_geom     one lane per photon: detector point, angles, path through
+ *                 the layers to the detector, escape probability, solid angle
+ *                 from the grid; off-grid points go to a queue
+ *   k_vr_cone     one wavefront per queued photon: the cone estimate of
+ *                 src/xmi_solid_angle_f.F90:432-710 with the trials split over
+ *                 the lanes (same draws, same hit count as the serial loop)
+ *   k_vr_scatter  one lane per photon: Rayleigh and Compton terms of every
+ *                 element of the photon's layer; the Compton Doppler draws
+ *                 continue the photon's estimator stream in element order
+ *   k_vr_lines    one lane per (element, fluorescence line) entry of a layer;
+ *                 the wavefront walks its photons one at a time, reading each
+ *                 photon's scalars from the lane that loaded it (readlane)
  *
- * Register arrays are indexed only by compile-time indices (NL = layer-count
- * bound); per-photon path lengths are kept in crossing order.  Tallies are
- * 128-bit fixed point, privatised in LDS when they fit; integer adds commute,
- * so the totals do not depend on the order of the adds. */
+ * Path lengths are kept in crossing order (k = 0: the photon's own layer) so
+ * that register arrays have compile-time indices only (NL bounds the layer
+ * count).  Tallies are 128-bit fixed point, privatised in LDS when they fit;
+ * integer adds commute, so totals do not depend on the order of the adds. */
 
 __device__ __forceinline__ double d_wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -600,169 +639,208 @@ __device__ __forceinline__ double d_readlane(double v, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+#define VR_ACTIVE 1
+#define VR_UP 2
+#define VR_NC_SHIFT 8
+
+template <int NL>
+__global__ void __launch_bounds__(ENG_BLOCK) k_vr_geom(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cap = (size_t)P.cap;
+  int active = (i < count) && S.layer[i] > 0 && S.energy[i] > 1.0;
+  int cur = 0, need_sa = 0, info = 0;
+  if (active) {
+    phx_stream s;
+    phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
+    cur = S.layer[i];
+    double radius = sqrt(phx_uniform(&s)) * P.det_radius;
+    double th = 2.0 * ENG_PI * phx_uniform(&s);
+    double dp[3] = {0.0, cos(th) * radius, sin(th) * radius};
+    double coords[3] = {S.x[i], S.y[i], S.z[i]};
+    double dirv_l[3] = {S.dx[i], S.dy[i], S.dz[i]};
+    double rel[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
+    double lpnt[3], dirv[3], ld[3];
+    d_matvec(P.det_inv, rel, lpnt);
+    d_matvec(P.det_inv, dirv_l, dirv);
+    for (int k = 0; k < 3; k++) ld[k] = dp[k] - lpnt[k];
+    if (ld[0] >= 0.0) {
+      active = 0;
+    } else {
+      double total_distance = d_distance(dp, lpnt);
+      d_normalize(dirv);
+      d_normalize(ld);
+      double nd[3];
+      d_matvec(P.det_new, ld, nd);
+      double c = d_dot(dirv, ld);
+      c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+      const double theta = acos(c);
+      double proj[3], ndd = d_dot(nd, dirv_l);
+      for (int k = 0; k < 3; k++) proj[k] = nd[k] - ndd * dirv_l[k];
+      d_normalize(proj);
+      double ev[3] = {S.ex[i], S.ey[i], S.ez[i]};
+      double en = d_norm(ev);
+      double evn[3] = {ev[0] / en, ev[1] / en, ev[2] / en};
+      c = d_dot(proj, evn);
+      c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+      const double phi = acos(c);
+      const int up = d_dot(nd, P.n_sample) > 0.0;
+      const int dir = up ? 1 : -1;
+      const int ncross = up ? P.n_layers - cur + 1 : cur;
+      /* path to the detector through the layers, in crossing order; the
+       * layer holding the detector point ends the walk */
+      double tc[3] = {coords[0], coords[1], coords[2]};
+      int done = 0, nc = 0;
+      double murhod = 0.0;
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        double dd = 0.0;
+        if (k < ncross && !done) {
+          const int l0 = cur - 1 + k * dir;
+          double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]}, ip[3];
+          d_intersect(pp, P.n_sample, tc, nd, ip);
+          dd = d_distance(tc, ip);
+          if (dd > total_distance) {
+            dd = total_distance;
+            done = 1;
+          } else {
+            tc[0] = ip[0];
+            tc[1] = ip[1];
+            tc[2] = ip[2];
+            total_distance -= dd;
+          }
+          murhod += S.mus[(size_t)l0 * cap + i] * P.layer_density[l0] * dd;
+          nc = k + 1;
+        }
+        V.dist[(size_t)k * cap + i] = dd;
+      }
+      /* solid angle, src/xmi_solid_angle_f.F90:712-801 */
+      double omega = 0.0;
+      double r = d_distance(P.p_det, coords);
+      double dv[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
+      d_normalize(dv);
+      double tt = acos(d_dot(dv, P.n_det));
+      if (tt > ENG_PI / 2.0) tt = ENG_PI - tt;
+      double sth = ENG_PI / 2.0 - tt;
+      if (sth >= P.sa_t[0]) {
+        int p1 = d_findpos_uniform(P.sa_r, P.sa_nr, P.sa_r0, P.sa_rinv, r);
+        int p2 = d_findpos_uniform(P.sa_t, P.sa_nt, P.sa_t0, P.sa_tinv, sth);
+        if (p1 < 0 || p2 < 0) {
+          need_sa = 1;
+          V.sar[i] = r;
+          V.sat[i] = sth;
+        } else {
+          const double *a1 = P.sa_r, *a2 = P.sa_t;
+          double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
+          double c1 = (a1[p1 + 1] - r) * (a2[p2 + 1] - sth) / denom;
+          double c2 = (r - a1[p1]) * (a2[p2 + 1] - sth) / denom;
+          double c3 = (a1[p1 + 1] - r) * (sth - a2[p2]) / denom;
+          double c4 = (r - a1[p1]) * (sth - a2[p2]) / denom;
+          const double *g0 = P.sa_grid + (size_t)p2 * P.sa_nr + p1;
+          const double *g1 = P.sa_grid + (size_t)(p2 + 1) * P.sa_nr + p1;
+          omega = c1 * g0[0] + c2 * g0[1] + c3 * g1[0] + c4 * g1[1];
+        }
+      }
+      V.theta[i] = theta;
+      V.phi[i] = phi;
+      V.omega[i] = omega;
+      V.pesc[i] = exp(-murhod);
+      V.blk[i] = s.ctr[0]; /* two uniforms drawn: block 1, nothing buffered */
+      info = VR_ACTIVE | (up ? VR_UP : 0) | (nc << VR_NC_SHIFT);
+    }
+  }
+  if (i < count) V.info[i] = info;
+  d_append(P, ENG_CNT_Q_SA, need_sa, i);
+  d_wave_count(d_ctr(P, ENG_CNT_FALLBACKS), need_sa);
+  d_wave_count(d_ctr(P, ENG_CNT_VR), active);
+  for (int L = 1; L <= P.n_layers; L++) d_wave_count(d_ctr(P, ENG_CNT_LAYER0 + L - 1), active && cur == L);
+}
+
+/* persistent: every wavefront takes queued photons w, w + W, ... of the
+ * off-grid queue (entries flattened over the ENG_SPREAD sub-queues) */
+__global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA S, VrSoA V, int n) {
+  const int lane = __lane_id();
+  const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned int waves = (gridDim.x * blockDim.x) >> 6;
+  /* sub-queue sizes, one per lane, and their inclusive prefix sums */
+  const unsigned long long sz = P.counters[(size_t)ENG_CNT_Q_SA * ENG_SPREAD + lane];
+  unsigned long long incl = sz;
+  for (int off = 1; off < 64; off <<= 1) {
+    unsigned long long v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  const unsigned long long total = __shfl(incl, 63);
+  const SaGeom g = d_sa_geom(P);
+  for (unsigned long long e = wave; e < total; e += waves) {
+    /* sub-queue holding entry e: first lane whose inclusive prefix exceeds e */
+    const int j = __ffsll((long long)__ballot(incl > e)) - 1;
+    const unsigned long long before = __shfl(incl - sz, j);
+    const int i = P.queue[((size_t)ENG_CNT_Q_SA * ENG_SPREAD + j) * P.qreg + (e - before)];
+    phx_stream s;
+    phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
+    s.ctr[0] = V.blk[i];
+    int open = 0;
+    const double om = d_cone_estimate_wave(g, V.sar[i], V.sat[i], P.hits_per_single, s.key, s.ctr, &open);
+    if (lane == 0) {
+      V.omega[i] = om;
+      if (open) V.blk[i] = s.ctr[0] + (unsigned int)P.hits_per_single;
+    }
+  }
+}
+
 template <int NL, bool use_lds>
-__global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA S, int n, int count) {
-  /* LDS tallies (when they fit): channels [nch][2], compact line bins
-   * [n_lbins][2], Rayleigh/Compton rows [nZ][2][2]; flushed once per workgroup */
+__global__ void __launch_bounds__(ENG_BLOCK) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
+  /* LDS tallies (when they fit): channels [nch][2], Rayleigh/Compton rows
+   * [nZ][2][2]; flushed once per workgroup */
   extern __shared__ unsigned long long lds[];
-  const size_t n_ch2 = (size_t)P.nch * 2, n_line2 = (size_t)P.n_lbins * 2, n_rc2 = (size_t)P.nZ * 4;
+  const size_t n_ch2 = (size_t)P.nch * 2, n_rc2 = (size_t)P.nZ * 4;
   const size_t cap = (size_t)P.cap;
   unsigned long long *gch = P.t_ch + (size_t)(n - 1) * n_ch2;
-  unsigned long long *gline = P.t_line + (size_t)(n - 1) * P.nZ * ENG_MAX_ACT * 2;
   unsigned long long *grc = P.t_rc + (size_t)(n - 1) * n_rc2;
-  unsigned long long *tch = gch, *tline = gline, *trc = grc;
+  unsigned long long *tch = gch, *trc = grc;
   if (use_lds) {
     tch = lds;
-    tline = lds + n_ch2;
-    trc = lds + n_ch2 + n_line2;
-    for (size_t k = threadIdx.x; k < n_ch2 + n_line2 + n_rc2; k += blockDim.x) lds[k] = 0ull;
+    trc = lds + n_ch2;
+    for (size_t k = threadIdx.x; k < n_ch2 + n_rc2; k += blockDim.x) lds[k] = 0ull;
     __syncthreads();
   }
   const int shift = P.tally_shift;
-  const int lane = __lane_id();
-  const int smax = P.use_M_lines ? 8 : 3;
-
-  /* grid-stride over photons; every lane runs the same trip count */
   const int stride = gridDim.x * blockDim.x;
   for (int base = blockIdx.x * blockDim.x; base < count; base += stride) {
     const int i = base + threadIdx.x;
-    int active = (i < count) && S.layer[i] > 0 && S.energy[i] > 1.0;
-    double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, Pesc_rayl = 0.0, mu_cur = 1.0, pe = 0.0;
-    double dist[NL]; /* escape path length in the k-th layer crossed (k = 0: own layer) */
-#pragma unroll
-    for (int k = 0; k < NL; k++) dist[k] = 0.0;
-    int cur = 0, dir = 1, nc = 0, prev_line = 0, prev_u = 0;
-    int need_sa = 0;
-    double sa_r = 0.0, sa_t = 0.0;
-    phx_stream s = {};
-    if (active) {
-      phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
-      E = S.energy[i];
-      w = S.weight[i];
-      cur = S.layer[i];
-      double radius = sqrt(phx_uniform(&s)) * P.det_radius;
-      double th = 2.0 * ENG_PI * phx_uniform(&s);
-      double dp[3] = {0.0, cos(th) * radius, sin(th) * radius};
-      double coords[3] = {S.x[i], S.y[i], S.z[i]};
-      double dirv_l[3] = {S.dx[i], S.dy[i], S.dz[i]};
-      double rel[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
-      double lpnt[3], dirv[3], ld[3];
-      d_matvec(P.det_inv, rel, lpnt);
-      d_matvec(P.det_inv, dirv_l, dirv);
-      for (int k = 0; k < 3; k++) ld[k] = dp[k] - lpnt[k];
-      if (ld[0] >= 0.0) {
-        active = 0;
-      } else {
-        double total_distance = d_distance(dp, lpnt);
-        d_normalize(dirv);
-        d_normalize(ld);
-        double nd[3];
-        d_matvec(P.det_new, ld, nd);
-        double c = d_dot(dirv, ld);
-        c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-        theta = acos(c);
-        double proj[3], ndd = d_dot(nd, dirv_l);
-        for (int k = 0; k < 3; k++) proj[k] = nd[k] - ndd * dirv_l[k];
-        d_normalize(proj);
-        double ev[3] = {S.ex[i], S.ey[i], S.ez[i]};
-        double en = d_norm(ev);
-        double evn[3] = {ev[0] / en, ev[1] / en, ev[2] / en};
-        c = d_dot(proj, evn);
-        c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-        phi = acos(c);
-        const int up = d_dot(nd, P.n_sample) > 0.0;
-        dir = up ? 1 : -1;
-        const int ncross = up ? P.n_layers - cur + 1 : cur;
-        /* path to the detector through the layers, in crossing order; the
-         * layer holding the detector point ends the walk */
-        double tc[3] = {coords[0], coords[1], coords[2]};
-        int done = 0;
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-          if (k < ncross && !done) {
-            const int l0 = cur - 1 + k * dir;
-            double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]}, ip[3];
-            d_intersect(pp, P.n_sample, tc, nd, ip);
-            double dd = d_distance(tc, ip);
-            if (dd > total_distance) {
-              dd = total_distance;
-              done = 1;
-            } else {
-              tc[0] = ip[0];
-              tc[1] = ip[1];
-              tc[2] = ip[2];
-              total_distance -= dd;
-            }
-            dist[k] = dd;
-            nc = k + 1;
-          }
-        }
-        double murhod = 0.0;
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-          if (k < nc) {
-            const int l0 = cur - 1 + k * dir;
-            murhod += S.mus[(size_t)l0 * cap + i] * P.layer_density[l0] * dist[k];
-          }
-        }
-        Pesc_rayl = exp(-murhod);
-        /* solid angle, src/xmi_solid_angle_f.F90:712-801 */
-        double r = d_distance(P.p_det, coords);
-        double dv[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
-        d_normalize(dv);
-        double tt = acos(d_dot(dv, P.n_det));
-        if (tt > ENG_PI / 2.0) tt = ENG_PI - tt;
-        double sth = ENG_PI / 2.0 - tt;
-        if (sth < P.sa_t[0]) {
-          omega = 0.0;
-        } else {
-          int p1 = d_findpos_uniform(P.sa_r, P.sa_nr, P.sa_r0, P.sa_rinv, r);
-          int p2 = d_findpos_uniform(P.sa_t, P.sa_nt, P.sa_t0, P.sa_tinv, sth);
-          if (p1 < 0 || p2 < 0) {
-            need_sa = 1;
-            sa_r = r;
-            sa_t = sth;
-          } else {
-            const double *a1 = P.sa_r, *a2 = P.sa_t;
-            double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
-            double c1 = (a1[p1 + 1] - r) * (a2[p2 + 1] - sth) / denom;
-            double c2 = (r - a1[p1]) * (a2[p2 + 1] - sth) / denom;
-            double c3 = (a1[p1 + 1] - r) * (sth - a2[p2]) / denom;
-            double c4 = (r - a1[p1]) * (sth - a2[p2]) / denom;
-            const double *g0 = P.sa_grid + (size_t)p2 * P.sa_nr + p1;
-            const double *g1 = P.sa_grid + (size_t)(p2 + 1) * P.sa_nr + p1;
-            omega = c1 * g0[0] + c2 * g0[1] + c3 * g1[0] + c4 * g1[1];
-          }
-        }
-        mu_cur = S.mus[(size_t)(cur - 1) * cap + i];
-        prev_line = S.prev_line[i];
-        prev_u = S.prev_u[i];
-        pe = xrl_energy(E).pe;
-      }
-    }
-    /* off-grid points: the whole wavefront runs each cone estimate */
-    if (__ballot(need_sa)) {
-      d_wave_solid_angle(d_sa_geom(P), need_sa, sa_r, sa_t, P.hits_per_single, &s, &omega);
-      d_wave_count(&P.counters[ENG_CNT_FALLBACKS], need_sa);
-    }
-    const double Pdir_fluo = omega / 4.0 / ENG_PI;
-    d_wave_count(&P.counters[ENG_CNT_VR], active);
-    for (int L = 1; L <= P.n_layers; L++) d_wave_count(&P.counters[ENG_CNT_LAYER0 + L - 1], active && cur == L);
-
-    /* phase A: scattering terms, element loop uniform over the wave per layer */
+    const int info = i < count ? V.info[i] : 0;
+    const int active = info & VR_ACTIVE;
+    const int cur = active ? S.layer[i] : 0;
     for (int L = 1; L <= P.n_layers; L++) {
       const int mine = active && cur == L;
       if (__ballot(mine) == 0ull) continue;
+      double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, pesc = 0.0, mu_cur = 1.0;
+      double dist[NL];
+      int dir = 1, nc = 0;
+      phx_stream s = {};
+      if (mine) {
+        E = S.energy[i];
+        w = S.weight[i];
+        theta = V.theta[i];
+        phi = V.phi[i];
+        omega = V.omega[i];
+        pesc = V.pesc[i];
+        mu_cur = S.mus[(size_t)(L - 1) * cap + i];
+        dir = (info & VR_UP) ? 1 : -1;
+        nc = info >> VR_NC_SHIFT;
+        phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
+        s.ctr[0] = V.blk[i];
+      }
+#pragma unroll
+      for (int k = 0; k < NL; k++) dist[k] = mine ? V.dist[(size_t)k * cap + i] : 0.0;
       const int off = P.layer_off[L - 1], nel = P.layer_nel[L - 1];
       for (int k = 0; k < nel; k++) {
         const int u = P.el_u[off + k];
         const xrl_atom *a = &P.atoms[u];
-        const double wi = P.el_w[off + k];
-        const double Pconv = wi / mu_cur;
+        const double Pconv = P.el_w[off + k] / mu_cur;
         /* Rayleigh */
         double twr = 0.0;
         if (mine) {
-          twr = Pconv * (omega * xrl_dcsp_rayl(a, E, theta, phi)) * Pesc_rayl * w;
+          twr = Pconv * (omega * xrl_dcsp_rayl(a, E, theta, phi)) * pesc * w;
           int ch = (E >= 1.0) ? (int)((E - P.zero) / P.gain) : -1;
           if (ch >= 0 && ch <= P.nch - 1 && twr > 0.0) fx_add_global(&tch[2 * ch], twr, shift);
         }
@@ -801,8 +879,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
 #pragma unroll
             for (int kk = 0; kk < NL; kk++) {
               if (kk < nc) {
-                const int l0 = cur - 1 + kk * dir;
-                m2 += d_layer_mu_c(P, l0, &ce) * P.layer_density[l0] * dist[kk];
+                const int l0 = L - 1 + kk * dir;
+                m2 += d_layer_mu_tab(P, l0, &ce) * P.layer_density[l0] * dist[kk];
               }
             }
             double Pesc_comp = exp(-m2);
@@ -810,15 +888,60 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
             int ch = (energy >= 1.0) ? (int)((energy - P.zero) / P.gain) : -1;
             if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) fx_add_global(&tch[2 * ch], twc, shift);
           } else {
-            atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
-            atomicAdd(&P.counters[ENG_CNT_ERR_VR_DOPPLER], 1ull);
+            atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+            atomicAdd(d_ctr(P, ENG_CNT_ERR_VR_DOPPLER), 1ull);
           }
         }
         d_tally_wave(&trc[((size_t)u * 2 + 1) * 2], twc, shift);
       }
     }
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (size_t k = threadIdx.x; k < n_ch2 / 2; k += blockDim.x)
+      if (tch[2 * k] | tch[2 * k + 1]) fx_merge_global(&gch[2 * k], tch[2 * k], tch[2 * k + 1]);
+    for (size_t k = threadIdx.x; k < n_rc2 / 2; k += blockDim.x)
+      if (trc[2 * k] | trc[2 * k + 1]) fx_merge_global(&grc[2 * k], trc[2 * k], trc[2 * k + 1]);
+  }
+}
 
-    /* phase B: fluorescence lines, one lane per (element, line) entry */
+template <int NL, bool use_lds>
+__global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
+  /* LDS tallies (when they fit): compact line bins [n_lbins][2] */
+  extern __shared__ unsigned long long lds[];
+  const size_t n_line2 = (size_t)P.n_lbins * 2;
+  const size_t cap = (size_t)P.cap;
+  unsigned long long *gline = P.t_line + (size_t)(n - 1) * P.nZ * ENG_MAX_ACT * 2;
+  unsigned long long *tline = gline;
+  if (use_lds) {
+    tline = lds;
+    for (size_t k = threadIdx.x; k < n_line2; k += blockDim.x) lds[k] = 0ull;
+    __syncthreads();
+  }
+  const int shift = P.tally_shift;
+  const int lane = __lane_id();
+  const int smax = P.use_M_lines ? 8 : 3;
+  const int stride = gridDim.x * blockDim.x;
+  for (int base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    /* this lane's photon: the wavefront's 64 photons are walked below */
+    const int i = base + threadIdx.x;
+    const int info = i < count ? V.info[i] : 0;
+    const int active = info & VR_ACTIVE;
+    int cur = 0, dir = 1, prev_line = 0, prev_u = 0;
+    double E = 0.0, pe = 0.0, gline = 0.0;
+    double dist[NL];
+    if (active) {
+      cur = S.layer[i];
+      dir = (info & VR_UP) ? 1 : -1;
+      E = S.energy[i];
+      pe = xrl_energy(E).pe;
+      /* P_dir * weight / mu(E): the per-photon factor of every line term */
+      gline = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(cur - 1) * cap + i];
+      prev_line = S.prev_line[i];
+      prev_u = S.prev_u[i];
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) dist[k] = active ? V.dist[(size_t)k * cap + i] : 0.0;
     for (int L = 1; L <= P.n_layers; L++) {
       const unsigned long long m = __ballot(active && cur == L);
       if (m == 0ull) continue;
@@ -826,9 +949,9 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
       for (int tb = 0; tb < T; tb += 64) {
         const int ok = tb + lane < T;
         const int t = ok ? t0 + tb + lane : t0;
-        const int4 info = P.fl_info[t];
-        const double2 wf = P.fl_wf[t];
-        const xrl_atom *a = &P.atoms[info.x];
+        const int4 info_t = P.fl_info[t];
+        const double wf = P.fl_wf[t];
+        const xrl_atom *a = &P.atoms[info_t.x];
         double edge[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) edge[q] = a->edge[q];
@@ -842,13 +965,15 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           mr_dn[kk] = ld >= 0 ? P.fl_mr[(size_t)t * P.n_layers + ld] : 0.0;
         }
         unsigned long long *bin =
-            use_lds ? &tline[(size_t)info.w * 2] : &tline[((size_t)info.x * ENG_MAX_ACT + info.y) * 2];
+            use_lds ? &tline[(size_t)info_t.w * 2] : &tline[((size_t)info_t.x * ENG_MAX_ACT + info_t.y) * 2];
+        /* this lane's line summed over the wave's photons (fixed lane order,
+         * so deterministic), then one fixed-point add per wavefront */
+        double acc = 0.0;
         unsigned long long mm = m;
         while (mm) {
           const int p = __ffsll((long long)mm) - 1;
           mm &= mm - 1ull;
-          const double Ep = d_readlane(E, p), pep = d_readlane(pe, p), mup = d_readlane(mu_cur, p);
-          const double pdp = d_readlane(Pdir_fluo, p), wp = d_readlane(w, p);
+          const double Ep = d_readlane(E, p), pep = d_readlane(pe, p), gp = d_readlane(gline, p);
           const int dirp = __builtin_amdgcn_readlane(dir, p);
           const int plp = __builtin_amdgcn_readlane(prev_line, p), pup = __builtin_amdgcn_readlane(prev_u, p);
           double dk[NL];
@@ -857,36 +982,31 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
           if (!ok) continue;
           double Ps;
           if (n > 1 && plp > 0) {
-            Ps = info.z <= smax
-                     ? P.precalc_xrf[(((size_t)info.x * 9 + info.z) * P.nZ + pup) * XMI_NLINES_VR_DEV + (plp - 1)]
+            Ps = info_t.z <= smax
+                     ? P.precalc_xrf[(((size_t)info_t.x * 9 + info_t.z) * P.nZ + pup) * XMI_NLINES_VR_DEV + (plp - 1)]
                      : 0.0;
           } else {
             int kE = 0;
 #pragma unroll
             for (int q = 0; q < 9; q++) kE += edge[q] > Ep ? 1 : 0;
-            Ps = (!P.use_M_lines && info.z >= 4) ? 0.0 : pep * shC[kE * 9 + info.z];
+            Ps = (!P.use_M_lines && info_t.z >= 4) ? 0.0 : pep * shC[kE * 9 + info_t.z];
           }
           if (Ps != 0.0) {
-            const double Pc = wf.x * Ps * wf.y / mup;
             double m2 = 0.0;
 #pragma unroll
             for (int kk = 0; kk < NL; kk++) m2 += (dirp > 0 ? mr_up[kk] : mr_dn[kk]) * dk[kk];
-            const double twf = Pc * pdp * exp(-m2) * wp;
-            fx_add_global(bin, twf, shift);
+            acc += wf * Ps * gp * exp(-m2);
           }
         }
+        if (ok && acc > 0.0) fx_add_global(bin, acc, shift);
       }
     }
   }
   if (use_lds) {
     __syncthreads();
-    for (size_t k = threadIdx.x; k < n_ch2 / 2; k += blockDim.x)
-      if (tch[2 * k] | tch[2 * k + 1]) fx_merge_global(&gch[2 * k], tch[2 * k], tch[2 * k + 1]);
     for (size_t k = threadIdx.x; k < n_line2 / 2; k += blockDim.x)
       if (tline[2 * k] | tline[2 * k + 1])
         fx_merge_global(&gline[(size_t)P.lbin_g[k] * 2], tline[2 * k], tline[2 * k + 1]);
-    for (size_t k = threadIdx.x; k < n_rc2 / 2; k += blockDim.x)
-      if (trc[2 * k] | trc[2 * k + 1]) fx_merge_global(&grc[2 * k], trc[2 * k], trc[2 * k + 1]);
   }
 }
 
@@ -894,9 +1014,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_varred(EngineParams P, PhotonSoA 
 /* interactions */
 
 __global__ void __launch_bounds__(ENG_BLOCK) k_rayleigh(EngineParams P, PhotonSoA S, int n) {
-  unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P.counters[0]) return;
-  int i = P.queue[q];
+  const int i = d_dequeue(P, 0, blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < 0) return;
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -932,9 +1051,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_rayleigh(EngineParams P, PhotonSo
 }
 
 __global__ void __launch_bounds__(ENG_BLOCK) k_compton(EngineParams P, PhotonSoA S, int n) {
-  unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P.counters[1]) return;
-  int i = P.queue[(size_t)P.cap + q];
+  const int i = d_dequeue(P, 1, blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < 0) return;
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -978,8 +1096,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_compton(EngineParams P, PhotonSoA
     }
   }
   if (!accepted) {
-    atomicAdd(&P.counters[ENG_CNT_ERRORS], 1ull);
-    atomicAdd(&P.counters[ENG_CNT_ERR_DOPPLER], 1ull);
+    atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+    atomicAdd(d_ctr(P, ENG_CNT_ERR_DOPPLER), 1ull);
     S.energy[i] = 0.0;
     return;
   }
@@ -1086,9 +1204,8 @@ __device__ int d_coster_kronig(const xrl_atom *a, phx_stream *s, int shell) {
 }
 
 __global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S, int n) {
-  unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P.counters[2]) return;
-  int i = P.queue[(size_t)2 * P.cap + q];
+  const int i = d_dequeue(P, 2, blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < 0) return;
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -1159,26 +1276,69 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S
 /* ------------------------------------------------------------------------ */
 /* host-side launch wrappers */
 
-size_t eng_varred_lds_bytes(const EngineParams *P) {
-  size_t need = ((size_t)P->nch * 2 + (size_t)P->n_lbins * 2 + (size_t)P->nZ * 4) * sizeof(unsigned long long);
-  return need <= ENG_LDS_BYTES ? need : 0;
+static size_t lds_if_fits(size_t words) {
+  size_t b = words * sizeof(unsigned long long);
+  return b <= ENG_LDS_BYTES ? b : 0;
+}
+static size_t scatter_lds_bytes(const EngineParams *P) { return lds_if_fits((size_t)P->nch * 2 + (size_t)P->nZ * 4); }
+static size_t lines_lds_bytes(const EngineParams *P) { return lds_if_fits((size_t)P->n_lbins * 2); }
+
+/* Workgroups of a grid-stride kernel: as many as can be resident at once,
+ * since every workgroup ends with one flush of its LDS tallies to global
+ * memory.  XMIMSIM_AMD_VR_WG_PER_CU overrides the resident count per CU
+ * (launch-shape sweeps). */
+static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks) {
+  static int env = -1;
+  if (env < 0) {
+    const char *e = getenv("XMIMSIM_AMD_VR_WG_PER_CU");
+    env = e ? atoi(e) : 0;
+  }
+  int per_cu = env;
+  if (per_cu <= 0 &&
+      (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, ENG_BLOCK, lds) != hipSuccess || per_cu < 1))
+    per_cu = 1;
+  int vb = n_cu * per_cu;
+  if (vb > blocks) vb = blocks;
+  return vb < 1 ? 1 : vb;
 }
 
 template <int NL>
-static void launch_varred_nl(const EngineParams *P, const PhotonSoA *S, int n, int count, int vb, hipStream_t st) {
-  size_t lds = eng_varred_lds_bytes(P);
-  if (lds) hipLaunchKernelGGL((k_varred<NL, true>), dim3(vb), dim3(ENG_BLOCK), lds, st, *P, *S, n, count);
-  else hipLaunchKernelGGL((k_varred<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count,
+                                int n_cu, int blocks, hipStream_t st, hipEvent_t *ev) {
+  hipLaunchKernelGGL((k_vr_geom<NL>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+  if (ev) (void)hipEventRecord(ev[0], st);
+  hipLaunchKernelGGL(k_vr_cone, dim3(n_cu * 4), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+  if (ev) (void)hipEventRecord(ev[1], st);
+  const size_t ls = scatter_lds_bytes(P);
+  if (ls) {
+    int vb = resident_blocks((const void *)k_vr_scatter<NL, true>, ls, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_scatter<NL, true>), dim3(vb), dim3(ENG_BLOCK), ls, st, *P, *S, *V, n, count);
+  } else {
+    int vb = resident_blocks((const void *)k_vr_scatter<NL, false>, 0, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_scatter<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+  }
+  if (ev) (void)hipEventRecord(ev[2], st);
+  const size_t ll = lines_lds_bytes(P);
+  if (ll) {
+    int vb = resident_blocks((const void *)k_vr_lines<NL, true>, ll, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n, count);
+  } else {
+    int vb = resident_blocks((const void *)k_vr_lines<NL, false>, 0, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+  }
+  if (ev) (void)hipEventRecord(ev[3], st);
 }
 
-/* the layer-count bound of the estimator: the smallest of 1, 2, 4, 8, 16 */
-static void launch_varred(const EngineParams *P, const PhotonSoA *S, int n, int count, int vb, hipStream_t st) {
+/* NL, the layer-count bound of the estimator and the step: the smallest of
+ * 1, 2, 4, 8, 16 that holds n_layers */
+static void launch_estimator(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
+                             int blocks, hipStream_t st, hipEvent_t *ev) {
   const int nl = P->n_layers;
-  if (nl <= 1) launch_varred_nl<1>(P, S, n, count, vb, st);
-  else if (nl <= 2) launch_varred_nl<2>(P, S, n, count, vb, st);
-  else if (nl <= 4) launch_varred_nl<4>(P, S, n, count, vb, st);
-  else if (nl <= 8) launch_varred_nl<8>(P, S, n, count, vb, st);
-  else launch_varred_nl<16>(P, S, n, count, vb, st);
+  if (nl <= 1) launch_estimator_nl<1>(P, S, V, n, count, n_cu, blocks, st, ev);
+  else if (nl <= 2) launch_estimator_nl<2>(P, S, V, n, count, n_cu, blocks, st, ev);
+  else if (nl <= 4) launch_estimator_nl<4>(P, S, V, n, count, n_cu, blocks, st, ev);
+  else if (nl <= 8) launch_estimator_nl<8>(P, S, V, n, count, n_cu, blocks, st, ev);
+  else launch_estimator_nl<16>(P, S, V, n, count, n_cu, blocks, st, ev);
 }
 
 extern "C" {
@@ -1198,21 +1358,26 @@ hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const So
   return hipGetLastError();
 }
 
-hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, int n, int count, int vr_blocks,
-                            hipStream_t st, hipEvent_t *ev /* 6 events or NULL */) {
+hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
+                            hipStream_t st, hipEvent_t *ev /* ENG_ORDER_EVENTS events or NULL */) {
   int blocks = (count + ENG_BLOCK - 1) / ENG_BLOCK;
   if (ev) (void)hipEventRecord(ev[0], st);
-  hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, st, *P);
-  hipLaunchKernelGGL(k_step, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+  hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
+  {
+    const int nl = P->n_layers;
+    if (nl <= 1) hipLaunchKernelGGL((k_step<1>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+    else if (nl <= 2) hipLaunchKernelGGL((k_step<2>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+    else if (nl <= 4) hipLaunchKernelGGL((k_step<4>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+    else if (nl <= 8) hipLaunchKernelGGL((k_step<8>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+    else hipLaunchKernelGGL((k_step<16>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+  }
   if (ev) (void)hipEventRecord(ev[1], st);
-  int vb = vr_blocks < blocks ? vr_blocks : blocks;
-  if (vb < 1) vb = 1;
-  launch_varred(P, S, n, count, vb, st);
-  if (ev) (void)hipEventRecord(ev[2], st);
-  hipLaunchKernelGGL(k_rayleigh, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
-  hipLaunchKernelGGL(k_compton, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
-  hipLaunchKernelGGL(k_photo, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
-  if (ev) (void)hipEventRecord(ev[3], st);
+  launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
+  const int qblocks = (int)(((size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
+  hipLaunchKernelGGL(k_rayleigh, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  hipLaunchKernelGGL(k_compton, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  hipLaunchKernelGGL(k_photo, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
 

@@ -1,0 +1,262 @@
+/*
+ * surrogate_kernels.hip -- gfx950 kernels of the net-line surrogate: a
+ * multilayer perceptron (log composition/thickness/density/energy -> log net
+ * line intensities) trained on spectra of the GPU Monte Carlo.
+ *
+ * Every product is one kernel shape, C[M x N] = A[M x K] * B[N x K]^T with both
+ * operands K-contiguous BF16 and FP32 accumulation on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16).  The forward pass, the input gradient and the
+ * weight gradient are all of this form because each activation and gradient is
+ * stored both row-major and transposed by the epilogue that produces it.
+ *
+ * Tile: 64 x 64 per workgroup of four wave64s (2 x 2 waves of 32 x 32, each
+ * 2 x 2 MFMA blocks of 16 x 16), K in steps of 32 staged through LDS.  The
+ * ensemble members run side by side as gridDim.z.  Sizes must be multiples of
+ * 64 (M, N) and 32 (K); the host checks this.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "surrogate_kernels.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define SG_TM 64
+#define SG_TN 64
+#define SG_TK 32
+#define SG_LDS_STRIDE (SG_TK + 8) /* bf16 per LDS row: 80 B, staggers the banks */
+
+/* bf16 <-> f32 with round-to-nearest-even, written out so that the host and the
+ * device round identically (NaN is not produced by these kernels) */
+__device__ __forceinline__ uint16_t sg_f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float sg_bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float sg_gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float sg_gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+struct SgEpi {
+  const float *bias;    /* [N] (+ net stride) */
+  long s_bias;
+  float *cf;            /* f32 result [M x N] */
+  long s_cf;
+  uint16_t *cb;         /* bf16 result [M x N] */
+  long s_cb;
+  uint16_t *ct;         /* bf16 transposed result [N x M] */
+  long s_ct;
+  const float *zprev;   /* SG_EPI_DX: pre-activation [M x N] */
+  long s_zprev;
+};
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_sg_gemm_nt(int M, int N, int K, const uint16_t *A, long lda, long s_a,
+                                                     const uint16_t *B, long ldb, long s_b, SgEpi e) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[SG_TM * SG_LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[SG_TN * SG_LDS_STRIDE];
+  const int net = blockIdx.z;
+  A += (size_t)net * s_a;
+  B += (size_t)net * s_b;
+  const int m0 = blockIdx.y * SG_TM, n0 = blockIdx.x * SG_TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  /* global -> LDS: thread t moves 8 bf16 (16 B) of row t/4, k (t%4)*8 */
+  const int lr = tid >> 2, lk = (tid & 3) * 8;
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += SG_TK) {
+    const uint4 va = *(const uint4 *)(A + (size_t)(m0 + lr) * lda + k0 + lk);
+    const uint4 vb = *(const uint4 *)(B + (size_t)(n0 + lr) * ldb + k0 + lk);
+    __syncthreads();
+    *(uint4 *)(As + lr * SG_LDS_STRIDE + lk) = va;
+    *(uint4 *)(Bs + lr * SG_LDS_STRIDE + lk) = vb;
+    __syncthreads();
+    /* operands: lane l holds row (l % 16) and the 8 consecutive k of group l / 16;
+     * A and B use the same k assignment, so the contraction is complete */
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 a[2], b[2];
+    for (int i = 0; i < 2; i++) a[i] = *(const bf16x8 *)(As + (wr + i * 16 + fr) * SG_LDS_STRIDE + fk);
+    for (int j = 0; j < 2; j++) b[j] = *(const bf16x8 *)(Bs + (wc + j * 16 + fr) * SG_LDS_STRIDE + fk);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  /* result layout of a 16 x 16 block: lane l holds column l % 16, rows 4 (l / 16) + v */
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int v = 0; v < 4; v++) {
+        const int row = m0 + wr + i * 16 + (lane >> 4) * 4 + v;
+        const int col = n0 + wc + j * 16 + (lane & 15);
+        const size_t o = (size_t)row * N + col, ot = (size_t)col * M + row;
+        float x = acc[i][j][v];
+        if (EPI == SG_EPI_HIDDEN) {
+          const float z = x + e.bias[(size_t)net * e.s_bias + col];
+          e.cf[(size_t)net * e.s_cf + o] = z;
+          const uint16_t h = sg_f2bf(sg_gelu(z));
+          e.cb[(size_t)net * e.s_cb + o] = h;
+          e.ct[(size_t)net * e.s_ct + ot] = h;
+        } else if (EPI == SG_EPI_OUT) {
+          e.cf[(size_t)net * e.s_cf + o] = x + e.bias[(size_t)net * e.s_bias + col];
+        } else if (EPI == SG_EPI_DX) {
+          const float g = x * sg_gelu_grad(e.zprev[(size_t)net * e.s_zprev + o]);
+          e.cf[(size_t)net * e.s_cf + o] = g;
+          const uint16_t gb = sg_f2bf(g);
+          e.cb[(size_t)net * e.s_cb + o] = gb;
+          e.ct[(size_t)net * e.s_ct + ot] = gb;
+        } else {
+          e.cf[(size_t)net * e.s_cf + o] = x;
+        }
+      }
+}
+
+/* f32 [rows x cols] -> bf16 row-major [rows x ld] (zero padded) and its
+ * transpose [ld x rows]; one thread per padded element */
+__global__ void k_sg_pack(int rows, int cols, int ld, const float *X, uint16_t *Xb, uint16_t *XbT) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)rows * ld) return;
+  const int r = (int)(idx / ld), c = (int)(idx % ld);
+  const uint16_t h = c < cols ? sg_f2bf(X[(size_t)r * cols + c]) : (uint16_t)0;
+  Xb[idx] = h;
+  if (XbT) XbT[(size_t)c * rows + r] = h;
+}
+
+/* masked mean-square loss of the output layer and its gradient:
+ * dY = 2 (Y - T) mask / count, stored f32, bf16 and transposed bf16;
+ * loss partials per (net, row block) */
+__global__ void k_sg_loss(int rows, int N, int nets, const float *Y, const float *T, const float *Mk, float inv_count,
+                          float *G, uint16_t *Gb, uint16_t *GT, float *partial) {
+  const int net = blockIdx.y;
+  const size_t base = (size_t)net * rows * N;
+  float s = 0.0f;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (size_t)rows * N;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(idx / N), c = (int)(idx % N);
+    const float d = (Y[base + idx] - T[idx]) * Mk[idx];
+    s += d * d;
+    const float g = 2.0f * d * inv_count;
+    G[base + idx] = g;
+    const uint16_t gb = sg_f2bf(g);
+    Gb[base + idx] = gb;
+    GT[base + (size_t)c * rows + r] = gb;
+  }
+  /* fixed-order block sum */
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)net * gridDim.x + blockIdx.x] = red[0];
+}
+
+/* column sums of G [rows x N] per net: the bias gradient (fixed order) */
+__global__ void k_sg_colsum(int rows, int N, const float *G, long s_g, float *db, long s_db) {
+  const int net = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const float *g = G + (size_t)net * s_g;
+  float s = 0.0f;
+  for (int r = 0; r < rows; r++) s += g[(size_t)r * N + c];
+  db[(size_t)net * s_db + c] = s;
+}
+
+/* Adam (Kingma & Ba 2015) on f32 master parameters; refreshes the bf16 copy
+ * (row-major [N x K]) and its transpose [K x N] when given */
+__global__ void k_sg_adam(size_t n, int N, int K, float *p, const float *g, float *m, float *v, float lr, float b1,
+                          float b2, float eps, float c1, float c2, uint16_t *pb, uint16_t *pt) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pi;
+  if (pb) {
+    /* n = nets * N * K; element (net, r, c) */
+    const size_t per = (size_t)N * K;
+    const size_t net = i / per, rem = i % per;
+    const int r = (int)(rem / K), c = (int)(rem % K);
+    const uint16_t h = sg_f2bf(pi);
+    pb[i] = h;
+    pt[net * per + (size_t)c * N + r] = h;
+  }
+}
+
+/* bf16 copy (row-major [N x K] per net) and its transpose from f32 parameters */
+__global__ void k_sg_refresh(size_t n, int N, int K, const float *p, uint16_t *pb, uint16_t *pt) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t per = (size_t)N * K;
+  const size_t net = i / per, rem = i % per;
+  const int r = (int)(rem / K), c = (int)(rem % K);
+  const uint16_t h = sg_f2bf(p[i]);
+  pb[i] = h;
+  pt[net * per + (size_t)c * N + r] = h;
+}
+
+/* ------------------------------------------------------------------------ */
+
+extern "C" {
+
+hipError_t sg_launch_refresh(size_t n, int N, int K, const float *p, uint16_t *pb, uint16_t *pt, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_refresh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, N, K, p, pb, pt);
+  return hipGetLastError();
+}
+
+hipError_t sg_launch_gemm(int epi, int nets, int M, int N, int K, const uint16_t *A, long lda, long s_a,
+                          const uint16_t *B, long ldb, long s_b, const SgEpiArgs *a, hipStream_t st) {
+  SgEpi e;
+  e.bias = a->bias;
+  e.s_bias = a->s_bias;
+  e.cf = a->cf;
+  e.s_cf = a->s_cf;
+  e.cb = a->cb;
+  e.s_cb = a->s_cb;
+  e.ct = a->ct;
+  e.s_ct = a->s_ct;
+  e.zprev = a->zprev;
+  e.s_zprev = a->s_zprev;
+  dim3 grid(N / SG_TN, M / SG_TM, nets);
+  switch (epi) {
+    case SG_EPI_HIDDEN: hipLaunchKernelGGL(k_sg_gemm_nt<SG_EPI_HIDDEN>, grid, dim3(256), 0, st, M, N, K, A, lda, s_a, B, ldb, s_b, e); break;
+    case SG_EPI_OUT: hipLaunchKernelGGL(k_sg_gemm_nt<SG_EPI_OUT>, grid, dim3(256), 0, st, M, N, K, A, lda, s_a, B, ldb, s_b, e); break;
+    case SG_EPI_DX: hipLaunchKernelGGL(k_sg_gemm_nt<SG_EPI_DX>, grid, dim3(256), 0, st, M, N, K, A, lda, s_a, B, ldb, s_b, e); break;
+    default: hipLaunchKernelGGL(k_sg_gemm_nt<SG_EPI_DW>, grid, dim3(256), 0, st, M, N, K, A, lda, s_a, B, ldb, s_b, e); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t sg_launch_pack(int rows, int cols, int ld, const float *X, uint16_t *Xb, uint16_t *XbT, hipStream_t st) {
+  size_t n = (size_t)rows * ld;
+  hipLaunchKernelGGL(k_sg_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rows, cols, ld, X, Xb, XbT);
+  return hipGetLastError();
+}
+
+hipError_t sg_launch_loss(int rows, int N, int nets, const float *Y, const float *T, const float *Mk, float inv_count,
+                          float *G, uint16_t *Gb, uint16_t *GT, float *partial, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_loss, dim3(blocks, nets), dim3(256), 0, st, rows, N, nets, Y, T, Mk, inv_count, G, Gb, GT,
+                     partial);
+  return hipGetLastError();
+}
+
+hipError_t sg_launch_colsum(int nets, int rows, int N, const float *G, long s_g, float *db, long s_db, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_colsum, dim3((N + 255) / 256, nets), dim3(256), 0, st, rows, N, G, s_g, db, s_db);
+  return hipGetLastError();
+}
+
+hipError_t sg_launch_adam(size_t n, int N, int K, float *p, const float *g, float *m, float *v, float lr, float b1,
+                          float b2, float eps, float c1, float c2, uint16_t *pb, uint16_t *pt, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, N, K, p, g, m, v, lr, b1, b2,
+                     eps, c1, c2, pb, pt);
+  return hipGetLastError();
+}
+
+} /* extern "C" */

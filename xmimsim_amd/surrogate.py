@@ -1,0 +1,172 @@
+"""Net-line surrogate: ctypes binding of the GPU MLP ensemble
+(xmi_surrogate_* in include/xmimsim_amd.h) and the XRF encoding of its inputs
+and outputs.
+
+Inputs (85): weight fractions of Z = 11..92, log10 thickness [cm],
+density / 20 [g/cm3], excitation energy / 40 [keV] of a single-layer sample.
+Outputs (256): log10 of net line intensities summed over interaction orders
+(the var_red_history rows of xmi_main_msim) for the lines in OUT_LINES;
+absent elements and zero intensities are masked out of the loss.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import api
+
+Z_MIN, Z_MAX = 11, 92
+N_IN = (Z_MAX - Z_MIN + 1) + 3
+# xraylib line numbers (var_red_history row = line - 1)
+KL3, KM3, L3M5, M5N7 = 3, 15, 90, 216
+
+
+def _out_lines() -> List[Tuple[int, int]]:
+    lines = []
+    for z in range(Z_MIN, Z_MAX + 1):
+        lines += [(z, KL3), (z, L3M5), (z, M5N7)]
+    lines += [(z, KM3) for z in range(Z_MIN, Z_MIN + 10)]
+    return lines
+
+
+OUT_LINES = _out_lines()
+N_OUT = len(OUT_LINES)  # 256
+DEFAULT_DIMS = (N_IN, 512, 512, 512, 512, N_OUT)
+
+
+def encode(Z: Sequence[int], w: Sequence[float], thickness: float, density: float, energy: float) -> np.ndarray:
+    x = np.zeros(N_IN, np.float32)
+    for z, f in zip(Z, w):
+        x[z - Z_MIN] += f
+    x[-3] = np.log10(thickness)
+    x[-2] = density / 20.0
+    x[-1] = energy / 40.0
+    return x
+
+
+def targets(var_red_history: np.ndarray, Z: Sequence[int]) -> Tuple[np.ndarray, np.ndarray]:
+    """log10 net-line intensities (summed over orders) and the loss mask."""
+    t = np.zeros(N_OUT, np.float32)
+    m = np.zeros(N_OUT, np.float32)
+    present = set(Z)
+    for k, (z, line) in enumerate(OUT_LINES):
+        if z not in present:
+            continue
+        v = float(var_red_history[z - 1, line - 1, :].sum())
+        if v > 0.0:
+            t[k] = np.log10(v)
+            m[k] = 1.0
+    return t, m
+
+
+def _lib():
+    lib = api.load_library()
+    if not getattr(lib, "_sg_ready", False):
+        P = C.POINTER
+        fp = P(C.c_float)
+        lib.xmi_surrogate_create.argtypes = [C.c_int, C.c_int, P(C.c_int), C.c_int, C.c_uint64]
+        lib.xmi_surrogate_create.restype = C.c_void_p
+        lib.xmi_surrogate_free.argtypes = [C.c_void_p]
+        lib.xmi_surrogate_param_count.argtypes = [C.c_void_p]
+        lib.xmi_surrogate_param_count.restype = C.c_long
+        lib.xmi_surrogate_padded_width.argtypes = [C.c_void_p, C.c_int]
+        lib.xmi_surrogate_padded_width.restype = C.c_int
+        lib.xmi_surrogate_train_step.argtypes = [C.c_void_p, fp, fp, fp, C.c_int, C.c_float, fp]
+        lib.xmi_surrogate_train_step.restype = C.c_int
+        lib.xmi_surrogate_gradients.argtypes = [C.c_void_p, fp, fp, fp, C.c_int, fp, fp]
+        lib.xmi_surrogate_gradients.restype = C.c_int
+        lib.xmi_surrogate_predict.argtypes = [C.c_void_p, fp, C.c_int, fp]
+        lib.xmi_surrogate_predict.restype = C.c_int
+        lib.xmi_surrogate_get_params.argtypes = [C.c_void_p, fp]
+        lib.xmi_surrogate_get_params.restype = C.c_int
+        lib.xmi_surrogate_set_params.argtypes = [C.c_void_p, fp]
+        lib.xmi_surrogate_set_params.restype = C.c_int
+        lib._sg_ready = True
+    return lib
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Surrogate:
+    """Ensemble of n_nets MLPs on the GPU (BF16 matrix-core products, FP32
+    master weights, Adam)."""
+
+    def __init__(self, n_nets: int = 4, dims: Sequence[int] = DEFAULT_DIMS, max_batch: int = 4096, seed: int = 0):
+        self.lib = _lib()
+        self.n_nets, self.dims, self.max_batch = n_nets, tuple(dims), max_batch
+        d = (C.c_int * len(dims))(*dims)
+        self.h = self.lib.xmi_surrogate_create(n_nets, len(dims) - 1, d, max_batch, seed)
+        if not self.h:
+            raise RuntimeError("xmi_surrogate_create failed: " + api.last_error())
+        self.n_params = self.lib.xmi_surrogate_param_count(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.xmi_surrogate_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def layer_shapes(self) -> List[Tuple[int, int]]:
+        """(N, K padded) of each weight matrix."""
+        return [(self.dims[l + 1], self.lib.xmi_surrogate_padded_width(self.h, l)) for l in range(len(self.dims) - 1)]
+
+    def _check(self, ok, what):
+        if not ok:
+            raise RuntimeError(f"{what} failed: " + api.last_error())
+
+    def train_step(self, X: np.ndarray, T: np.ndarray, M: np.ndarray, lr: float) -> np.ndarray:
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        loss = np.zeros(self.n_nets, np.float32)
+        self._check(self.lib.xmi_surrogate_train_step(self.h, _fp(X), _fp(T), _fp(M), X.shape[0], lr, _fp(loss)),
+                    "train_step")
+        return loss
+
+    def gradients(self, X: np.ndarray, T: np.ndarray, M: np.ndarray):
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        loss = np.zeros(self.n_nets, np.float32)
+        g = np.zeros((self.n_nets, self.n_params), np.float32)
+        self._check(self.lib.xmi_surrogate_gradients(self.h, _fp(X), _fp(T), _fp(M), X.shape[0], _fp(loss), _fp(g)),
+                    "gradients")
+        return loss, g
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        """Y[n_nets][rows][out]; rows are padded to the kernels' multiple of 64."""
+        X = np.ascontiguousarray(X, np.float32)
+        n = X.shape[0]
+        out = np.zeros((self.n_nets, n, self.dims[-1]), np.float32)
+        for s in range(0, n, self.max_batch):
+            xb = X[s:s + self.max_batch]
+            b = xb.shape[0]
+            bp = max(64, (b + 63) // 64 * 64)
+            if bp != b:
+                xb = np.concatenate([xb, np.zeros((bp - b, X.shape[1]), np.float32)])
+            y = np.zeros((self.n_nets, bp, self.dims[-1]), np.float32)
+            self._check(self.lib.xmi_surrogate_predict(self.h, _fp(np.ascontiguousarray(xb)), bp, _fp(y)), "predict")
+            out[:, s:s + b] = y[:, :b]
+        return out
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros((self.n_nets, self.n_params), np.float32)
+        self._check(self.lib.xmi_surrogate_get_params(self.h, _fp(p)), "get_params")
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        self._check(self.lib.xmi_surrogate_set_params(self.h, _fp(p)), "set_params")
+
+    def unpack(self, flat: np.ndarray):
+        """[(W [N][K padded], b [N])] of one net's flat parameter vector."""
+        out, o = [], 0
+        for N, K in self.layer_shapes():
+            W = flat[o:o + N * K].reshape(N, K)
+            o += N * K
+            b = flat[o:o + N]
+            o += N
+            out.append((W, b))
+        return out
