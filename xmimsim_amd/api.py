@@ -307,7 +307,8 @@ class Prepared:
     """An input after xmi_input_C2F + xmi_init_input + table loading (the
     handles xmi_main_msim takes), plus an optional solid-angle grid."""
 
-    def __init__(self, inp: Input, options: Optional[xmi_main_options] = None, hdf5_file: Optional[str] = None):
+    def __init__(self, inp: Input, options: Optional[xmi_main_options] = None, hdf5_file: Optional[str] = None,
+                 tables: Optional["SharedTables"] = None):
         lib = load_library()
         self.lib = lib
         self.inp = inp
@@ -315,11 +316,17 @@ class Prepared:
         self.cinput = CInput(inp)
         self.inputF = C.c_void_p()
         self.hdf5F = C.c_void_p()
+        self._owns_tables = tables is None
         self.solid_angle = None
         if not lib.xmi_input_C2F(C.byref(self.cinput.input), C.byref(self.inputF)):
             raise RuntimeError("xmi_input_C2F failed: " + last_error())
         if not lib.xmi_init_input(C.byref(self.inputF)):
             raise RuntimeError("xmi_init_input failed: " + last_error())
+        if tables is not None:
+            self.hdf5F = tables.hdf5F
+            if not lib.xmi_update_input_from_hdf5(self.inputF, self.hdf5F):
+                raise RuntimeError("xmi_update_input_from_hdf5 failed: " + last_error())
+            return
         if hdf5_file is not None:
             ok = lib.xmi_init_from_hdf5(hdf5_file.encode(), self.inputF, C.byref(self.hdf5F), C.byref(self.options))
         else:
@@ -381,16 +388,37 @@ class Prepared:
     def close(self):
         lib = self.lib
         self.free_solid_angle()
-        if self.hdf5F:
+        if self.hdf5F and self._owns_tables:
             lib.xmi_free_hdf5_F(C.byref(self.hdf5F))
+        self.hdf5F = C.c_void_p()
         if self.inputF:
             lib.xmi_free_input_F(C.byref(self.inputF))
+            self.inputF = C.c_void_p()
 
     def __del__(self):
         try:
             self.close()
         except Exception:
             pass
+
+
+class SharedTables:
+    """Interaction tables of a set of elements, built once and bound to many
+    inputs (each Prepared made with tables=... uses them without a copy; the
+    engine keeps one device copy per table handle)."""
+
+    def __init__(self, Zs: Sequence[int], hdf5_file: Optional[str] = None):
+        Zs = sorted(set(int(z) for z in Zs))
+        self.Zs = Zs
+        self._holder = Prepared(Input(layers=[Layer(Zs, [1.0 / len(Zs)] * len(Zs), 1.0, 0.1)],
+                                      discrete=[DiscreteLine(20.0, 1e9)], n_photons_line=10), hdf5_file=hdf5_file)
+        self.hdf5F = self._holder.hdf5F
+
+    def close(self):
+        if self._holder is not None:
+            self._holder.close()
+            self._holder = None
+            self.hdf5F = C.c_void_p()
 
 
 def _collect(lib, inp: Input, ch, bh, vr):

@@ -449,10 +449,14 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
   }
   P.lay_eoff[NL] = (int)lay_edge.size();
 
-  /* attenuation at the fluorescence lines, and the estimator's active lines */
+  /* attenuation at the fluorescence lines, and the estimator's active lines,
+   * for the elements of this input (a table handle may hold more) */
+  std::vector<char> present(nZ, 0);
+  for (int u : elU) present[u] = 1;
   std::vector<double> pmu((size_t)NL * nZ * XMI_NLINES_VR, 0.0);
   for (int l = 0; l < NL; l++)
     for (int u = 0; u < nZ; u++) {
+      if (!present[u]) continue;
       const xrl_atom *a = xrl_atom_get(T->Zs[u]);
       for (int line = 1; line <= XMI_NLINES_VR; line++) {
         double E = a->line_energy[line];
@@ -464,6 +468,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
   std::vector<double> actf((size_t)nZ * ENG_MAX_ACT, 0.0), actE((size_t)nZ * ENG_MAX_ACT, 0.0),
       actm((size_t)NL * nZ * ENG_MAX_ACT, 0.0);
   for (int u = 0; u < nZ; u++) {
+    if (!present[u]) continue;
     const xrl_atom *a = xrl_atom_get(T->Zs[u]);
     int k = 0;
     for (int line = 1; line <= line_last; line++) {
