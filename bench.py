@@ -66,16 +66,20 @@ def make_input(args):
 
 def vr_bytes_per_event(inp, layer_events, n_trials=2.0):
     """Algorithmic bytes of one detection-weight evaluation (8 B per table
-    scalar needed, 8 B per tally add), summed over the layers it happens in:
-    32 (solid angle) + E*[16 (form factor) + 16*L*E (attenuation at the
-    Compton energy) + 16 (scattering function) + 16*n_trials (Doppler ICDF)
-    + 72 (shell cross sections) + A*(16 + 8*L)] + 8*(2E + 2E*A)."""
-    total_events = sum(layer_events)
-    if total_events == 0:
-        return 0.0, 0.0
-    # active lines per element: lines up to M5P5 with E >= 1 keV and a non-zero rate
+    scalar needed, 8 B per tally add), summed over the layers it happens in,
+    split by estimator kernel:
+      geometry  32 (solid-angle bilinear lookup)
+      scatter   E*[16 (form factor) + 16*L*E (attenuation at the Compton
+                energy) + 16 (scattering function) + 16*n_trials (Doppler
+                ICDF)] + 8*2E (Rayleigh and Compton tallies)
+      lines     E*[72 (shell cross sections) + A*(16 + 8*L)] + 8*2E*A
+    E elements of the layer, L layers on the escape path, A active lines per
+    element.  Returns {kernel: total bytes} and the per-event total."""
+    total = {"geom": 0.0, "scatter": 0.0, "lines": 0.0}
+    n_ev = sum(layer_events)
+    if n_ev == 0:
+        return total, 0.0
     active = _active_lines(inp)
-    total = 0.0
     for l, ev in enumerate(layer_events):
         if ev == 0:
             continue
@@ -83,9 +87,10 @@ def vr_bytes_per_event(inp, layer_events, n_trials=2.0):
         E = len(layer.Z)
         L = l + 1  # reflection geometry: the escape path crosses the layers above
         A = sum(active[z] for z in layer.Z) / E
-        b = 32 + E * (16 + 16 * L * E + 16 + 16 * n_trials + 72 + A * (16 + 8 * L)) + 8 * (2 * E + 2 * E * A)
-        total += ev * b
-    return total, total / total_events
+        total["geom"] += ev * 32
+        total["scatter"] += ev * (E * (16 + 16 * L * E + 16 + 16 * n_trials) + 8 * 2 * E)
+        total["lines"] += ev * (E * (72 + A * (16 + 8 * L)) + 8 * 2 * E * A)
+    return total, sum(total.values()) / n_ev
 
 
 def _active_lines(inp):
@@ -98,30 +103,26 @@ def _active_lines(inp):
 
 def cpu_baseline(args, prep_gpu):
     """Oracle (reference structure: one RNG stream per thread, thread 0 takes
-    the remainder) on a bounded sample of the workload."""
+    the remainder) on a bounded sample of the workload, grown until its
+    transport takes about --cpu-seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from xmimsim_amd import api
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     # a job's CPU share is what OMP_NUM_THREADS grants it, when set
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores)
-    # pilot to size the sample
-    pilot_n = 50 if args.workload == "three_layer_kramers" else 5000
-    sample = make_input(argparse.Namespace(**{**vars(args), "photons": pilot_n}))
-    prep = api.Prepared(sample)
-    prep.solid_angle = prep_gpu.solid_angle
-    prep._sa_owner = "borrowed"
-    _, _, _, st = oracle.run(prep, seed=args.seed, rng_mode=oracle.RNG_PER_THREAD, n_threads=cores)
-    # size from the pilot's transport time (its set-up time does not scale)
-    pilot_s = max(st.seconds, 1e-3)
-    n = max(pilot_n, int(pilot_n * args.cpu_seconds / pilot_s))
-    sample = make_input(argparse.Namespace(**{**vars(args), "photons": n}))
-    prep2 = api.Prepared(sample)
-    prep2.solid_angle = prep_gpu.solid_angle
-    prep2._sa_owner = "borrowed"
-    _, _, _, st = oracle.run(prep2, seed=args.seed, rng_mode=oracle.RNG_PER_THREAD, n_threads=cores)
-    prep.solid_angle = None
-    prep2.solid_angle = None
+    n = 50 if args.workload == "three_layer_kramers" else 5000
+    for attempt in range(4):
+        sample = make_input(argparse.Namespace(**{**vars(args), "photons": n}))
+        prep = api.Prepared(sample)
+        prep.solid_angle = prep_gpu.solid_angle
+        prep._sa_owner = "borrowed"
+        _, _, _, st = oracle.run(prep, seed=args.seed, rng_mode=oracle.RNG_PER_THREAD, n_threads=cores)
+        prep.solid_angle = None
+        prep.close()
+        if st.seconds >= 0.5 * args.cpu_seconds or attempt == 3:
+            break
+        n = max(n + 1, int(n * min(8.0, args.cpu_seconds / max(st.seconds, 1e-3))))
     return {
         "value": st.histories / st.seconds,
         "unit": "histories/s",
@@ -130,6 +131,18 @@ def cpu_baseline(args, prep_gpu):
         "sample": f"{args.workload} with {n} photons per interval/line ({st.histories} histories, "
                   f"{st.seconds:.1f} s, transport only)",
     }
+
+
+def _measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), from the
+    committed profile summary of this workload, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("bytes_per_launch")
 
 
 def main():
@@ -155,8 +168,10 @@ def main():
         lib.xmi_msim_set_profiling(1 if w == 0 else 0)
         _, _, _, st = prep.run(seed=args.seed + 1000 + w)
         if w == 0:
-            prof = (st.vr_kernel_ms, st.step_kernel_ms, st.interaction_kernel_ms, st.launches,
-                    list(prep.layer_events), st.histories, st.solid_angle_fallbacks)
+            prof = {"geom": st.vr_geom_ms, "cone": st.vr_cone_ms, "scatter": st.vr_scatter_ms,
+                    "lines": st.vr_lines_ms, "step": st.step_kernel_ms, "interactions": st.interaction_kernel_ms,
+                    "launches": st.launches, "layer_events": list(prep.layer_events),
+                    "fallbacks": st.solid_angle_fallbacks}
         print(f"[bench] warmup {w}: {st.histories} histories, {st.transport_ms:.1f} ms", file=sys.stderr, flush=True)
     lib.xmi_msim_set_profiling(0)
 
@@ -173,18 +188,22 @@ def main():
     value = hist / elapsed
 
     roofline = None
-    if prof is not None and prof[0] > 0:
-        vr_ms, step_ms, int_ms, launches, layer_events, _, fallbacks = prof
-        tot_bytes, per_event = vr_bytes_per_event(inp, layer_events)
-        achieved = tot_bytes / (vr_ms * 1e-3) / 1e9
+    if prof is not None and prof["scatter"] > 0:
+        tot_bytes, per_event = vr_bytes_per_event(inp, prof["layer_events"])
+        launches = max(prof["launches"], 1)
+        kernel_ms = {k: round(prof[k], 2) for k in ("geom", "cone", "scatter", "lines", "step", "interactions")}
+        # the dominant kernel of the step
+        dom = max(("scatter", "lines", "geom"), key=lambda k: prof[k])
+        names = {"scatter": "k_vr_scatter", "lines": "k_vr_lines", "geom": "k_vr_geom"}
+        achieved = tot_bytes[dom] / (prof[dom] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_varred (detection-weight estimator)",
-                    "bytes_per_event": round(per_event, 1), "events": int(sum(layer_events)),
-                    "launches": int(launches), "avg_launch_ms": round(vr_ms / max(launches, 1), 4),
-                    "solid_angle_fallbacks": int(fallbacks),
-                    "kernel_ms": {"k_varred": round(vr_ms, 2), "k_step": round(step_ms, 2),
-                                  "interactions": round(int_ms, 2)}}
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _measured_traffic(names[dom], args.workload),
+                    "kernel": names[dom] + " (detection weight: " + dom + ")",
+                    "bytes_per_launch": round(tot_bytes[dom] / launches),
+                    "avg_launch_ms": round(prof[dom] / launches, 4),
+                    "estimator_bytes_per_event": round(per_event, 1), "events": int(sum(prof["layer_events"])),
+                    "launches": int(launches), "solid_angle_fallbacks": int(prof["fallbacks"]),
+                    "kernel_ms": kernel_ms}
 
     cpu = None
     if not args.no_cpu:

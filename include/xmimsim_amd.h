@@ -278,6 +278,42 @@ int xmi_provider_active_lines(int Z, int line_last);
 /* Last error message of this thread, or "" */
 const char *xmi_msim_last_error(void);
 
+/* ---- detector response ------------------------------------------------ */
+
+/* Escape ratios of the detector crystal (include/xmi_detector.h:28-40). */
+typedef struct _xmi_escape_ratios {
+  int n_elements;
+  int n_fluo_input_energies;
+  int n_compton_input_energies;
+  int n_compton_output_energies;
+  int *Z;
+  double *fluo_escape_ratios;          /* Fortran (element, line 1..109, input energy) */
+  double *fluo_escape_input_energies;
+  double *compton_escape_ratios;       /* Fortran (input energy, output energy) */
+  double *compton_escape_input_energies;
+  double *compton_escape_output_energies;
+  char *xmi_input_string;
+} xmi_escape_ratios;
+
+/* Detector response of every interaction order: absorption in the window and
+ * crystal, escape peaks (when options->use_escape_peaks and escape_ratios is
+ * given), pile-up (use_sum_peaks), Gaussian response with low-energy tail and
+ * Poisson noise (use_poisson); the histories get the detector absorption at
+ * their line energies in place.  channels_noconv[n] are the spectra of
+ * xmi_main_msim per order; channels_conv[n] receive malloc'ed results (order
+ * 0 is zero-filled when zero_interaction is 0).  Replaces
+ * xmi_detector_convolute_all (include/xmi_main.h:33,
+ * src/xmi_detector_f.F90:219-580); xmi_detector_convolute_all_custom is the
+ * same function under the name the reference loads a custom detector
+ * response by (bin/xmimsim.c:516). */
+void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **channels_noconv, double **channels_conv,
+                                double *brute_history, double *var_red_history, xmi_main_options *options,
+                                xmi_escape_ratios *escape_ratios, int n_interactions_all, int zero_interaction);
+void xmi_detector_convolute_all_custom(xmi_inputFPtr inputFPtr, double **channels_noconv, double **channels_conv,
+                                       double *brute_history, double *var_red_history, xmi_main_options *options,
+                                       xmi_escape_ratios *escape_ratios, int n_interactions_all,
+                                       int zero_interaction);
+
 /* ---- net-line surrogate ------------------------------------------------
  * No counterpart in the reference.  An ensemble of multilayer perceptrons
  * (dims[0] inputs, GELU hidden layers, linear outputs) trained on the GPU

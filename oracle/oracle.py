@@ -55,6 +55,7 @@ def load():
     lib.xmi_oracle_fpm_first_order.restype = C.c_double
     lib.xmi_oracle_scatter_first_order.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.xmi_oracle_scatter_first_order.restype = C.c_double
+    lib.xmi_oracle_detector_convolute.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.xmi_oracle_poly_solve_quadratic.argtypes = [C.c_double] * 3 + [P(C.c_double)] * 2
     lib.xmi_oracle_poly_solve_quadratic.restype = C.c_int
     lib.xmi_oracle_energy_from_q.argtypes = [C.c_double] * 3
@@ -120,3 +121,12 @@ def fpm_first_order(prepared, Z: int, line: int, cascade: int = 4) -> float:
 def scatter_first_order(prepared, Z: int, kind: str) -> float:
     """Closed-form first-order Rayleigh ("rayl") or Compton ("compt") intensity."""
     return load().xmi_oracle_scatter_first_order(prepared.inputF, Z, {"rayl": 0, "compt": 1}[kind])
+
+
+def detector_convolute(prepared, noconv):
+    """Absorption and Gaussian response of one order's spectrum."""
+    x = np.ascontiguousarray(noconv, np.float64)
+    out = np.zeros_like(x)
+    load().xmi_oracle_detector_convolute(prepared.inputF, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                         out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out

@@ -1358,3 +1358,63 @@ double xmi_oracle_scatter_first_order(xmi_inputFPtr inputFPtr, int Z, int kind) 
   double dcs_v = kind == 0 ? xrl_dcsp_rayl(a, E0, theta, phi_v) : xrl_dcsp_compt(a, E0, theta, phi_v);
   return (Ih * dcs_h + Iv * dcs_v) * Omega / sin_alpha * w * (1.0 - exp(-chi * L->density * L->thickness)) / chi;
 }
+
+/* ------------------------------------------------------------------------ */
+/* detector response of one interaction order, src/xmi_detector_f.F90:337-530
+ * (absorption per channel, Gaussian response with the low-energy tail; no
+ * escape peaks, pile-up or Poisson noise) */
+
+static double det_layer_mu(const xmi_layerF *L, double E) {
+  double rv = 0.0;
+  for (int i = 0; i < L->n_elements; i++) rv += xrl_cs_total(xrl_atom_get(L->Z[i]), E) * L->weight[i];
+  return rv;
+}
+
+void xmi_oracle_detector_convolute(xmi_inputFPtr inputFPtr, const double *noconv, double *conv) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const int nch = in->detector.nchannels;
+  const long nlim = nch - 1;
+  const double a = in->detector.noise * in->detector.noise;
+  const double b = (double)((2.3548f * 2.3548f) * 3.85f) * in->detector.fano / 1000.0;
+  const double c = sqrt(2.0) / (2.0 * sqrt(2.0 * log(2.0)));
+  double *temp = (double *)malloc(sizeof(double) * nch);
+  double *R = (double *)malloc(sizeof(double) * (nch + 101));
+  for (int i = 0; i < nch; i++) {
+    const double E = i * in->detector.gain + in->detector.zero;
+    double det_corr = 1.0;
+    for (int j = 0; j < in->n_det_layers; j++)
+      det_corr *= exp(-1.0 * in->det_layers[j].density * in->det_layers[j].thickness * det_layer_mu(&in->det_layers[j], E));
+    for (int j = 0; j < in->n_crystal_layers; j++)
+      det_corr = -1.0 * det_corr *
+                 expm1(-1.0 * in->crystal_layers[j].density * in->crystal_layers[j].thickness *
+                       det_layer_mu(&in->crystal_layers[j], E));
+    temp[i] = noconv[i] * det_corr;
+    conv[i] = 0.0;
+  }
+  for (long I0 = 0; I0 <= nlim; I0++) {
+    const double E0 = in->detector.zero + in->detector.gain * I0;
+    if (E0 < 1.0) continue;
+    const double FWHM = sqrt(a + b * E0);
+    const double B0 = c * FWHM;
+    const double A0 = 1.0 / (B0 * sqrt(ORC_PI));
+    const double A3 = 2.73E-3 * exp(-0.21 * E0) + 1.E-4;
+    const double A4 = 0.000188 * exp(-0.00296 * pow(E0, 0.763)) + 1.355E-5 * exp(0.968 * pow(E0, 0.498));
+    const double ALFA = 1.179 * exp(8.6E-4 * pow(E0, 1.877)) - 7.793 * exp(-3.81 * pow(E0, -0.0716));
+    double my_sum = 0.0;
+    for (long I = 0; I <= I0 + 100 && I < nch; I++) {
+      const double E = in->detector.zero + in->detector.gain * I;
+      const double X = (E - E0) / B0;
+      const double G = exp(-X * X);
+      const double F = erfc(X);
+      if (E0 > 50.0) R[I] = A0 * G;
+      else if (in->detector.detector_type == XMI_DETECTOR_CONVOLUTION_PROFILE_SI_SDD)
+        R[I] = A0 * G + 1.0 * (0.63 * A3 + 15.0 * A4 * exp(ALFA * (E - E0))) * F;
+      else
+        R[I] = A0 * G + 1.0 * (2.7 * A3 + 15.0 * A4 * exp(ALFA * (E - E0))) * F;
+      my_sum += R[I];
+    }
+    for (long I = 0; I <= I0 + 100 && I < nch; I++) conv[I] += R[I] * temp[I0] / my_sum;
+  }
+  free(temp);
+  free(R);
+}

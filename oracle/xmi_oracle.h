@@ -53,6 +53,8 @@ double xmi_oracle_single_solid_angle(xmi_inputFPtr inputFPtr, double r1, double 
 double xmi_oracle_fpm_first_order(xmi_inputFPtr inputFPtr, int Z, int line, int cascade_type);
 /* first-order Rayleigh (kind 0) / Compton (kind 1) scatter of element Z, closed form */
 double xmi_oracle_scatter_first_order(xmi_inputFPtr inputFPtr, int Z, int kind);
+/* detector absorption and Gaussian response of one order (no escape, pile-up, noise) */
+void xmi_oracle_detector_convolute(xmi_inputFPtr inputFPtr, const double *noconv, double *conv);
 
 /* small numerical helpers exposed for unit tests */
 int xmi_oracle_poly_solve_quadratic(double a, double b, double c, double *r1, double *r2);

@@ -110,7 +110,8 @@ EXPORTED_SYMBOLS = [
     "xmi_init_from_hdf5", "xmi_init_from_provider", "xmi_update_input_from_hdf5", "xmi_free_hdf5_F",
     "xmi_solid_angle_calculation_cl", "xmi_free_solid_angle", "xmi_hits_per_single", "xmi_main_msim",
     "xmi_free", "xmi_set_seed", "xmi_msim_get_stats", "xmi_msim_last_error", "xmi_msim_set_profiling",
-    "xmi_msim_get_layer_events", "xmi_provider_active_lines",
+    "xmi_msim_get_layer_events", "xmi_provider_active_lines", "xmi_detector_convolute_all",
+    "xmi_detector_convolute_all_custom", "xmi_surrogate_create", "xmi_surrogate_train_step",
 ]
 
 _lib = None
@@ -400,6 +401,45 @@ class Prepared:
             self.close()
         except Exception:
             pass
+
+
+class xmi_escape_ratios(C.Structure):
+    _fields_ = [("n_elements", C.c_int), ("n_fluo_input_energies", C.c_int), ("n_compton_input_energies", C.c_int),
+                ("n_compton_output_energies", C.c_int), ("Z", C.POINTER(C.c_int)),
+                ("fluo_escape_ratios", C.POINTER(C.c_double)), ("fluo_escape_input_energies", C.POINTER(C.c_double)),
+                ("compton_escape_ratios", C.POINTER(C.c_double)),
+                ("compton_escape_input_energies", C.POINTER(C.c_double)),
+                ("compton_escape_output_energies", C.POINTER(C.c_double)), ("xmi_input_string", C.c_char_p)]
+
+
+def detector_response(prep: "Prepared", channels: np.ndarray, var_red: Optional[np.ndarray] = None,
+                      escape_ratios=None, zero_interaction: bool = False, options: Optional[xmi_main_options] = None):
+    """xmi_detector_convolute_all on the spectra of every order (channels
+    [n_int+1][nch] from Prepared.run); returns the convoluted spectra and
+    applies the detector absorption to var_red in place."""
+    lib = load_library()
+    n1, nch = channels.shape
+    ch = [np.ascontiguousarray(channels[k], np.float64) for k in range(n1)]
+    noconv = (C.POINTER(C.c_double) * n1)(*[c.ctypes.data_as(C.POINTER(C.c_double)) for c in ch])
+    conv = (C.POINTER(C.c_double) * n1)()
+    vr_ptr = None
+    if var_red is not None:
+        assert var_red.flags["C_CONTIGUOUS"] and var_red.dtype == np.float64
+        vr_ptr = var_red.ctypes.data_as(C.POINTER(C.c_double))
+    opts = options or prep.options
+    lib.xmi_detector_convolute_all.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)),
+                                               C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(xmi_main_options),
+                                               C.POINTER(xmi_escape_ratios), C.c_int, C.c_int]
+    lib.xmi_detector_convolute_all(prep.inputF, noconv, conv, None, vr_ptr, C.byref(opts),
+                                   C.byref(escape_ratios) if escape_ratios is not None else None, n1 - 1,
+                                   1 if zero_interaction else 0)
+    out = np.zeros((n1, nch))
+    for k in range(n1):
+        if conv[k]:
+            out[k] = np.ctypeslib.as_array(conv[k], shape=(nch,))
+            lib.xmi_free(conv[k])
+    return out
 
 
 class SharedTables:

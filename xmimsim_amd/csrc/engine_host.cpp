@@ -48,6 +48,17 @@ extern "C" void xmi_set_seed(uint64_t seed) {
   g_seed_set = 1;
 }
 
+/* the seed the engine uses under these options (1 with default seeds, as the
+ * reference's use_default_seeds; else the xmi_set_seed or time-based one) */
+extern "C" uint64_t xmi_engine_current_seed(const xmi_main_options *options) {
+  if (options && options->use_default_seeds) return 1;
+  if (!g_seed_set) {
+    g_seed = (uint64_t)time(NULL) * 6364136223846793005ull + (uint64_t)clock();
+    g_seed_set = 1;
+  }
+  return g_seed;
+}
+
 extern "C" void xmi_msim_get_stats(xmi_msim_stats *out) {
   if (out) *out = g_stats;
 }
@@ -380,14 +391,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
   P.cascade = cascade_of(options);
   P.use_M_lines = options->use_M_lines;
   P.hits_per_single = xmi_hits_per_single;
-  if (options->use_default_seeds) P.seed = 1;
-  else {
-    if (!g_seed_set) {
-      g_seed = (uint64_t)time(NULL) * 6364136223846793005ull + (uint64_t)clock();
-      g_seed_set = 1;
-    }
-    P.seed = g_seed;
-  }
+  P.seed = xmi_engine_current_seed(options);
   P.sample = 0;
   P.nZ = nZ;
 
