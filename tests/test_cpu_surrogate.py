@@ -53,7 +53,8 @@ def test_xrf_encoding():
     from xmimsim_amd import surrogate as S
     assert S.N_IN == 85 and S.N_OUT == 256 and len(set(S.OUT_LINES)) == 256
     x = S.encode([26, 82], [0.7, 0.3], 0.01, 7.9, 20.0)
-    assert x[26 - 11] == np.float32(0.7) and x[82 - 11] == np.float32(0.3)
+    assert x[26 - 11] == pytest.approx(1 + np.log10(0.7) / 4) and x[82 - 11] == pytest.approx(1 + np.log10(0.3) / 4)
+    assert x[12 - 11] == 0.0
     assert x[-3] == np.float32(-2.0) and x[-1] == np.float32(0.5)
     vr = np.zeros((100, 385, 4))
     vr[25, 2, :] = [1e3, 10.0, 1.0, 0.0]   # Fe KL3

@@ -2,8 +2,10 @@
 (xmi_surrogate_* in include/xmimsim_amd.h) and the XRF encoding of its inputs
 and outputs.
 
-Inputs (85): weight fractions of Z = 11..92, log10 thickness [cm],
-density / 20 [g/cm3], excitation energy / 40 [keV] of a single-layer sample.
+Inputs (85): weight fractions of Z = 11..92 on a logarithmic scale
+(1 + log10(w) / 4 for present elements, 0 for absent ones: line intensities
+go roughly as log w), log10 thickness [cm], density / 20 [g/cm3], excitation
+energy / 40 [keV] of a single-layer sample.
 Outputs (256): log10 of net line intensities summed over interaction orders
 (the var_red_history rows of xmi_main_msim) for the lines in OUT_LINES;
 absent elements and zero intensities are masked out of the loss.
@@ -39,7 +41,7 @@ DEFAULT_DIMS = (N_IN, 512, 512, 512, 512, N_OUT)
 def encode(Z: Sequence[int], w: Sequence[float], thickness: float, density: float, energy: float) -> np.ndarray:
     x = np.zeros(N_IN, np.float32)
     for z, f in zip(Z, w):
-        x[z - Z_MIN] += f
+        x[z - Z_MIN] = 1.0 + np.log10(max(float(f), 1e-4)) / 4.0
     x[-3] = np.log10(thickness)
     x[-2] = density / 20.0
     x[-1] = energy / 40.0
