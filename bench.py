@@ -31,7 +31,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="three_layer_kramers",
-                    choices=["three_layer_kramers", "single_layer_steel", "quantify"])
+                    choices=["three_layer_kramers", "single_layer_steel", "quantify", "batch"])
+    ap.add_argument("--batch-samples", type=int, default=1024, help="batch: independent samples")
+    ap.add_argument("--streams", type=int, default=4, help="batch: engines (HIP streams) serving the sample queue")
     ap.add_argument("--train-samples", type=int, default=32768, help="quantify: Monte Carlo training samples")
     ap.add_argument("--train-steps", type=int, default=8000, help="quantify: Adam steps")
     ap.add_argument("--quant-samples", type=int, default=64, help="quantify: unknown samples")
@@ -148,6 +150,67 @@ def _measured_traffic(kernel, workload):
     return t.get("bytes_per_launch")
 
 
+def batch(args):
+    """Independent samples (1-5 layers, 2-30 elements, budgets 1e5-1e7, one
+    line each) through xmi_main_msim_batch: aggregate histories/s and the
+    per-sample completion times, with one engine and with --streams engines.
+    All samples share the default geometry, hence one solid-angle grid
+    (1024 x 1024 x --grid-hits) whose range covers 5 cm of a light matrix."""
+    import numpy as np
+    from xmimsim_amd import api, workloads
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    inps = workloads.batch_samples(args.batch_samples, seed=args.seed % 100000)
+    Zs = sorted({z for inp in inps for l in inp.layers for z in l.Z})
+    t = time.time()
+    tables = api.SharedTables(Zs)
+    log(f"tables of {len(Zs)} elements in {time.time() - t:.1f} s")
+    deep = api.Input(layers=[api.Layer([Zs[0]], [1.0], 1.0, 1.0)] * 5, discrete=[api.DiscreteLine(40.0, 1e9)],
+                     n_photons_line=10)
+    grid = api.Prepared(deep, tables=tables)
+    grid.compute_solid_angle(hits_per_single=args.grid_hits, seed=args.seed)
+    preps = []
+    for inp in inps:
+        p = api.Prepared(inp, tables=tables)
+        p.solid_angle = grid.solid_angle
+        p._sa_owner = "borrowed"
+        preps.append(p)
+    hist = sum(inp.n_photons_line for inp in inps)
+    api.run_batch(preps[:8], tables, seed=args.seed, n_streams=args.streams)  # warm-up: engines, table upload
+    runs = {}
+    for ns in (1, args.streams):
+        t = time.perf_counter()
+        _, done = api.run_batch(preps, tables, seed=args.seed, n_streams=ns)
+        dt = time.perf_counter() - t
+        runs[ns] = {"seconds": round(dt, 3), "histories_per_s": round(hist / dt, 1),
+                    "p50_ms": round(float(np.percentile(done, 50)), 1), "p99_ms": round(float(np.percentile(done, 99)), 1)}
+        log(f"{ns} stream(s): {runs[ns]}")
+    for p in preps:
+        p.solid_angle = None
+        p.close()
+    grid.close()
+    tables.close()
+    best = runs[args.streams]
+    out = {
+        "metric": "photon histories/s on 1 GPU (batched samples)",
+        "value": best["histories_per_s"],
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(best["seconds"] * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded independent samples)",
+        "config": {"workload": "batch", "samples": len(inps), "histories": int(hist), "streams": args.streams,
+                   "elements": len(Zs), "solid_angle_grid": f"shared, 1024x1024x{args.grid_hits}"},
+        "per_sample_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"]},
+        "one_stream": runs[1],
+    }
+    print(json.dumps(out))
+
+
 def quantify(args):
     """Surrogate-screened vs plain Monte Carlo quantification of single-layer
     samples: training data from the GPU Monte Carlo, training of the BF16
@@ -188,6 +251,8 @@ def main():
         sys.exit(2)
     if args.workload == "quantify":
         return quantify(args)
+    if args.workload == "batch":
+        return batch(args)
     from xmimsim_amd import api
     inp = make_input(args)
     prep = api.Prepared(inp)

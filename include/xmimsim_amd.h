@@ -226,6 +226,20 @@ int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_host
                   xmi_main_options *options, double **brute_history, double **var_red_history,
                   xmi_solid_angle *solid_angles);
 
+/* Many samples in one call; no counterpart in the reference, whose batch mode
+ * (src/xmi_batch.c) runs one xmimsim process per sample.  A work queue of
+ * samples, largest photon budget first, is served by n_streams engines (at
+ * most 8), each with its own HIP stream, so that samples overlap on the GPU.
+ * All inputs must be bound to the one table handle; each sample has its
+ * solid-angle grid (one grid may serve every sample of one geometry).
+ * Outputs per sample as xmi_main_msim's (channels[k], var_red_history[k],
+ * malloc'ed); done_ms[k], when given, receives the sample's completion time in
+ * ms after the start.  Each sample's result equals xmi_main_msim's for the
+ * same seed.  Returns 1 on success, 0 (no outputs) on failure. */
+int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
+                        xmi_main_options *options, xmi_solid_angle **solid_angles, double **channels,
+                        double **var_red_history, int n_streams, double *done_ms);
+
 /* Releases memory returned by this library (src/xmi_aux.c:217). */
 void xmi_free(void *ptr);
 

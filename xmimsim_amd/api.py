@@ -111,7 +111,7 @@ EXPORTED_SYMBOLS = [
     "xmi_solid_angle_calculation_cl", "xmi_free_solid_angle", "xmi_hits_per_single", "xmi_main_msim",
     "xmi_free", "xmi_set_seed", "xmi_msim_get_stats", "xmi_msim_last_error", "xmi_msim_set_profiling",
     "xmi_msim_get_layer_events", "xmi_provider_active_lines", "xmi_detector_convolute_all",
-    "xmi_detector_convolute_all_custom", "xmi_surrogate_create", "xmi_surrogate_train_step",
+    "xmi_detector_convolute_all_custom", "xmi_surrogate_create", "xmi_surrogate_train_step", "xmi_main_msim_batch",
 ]
 
 _lib = None
@@ -440,6 +440,37 @@ def detector_response(prep: "Prepared", channels: np.ndarray, var_red: Optional[
             out[k] = np.ctypeslib.as_array(conv[k], shape=(nch,))
             lib.xmi_free(conv[k])
     return out
+
+
+def run_batch(preps: Sequence["Prepared"], tables: "SharedTables", seed: int, n_streams: int = 4,
+              options: Optional[xmi_main_options] = None):
+    """xmi_main_msim_batch over inputs bound to `tables`, each with a
+    solid-angle grid; returns ([(channels, var_red_history)], done_ms)."""
+    lib = load_library()
+    n = len(preps)
+    P = C.POINTER
+    lib.xmi_main_msim_batch.argtypes = [C.c_int, P(C.c_void_p), C.c_void_p, C.c_int, P(xmi_main_options),
+                                        P(P(xmi_solid_angle)), P(P(C.c_double)), P(P(C.c_double)), C.c_int,
+                                        P(C.c_double)]
+    lib.xmi_main_msim_batch.restype = C.c_int
+    inputs = (C.c_void_p * n)(*[p.inputF.value for p in preps])
+    sas = (P(xmi_solid_angle) * n)(*[p.solid_angle for p in preps])
+    ch = (P(C.c_double) * n)()
+    vr = (P(C.c_double) * n)()
+    done = (C.c_double * n)()
+    opts = options or preps[0].options
+    lib.xmi_set_seed(seed)
+    if not lib.xmi_main_msim_batch(n, inputs, tables.hdf5F, 1, C.byref(opts), sas, ch, vr, n_streams, done):
+        raise RuntimeError("xmi_main_msim_batch failed: " + last_error())
+    out = []
+    for k, p in enumerate(preps):
+        n_int, nch = p.inp.n_interactions_trajectory, p.inp.nchannels
+        c = np.ctypeslib.as_array(ch[k], shape=(n_int + 1, nch)).copy()
+        v = np.ctypeslib.as_array(vr[k], shape=(100, 385, n_int)).copy()
+        lib.xmi_free(ch[k])
+        lib.xmi_free(vr[k])
+        out.append((c, v))
+    return out, np.array(done)
 
 
 class SharedTables:

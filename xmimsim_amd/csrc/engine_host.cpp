@@ -22,6 +22,9 @@
 #include <chrono>
 #include <mutex>
 #include <algorithm>
+#include <atomic>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "engine_kernels.h"
@@ -116,12 +119,19 @@ struct DeviceTables {
   double ip_acc_E0 = 0, ip_acc_inv = 0;
 };
 
-/* per-process engine state: one device, one stream */
+/* device copies of the tables of each xmi_hdf5F handle, shared by all engines */
+static std::mutex g_tables_lock;
+static std::vector<DeviceTables *> g_tables;
+
+#define ENG_MAX_ORDERS_PROFILED 16
+
+/* one engine: a stream and its working buffers (the default engine serves
+ * xmi_main_msim; xmi_main_msim_batch runs several side by side) */
 struct Engine {
   std::mutex lock;
   bool ready = false;
   hipStream_t stream = nullptr;
-  std::vector<DeviceTables *> tables;
+  hipEvent_t prof_ev[ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED] = {};
   /* solid-angle grid cache */
   const void *sa_key = nullptr;
   uint64_t sa_hash = 0;
@@ -135,30 +145,29 @@ struct Engine {
   DevBuf state, vr, tallies, counters, queue;
 };
 
-#define ENG_MAX_ORDERS_PROFILED 16
-static hipEvent_t prof_ev[ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED];
-
 static Engine g_engine;
 
-static int engine_init() {
-  if (g_engine.ready) return 1;
+static int engine_init(Engine &E) {
+  if (E.ready) return 1;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     xmi_set_error("no HIP device available");
     return 0;
   }
-  if (hipStreamCreateWithFlags(&g_engine.stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess) {
     xmi_set_error("hipStreamCreate failed");
     return 0;
   }
-  g_engine.ready = true;
+  E.ready = true;
   return 1;
 }
+static int engine_init() { return engine_init(g_engine); }
 
 static double uniform_inv(const double *a, int n) { return (double)(n - 1) / (a[n - 1] - a[0]); }
 
 static DeviceTables *get_tables(const xmi_hdf5F *h) {
-  for (DeviceTables *t : g_engine.tables)
+  std::lock_guard<std::mutex> g(g_tables_lock);
+  for (DeviceTables *t : g_tables)
     if (t->owner == h) return t;
   DeviceTables *t = new DeviceTables();
   t->owner = h;
@@ -246,17 +255,17 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
     delete t;
     return nullptr;
   }
-  g_engine.tables.push_back(t);
+  g_tables.push_back(t);
   return t;
 }
 
 /* released through xmi_hdf5F's destructor hook */
 extern "C" void xmi_engine_forget_tables(const void *h) {
-  std::lock_guard<std::mutex> g(g_engine.lock);
-  for (size_t k = 0; k < g_engine.tables.size(); k++)
-    if (g_engine.tables[k]->owner == h) {
-      delete g_engine.tables[k];
-      g_engine.tables.erase(g_engine.tables.begin() + k);
+  std::lock_guard<std::mutex> g(g_tables_lock);
+  for (size_t k = 0; k < g_tables.size(); k++)
+    if (g_tables[k]->owner == h) {
+      delete g_tables[k];
+      g_tables.erase(g_tables.begin() + k);
       return;
     }
 }
@@ -276,20 +285,20 @@ static uint64_t sampled_hash(const xmi_solid_angle *sa) {
   return hsh;
 }
 
-static int ensure_grid(const xmi_solid_angle *sa) {
+static int ensure_grid(Engine &E, const xmi_solid_angle *sa) {
   uint64_t hsh = sampled_hash(sa);
-  if (g_engine.sa_key == sa && g_engine.sa_hash == hsh) return 1;
+  if (E.sa_key == sa && E.sa_hash == hsh) return 1;
   size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
-  if (upload(g_engine.sa_grid, sa->solid_angles, tot) != hipSuccess ||
-      upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n) != hipSuccess ||
-      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n) != hipSuccess) {
+  if (upload(E.sa_grid, sa->solid_angles, tot) != hipSuccess ||
+      upload(E.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n) != hipSuccess ||
+      upload(E.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n) != hipSuccess) {
     xmi_set_error("solid-angle grid upload failed");
     return 0;
   }
-  g_engine.sa_key = sa;
-  g_engine.sa_hash = hsh;
-  g_engine.sa_nr = (int)sa->grid_dims_r_n;
-  g_engine.sa_nt = (int)sa->grid_dims_theta_n;
+  E.sa_key = sa;
+  E.sa_hash = hsh;
+  E.sa_nr = (int)sa->grid_dims_r_n;
+  E.sa_nt = (int)sa->grid_dims_theta_n;
   return 1;
 }
 
@@ -332,9 +341,9 @@ typedef unsigned __int128 u128;
 static inline u128 fx_get(const unsigned long long *b) { return ((u128)b[1] << 64) | (u128)b[0]; }
 static inline double fx_to_double(u128 v, int shift) { return ldexp((double)v, -shift); }
 
-extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
-                             xmi_main_options *options, double **brute_historyPtr, double **var_red_historyPtr,
-                             xmi_solid_angle *solid_angles) {
+static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
+                    double **channelsPtr, xmi_main_options *options, double **brute_historyPtr,
+                    double **var_red_historyPtr, xmi_solid_angle *solid_angles) {
   auto t_wall0 = std::chrono::steady_clock::now();
   memset(&g_stats, 0, sizeof(g_stats));
   xmi_inputF *in = (xmi_inputF *)inputFPtr;
@@ -364,12 +373,11 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     return 0;
   }
   if (n_mpi_hosts < 1) n_mpi_hosts = 1;
-  std::lock_guard<std::mutex> guard(g_engine.lock);
-  if (!engine_init()) return 0;
+  if (!engine_init(E)) return 0;
   DeviceTables *T = get_tables(h);
   if (!T) return 0;
-  if (!ensure_grid(solid_angles)) return 0;
-  hipStream_t st = g_engine.stream;
+  if (!ensure_grid(E, solid_angles)) return 0;
+  hipStream_t st = E.stream;
   const int nZ = T->nZ;
   const int n_int = in->general.n_interactions_trajectory;
   const int nch = in->detector.nchannels;
@@ -452,6 +460,16 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     }
   }
   P.lay_eoff[NL] = (int)lay_edge.size();
+  /* the same lookup on the host (as d_layer_mu_tab on the device) */
+  auto layer_mu_tab = [&](int l, double E) -> double {
+    const xrl_ectx c = xrl_energy(E);
+    if (!(c.E > 0.0)) return 0.0;
+    const double *T = &lay_rc[(size_t)l * XRL_NCS];
+    const int eo = P.lay_eoff[l], ne = P.lay_eoff[l + 1] - eo;
+    int K = lay_kstart[(size_t)l * XRL_NCS + c.ci];
+    while (K < ne && lay_edge[eo + K] > c.E) K++;
+    return c.pe * lay_pb[eo + l + K] + (T[c.ci] + c.cf * (T[c.ci + 1] - T[c.ci]));
+  };
 
   /* attenuation at the fluorescence lines, and the estimator's active lines,
    * for the elements of this input (a table handle may hold more) */
@@ -464,7 +482,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
       const xrl_atom *a = xrl_atom_get(T->Zs[u]);
       for (int line = 1; line <= XMI_NLINES_VR; line++) {
         double E = a->line_energy[line];
-        pmu[((size_t)l * nZ + u) * XMI_NLINES_VR + line - 1] = E > 0.0 ? layer_mu_host(&in->layers[l], E) : 0.0;
+        pmu[((size_t)l * nZ + u) * XMI_NLINES_VR + line - 1] = E > 0.0 ? layer_mu_tab(l, E) : 0.0;
       }
     }
   std::vector<int> actn(nZ, 0), actl((size_t)nZ * ENG_MAX_ACT, 0), acts((size_t)nZ * ENG_MAX_ACT, 0),
@@ -612,7 +630,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
   P.tally_shift = shift;
 
   {
-    Engine &G = g_engine;
+    Engine &G = E;
     /* histories: each source's first `count` ids, all sources back to back in
      * one run order, so that every batch is full whatever the source sizes */
     int64_t total_hist = 0;
@@ -783,7 +801,7 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
     HIP_TRY(hipEventCreate(&evs[1]));
     if (g_profile)
       for (int k = 0; k < ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED; k++)
-        if (!prof_ev[k]) HIP_TRY(hipEventCreate(&prof_ev[k]));
+        if (!G.prof_ev[k]) HIP_TRY(hipEventCreate(&G.prof_ev[k]));
     HIP_TRY(hipEventRecord(evs[0], st));
     /* batches of `cap` consecutive positions of the run order */
     {
@@ -793,12 +811,12 @@ extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int
         int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
         for (int n = 1; n <= n_int; n++)
           HIP_TRY(eng_launch_order(&P, &S, &V, n, count, n_cu, st,
-                                   prof ? &prof_ev[ENG_ORDER_EVENTS * (n - 1)] : nullptr));
+                                   prof ? &G.prof_ev[ENG_ORDER_EVENTS * (n - 1)] : nullptr));
         if (prof) {
           HIP_TRY(hipStreamSynchronize(st));
           for (int n = 0; n < n_int; n++) {
             float t[ENG_ORDER_EVENTS - 1];
-            const hipEvent_t *e = &prof_ev[ENG_ORDER_EVENTS * n];
+            const hipEvent_t *e = &G.prof_ev[ENG_ORDER_EVENTS * n];
             for (int k = 0; k + 1 < ENG_ORDER_EVENTS; k++) HIP_TRY(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
             g_stats.step_kernel_ms += t[0];
             g_stats.vr_geom_ms += t[1];
@@ -901,6 +919,93 @@ fail:
 /* ------------------------------------------------------------------------ */
 /* solid-angle grid (xmi_solid_angle_inputs_f, src/xmi_solid_angle_f.F90:62-301,
  * then the per-cell estimate of :432-710 on the GPU) */
+
+extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
+                             xmi_main_options *options, double **brute_historyPtr, double **var_red_historyPtr,
+                             xmi_solid_angle *solid_angles) {
+  std::lock_guard<std::mutex> guard(g_engine.lock);
+  return msim_run(g_engine, inputFPtr, hdf5FPtr, n_mpi_hosts, channelsPtr, options, brute_historyPtr,
+                  var_red_historyPtr, solid_angles);
+}
+
+/* ------------------------------------------------------------------------ */
+/* many samples: a work queue of samples served by several engines, each with
+ * its own HIP stream and buffers, so that one sample's host set-up and small
+ * launches overlap other samples' kernels; the device tables of the shared
+ * table handle are uploaded once */
+
+static std::mutex g_pool_lock;
+static std::vector<Engine *> g_pool;
+
+static double sample_histories(const xmi_inputF *in, int n_mpi_hosts) {
+  double n = (double)in->general.n_photons_line / n_mpi_hosts * in->n_discrete;
+  if (in->n_continuous > 1) n += (double)in->general.n_photons_interval / n_mpi_hosts * (in->n_continuous - 1);
+  return n;
+}
+
+extern "C" int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
+                                   xmi_main_options *options, xmi_solid_angle **solid_angles, double **channels,
+                                   double **var_red_history, int n_streams, double *done_ms) {
+  if (n_samples < 0 || (n_samples > 0 && (!inputs || !hdf5FPtr || !options || !solid_angles || !channels ||
+                                          !var_red_history))) {
+    xmi_set_error("xmi_main_msim_batch: NULL argument");
+    return 0;
+  }
+  if (n_streams < 1) n_streams = 1;
+  if (n_streams > 8) n_streams = 8;
+  if (n_mpi_hosts < 1) n_mpi_hosts = 1;
+  {
+    std::lock_guard<std::mutex> g(g_pool_lock);
+    while ((int)g_pool.size() < n_streams) g_pool.push_back(new Engine());
+  }
+  /* longest first: the queue hands out the largest budgets before the small ones */
+  std::vector<int> order(n_samples);
+  for (int k = 0; k < n_samples; k++) {
+    order[k] = k;
+    channels[k] = nullptr;
+    var_red_history[k] = nullptr;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return sample_histories((const xmi_inputF *)inputs[a], n_mpi_hosts) >
+           sample_histories((const xmi_inputF *)inputs[b], n_mpi_hosts);
+  });
+  std::atomic<int> next(0), failed(0);
+  std::string first_error;
+  std::mutex err_lock;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&](int w) {
+    Engine &E = *g_pool[w];
+    std::lock_guard<std::mutex> guard(E.lock);
+    for (;;) {
+      const int q = next.fetch_add(1);
+      if (q >= n_samples || failed.load()) break;
+      const int k = order[q];
+      double *brute = nullptr;
+      if (!msim_run(E, inputs[k], hdf5FPtr, n_mpi_hosts, &channels[k], options, &brute, &var_red_history[k],
+                    solid_angles[k])) {
+        std::lock_guard<std::mutex> g(err_lock);
+        if (!failed.exchange(1)) first_error = xmi_msim_last_error();
+        break;
+      }
+      free(brute);
+      if (done_ms)
+        done_ms[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int w = 0; w < n_streams; w++) threads.emplace_back(worker, w);
+  for (std::thread &t : threads) t.join();
+  if (failed.load()) {
+    for (int k = 0; k < n_samples; k++) {
+      free(channels[k]);
+      free(var_red_history[k]);
+      channels[k] = var_red_history[k] = nullptr;
+    }
+    xmi_set_error("xmi_main_msim_batch: %s", first_error.c_str());
+    return 0;
+  }
+  return 1;
+}
 
 static double depth_quantile(const xmi_inputF *in, double energy, double R) {
   int nl = in->n_layers;

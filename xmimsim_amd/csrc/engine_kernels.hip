@@ -788,7 +788,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA
 }
 
 template <int NL, bool use_lds>
-__global__ void __launch_bounds__(ENG_BLOCK) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
+__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
   /* LDS tallies (when they fit): channels [nch][2], Rayleigh/Compton rows
    * [nZ][2][2]; flushed once per workgroup */
   extern __shared__ unsigned long long lds[];
@@ -814,7 +814,6 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_scatter(EngineParams P, Photon
       const int mine = active && cur == L;
       if (__ballot(mine) == 0ull) continue;
       double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, pesc = 0.0, mu_cur = 1.0;
-      double dist[NL];
       int dir = 1, nc = 0;
       phx_stream s = {};
       if (mine) {
@@ -830,8 +829,6 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_scatter(EngineParams P, Photon
         phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
         s.ctr[0] = V.blk[i];
       }
-#pragma unroll
-      for (int k = 0; k < NL; k++) dist[k] = mine ? V.dist[(size_t)k * cap + i] : 0.0;
       const int off = P.layer_off[L - 1], nel = P.layer_nel[L - 1];
       for (int k = 0; k < nel; k++) {
         const int u = P.el_u[off + k];
@@ -876,12 +873,12 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_scatter(EngineParams P, Photon
           if (!failed) {
             double m2 = 0.0;
             xrl_ectx ce = xrl_energy(energy);
-#pragma unroll
-            for (int kk = 0; kk < NL; kk++) {
-              if (kk < nc) {
-                const int l0 = L - 1 + kk * dir;
-                m2 += d_layer_mu_tab(P, l0, &ce) * P.layer_density[l0] * dist[kk];
-              }
+            /* path lengths from the record (cached), not registers: the loop
+             * stays rolled whatever the layer count */
+#pragma unroll 1
+            for (int kk = 0; kk < nc; kk++) {
+              const int l0 = L - 1 + kk * dir;
+              m2 += d_layer_mu_tab(P, l0, &ce) * P.layer_density[l0] * V.dist[(size_t)kk * cap + i];
             }
             double Pesc_comp = exp(-m2);
             twc = Pconv * (omega * xrl_dcsp_compt(a, E, theta, phi)) * Pesc_comp * w / 1.0;

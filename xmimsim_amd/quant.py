@@ -10,9 +10,10 @@ net-line intensities by damped Gauss-Newton on log10 intensities:
   - screened: Gauss-Newton on the surrogate ensemble (finite-difference
     Jacobians, one batched forward pass per iteration for all samples); then a
     Monte Carlo run at the answer confirms it, refined while needed by steps
-    with the Monte Carlo residual and the surrogate Jacobian; samples whose
-    ensemble spread exceeds the gate, or that stay above the tolerance, fall
-    back to the plain loop;
+    with the Monte Carlo residual and the surrogate Jacobian (corrected by
+    the secants of successive Monte Carlo runs); samples whose ensemble
+    spread exceeds the gate get twice the refinement steps, and samples still
+    above the tolerance finish with a few plain iterations;
   - plain: the same Gauss-Newton loop with every function value and Jacobian
     column from a Monte Carlo run (common random numbers across columns).
 """
@@ -177,12 +178,15 @@ def _surrogate_jacobians(sg: S.Surrogate, samples, W, h=0.2):
 
 
 def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], meas, meas_mask, photons: int,
-                      iters: int = 15, gate: float = 0.02, tol: float = 0.02, hybrid_iters: int = 4, seed: int = 0):
+                      iters: int = 15, gate: float = 0.02, tol: float = 0.02, hybrid_iters: int = 4,
+                      fallback_iters: int = 3, seed: int = 0):
     """Gauss-Newton on the surrogate for all samples at once; then per sample
     Monte Carlo confirmation, refined while needed by Gauss-Newton steps that
     take the residual from the Monte Carlo run and the Jacobian from the
-    surrogate; samples gated by ensemble spread, or still above tol, fall back
-    to the plain Monte Carlo loop.  Returns (w_hat list, info dict)."""
+    surrogate, Broyden-corrected by the Monte Carlo secants; samples gated by
+    ensemble spread get twice the refinement steps; samples still above tol
+    finish with fallback_iters plain Monte Carlo iterations.  Returns (w_hat
+    list, info dict)."""
     n = len(samples)
     W = [np.full(len(s.Z), 1.0 / len(s.Z)) for s in samples]
     for it in range(iters):
@@ -201,25 +205,42 @@ def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], me
     line_err = np.full(n, np.inf)
     mc_calls = np.zeros(n, int)
     fallback = 0
+    trace = []
     for i, s in enumerate(samples):
         mk = meas_mask[i] > 0
         # the Monte Carlo checks every answer; where the ensemble disagrees it
-        # also supplies the residual of each refinement step (more of them)
+        # also supplies the residual of each refinement step (more of them).
+        # The refinement runs use one seed (common random numbers), so that
+        # successive runs differ by the composition change only, and each
+        # run's secant corrects the surrogate Jacobian (Broyden's update).
         n_ref = hybrid_iters * (2 if gated[i] else 1)
+        J = t_prev = u_prev = None
+        errs = []
         for h_it in range(n_ref + 1):
-            t, _ = engine.lines(s, W[i], photons, seed=seed + 7919 * i + h_it)
+            t, _ = engine.lines(s, W[i], photons, seed=seed + 7919 * i)
             mc_calls[i] += 1
             line_err[i] = _line_err(t, meas[i], mk)
+            errs.append(line_err[i])
             if line_err[i] <= tol or h_it == n_ref:
                 break
-            _, J, _ = _surrogate_jacobians(sg, [s], [W[i]])[0]
-            W[i] = _gn_update(W[i], J[mk], t[mk] - meas[i][mk], 1e-3)
+            u = np.log(W[i])
+            if J is None:
+                J = _surrogate_jacobians(sg, [s], [W[i]])[0][1][mk]
+            else:
+                du, df = u - u_prev, t[mk] - t_prev
+                J = J + np.outer(df - J @ du, du) / max(float(du @ du), 1e-30)
+            t_prev, u_prev = t[mk], u
+            W[i] = _gn_update(W[i], J, t[mk] - meas[i][mk], 1e-3)
         if line_err[i] > tol:
+            # a few plain Monte Carlo iterations from the refined composition
             fallback += 1
-            W[i], line_err[i], k = _mc_gn(engine, s, meas[i], meas_mask[i], photons, seed + 7919 * i + 100, W[i], tol)
+            W[i], line_err[i], k = _mc_gn(engine, s, meas[i], meas_mask[i], photons, seed + 7919 * i + 100, W[i], tol,
+                                          iters=fallback_iters)
             mc_calls[i] += k
+            errs.append(line_err[i])
+        trace.append(errs)
     return W, {"spread": spread, "gated": int(gated.sum()), "fallback": fallback, "line_err": line_err,
-               "mc_calls": mc_calls}
+               "mc_calls": mc_calls, "trace": trace}
 
 
 def _mc_gn(engine: Engine, s: Sample, meas, meas_mask, photons, seed, w0, tol, iters: int = 8):
@@ -303,6 +324,7 @@ def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int 
                                                                for w, s in zip(W, samples)]))}
         if name == "screened":
             out[name].update({"gated": info["gated"], "fallback": info["fallback"]})
+        out[name]["mc_calls"] = [int(c) for c in info["mc_calls"]]
         log(f"{name}: {out[name]}")
     out["speedup"] = out["screened"]["samples_per_s"] / out["plain"]["samples_per_s"]
     sg.close()

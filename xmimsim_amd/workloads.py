@@ -14,6 +14,8 @@ casO4_kat            the reference's CaSO4 known-answer set-up
 """
 from __future__ import annotations
 
+from typing import List
+
 import numpy as np
 
 from .api import ContinuousPoint, DiscreteLine, Input, Layer
@@ -87,3 +89,20 @@ def random_sample(rng: np.random.Generator, n_layers=(1, 5), n_elements=(2, 30),
         layers.append(Layer(Z, w, rho, d))
     E = float(rng.uniform(*energy))
     return Input(layers=layers, discrete=[DiscreteLine(E, 1e9, 0.0)], gain=0.02, nchannels=2048)
+
+
+def batch_samples(n: int = 1024, seed: int = 7, budget=(1e5, 1e7), n_layers=(1, 5), n_elements=(2, 30),
+                  zrange=(11, 92), alpha=0.5, density=(1.0, 20.0), thickness=(1e-4, 1.0),
+                  energy=(10.0, 40.0)) -> List[Input]:
+    """Independent samples for the batched engine: 1-5 layers of 2-30 elements
+    (Z 11-92, Dirichlet(0.5) weight fractions, density uniform 1-20 g/cm3,
+    thickness log-uniform 1 um - 1 cm), one monochromatic line uniform in
+    10-40 keV, photon budget log-uniform 1e5-1e7, 4 interactions."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        inp = random_sample(rng, n_layers, n_elements, zrange, alpha, density, thickness, energy)
+        inp.n_photons_line = int(np.exp(rng.uniform(np.log(budget[0]), np.log(budget[1]))))
+        inp.n_interactions_trajectory = 4
+        out.append(inp)
+    return out
