@@ -129,6 +129,12 @@ struct EngineParams {
   const int4 *fl_info;          /* {u, q, shell, compact bin} */
   const double *fl_wf;          /* weight fraction of u in the layer * yield * rate */
   const double *fl_mr;          /* [entry][n_layers] mu*rho of the line in each layer */
+  /* first-order shell factor of each entry as a step function of E:
+   * coefficient fl_cf[j] below breakpoint fl_bp[j - 1] (descending), fl_cf[0]
+   * above all; fl_nb breakpoints per entry (4 or 9), unused ones -inf */
+  const double *fl_bp;          /* [entry][fl_nb] */
+  const double *fl_cf;          /* [entry][fl_nb + 1] */
+  int fl_nb;
   const int *lbin_g;            /* compact bin -> u * ENG_MAX_ACT + q */
   int n_lbins;
   /* azimuth and solid angle */
@@ -150,6 +156,13 @@ struct EngineParams {
   unsigned long long *counters; /* [slot ENG_CNT_*][ENG_SPREAD]: queue sizes, events, fallbacks, errors */
   int *queue;                   /* [ENG_NQUEUES][ENG_SPREAD][qreg] */
   int qreg;                     /* entries per sub-queue */
+  /* layer lists of an order's estimator events: the slots of the photons
+   * whose interaction lies in layer l, in slot order, so that every
+   * wavefront of the scatter and line kernels holds photons of one layer */
+  int *lay_list;                /* [n_layers][cap] */
+  int *lay_woff;                /* [n_layers][n_waves]: per-wave counts, then exclusive offsets */
+  int *lay_total;               /* [n_layers] entries of each list */
+  int n_waves;                  /* row length of lay_woff: waves of a one-thread-per-slot launch over cap */
   int cap;
 };
 

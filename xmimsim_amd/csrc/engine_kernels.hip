@@ -482,6 +482,31 @@ __device__ __forceinline__ int d_dequeue(const EngineParams &P, int t, unsigned 
   return P.queue[((size_t)t * ENG_SPREAD + j) * P.qreg + r];
 }
 
+/* layer lists: k_step stores each wave's number of events per layer (0 = no
+ * event), k_layer_scan turns them into offsets, k_vr_geom writes the slots.
+ * Both launches run one thread per slot with the same workgroups, so a slot
+ * has the same wave and lane in both.  Call from converged code. */
+__device__ __forceinline__ void d_layer_count(const EngineParams &P, int layer) {
+  const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  for (int L = 1; L <= P.n_layers; L++) {
+    const unsigned long long m = __ballot(layer == L);
+    if (__lane_id() == 0) P.lay_woff[(size_t)(L - 1) * P.n_waves + w] = __popcll(m);
+  }
+}
+
+__device__ __forceinline__ void d_layer_list(const EngineParams &P, int layer, int idx) {
+  const unsigned int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned long long below = (1ull << __lane_id()) - 1ull;
+  for (int L = 1; L <= P.n_layers; L++) {
+    const unsigned long long m = __ballot(layer == L);
+    if (layer == L) {
+      const int pos = P.lay_woff[(size_t)(L - 1) * P.n_waves + w] + __popcll(m & below);
+      if ((unsigned int)pos < (unsigned int)P.cap) P.lay_list[(size_t)(L - 1) * P.cap + pos] = idx;
+      else atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull); /* counts of k_step and k_vr_geom disagree */
+    }
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* forced interaction + atom and interaction-type selection */
 
@@ -491,7 +516,7 @@ __device__ __forceinline__ int d_dequeue(const EngineParams &P, int t, unsigned 
 template <int NL>
 __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S, int n, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int type = -1;
+  int type = -1, ev_layer = 0;
   const size_t cap = (size_t)P.cap;
   if (i < count && S.layer[i] > 0) {
     double E = S.energy[i];
@@ -560,6 +585,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
       S.z[i] = coords[2] + tsum * dirv[2];
       S.weight[i] = w;
       S.layer[i] = my;
+      /* an estimator event of this order (the test of k_vr_geom) */
+      ev_layer = E > 1.0 ? my : 0;
 
       /* atom selection, src/xmi_main.F90:1558-1572 */
       double R2 = phx_uniform(&s);
@@ -597,6 +624,33 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
     }
   }
   d_enqueue(P, type, i);
+  d_layer_count(P, ev_layer);
+}
+
+/* exclusive scan of one layer's per-wave counts (workgroup l - 1 scans layer
+ * l), in place, and the layer's total; nw = waves of the k_step launch */
+__global__ void __launch_bounds__(1024) k_layer_scan(EngineParams P, int nw) {
+  __shared__ int part[1024];
+  int *row = P.lay_woff + (size_t)blockIdx.x * P.n_waves;
+  const int per = (nw + 1023) / 1024;
+  const int b = threadIdx.x * per, e = min(b + per, nw);
+  int sum = 0;
+  for (int k = b; k < e; k++) sum += row[k];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - sum;
+  for (int k = b; k < e; k++) {
+    const int c = row[k];
+    row[k] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) P.lay_total[blockIdx.x] = part[1023];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -648,6 +702,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_geom(EngineParams P, PhotonSoA
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t cap = (size_t)P.cap;
   int active = (i < count) && S.layer[i] > 0 && S.energy[i] > 1.0;
+  const int ev_layer = active ? S.layer[i] : 0;
   int cur = 0, need_sa = 0, info = 0;
   if (active) {
     phx_stream s;
@@ -749,6 +804,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_geom(EngineParams P, PhotonSoA
     }
   }
   if (i < count) V.info[i] = info;
+  d_layer_list(P, ev_layer, i);
   d_append(P, ENG_CNT_Q_SA, need_sa, i);
   d_wave_count(d_ctr(P, ENG_CNT_FALLBACKS), need_sa);
   d_wave_count(d_ctr(P, ENG_CNT_VR), active);
@@ -787,8 +843,26 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA
   }
 }
 
+/* walk over the layer lists in chunks of 64 entries: chunk c of the
+ * concatenation of every layer's chunks (layer 1 first); a wavefront takes
+ * chunks in increasing order, so the layer only moves forward (wave-uniform) */
+struct LayerChunks {
+  int L, tot;
+  long long c_begin, c_end;
+};
+
+__device__ __forceinline__ bool d_chunk_next(const EngineParams &P, LayerChunks &w, long long c) {
+  while (c >= w.c_end && w.L < P.n_layers) {
+    w.L++;
+    w.tot = P.lay_total[w.L - 1];
+    w.c_begin = w.c_end;
+    w.c_end += (w.tot + 63) / 64;
+  }
+  return c < w.c_end;
+}
+
 template <int NL, bool use_lds>
-__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
+__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): channels [nch][2], Rayleigh/Compton rows
    * [nZ][2][2]; flushed once per workgroup */
   extern __shared__ unsigned long long lds[];
@@ -804,16 +878,17 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
   }
   const int shift = P.tally_shift;
-  const int stride = gridDim.x * blockDim.x;
-  for (int base = blockIdx.x * blockDim.x; base < count; base += stride) {
-    const int i = base + threadIdx.x;
-    const int info = i < count ? V.info[i] : 0;
-    const int active = info & VR_ACTIVE;
-    const int cur = active ? S.layer[i] : 0;
-    for (int L = 1; L <= P.n_layers; L++) {
-      const int mine = active && cur == L;
-      if (__ballot(mine) == 0ull) continue;
-      double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, pesc = 0.0, mu_cur = 1.0;
+  const long long wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+  LayerChunks lc = {0, 0, 0, 0};
+  for (long long c = wave; d_chunk_next(P, lc, c); c += waves) {
+    {
+      const int L = lc.L;
+      const int e = (int)((c - lc.c_begin) * 64) + __lane_id();
+      const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
+      const int info = e < lc.tot ? V.info[i] : 0;
+      const int mine = info & VR_ACTIVE;
+      double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, pesc = 0.0, mu_cur = 1.0, sth2 = 0.0;
+      xrl_dcsp_ctx dc = {0.0, 0.0, 0.0};
       int dir = 1, nc = 0;
       phx_stream s = {};
       if (mine) {
@@ -828,6 +903,9 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         nc = info >> VR_NC_SHIFT;
         phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
         s.ctr[0] = V.blk[i];
+        /* the angular factors are the same for every element */
+        dc = xrl_dcsp_setup(E, theta, phi);
+        sth2 = sin(theta / 2.0);
       }
       const int off = P.layer_off[L - 1], nel = P.layer_nel[L - 1];
       for (int k = 0; k < nel; k++) {
@@ -837,7 +915,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         /* Rayleigh */
         double twr = 0.0;
         if (mine) {
-          twr = Pconv * (omega * xrl_dcsp_rayl(a, E, theta, phi)) * pesc * w;
+          twr = Pconv * (omega * xrl_dcsp_rayl_c(a, &dc)) * pesc * w;
           int ch = (E >= 1.0) ? (int)((E - P.zero) / P.gain) : -1;
           if (ch >= 0 && ch <= P.nch - 1 && twr > 0.0) fx_add_global(&tch[2 * ch], twr, shift);
         }
@@ -848,7 +926,6 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
           const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
           double energy = E * 1000.0;
           double c_lamb0 = cc / energy;
-          double sth2 = sin(theta / 2.0);
           const double *rn = P.cp_rn;
           const double *icdf = P.cp_icdf + (size_t)u * P.cp_len;
           const double drn = rn[1] - rn[0];
@@ -881,7 +958,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
               m2 += d_layer_mu_tab(P, l0, &ce) * P.layer_density[l0] * V.dist[(size_t)kk * cap + i];
             }
             double Pesc_comp = exp(-m2);
-            twc = Pconv * (omega * xrl_dcsp_compt(a, E, theta, phi)) * Pesc_comp * w / 1.0;
+            twc = Pconv * (omega * xrl_dcsp_compt_c(a, &dc)) * Pesc_comp * w / 1.0;
             int ch = (energy >= 1.0) ? (int)((energy - P.zero) / P.gain) : -1;
             if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) fx_add_global(&tch[2 * ch], twc, shift);
           } else {
@@ -902,8 +979,18 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-template <int NL, bool use_lds>
-__global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
+/* lines phase.  Per (line t, photon) term:
+ *   wf_t * P_shell(t, E) * g * exp(-sum_k mu_rho(t, layer k) * d_k)
+ * with g = Omega / 4 pi * weight / mu(E) of the photon.  P_shell of a first
+ * interaction (or of a photon not coming from fluorescence) is
+ * E^-p * shell_C[cascade][k(E)][shell], k(E) the number of edges above E: a
+ * step function of E, held per line as breakpoints (descending) and the
+ * coefficient of every piece (MAXB breakpoints: 4 covers the K and L shells, 9
+ * the M shells), so that it costs a few compares and selects instead of a
+ * count over nine edges and a gather.  Photons coming from fluorescence use
+ * the precalculated cascade table of src/xmi_variance_reduction.F90:540-580. */
+template <int NL, int MAXB, bool use_lds>
+__global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): compact line bins [n_lbins][2] */
   extern __shared__ unsigned long long lds[];
   const size_t n_line2 = (size_t)P.n_lbins * 2;
@@ -918,85 +1005,91 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
   const int shift = P.tally_shift;
   const int lane = __lane_id();
   const int smax = P.use_M_lines ? 8 : 3;
-  const int stride = gridDim.x * blockDim.x;
-  for (int base = blockIdx.x * blockDim.x; base < count; base += stride) {
-    /* this lane's photon: the wavefront's 64 photons are walked below */
-    const int i = base + threadIdx.x;
-    const int info = i < count ? V.info[i] : 0;
+  const long long wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+  LayerChunks lc = {0, 0, 0, 0};
+  for (long long c = wave; d_chunk_next(P, lc, c); c += waves) {
+    /* this lane's photon: the chunk's 64 photons (all in layer L) are walked below */
+    const int L = lc.L;
+    const int e = (int)((c - lc.c_begin) * 64) + lane;
+    const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
+    const int info = e < lc.tot ? V.info[i] : 0;
     const int active = info & VR_ACTIVE;
-    int cur = 0, dir = 1, prev_line = 0, prev_u = 0;
-    double E = 0.0, pe = 0.0, gline = 0.0;
+    /* packed per-photon integers: layers crossed (bits 0-4), upwards (5),
+     * first-order shell factor (6), previous line (7-15), its element (16-) */
+    int pk = 0;
+    double E = 0.0, gq = 0.0;
     double dist[NL];
     if (active) {
-      cur = S.layer[i];
-      dir = (info & VR_UP) ? 1 : -1;
       E = S.energy[i];
-      pe = xrl_energy(E).pe;
-      /* P_dir * weight / mu(E): the per-photon factor of every line term */
-      gline = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(cur - 1) * cap + i];
-      prev_line = S.prev_line[i];
-      prev_u = S.prev_u[i];
+      const double g = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(L - 1) * cap + i];
+      const int prev_line = S.prev_line[i], prev_u = S.prev_u[i];
+      const int fo = !(n > 1 && prev_line > 0);
+      /* E^-p joins the photon factor on the first-order path */
+      gq = fo ? xrl_energy(E).pe * g : g;
+      pk = (info >> VR_NC_SHIFT) | ((info & VR_UP) ? 32 : 0) | (fo ? 64 : ((prev_line << 7) | (prev_u << 16)));
     }
 #pragma unroll
     for (int k = 0; k < NL; k++) dist[k] = active ? V.dist[(size_t)k * cap + i] : 0.0;
-    for (int L = 1; L <= P.n_layers; L++) {
-      const unsigned long long m = __ballot(active && cur == L);
-      if (m == 0ull) continue;
-      const int t0 = P.fl_off[L - 1], T = P.fl_off[L] - t0;
-      for (int tb = 0; tb < T; tb += 64) {
-        const int ok = tb + lane < T;
-        const int t = ok ? t0 + tb + lane : t0;
-        const int4 info_t = P.fl_info[t];
-        const double wf = P.fl_wf[t];
-        const xrl_atom *a = &P.atoms[info_t.x];
-        double edge[9];
+    const unsigned long long m = __ballot(active);
+    if (m == 0ull) continue;
+    const int t0 = P.fl_off[L - 1], T = P.fl_off[L] - t0;
+    for (int tb = 0; tb < T; tb += 64) {
+      const int ok = tb + lane < T;
+      const int t = ok ? t0 + tb + lane : t0;
+      const int4 info_t = P.fl_info[t];
+      const double wf = P.fl_wf[t];
+      double bp[MAXB], cf[MAXB + 1];
 #pragma unroll
-        for (int q = 0; q < 9; q++) edge[q] = a->edge[q];
-        const double *shC = &a->shell_C[P.cascade - 1][0][0];
-        /* mu*rho of this line in the k-th layer crossed, upwards and downwards */
-        double mr_up[NL], mr_dn[NL];
+      for (int j = 0; j < MAXB; j++) bp[j] = P.fl_bp[(size_t)t * MAXB + j];
 #pragma unroll
-        for (int kk = 0; kk < NL; kk++) {
-          const int lu = L - 1 + kk, ld = L - 1 - kk;
-          mr_up[kk] = lu < P.n_layers ? P.fl_mr[(size_t)t * P.n_layers + lu] : 0.0;
-          mr_dn[kk] = ld >= 0 ? P.fl_mr[(size_t)t * P.n_layers + ld] : 0.0;
-        }
-        unsigned long long *bin =
-            use_lds ? &tline[(size_t)info_t.w * 2] : &tline[((size_t)info_t.x * ENG_MAX_ACT + info_t.y) * 2];
-        /* this lane's line summed over the wave's photons (fixed lane order,
-         * so deterministic), then one fixed-point add per wavefront */
-        double acc = 0.0;
-        unsigned long long mm = m;
-        while (mm) {
-          const int p = __ffsll((long long)mm) - 1;
-          mm &= mm - 1ull;
-          const double Ep = d_readlane(E, p), pep = d_readlane(pe, p), gp = d_readlane(gline, p);
-          const int dirp = __builtin_amdgcn_readlane(dir, p);
-          const int plp = __builtin_amdgcn_readlane(prev_line, p), pup = __builtin_amdgcn_readlane(prev_u, p);
-          double dk[NL];
+      for (int j = 0; j <= MAXB; j++) cf[j] = P.fl_cf[(size_t)t * (MAXB + 1) + j];
+      /* mu*rho of this line in the k-th layer crossed, upwards and downwards */
+      double mr_up[NL], mr_dn[NL];
 #pragma unroll
-          for (int kk = 0; kk < NL; kk++) dk[kk] = d_readlane(dist[kk], p);
-          if (!ok) continue;
-          double Ps;
-          if (n > 1 && plp > 0) {
-            Ps = info_t.z <= smax
-                     ? P.precalc_xrf[(((size_t)info_t.x * 9 + info_t.z) * P.nZ + pup) * XMI_NLINES_VR_DEV + (plp - 1)]
-                     : 0.0;
-          } else {
-            int kE = 0;
-#pragma unroll
-            for (int q = 0; q < 9; q++) kE += edge[q] > Ep ? 1 : 0;
-            Ps = (!P.use_M_lines && info_t.z >= 4) ? 0.0 : pep * shC[kE * 9 + info_t.z];
-          }
-          if (Ps != 0.0) {
-            double m2 = 0.0;
-#pragma unroll
-            for (int kk = 0; kk < NL; kk++) m2 += (dirp > 0 ? mr_up[kk] : mr_dn[kk]) * dk[kk];
-            acc += wf * Ps * gp * exp(-m2);
-          }
-        }
-        if (ok && acc > 0.0) fx_add_global(bin, acc, shift);
+      for (int kk = 0; kk < NL; kk++) {
+        const int lu = L - 1 + kk, ld = L - 1 - kk;
+        mr_up[kk] = lu < P.n_layers ? P.fl_mr[(size_t)t * P.n_layers + lu] : 0.0;
+        mr_dn[kk] = ld >= 0 ? P.fl_mr[(size_t)t * P.n_layers + ld] : 0.0;
       }
+      const double *pre = P.precalc_xrf + ((size_t)info_t.x * 9 + info_t.z) * P.nZ * XMI_NLINES_VR_DEV;
+      unsigned long long *bin =
+          use_lds ? &tline[(size_t)info_t.w * 2] : &tline[((size_t)info_t.x * ENG_MAX_ACT + info_t.y) * 2];
+      /* this lane's line summed over the chunk's photons (fixed lane order,
+       * so deterministic), then one fixed-point add per wavefront */
+      double acc = 0.0;
+      unsigned long long mm = m;
+      while (mm) {
+        const int p = __ffsll((long long)mm) - 1;
+        mm &= mm - 1ull;
+        /* the photon's scalars are wave-uniform from here on */
+        const double Ep = d_readlane(E, p), gp = d_readlane(gq, p);
+        const int pkp = __builtin_amdgcn_readlane(pk, p);
+        const int ncp = pkp & 31;
+        double dk[NL];
+#pragma unroll
+        for (int kk = 0; kk < NL; kk++) dk[kk] = kk < ncp ? d_readlane(dist[kk], p) : 0.0;
+        double Ps;
+        if (pkp & 64) {
+          Ps = cf[0];
+#pragma unroll
+          for (int j = 0; j < MAXB; j++)
+            if (Ep < bp[j]) Ps = cf[j + 1];
+        } else {
+          Ps = info_t.z <= smax ? pre[(size_t)(pkp >> 16) * XMI_NLINES_VR_DEV + (((pkp >> 7) & 511) - 1)] : 0.0;
+        }
+        double m2 = 0.0;
+        if (pkp & 32) {
+#pragma unroll
+          for (int kk = 0; kk < NL; kk++)
+            if (kk < ncp) m2 += mr_up[kk] * dk[kk];
+        } else {
+#pragma unroll
+          for (int kk = 0; kk < NL; kk++)
+            if (kk < ncp) m2 += mr_dn[kk] * dk[kk];
+        }
+        if (ok && Ps != 0.0) acc += wf * Ps * gp * exp(-m2);
+      }
+      if (ok && acc > 0.0) fx_add_global(bin, acc, shift);
     }
   }
   if (use_lds) {
@@ -1299,6 +1392,19 @@ static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks)
   return vb < 1 ? 1 : vb;
 }
 
+template <int NL, int MAXB>
+static void launch_lines(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
+                         hipStream_t st) {
+  const size_t ll = lines_lds_bytes(P);
+  if (ll) {
+    int vb = resident_blocks((const void *)k_vr_lines<NL, MAXB, true>, ll, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, MAXB, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
+  } else {
+    int vb = resident_blocks((const void *)k_vr_lines<NL, MAXB, false>, 0, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, MAXB, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+  }
+}
+
 template <int NL>
 static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count,
                                 int n_cu, int blocks, hipStream_t st, hipEvent_t *ev) {
@@ -1309,20 +1415,14 @@ static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const
   const size_t ls = scatter_lds_bytes(P);
   if (ls) {
     int vb = resident_blocks((const void *)k_vr_scatter<NL, true>, ls, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_scatter<NL, true>), dim3(vb), dim3(ENG_BLOCK), ls, st, *P, *S, *V, n, count);
+    hipLaunchKernelGGL((k_vr_scatter<NL, true>), dim3(vb), dim3(ENG_BLOCK), ls, st, *P, *S, *V, n);
   } else {
     int vb = resident_blocks((const void *)k_vr_scatter<NL, false>, 0, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_scatter<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+    hipLaunchKernelGGL((k_vr_scatter<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   }
   if (ev) (void)hipEventRecord(ev[2], st);
-  const size_t ll = lines_lds_bytes(P);
-  if (ll) {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, true>, ll, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n, count);
-  } else {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, false>, 0, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
-  }
+  if (P->fl_nb <= 4) launch_lines<NL, 4>(P, S, V, n, n_cu, blocks, st);
+  else launch_lines<NL, 9>(P, S, V, n, n_cu, blocks, st);
   if (ev) (void)hipEventRecord(ev[3], st);
 }
 
@@ -1368,6 +1468,7 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
     else if (nl <= 8) hipLaunchKernelGGL((k_step<8>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
     else hipLaunchKernelGGL((k_step<16>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
   }
+  hipLaunchKernelGGL(k_layer_scan, dim3(P->n_layers), dim3(1024), 0, st, *P, blocks * (ENG_BLOCK / 64));
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
   const int qblocks = (int)(((size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);

@@ -249,18 +249,42 @@ XRL_FN double xrl_dcs_compt(const xrl_atom *a, double E, double theta) {
   return kn * xrl_sf_compt(a, xrl_moment_transf(E, theta)) * XRL_AVOG_BARN / a->A;
 }
 
-/* linearly polarised differential cross sections (DCSP_*), cm2/g/sr */
+/* linearly polarised differential cross sections (DCSP_*), cm2/g/sr.  The
+ * factors that depend on (E, theta, phi) only are formed once in an
+ * xrl_dcsp_ctx and shared by every element scattering at that geometry. */
+typedef struct xrl_dcsp_ctx {
+  double x;       /* momentum transfer */
+  double thomson; /* r_e^2 (1 - sin^2 theta cos^2 phi) */
+  double kn;      /* polarised Klein-Nishina */
+} xrl_dcsp_ctx;
+
+XRL_FN xrl_dcsp_ctx xrl_dcsp_setup(double E, double theta, double phi) {
+  xrl_dcsp_ctx d;
+  double c = cos(theta), st = sin(theta), cp = cos(phi);
+  double k = 1.0 / (1.0 + E / XRL_MEC2 * (1.0 - c));
+  d.x = xrl_moment_transf(E, theta);
+  d.thomson = XRL_RE2 * (1.0 - st * st * cp * cp);
+  d.kn = XRL_RE2 * 0.5 * k * k * (k + 1.0 / k - 2.0 * st * st * cp * cp);
+  return d;
+}
+
+XRL_FN double xrl_dcsp_rayl_c(const xrl_atom *a, const xrl_dcsp_ctx *d) {
+  double F = xrl_ff_rayl(a, d->x);
+  return d->thomson * F * F * XRL_AVOG_BARN / a->A;
+}
+
+XRL_FN double xrl_dcsp_compt_c(const xrl_atom *a, const xrl_dcsp_ctx *d) {
+  return d->kn * xrl_sf_compt(a, d->x) * XRL_AVOG_BARN / a->A;
+}
+
 XRL_FN double xrl_dcsp_rayl(const xrl_atom *a, double E, double theta, double phi) {
-  double st = sin(theta), cp = cos(phi);
-  double F = xrl_ff_rayl(a, xrl_moment_transf(E, theta));
-  return XRL_RE2 * (1.0 - st * st * cp * cp) * F * F * XRL_AVOG_BARN / a->A;
+  xrl_dcsp_ctx d = xrl_dcsp_setup(E, theta, phi);
+  return xrl_dcsp_rayl_c(a, &d);
 }
 
 XRL_FN double xrl_dcsp_compt(const xrl_atom *a, double E, double theta, double phi) {
-  double c = cos(theta), st = sin(theta), cp = cos(phi);
-  double k = 1.0 / (1.0 + E / XRL_MEC2 * (1.0 - c));
-  double kn = XRL_RE2 * 0.5 * k * k * (k + 1.0 / k - 2.0 * st * st * cp * cp);
-  return kn * xrl_sf_compt(a, xrl_moment_transf(E, theta)) * XRL_AVOG_BARN / a->A;
+  xrl_dcsp_ctx d = xrl_dcsp_setup(E, theta, phi);
+  return xrl_dcsp_compt_c(a, &d);
 }
 
 /* total Compton profile (electrons per atomic unit of momentum) */
