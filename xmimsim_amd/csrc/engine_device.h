@@ -128,13 +128,7 @@ struct EngineParams {
   int fl_off[ENG_MAX_LAYERS + 1];
   const int4 *fl_info;          /* {u, q, shell, compact bin} */
   const double *fl_wf;          /* weight fraction of u in the layer * yield * rate */
-  const double *fl_mr;          /* [entry][n_layers] mu*rho of the line in each layer */
-  /* first-order shell factor of each entry as a step function of E:
-   * coefficient fl_cf[j] below breakpoint fl_bp[j - 1] (descending), fl_cf[0]
-   * above all; fl_nb breakpoints per entry (4 or 9), unused ones -inf */
-  const double *fl_bp;          /* [entry][fl_nb] */
-  const double *fl_cf;          /* [entry][fl_nb + 1] */
-  int fl_nb;
+  const double *fl_mr;          /* [entry][layer bound of the kernels] mu*rho of the line per layer, 0 past the stack */
   const int *lbin_g;            /* compact bin -> u * ENG_MAX_ACT + q */
   int n_lbins;
   /* azimuth and solid angle */

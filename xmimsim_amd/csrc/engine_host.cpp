@@ -139,7 +139,7 @@ struct Engine {
   int sa_nr = 0, sa_nt = 0;
   /* per-call buffers (grow only) */
   DevBuf el_Z, el_w, el_u, precalc_mu, act_n, act_line, act_shell, act_factor, act_E, act_ch, act_murho;
-  DevBuf fl_info, fl_wf, fl_mr, fl_bp, fl_cf, lbin_g;
+  DevBuf fl_info, fl_wf, fl_mr, lbin_g;
   DevBuf lay_rc, lay_edge, lay_pb, lay_kstart;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total;
@@ -527,37 +527,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   std::vector<int4> fl_info;
   std::vector<double> fl_wf;
   std::vector<double> fl_mr;
-  /* first-order shell factor shell_C[cascade][k(E)][shell] of each entry as
-   * a step function: breakpoints at the element's distinct edges
-   * (descending), pieces with equal coefficients merged; k(E) counts the edges
-   * above E, as xrl_edges_above does */
-  const int fl_nb = options->use_M_lines ? 9 : 4;
-  std::vector<double> fl_bp, fl_cf;
-  auto shell_steps = [&](const xrl_atom *a, int z) -> bool {
-    std::vector<double> ev;
-    for (int q = 0; q < 9; q++)
-      if (a->edge[q] > 0.0) ev.push_back(a->edge[q]);
-    std::sort(ev.begin(), ev.end(), [](double x, double y) { return x > y; });
-    ev.erase(std::unique(ev.begin(), ev.end()), ev.end());
-    const double *C = &a->shell_C[P.cascade - 1][0][0];
-    std::vector<double> bp, cf{C[0 * 9 + z]};
-    for (double b : ev) {
-      int k = 0; /* edges above any E just below b */
-      for (int q = 0; q < 9; q++) k += a->edge[q] >= b ? 1 : 0;
-      const double c = C[k * 9 + z];
-      if (c == cf.back() && std::signbit(c) == std::signbit(cf.back())) continue;
-      bp.push_back(b);
-      cf.push_back(c);
-    }
-    if ((int)bp.size() > fl_nb) return false;
-    while ((int)bp.size() < fl_nb) {
-      bp.push_back(-INFINITY);
-      cf.push_back(cf.back());
-    }
-    fl_bp.insert(fl_bp.end(), bp.begin(), bp.end());
-    fl_cf.insert(fl_cf.end(), cf.begin(), cf.end());
-    return true;
-  };
+  const int fl_stride = NL <= 1 ? 1 : NL <= 2 ? 2 : NL <= 4 ? 4 : NL <= 8 ? 8 : 16;
   for (int l = 0; l < NL; l++) {
     P.fl_off[l] = (int)fl_info.size();
     for (int k = 0; k < P.layer_nel[l]; k++) {
@@ -566,17 +536,13 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         size_t o = (size_t)u * ENG_MAX_ACT + q;
         fl_info.push_back(make_int4(u, q, acts[o], act_base[u] + q));
         fl_wf.push_back(elW[(size_t)P.layer_off[l] + k] * actf[o]);
-        if (!shell_steps(xrl_atom_get(T->Zs[u]), acts[o])) {
-          xmi_set_error("xmi_main_msim: shell factor of Z=%d shell %d has more than %d steps", T->Zs[u], acts[o],
-                        fl_nb);
-          return 0;
-        }
-        for (int l2 = 0; l2 < NL; l2++) fl_mr.push_back(actm[((size_t)l2 * nZ + u) * ENG_MAX_ACT + q]);
+        /* padded to the kernels' layer bound, zeros past the stack */
+        for (int l2 = 0; l2 < fl_stride; l2++)
+          fl_mr.push_back(l2 < NL ? actm[((size_t)l2 * nZ + u) * ENG_MAX_ACT + q] : 0.0);
       }
     }
   }
   P.fl_off[NL] = (int)fl_info.size();
-  P.fl_nb = fl_nb;
 
   /* sources: continuous intervals, then discrete lines */
   std::vector<SourceDesc> src;
@@ -697,6 +663,10 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     /* layer lists: a row of cap slots per layer; per-wave offsets of the
      * k_step launch (ceil(cap / ENG_BLOCK) workgroups of ENG_BLOCK / 64 waves) */
     P.n_waves = step_waves;
+    if (step_waves > 32 * 1024) {
+      xmi_set_error("xmi_main_msim: %d waves per batch exceed the layer scan's 32768", step_waves);
+      return 0;
+    }
     HIP_TRY(G.lay_list.ensure((size_t)NL * cap * sizeof(int)));
     HIP_TRY(G.lay_woff.ensure((size_t)NL * step_waves * sizeof(int)));
     HIP_TRY(G.lay_total.ensure((size_t)ENG_MAX_LAYERS * sizeof(int)));
@@ -714,8 +684,6 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size()));
     HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size()));
     HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size()));
-    HIP_TRY(upload(G.fl_bp, fl_bp.data(), fl_bp.size()));
-    HIP_TRY(upload(G.fl_cf, fl_cf.data(), fl_cf.size()));
     HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size()));
     HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size()));
     HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size()));
@@ -766,8 +734,6 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.fl_info = G.fl_info.as<int4>();
     P.fl_wf = G.fl_wf.as<double>();
     P.fl_mr = G.fl_mr.as<double>();
-    P.fl_bp = G.fl_bp.as<double>();
-    P.fl_cf = G.fl_cf.as<double>();
     P.lbin_g = G.lbin_g.as<int>();
     P.lay_rc = G.lay_rc.as<double>();
     P.lay_edge = G.lay_edge.as<double>();

@@ -628,29 +628,37 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
 }
 
 /* exclusive scan of one layer's per-wave counts (workgroup l - 1 scans layer
- * l), in place, and the layer's total; nw = waves of the k_step launch */
+ * l), in place, and the layer's total; nw = waves of the k_step launch, at
+ * most ENG_SCAN_MAX (32 per thread: the thread's counts are loaded at once
+ * and summed, then one wave scan and one pass over the 16 wave totals) */
+#define ENG_SCAN_MAX (1024 * 32)
 __global__ void __launch_bounds__(1024) k_layer_scan(EngineParams P, int nw) {
-  __shared__ int part[1024];
+  __shared__ int wsum[16];
   int *row = P.lay_woff + (size_t)blockIdx.x * P.n_waves;
-  const int per = (nw + 1023) / 1024;
-  const int b = threadIdx.x * per, e = min(b + per, nw);
-  int sum = 0;
-  for (int k = b; k < e; k++) sum += row[k];
-  part[threadIdx.x] = sum;
+  const int b = threadIdx.x * 32, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int v[32], s = 0;
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    v[j] = b + j < nw ? row[b + j] : 0;
+    s += v[j];
+  }
+  int incl = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[w] = incl;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  int base = 0;
+  for (int k = 0; k < w; k++) base += wsum[k];
+  int run = base + incl - s;
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    if (b + j < nw) row[b + j] = run;
+    run += v[j];
   }
-  int run = part[threadIdx.x] - sum;
-  for (int k = b; k < e; k++) {
-    const int c = row[k];
-    row[k] = run;
-    run += c;
-  }
-  if (threadIdx.x == 1023) P.lay_total[blockIdx.x] = part[1023];
+  if (threadIdx.x == 1023) P.lay_total[blockIdx.x] = base + incl;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -691,6 +699,56 @@ __device__ __forceinline__ double d_readlane(double v, int lane) {
   unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
   unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* read-only tables through the constant address space: wave-uniform reads
+ * become scalar loads (the tables are not written while a kernel runs) */
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *d_const(const T *p) {
+  return (const __attribute__((address_space(4))) T *)p;
+}
+
+/* one DPP lane move of a double (two 32-bit moves); lanes the row mask
+ * leaves out, or whose source lane lies outside the row, read 0 */
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double d_dpp0(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+
+/* sum of v over the wavefront, in a fixed order (row prefix sums, then the
+ * row totals), returned wave-uniform: DPP moves keep the reduction in the
+ * VALU, without the LDS round trips of a shuffle butterfly */
+__device__ __forceinline__ double d_wave_total(double v) {
+  v += d_dpp0<0x111, 0xf>(v); /* row_shr:1 */
+  v += d_dpp0<0x112, 0xf>(v); /* row_shr:2 */
+  v += d_dpp0<0x114, 0xf>(v); /* row_shr:4 */
+  v += d_dpp0<0x118, 0xf>(v); /* row_shr:8: lane 15 of a row holds its sum */
+  v += d_dpp0<0x142, 0xa>(v); /* row_bcast:15 into rows 1 and 3 */
+  v += d_dpp0<0x143, 0xc>(v); /* row_bcast:31 into rows 2 and 3: lane 63 holds the total */
+  return d_readlane(v, 63);
+}
+
+/* e^x for x <= 0, table-driven (P. T. P. Tang, "Table-driven implementation
+ * of the exponential function in IEEE floating-point arithmetic", ACM TOMS
+ * 15(2), 1989): x = (64 m + j) ln2/64 + r with |r| <= ln2/128, so
+ * e^x = 2^m * 2^(j/64) * e^r; e^r - 1 is its Taylor polynomial of degree 5
+ * (truncation below 4e-17 relative).  tab[j] = 2^(j/64), j = 0..63.
+ * Arguments below -746 give 0, as e^x underflows there. */
+__device__ __forceinline__ double d_exp_neg(double x, const double *tab) {
+  x = fmax(x, -746.0);
+  const double n = rint(x * 92.33248261689366);          /* 64 / ln 2 */
+  double r = fma(-n, 0.010830424696249145, x);           /* ln2/64, leading part */
+  r = fma(-n, 3.623510646634843e-19, r);                 /* ln2/64, trailing part */
+  const int ni = (int)n;
+  double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  q = fma(r, q, 1.0 / 6.0);
+  q = fma(r, q, 0.5);
+  const double p = fma(r * r, q, r);
+  const double T = tab[ni & 63];
+  return ldexp(fma(T, p, T), ni >> 6);
 }
 
 #define VR_ACTIVE 1
@@ -878,7 +936,10 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
     __syncthreads();
   }
   const int shift = P.tally_shift;
-  const long long wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+  /* wave index through readfirstlane: wave-uniform for the compiler too,
+   * so that the chunk walk and the layer's data stay in scalar registers */
+  const long long wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
+                  waves = (gridDim.x * blockDim.x) >> 6;
   LayerChunks lc = {0, 0, 0, 0};
   for (long long c = wave; d_chunk_next(P, lc, c); c += waves) {
     {
@@ -907,11 +968,12 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         dc = xrl_dcsp_setup(E, theta, phi);
         sth2 = sin(theta / 2.0);
       }
-      const int off = P.layer_off[L - 1], nel = P.layer_nel[L - 1];
+      const int off = __builtin_amdgcn_readfirstlane(P.layer_off[L - 1]);
+      const int nel = __builtin_amdgcn_readfirstlane(P.layer_nel[L - 1]);
       for (int k = 0; k < nel; k++) {
-        const int u = P.el_u[off + k];
+        const int u = d_const(P.el_u)[off + k];
         const xrl_atom *a = &P.atoms[u];
-        const double Pconv = P.el_w[off + k] / mu_cur;
+        const double Pconv = d_const(P.el_w)[off + k] / mu_cur;
         /* Rayleigh */
         double twr = 0.0;
         if (mine) {
@@ -979,117 +1041,124 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-/* lines phase.  Per (line t, photon) term:
- *   wf_t * P_shell(t, E) * g * exp(-sum_k mu_rho(t, layer k) * d_k)
- * with g = Omega / 4 pi * weight / mu(E) of the photon.  P_shell of a first
- * interaction (or of a photon not coming from fluorescence) is
- * E^-p * shell_C[cascade][k(E)][shell], k(E) the number of edges above E: a
- * step function of E, held per line as breakpoints (descending) and the
- * coefficient of every piece (MAXB breakpoints: 4 covers the K and L shells, 9
- * the M shells), so that it costs a few compares and selects instead of a
- * count over nine edges and a gather.  Photons coming from fluorescence use
- * the precalculated cascade table of src/xmi_variance_reduction.F90:540-580. */
-template <int NL, int MAXB, bool use_lds>
+/* lines phase: one lane per photon of a 64-photon chunk (all in layer L);
+ * the wavefront walks the layer's (element, line) entries.  Per (line t,
+ * photon) term
+ *   wf_t * P_shell(t, photon) * g * exp(-sum_l mu_rho(t, l) * d_l)
+ * with g = Omega / 4 pi * weight / mu(E) and d_l the photon's path in layer l
+ * on its way to the detector.  The line's data are wave-uniform (scalar
+ * loads); P_shell is formed once per (element, shell) of the walk: a first
+ * interaction (or a photon not coming from fluorescence) uses
+ * E^-p * shell_C[cascade][edges above E][shell], a fluorescence photon the
+ * precalculated cascade table (src/xmi_variance_reduction.F90:540-580).  The
+ * terms of a line are summed over the chunk's lanes in a fixed butterfly
+ * order (deterministic) and added once to the line's fixed-point bin. */
+template <int NL, bool use_lds>
 __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): compact line bins [n_lbins][2] */
   extern __shared__ unsigned long long lds[];
+  __shared__ double exp_tab[64];
   const size_t n_line2 = (size_t)P.n_lbins * 2;
   const size_t cap = (size_t)P.cap;
   unsigned long long *gline = P.t_line + (size_t)(n - 1) * P.nZ * ENG_MAX_ACT * 2;
   unsigned long long *tline = gline;
+  if (threadIdx.x < 64) exp_tab[threadIdx.x] = exp2((double)threadIdx.x / 64.0);
   if (use_lds) {
     tline = lds;
     for (size_t k = threadIdx.x; k < n_line2; k += blockDim.x) lds[k] = 0ull;
-    __syncthreads();
   }
+  __syncthreads();
   const int shift = P.tally_shift;
-  const int lane = __lane_id();
   const int smax = P.use_M_lines ? 8 : 3;
-  const long long wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, waves = (gridDim.x * blockDim.x) >> 6;
+  /* wave index through readfirstlane: wave-uniform for the compiler too,
+   * so that the chunk walk and the layer's data stay in scalar registers */
+  const long long wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
+                  waves = (gridDim.x * blockDim.x) >> 6;
   LayerChunks lc = {0, 0, 0, 0};
   for (long long c = wave; d_chunk_next(P, lc, c); c += waves) {
-    /* this lane's photon: the chunk's 64 photons (all in layer L) are walked below */
     const int L = lc.L;
-    const int e = (int)((c - lc.c_begin) * 64) + lane;
+    const int e = (int)((c - lc.c_begin) * 64) + __lane_id();
     const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
     const int info = e < lc.tot ? V.info[i] : 0;
     const int active = info & VR_ACTIVE;
-    /* packed per-photon integers: layers crossed (bits 0-4), upwards (5),
-     * first-order shell factor (6), previous line (7-15), its element (16-) */
-    int pk = 0;
-    double E = 0.0, gq = 0.0;
-    double dist[NL];
+    if (__ballot(active) == 0ull) continue;
+    double E = 2.0, g = 0.0, pe = 0.0;
+    int fo = 1, prev_line = 0, prev_u = 0;
+    /* path length per layer, by layer index (0 where not crossed) */
+    double dl[NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) dl[l] = 0.0;
     if (active) {
       E = S.energy[i];
-      const double g = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(L - 1) * cap + i];
-      const int prev_line = S.prev_line[i], prev_u = S.prev_u[i];
-      const int fo = !(n > 1 && prev_line > 0);
-      /* E^-p joins the photon factor on the first-order path */
-      gq = fo ? xrl_energy(E).pe * g : g;
-      pk = (info >> VR_NC_SHIFT) | ((info & VR_UP) ? 32 : 0) | (fo ? 64 : ((prev_line << 7) | (prev_u << 16)));
+      pe = xrl_energy(E).pe;
+      g = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(L - 1) * cap + i];
+      prev_line = S.prev_line[i];
+      prev_u = S.prev_u[i];
+      fo = !(n > 1 && prev_line > 0);
+      const int nc = info >> VR_NC_SHIFT, dir = (info & VR_UP) ? 1 : -1;
+#pragma unroll
+      for (int k = 0; k < NL; k++) {
+        const double d = k < nc ? V.dist[(size_t)k * cap + i] : 0.0;
+        const int l0 = L - 1 + k * dir;
+#pragma unroll
+        for (int l = 0; l < NL; l++)
+          if (k < nc && l == l0) dl[l] = d;
+      }
     }
+    const int t0 = __builtin_amdgcn_readfirstlane(P.fl_off[L - 1]), t1 = __builtin_amdgcn_readfirstlane(P.fl_off[L]);
+    int u_cur = -1, z_cur = -1, kE = 0, any = 0;
+    double Ps = 0.0;
+    /* lane j holds the chunk's total of line t0 + 64 g + j of the current
+     * group g of 64 lines; the group is added to the tallies in one pass */
+    double lsum = 0.0;
+    const auto *fl_info = d_const((const int *)P.fl_info);
+    const auto *fl_wf = d_const(P.fl_wf);
+    const auto *fl_mr = d_const(P.fl_mr);
+    for (int t = t0; t < t1; t++) {
+      /* {element, active line, shell, compact bin} */
+      const int4 it = make_int4(fl_info[4 * t], fl_info[4 * t + 1], fl_info[4 * t + 2], fl_info[4 * t + 3]);
+      if (it.x != u_cur) {
+        /* edges of the element above the photon's energy */
+        const auto *a = d_const(&P.atoms[it.x]);
+        kE = 0;
 #pragma unroll
-    for (int k = 0; k < NL; k++) dist[k] = active ? V.dist[(size_t)k * cap + i] : 0.0;
-    const unsigned long long m = __ballot(active);
-    if (m == 0ull) continue;
-    const int t0 = P.fl_off[L - 1], T = P.fl_off[L] - t0;
-    for (int tb = 0; tb < T; tb += 64) {
-      const int ok = tb + lane < T;
-      const int t = ok ? t0 + tb + lane : t0;
-      const int4 info_t = P.fl_info[t];
-      const double wf = P.fl_wf[t];
-      double bp[MAXB], cf[MAXB + 1];
-#pragma unroll
-      for (int j = 0; j < MAXB; j++) bp[j] = P.fl_bp[(size_t)t * MAXB + j];
-#pragma unroll
-      for (int j = 0; j <= MAXB; j++) cf[j] = P.fl_cf[(size_t)t * (MAXB + 1) + j];
-      /* mu*rho of this line in the k-th layer crossed, upwards and downwards */
-      double mr_up[NL], mr_dn[NL];
-#pragma unroll
-      for (int kk = 0; kk < NL; kk++) {
-        const int lu = L - 1 + kk, ld = L - 1 - kk;
-        mr_up[kk] = lu < P.n_layers ? P.fl_mr[(size_t)t * P.n_layers + lu] : 0.0;
-        mr_dn[kk] = ld >= 0 ? P.fl_mr[(size_t)t * P.n_layers + ld] : 0.0;
+        for (int q = 0; q < 9; q++) kE += a->edge[q] > E ? 1 : 0;
+        u_cur = it.x;
+        z_cur = -1;
       }
-      const double *pre = P.precalc_xrf + ((size_t)info_t.x * 9 + info_t.z) * P.nZ * XMI_NLINES_VR_DEV;
-      unsigned long long *bin =
-          use_lds ? &tline[(size_t)info_t.w * 2] : &tline[((size_t)info_t.x * ENG_MAX_ACT + info_t.y) * 2];
-      /* this lane's line summed over the chunk's photons (fixed lane order,
-       * so deterministic), then one fixed-point add per wavefront */
-      double acc = 0.0;
-      unsigned long long mm = m;
-      while (mm) {
-        const int p = __ffsll((long long)mm) - 1;
-        mm &= mm - 1ull;
-        /* the photon's scalars are wave-uniform from here on */
-        const double Ep = d_readlane(E, p), gp = d_readlane(gq, p);
-        const int pkp = __builtin_amdgcn_readlane(pk, p);
-        const int ncp = pkp & 31;
-        double dk[NL];
-#pragma unroll
-        for (int kk = 0; kk < NL; kk++) dk[kk] = kk < ncp ? d_readlane(dist[kk], p) : 0.0;
-        double Ps;
-        if (pkp & 64) {
-          Ps = cf[0];
-#pragma unroll
-          for (int j = 0; j < MAXB; j++)
-            if (Ep < bp[j]) Ps = cf[j + 1];
+      if (it.z != z_cur) {
+        z_cur = it.z;
+        double v;
+        if (fo) {
+          v = (!P.use_M_lines && z_cur >= 4) ? 0.0 : pe * P.atoms[u_cur].shell_C[P.cascade - 1][kE][z_cur];
         } else {
-          Ps = info_t.z <= smax ? pre[(size_t)(pkp >> 16) * XMI_NLINES_VR_DEV + (((pkp >> 7) & 511) - 1)] : 0.0;
+          v = z_cur <= smax ? P.precalc_xrf[(((size_t)u_cur * 9 + z_cur) * P.nZ + prev_u) * XMI_NLINES_VR_DEV +
+                                            (prev_line - 1)]
+                            : 0.0;
         }
+        Ps = active ? v : 0.0;
+        any = __ballot(Ps != 0.0) != 0ull;
+      }
+      const int j = (t - t0) & 63;
+      if (any) {
+        /* attenuation rows are padded to NL entries (zeros past the stack) */
         double m2 = 0.0;
-        if (pkp & 32) {
 #pragma unroll
-          for (int kk = 0; kk < NL; kk++)
-            if (kk < ncp) m2 += mr_up[kk] * dk[kk];
-        } else {
-#pragma unroll
-          for (int kk = 0; kk < NL; kk++)
-            if (kk < ncp) m2 += mr_dn[kk] * dk[kk];
-        }
-        if (ok && Ps != 0.0) acc += wf * Ps * gp * exp(-m2);
+        for (int l = 0; l < NL; l++) m2 += fl_mr[(size_t)t * NL + l] * dl[l];
+        double term = Ps != 0.0 ? fl_wf[t] * Ps * g * d_exp_neg(-m2, exp_tab) : 0.0;
+        term = d_wave_total(term);
+        if (__lane_id() == j) lsum = term;
       }
-      if (ok && acc > 0.0) fx_add_global(bin, acc, shift);
+      if (j == 63 || t == t1 - 1) {
+        const int tl = t - j + __lane_id();
+        if (tl <= t && lsum > 0.0) {
+          const int4 il = P.fl_info[tl];
+          unsigned long long *bin =
+              use_lds ? &tline[(size_t)il.w * 2] : &tline[((size_t)il.x * ENG_MAX_ACT + il.y) * 2];
+          fx_add_global(bin, lsum, shift);
+        }
+        lsum = 0.0;
+      }
     }
   }
   if (use_lds) {
@@ -1392,16 +1461,16 @@ static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks)
   return vb < 1 ? 1 : vb;
 }
 
-template <int NL, int MAXB>
+template <int NL>
 static void launch_lines(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
                          hipStream_t st) {
   const size_t ll = lines_lds_bytes(P);
   if (ll) {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, MAXB, true>, ll, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, MAXB, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
+    int vb = resident_blocks((const void *)k_vr_lines<NL, true>, ll, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
   } else {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, MAXB, false>, 0, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, MAXB, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+    int vb = resident_blocks((const void *)k_vr_lines<NL, false>, 0, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   }
 }
 
@@ -1421,8 +1490,7 @@ static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const
     hipLaunchKernelGGL((k_vr_scatter<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   }
   if (ev) (void)hipEventRecord(ev[2], st);
-  if (P->fl_nb <= 4) launch_lines<NL, 4>(P, S, V, n, n_cu, blocks, st);
-  else launch_lines<NL, 9>(P, S, V, n, n_cu, blocks, st);
+  launch_lines<NL>(P, S, V, n, n_cu, blocks, st);
   if (ev) (void)hipEventRecord(ev[3], st);
 }
 

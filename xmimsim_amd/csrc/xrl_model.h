@@ -151,14 +151,16 @@ XRL_FN xrl_ectx xrl_energy(double E) {
     c.ci = 0;
     return c;
   }
-  const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
-  double x = (log(E) - XRL_CS_LNE0) / h;
+  /* one logarithm serves the grid position and E^-p = exp(-p ln E) */
+  const double inv_h = (double)(XRL_NCS - 1) / (XRL_CS_LNE1 - XRL_CS_LNE0);
+  const double lnE = log(E);
+  double x = (lnE - XRL_CS_LNE0) * inv_h;
   int i = (int)floor(x);
   if (i < 0) i = 0;
   if (i > XRL_NCS - 2) i = XRL_NCS - 2;
   c.ci = i;
   c.cf = x - (double)i;
-  c.pe = pow(E, -XRL_PHOTO_EXPONENT);
+  c.pe = exp(-XRL_PHOTO_EXPONENT * lnE);
   return c;
 }
 
