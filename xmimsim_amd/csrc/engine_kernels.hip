@@ -924,113 +924,145 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
   /* LDS tallies (when they fit): channels [nch][2], Rayleigh/Compton rows
    * [nZ][2][2]; flushed once per workgroup */
   extern __shared__ unsigned long long lds[];
+  __shared__ double exp_tab[64];
   const size_t n_ch2 = (size_t)P.nch * 2, n_rc2 = (size_t)P.nZ * 4;
   const size_t cap = (size_t)P.cap;
   unsigned long long *gch = P.t_ch + (size_t)(n - 1) * n_ch2;
   unsigned long long *grc = P.t_rc + (size_t)(n - 1) * n_rc2;
   unsigned long long *tch = gch, *trc = grc;
+  if (threadIdx.x < 64) exp_tab[threadIdx.x] = exp2((double)threadIdx.x / 64.0);
   if (use_lds) {
     tch = lds;
     trc = lds + n_ch2;
     for (size_t k = threadIdx.x; k < n_ch2 + n_rc2; k += blockDim.x) lds[k] = 0ull;
-    __syncthreads();
   }
+  __syncthreads();
   const int shift = P.tally_shift;
+  const int lane = __lane_id();
+  const double inv_gain = 1.0 / P.gain;
+  /* the Compton profile's random-number grid is uniform */
+  const double inv_drn = 1.0 / (P.cp_rn[1] - P.cp_rn[0]);
+  /* single-precision literals of src/xmi_variance_reduction.F90:949-1008 */
+  const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
   /* wave index through readfirstlane: wave-uniform for the compiler too,
    * so that the chunk walk and the layer's data stay in scalar registers */
   const long long wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
                   waves = (gridDim.x * blockDim.x) >> 6;
   LayerChunks lc = {0, 0, 0, 0};
   for (long long c = wave; d_chunk_next(P, lc, c); c += waves) {
-    {
-      const int L = lc.L;
-      const int e = (int)((c - lc.c_begin) * 64) + __lane_id();
-      const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
-      const int info = e < lc.tot ? V.info[i] : 0;
-      const int mine = info & VR_ACTIVE;
-      double E = 0.0, w = 0.0, theta = 0.0, phi = 0.0, omega = 0.0, pesc = 0.0, mu_cur = 1.0, sth2 = 0.0;
-      xrl_dcsp_ctx dc = {0.0, 0.0, 0.0};
-      int dir = 1, nc = 0;
-      phx_stream s = {};
-      if (mine) {
-        E = S.energy[i];
-        w = S.weight[i];
-        theta = V.theta[i];
-        phi = V.phi[i];
-        omega = V.omega[i];
-        pesc = V.pesc[i];
-        mu_cur = S.mus[(size_t)(L - 1) * cap + i];
-        dir = (info & VR_UP) ? 1 : -1;
-        nc = info >> VR_NC_SHIFT;
-        phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
-        s.ctr[0] = V.blk[i];
-        /* the angular factors are the same for every element */
-        dc = xrl_dcsp_setup(E, theta, phi);
-        sth2 = sin(theta / 2.0);
+    const int L = lc.L;
+    const int e = (int)((c - lc.c_begin) * 64) + lane;
+    const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
+    const int info = e < lc.tot ? V.info[i] : 0;
+    const int mine = info & VR_ACTIVE;
+    /* per-photon factors, shared by every element of the layer */
+    double E = 0.0, wo = 0.0, pesc = 0.0, inv_mu = 0.0, c_lamb0 = 1.0, A0 = 0.0, A1 = 0.0;
+    /* rho * path in the k-th layer crossed: in registers for up to four
+     * layers, read from the records beyond (register budget) */
+    constexpr int NR = NL <= 4 ? NL : 1;
+    double rd[NR];
+    xrl_dcsp_ctx dc = {0.0, 0.0, 0.0};
+    int dir = 1, nc = 0, ch_r = -1;
+    phx_stream s = {};
+#pragma unroll
+    for (int k = 0; k < NR; k++) rd[k] = 0.0;
+    if (mine) {
+      E = S.energy[i];
+      const double theta = V.theta[i], phi = V.phi[i];
+      wo = V.omega[i] * S.weight[i];
+      pesc = V.pesc[i];
+      inv_mu = 1.0 / S.mus[(size_t)(L - 1) * cap + i];
+      dir = (info & VR_UP) ? 1 : -1;
+      nc = info >> VR_NC_SHIFT;
+      if (NL <= 4) {
+#pragma unroll
+        for (int k = 0; k < NR; k++)
+          if (k < nc) rd[k] = P.layer_density[L - 1 + k * dir] * V.dist[(size_t)k * cap + i];
       }
-      const int off = __builtin_amdgcn_readfirstlane(P.layer_off[L - 1]);
-      const int nel = __builtin_amdgcn_readfirstlane(P.layer_nel[L - 1]);
-      for (int k = 0; k < nel; k++) {
-        const int u = d_const(P.el_u)[off + k];
-        const xrl_atom *a = &P.atoms[u];
-        const double Pconv = d_const(P.el_w)[off + k] / mu_cur;
-        /* Rayleigh */
-        double twr = 0.0;
-        if (mine) {
-          twr = Pconv * (omega * xrl_dcsp_rayl_c(a, &dc)) * pesc * w;
-          int ch = (E >= 1.0) ? (int)((E - P.zero) / P.gain) : -1;
-          if (ch >= 0 && ch <= P.nch - 1 && twr > 0.0) fx_add_global(&tch[2 * ch], twr, shift);
-        }
-        d_tally_wave(&trc[((size_t)u * 2 + 0) * 2], twr, shift);
-        /* Compton, xmi_compton_varred2 */
-        double twc = 0.0;
-        if (mine) {
-          const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
-          double energy = E * 1000.0;
-          double c_lamb0 = cc / energy;
-          const double *rn = P.cp_rn;
-          const double *icdf = P.cp_icdf + (size_t)u * P.cp_len;
-          const double drn = rn[1] - rn[0];
-          int it = 0, failed = 0;
-          for (;;) {
-            double r = phx_uniform(&s);
-            int pos = (int)(r / drn);
-            if (pos == P.cp_len - 2) continue;
-            double pz = d_interp(rn[pos], icdf[pos], rn[pos + 1], icdf[pos + 1], r);
-            if (phx_uniform(&s) < 0.5) pz = -pz;
-            double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
-            energy = cc / (c_lamb0 + dlamb) / 1000.0;
-            /* a large positive pz can give a negative energy: accepted, as in
-             * the reference, where it then only reaches the history row */
-            if (energy <= E) break;
-            if (it == 100) {
-              failed = 1; /* the reference stops with an error here */
-              break;
-            }
-            it++;
+      phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
+      s.ctr[0] = V.blk[i];
+      dc = xrl_dcsp_setup(E, theta, phi);
+      const double sth2 = sin(theta / 2.0);
+      c_lamb0 = cc / (E * 1000.0);
+      A0 = c0 * sth2 * sth2;
+      A1 = c1 * c_lamb0 * sth2;
+      ch_r = E >= 1.0 ? (int)((E - P.zero) * inv_gain) : -1;
+    }
+    const int off = __builtin_amdgcn_readfirstlane(P.layer_off[L - 1]);
+    const int nel = __builtin_amdgcn_readfirstlane(P.layer_nel[L - 1]);
+    double twr_sum = 0.0; /* Rayleigh keeps the energy: one channel per photon */
+    double rsum = 0.0;    /* lane 2j + kind: row total of element k0 + j of this group */
+    for (int k = 0; k < nel; k++) {
+      const int u = d_const(P.el_u)[off + k];
+      const xrl_atom *a = &P.atoms[u];
+      const double Pconv = d_const(P.el_w)[off + k] * inv_mu;
+      /* Rayleigh */
+      double twr = 0.0;
+      if (mine) {
+        twr = Pconv * (xrl_dcsp_rayl_c(a, &dc) * wo) * pesc;
+        twr_sum += twr;
+      }
+      /* Compton, xmi_compton_varred2: Doppler-broadened energy from the
+       * element's Compton profile, continuing the photon's estimator stream */
+      double twc = 0.0;
+      if (mine) {
+        const double *rn = P.cp_rn;
+        const double *icdf = P.cp_icdf + (size_t)u * P.cp_len;
+        double energy = 0.0;
+        int it = 0, failed = 0;
+        for (;;) {
+          const double r = phx_uniform(&s);
+          const int pos = (int)(r * inv_drn);
+          if (pos == P.cp_len - 2) continue;
+          double pz = icdf[pos] + (icdf[pos + 1] - icdf[pos]) * ((r - rn[pos]) * inv_drn);
+          if (phx_uniform(&s) < 0.5) pz = -pz;
+          energy = (cc / 1000.0) / (c_lamb0 + (A0 - A1 * pz));
+          /* a large positive pz can give a negative energy: accepted, as in
+           * the reference, where it then only reaches the history row */
+          if (energy <= E) break;
+          if (it == 100) {
+            failed = 1; /* the reference stops with an error here */
+            break;
           }
-          if (!failed) {
-            double m2 = 0.0;
-            xrl_ectx ce = xrl_energy(energy);
-            /* path lengths from the record (cached), not registers: the loop
-             * stays rolled whatever the layer count */
+          it++;
+        }
+        if (!failed) {
+          const xrl_ectx ce = xrl_energy(energy);
+          double m2 = 0.0;
+          if (NL <= 4) {
+#pragma unroll
+            for (int kk = 0; kk < NR; kk++)
+              if (kk < nc) m2 += d_layer_mu_tab(P, L - 1 + kk * dir, &ce) * rd[kk];
+          } else {
 #pragma unroll 1
             for (int kk = 0; kk < nc; kk++) {
               const int l0 = L - 1 + kk * dir;
-              m2 += d_layer_mu_tab(P, l0, &ce) * P.layer_density[l0] * V.dist[(size_t)kk * cap + i];
+              m2 += d_layer_mu_tab(P, l0, &ce) * (P.layer_density[l0] * V.dist[(size_t)kk * cap + i]);
             }
-            double Pesc_comp = exp(-m2);
-            twc = Pconv * (omega * xrl_dcsp_compt_c(a, &dc)) * Pesc_comp * w / 1.0;
-            int ch = (energy >= 1.0) ? (int)((energy - P.zero) / P.gain) : -1;
-            if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) fx_add_global(&tch[2 * ch], twc, shift);
-          } else {
-            atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
-            atomicAdd(d_ctr(P, ENG_CNT_ERR_VR_DOPPLER), 1ull);
           }
+          twc = Pconv * (xrl_dcsp_compt_c(a, &dc) * wo) * d_exp_neg(-m2, exp_tab);
+          const int ch = energy >= 1.0 ? (int)((energy - P.zero) * inv_gain) : -1;
+          if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) fx_add_global(&tch[2 * ch], twc, shift);
+        } else {
+          atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
+          atomicAdd(d_ctr(P, ENG_CNT_ERR_VR_DOPPLER), 1ull);
         }
-        d_tally_wave(&trc[((size_t)u * 2 + 1) * 2], twc, shift);
+      }
+      /* history rows: wave totals collected in lanes, 32 elements a group */
+      const double tr = d_wave_total(twr), tc = d_wave_total(twc);
+      const int j = k & 31;
+      if (lane == 2 * j) rsum = tr;
+      if (lane == 2 * j + 1) rsum = tc;
+      if (j == 31 || k == nel - 1) {
+        const int kl = k - j + (lane >> 1);
+        if (kl <= k && rsum > 0.0) {
+          const int ul = d_const(P.el_u)[off + kl];
+          fx_add_global(&trc[((size_t)ul * 2 + (lane & 1)) * 2], rsum, shift);
+        }
+        rsum = 0.0;
       }
     }
+    if (ch_r >= 0 && ch_r <= P.nch - 1 && twr_sum > 0.0) fx_add_global(&tch[2 * ch_r], twr_sum, shift);
   }
   if (use_lds) {
     __syncthreads();
