@@ -16,6 +16,8 @@ typedef struct {
   uint32_t c0, c1, c2, c3;
   double pending;
   int has_pending;
+  uint32_t words[4]; /* word mode: the rest of the current block */
+  int n_words;
 } orc_rng;
 
 static inline void orc_philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -43,6 +45,7 @@ static inline void orc_rng_init(orc_rng *r, uint64_t seed, uint32_t word1, uint6
   r->c2 = (uint32_t)history;
   r->c3 = (uint32_t)(history >> 32);
   r->has_pending = 0;
+  r->n_words = 0;
 }
 
 static inline double orc_word_to_double(uint32_t hi, uint32_t lo) {
@@ -60,6 +63,21 @@ static inline double orc_uniform(orc_rng *r) {
   r->pending = orc_word_to_double(c[2], c[3]);
   r->has_pending = 1;
   return orc_word_to_double(c[0], c[1]);
+}
+
+/* 32-bit words from the following blocks, four per block in order (the
+ * estimator's Doppler trials draw one word each: 31 bits of random number and
+ * a sign bit); a pending double of orc_uniform is dropped */
+static inline uint32_t orc_word(orc_rng *r) {
+  if (r->n_words == 0) {
+    uint32_t c[4] = {r->c0, r->c1, r->c2, r->c3};
+    orc_philox(c, r->k0, r->k1);
+    r->c0++;
+    for (int k = 0; k < 4; k++) r->words[k] = c[3 - k];
+    r->n_words = 4;
+    r->has_pending = 0;
+  }
+  return r->words[--r->n_words];
 }
 
 #endif

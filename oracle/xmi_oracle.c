@@ -718,12 +718,14 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
       int it = 0, failed = 0;
       int nrn = z->cp_len;
       for (;;) {
-        double r = orc_uniform(rng);
+        /* one 32-bit word per trial: 31 bits of random number, a sign bit */
+        const uint32_t w = orc_word(rng);
+        double r = (double)(w >> 1) * 0x1.0p-31;
         int pos = (int)(r / (z->cp_random_numbers[1] - z->cp_random_numbers[0]));
         if (pos == nrn - 2) continue; /* last interval excluded */
         double pz = interpolate_simple(z->cp_random_numbers[pos], z->cp_total_icdf[pos], z->cp_random_numbers[pos + 1],
                                        z->cp_total_icdf[pos + 1], r);
-        if (orc_uniform(rng) < 0.5) pz = -pz;
+        if (w & 1u) pz = -pz;
         double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
         energy = cc / (c_lamb0 + dlamb) / 1000.0;
         if (energy <= p->energy) break; /* also accepts negative energies, as the reference does */

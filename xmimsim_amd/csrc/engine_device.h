@@ -153,6 +153,7 @@ struct EngineParams {
   /* layer lists of an order's estimator events: the slots of the photons
    * whose interaction lies in layer l, in slot order, so that every
    * wavefront of the scatter and line kernels holds photons of one layer */
+  unsigned long long *cone_acc; /* [ENG_SPREAD * qreg] off-grid cone estimates: completed segments << 40 | hits */
   int *lay_list;                /* [n_layers][cap] */
   int *lay_woff;                /* [n_layers][n_waves]: per-wave counts, then exclusive offsets */
   int *lay_total;               /* [n_layers] entries of each list */

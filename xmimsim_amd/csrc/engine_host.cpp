@@ -142,7 +142,7 @@ struct Engine {
   DevBuf fl_info, fl_wf, fl_mr, lbin_g;
   DevBuf lay_rc, lay_edge, lay_pb, lay_kstart;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
-  DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total;
+  DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, cone_acc;
 };
 
 static Engine g_engine;
@@ -670,6 +670,14 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(G.lay_list.ensure((size_t)NL * cap * sizeof(int)));
     HIP_TRY(G.lay_woff.ensure((size_t)NL * step_waves * sizeof(int)));
     HIP_TRY(G.lay_total.ensure((size_t)ENG_MAX_LAYERS * sizeof(int)));
+    /* off-grid cone accumulators: zero here, and cleared again by the
+     * kernel after each photon's estimate */
+    {
+      const size_t cone_bytes = (size_t)ENG_SPREAD * P.qreg * sizeof(unsigned long long);
+      const bool fresh = G.cone_acc.n < cone_bytes || !G.cone_acc.p;
+      HIP_TRY(G.cone_acc.ensure(cone_bytes));
+      if (fresh) HIP_TRY(hipMemsetAsync(G.cone_acc.p, 0, cone_bytes, st));
+    }
     HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size()));
     HIP_TRY(upload(G.el_w, elW.data(), elW.size()));
     HIP_TRY(upload(G.el_u, elU.data(), elU.size()));
@@ -766,6 +774,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.lay_list = G.lay_list.as<int>();
     P.lay_woff = G.lay_woff.as<int>();
     P.lay_total = G.lay_total.as<int>();
+    P.cone_acc = G.cone_acc.as<unsigned long long>();
 
     PhotonSoA S;
     char *sp = G.state.as<char>();

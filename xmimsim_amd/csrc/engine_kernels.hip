@@ -178,12 +178,16 @@ __device__ __forceinline__ int d_sa_setup(const SaGeom g, double r1, double thet
   return 1;
 }
 
-/* one cone direction from two uniforms; 1 if it reaches the detector window */
+/* one cone direction from two uniforms; 1 if it reaches the detector window.
+ * theta = acos(1 - u1 (1 - cos apex)) enters only through its cosine and
+ * sine, taken directly (cos theta is the argument, sin theta = sqrt(1 - cos^2)
+ * >= 0 on [0, pi]); phi = 2 pi u2 through sincospi (no argument reduction) */
 __device__ __forceinline__ int d_sa_trial(const SaCone &c, double u1, double u2) {
-  double th = acos(1.0 - u1 * (1.0 - c.cos_apex));
-  double ph = u2 * 2.0 * ENG_PI;
-  double sth = sin(th);
-  double c0 = sth * cos(ph), c1 = sth * sin(ph), c2 = cos(th);
+  const double x = 1.0 - u1 * (1.0 - c.cos_apex);
+  const double sth = sqrt((1.0 - x) * (1.0 + x));
+  double sp, cp;
+  sincospi(2.0 * u2, &sp, &cp);
+  double c0 = sth * cp, c1 = sth * sp, c2 = x;
   /* rotation matrix rows (1,0,0), (0,-sin,-cos), (0,cos,-sin) */
   double v0 = 1.0 * c0 + 0.0 * c1 + 0.0 * c2;
   double v1 = 0.0 * c0 + (-c.st) * c1 + (-c.ct) * c2;
@@ -213,28 +217,6 @@ __device__ __forceinline__ double d_single_solid_angle_impl(const SaGeom g, doub
   return c.cone_sa * (double)nhit / (double)hits;
 }
 
-/* One off-grid cone estimate run by a whole wavefront (all arguments
- * wave-uniform): trial t uses Philox block `first + t` of the photon's
- * estimator stream (key, ctr1..3), i.e. exactly the uniforms 2t and 2t+1 of
- * the sequential loop that starts at block `first`, and the lanes' hit counts
- * are summed as integers.  Returns the estimate; *open = 0 when the
- * collimator hides the window (no draws). */
-__device__ __forceinline__ double d_cone_estimate_wave(const SaGeom g, double r1, double theta1, long hits,
-                                                       const uint32_t key[2], const uint32_t ctr[4], int *open) {
-  SaCone c;
-  *open = d_sa_setup(g, r1, theta1, &c);
-  long nhit = 0;
-  if (*open) {
-    for (long t = __lane_id(); t < hits; t += 64) {
-      uint32_t o[4];
-      uint32_t cc[4] = {ctr[0] + (uint32_t)t, ctr[1], ctr[2], ctr[3]};
-      phx_4x32_10(cc, key, o);
-      nhit += d_sa_trial(c, phx_u01(o[0], o[1]), phx_u01(o[2], o[3]));
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) nhit += __shfl_xor(nhit, off);
-  return *open ? c.cone_sa * (double)nhit / (double)hits : 0.0;
-}
 
 /* ------------------------------------------------------------------------ */
 /* solid-angle grid: one thread per (r, theta) cell, Philox stream per cell */
@@ -869,34 +851,63 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_geom(EngineParams P, PhotonSoA
   for (int L = 1; L <= P.n_layers; L++) d_wave_count(d_ctr(P, ENG_CNT_LAYER0 + L - 1), active && cur == L);
 }
 
-/* persistent: every wavefront takes queued photons w, w + W, ... of the
- * off-grid queue (entries flattened over the ENG_SPREAD sub-queues) */
+/* persistent: the off-grid queue's photons (entries flattened over the
+ * ENG_SPREAD sub-queues) times ENG_CONE_SEGS(hits) segments of
+ * ENG_CONE_SEG trials are the work items; a wavefront takes items w, w + W,
+ * ...  Trial t of a photon uses block ctr0 + t of its estimator stream, as
+ * the serial loop does, and each segment adds its integer hit count and one
+ * completion to the photon's packed accumulator (completions in bits 40 and
+ * up); the wavefront that completes the photon writes its estimate and clears
+ * the accumulator for the next launch.  Integer sums: the estimate equals the
+ * serial one whatever the order of the segments. */
+#define ENG_CONE_SEG 1024
+#define ENG_CONE_DONE_SHIFT 40
 __global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   const int lane = __lane_id();
-  const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const unsigned int waves = (gridDim.x * blockDim.x) >> 6;
   /* sub-queue sizes, one per lane, and their inclusive prefix sums */
   const unsigned long long sz = P.counters[(size_t)ENG_CNT_Q_SA * ENG_SPREAD + lane];
   unsigned long long incl = sz;
   for (int off = 1; off < 64; off <<= 1) {
-    unsigned long long v = __shfl_up(incl, off);
+    const unsigned long long v = __shfl_up(incl, off);
     if (lane >= off) incl += v;
   }
   const unsigned long long total = __shfl(incl, 63);
+  const long hits = P.hits_per_single;
+  const unsigned long long nseg = hits > 0 ? (unsigned long long)((hits + ENG_CONE_SEG - 1) / ENG_CONE_SEG) : 1ull;
   const SaGeom g = d_sa_geom(P);
-  for (unsigned long long e = wave; e < total; e += waves) {
+  for (unsigned long long item = wave; item < total * nseg; item += waves) {
+    const unsigned long long e = item / nseg, sg = item - e * nseg;
     /* sub-queue holding entry e: first lane whose inclusive prefix exceeds e */
     const int j = __ffsll((long long)__ballot(incl > e)) - 1;
     const unsigned long long before = __shfl(incl - sz, j);
     const int i = P.queue[((size_t)ENG_CNT_Q_SA * ENG_SPREAD + j) * P.qreg + (e - before)];
     phx_stream s;
     phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
-    s.ctr[0] = V.blk[i];
-    int open = 0;
-    const double om = d_cone_estimate_wave(g, V.sar[i], V.sat[i], P.hits_per_single, s.key, s.ctr, &open);
+    const uint32_t ctr0 = V.blk[i];
+    SaCone cone;
+    const int open = d_sa_setup(g, V.sar[i], V.sat[i], &cone);
+    unsigned long long nhit = 0;
+    if (open) {
+      const long t1 = min((long)((sg + 1) * ENG_CONE_SEG), hits);
+      for (long t = (long)(sg * ENG_CONE_SEG) + lane; t < t1; t += 64) {
+        uint32_t o[4];
+        const uint32_t cc[4] = {ctr0 + (uint32_t)t, s.ctr[1], s.ctr[2], s.ctr[3]};
+        phx_4x32_10(cc, s.key, o);
+        nhit += d_sa_trial(cone, phx_u01(o[0], o[1]), phx_u01(o[2], o[3]));
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) nhit += __shfl_xor(nhit, off);
     if (lane == 0) {
-      V.omega[i] = om;
-      if (open) V.blk[i] = s.ctr[0] + (unsigned int)P.hits_per_single;
+      unsigned long long *acc = P.cone_acc + e;
+      const unsigned long long old = atomicAdd(acc, (1ull << ENG_CONE_DONE_SHIFT) + nhit);
+      if ((old >> ENG_CONE_DONE_SHIFT) + 1ull == nseg) {
+        const unsigned long long all = (old & ((1ull << ENG_CONE_DONE_SHIFT) - 1ull)) + nhit;
+        V.omega[i] = open ? cone.cone_sa * (double)all / (double)hits : 0.0;
+        if (open) V.blk[i] = ctr0 + (uint32_t)hits;
+        *acc = 0ull;
+      }
     }
   }
 }
@@ -963,7 +974,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
     double rd[NR];
     xrl_dcsp_ctx dc = {0.0, 0.0, 0.0};
     int dir = 1, nc = 0, ch_r = -1;
-    phx_stream s = {};
+    phx_wstream s = {};
 #pragma unroll
     for (int k = 0; k < NR; k++) rd[k] = 0.0;
     if (mine) {
@@ -979,8 +990,19 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         for (int k = 0; k < NR; k++)
           if (k < nc) rd[k] = P.layer_density[L - 1 + k * dir] * V.dist[(size_t)k * cap + i];
       }
-      phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
-      s.ctr[0] = V.blk[i];
+      /* the estimator stream from the block after the geometry and cone
+       * draws, as words */
+      {
+        phx_stream s0;
+        phx_stream_init(&s0, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
+        s.key[0] = s0.key[0];
+        s.key[1] = s0.key[1];
+        s.ctr[0] = V.blk[i];
+        s.ctr[1] = s0.ctr[1];
+        s.ctr[2] = s0.ctr[2];
+        s.ctr[3] = s0.ctr[3];
+        s.avail = 0;
+      }
       dc = xrl_dcsp_setup(E, theta, phi);
       const double sth2 = sin(theta / 2.0);
       c_lamb0 = cc / (E * 1000.0);
@@ -1011,11 +1033,13 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         double energy = 0.0;
         int it = 0, failed = 0;
         for (;;) {
-          const double r = phx_uniform(&s);
+          /* one 32-bit word per trial: 31 bits of random number, a sign bit */
+          const uint32_t wd = phx_word(&s);
+          const double r = (double)(wd >> 1) * 0x1.0p-31;
           const int pos = (int)(r * inv_drn);
           if (pos == P.cp_len - 2) continue;
           double pz = icdf[pos] + (icdf[pos + 1] - icdf[pos]) * ((r - rn[pos]) * inv_drn);
-          if (phx_uniform(&s) < 0.5) pz = -pz;
+          if (wd & 1u) pz = -pz;
           energy = (cc / 1000.0) / (c_lamb0 + (A0 - A1 * pz));
           /* a large positive pz can give a negative energy: accepted, as in
            * the reference, where it then only reaches the history row */

@@ -114,4 +114,30 @@ PHX_FN double phx_uniform(phx_stream *s) {
   return s->buf[1];
 }
 
+/* a sequential stream of 32-bit words, four per block, for draws that need
+ * at most 32 random bits (the estimator's Doppler trials: 31 bits for the
+ * profile's random number, one for the sign of the momentum); it starts at
+ * the counter it is given and does not share blocks with a phx_stream */
+typedef struct phx_wstream {
+  uint32_t key[2];
+  uint32_t ctr[4];
+  uint32_t buf[4];
+  int avail;
+} phx_wstream;
+
+PHX_FN uint32_t phx_word(phx_wstream *s) {
+  if (s->avail == 0) {
+    phx_4x32_10(s->ctr, s->key, s->buf);
+    s->ctr[0]++;
+    s->avail = 4;
+  }
+  /* words in block order: buf[0] first, then shift */
+  const uint32_t w = s->buf[0];
+  s->buf[0] = s->buf[1];
+  s->buf[1] = s->buf[2];
+  s->buf[2] = s->buf[3];
+  s->avail--;
+  return w;
+}
+
 #endif
