@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="three_layer_kramers",
-                    choices=["three_layer_kramers", "single_layer_steel", "quantify", "batch"])
+                    choices=["three_layer_kramers", "single_layer_steel", "quantify", "batch", "tables", "escape"])
     ap.add_argument("--batch-samples", type=int, default=1024, help="batch: independent samples")
     ap.add_argument("--streams", type=int, default=4, help="batch: engines (HIP streams) serving the sample queue")
     ap.add_argument("--train-samples", type=int, default=32768, help="quantify: Monte Carlo training samples")
@@ -249,6 +249,94 @@ def batch(args):
     print(json.dumps(out))
 
 
+def tables(args):
+    """Interaction tables of Z 11-92 (the element set of the batch and
+    quantification workloads): the angle, Compton-profile and azimuth inverse
+    CDFs on the GPU, with the host's light parts; elements/s.  The host
+    generator is timed beside it on a sample of four elements."""
+    from xmimsim_amd import api
+    lib = api.load_library()
+    Zs = list(range(11, 93))
+    lib.xmi_tables_set_device(1)
+    try:
+        warm = api.SharedTables([26])  # device start-up outside the timed region
+        warm.close()
+        t0 = time.perf_counter()
+        T = api.SharedTables(Zs)
+        dt = time.perf_counter() - t0
+        T.close()
+    finally:
+        lib.xmi_tables_set_device(-1)
+    cpu = None
+    if not args.no_cpu:
+        sample = [14, 26, 50, 82]
+        lib.xmi_tables_set_device(0)
+        try:
+            t0 = time.perf_counter()
+            T = api.SharedTables(sample)
+            dc = time.perf_counter() - t0
+            T.close()
+        finally:
+            lib.xmi_tables_set_device(-1)
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores)
+        cpu = {"value": round(len(sample) / dc, 3), "unit": "elements/s", "cores": cores, "kind": "port",
+               "sample": f"host generator on Z {sample} ({dc:.1f} s)"}
+    print(json.dumps({
+        "metric": "interaction tables generated (elements/s)", "value": round(len(Zs) / dt, 3),
+        "unit": "elements/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(dt * 1e3, 1),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "analytic physics provider, Z 11-92",
+        "config": {"workload": "tables", "elements": len(Zs), "energies": 400, "random_numbers": 2000,
+                   "profile_points": 10001},
+        "roofline": None, "cpu_baseline": cpu}))
+
+
+def escape(args):
+    """Escape ratios of a 5 mm Ge crystal with the reference's default options
+    (1990 input energies from 1 keV by 0.1 keV, 5e5 photons each, 1999 Compton
+    output energies): photons/s of the whole call (tables, set-up, transport,
+    read-back) and of the device transport.  The oracle's restatement is timed
+    beside it on a sample of the same run."""
+    from xmimsim_amd import api
+    lib = api.load_library()
+    inp = api.Input(layers=[api.Layer([26], [1.0], 7.9, 0.5)], discrete=[api.DiscreteLine(20.0, 1e9, 0.0)],
+                    crystal=[api.Layer([32], [1.0], 5.32, 0.5)])
+    warm = api.escape_ratios(inp, n_photons=1000, n_input_energies=10, seed=args.seed)
+    api.free_escape_ratios(warm[3])
+    n_e, n_ph = 1990, 500_000
+    t0 = time.perf_counter()
+    Z, fluo, comp, h = api.escape_ratios(inp, n_photons=n_ph, n_input_energies=n_e, seed=args.seed)
+    dt = time.perf_counter() - t0
+    api.free_escape_ratios(h)
+    st = api.get_stats()
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        o = api.default_options(use_M_lines=0, use_cascade_auger=0, use_cascade_radiative=0,
+                                use_variance_reduction=0, escape_ratios_mode=1)
+        prep = api.Prepared(api.escape_input(inp), options=o)
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores)
+        n_s, e_s = 20_000, 64
+        t0 = time.perf_counter()
+        oracle.escape_ratios(prep, n_s, e_s, 1.0, 3.1, 1999, 0.1, 0.1, seed=args.seed, n_threads=cores)
+        dc = time.perf_counter() - t0
+        prep.close()
+        cpu = {"value": round(n_s * e_s / dc, 1), "unit": "photons/s", "cores": cores, "kind": "port",
+               "sample": f"{e_s} input energies 1-196 keV x {n_s} photons ({dc:.1f} s)"}
+    print(json.dumps({
+        "metric": "escape-ratio photons/s (detector crystal)", "value": round(n_e * n_ph / dt, 1),
+        "unit": "photons/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(dt * 1e3, 1),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "data": "Ge crystal 5 mm, analytic physics provider",
+        "config": {"workload": "escape", "input_energies": n_e, "photons_per_energy": n_ph,
+                   "compton_output_energies": 1999, "device_transport_ms": round(st.transport_ms, 1),
+                   "device_photons_per_s": round(n_e * n_ph / (st.transport_ms * 1e-3), 1) if st.transport_ms else None},
+        "roofline": None, "cpu_baseline": cpu}))
+
+
 def quantify(args):
     """Surrogate-screened vs plain Monte Carlo quantification of single-layer
     samples: training data from the GPU Monte Carlo, training of the BF16
@@ -291,6 +379,10 @@ def main():
         return quantify(args)
     if args.workload == "batch":
         return batch(args)
+    if args.workload == "tables":
+        return tables(args)
+    if args.workload == "escape":
+        return escape(args)
     from xmimsim_amd import api
     inp = make_input(args)
     prep = api.Prepared(inp)

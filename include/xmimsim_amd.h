@@ -190,6 +190,21 @@ int xmi_init_from_hdf5(char *hdf5_file, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *h
 /* Same tables without a file round trip (generated in memory). */
 int xmi_init_from_provider(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *hdf5FPtr, xmi_main_options *options);
 
+/* Where xmi_init_from_provider and xmi_db run the heavy loops of the table
+ * generator (angle, Compton-profile and azimuth inverse CDFs;
+ * src/xmi_data_f.F90:996-1237,1483-1549): -1 (default) on the GPU when one is
+ * usable, else on the host; 0 on the host; 1 on the GPU (an error without
+ * one).  No counterpart in the reference, whose generator runs on the host. */
+void xmi_tables_set_device(int mode);
+
+/* Copies one table of a table handle: which = 0 Rayleigh and 1 Compton angle
+ * inverse CDF of element Z ([400][2000]), 2 its total Compton-profile inverse
+ * CDF, 3 its partial-profile inversions ([shells][10001]), 4 the azimuth
+ * inverse CDF ([200][2000], Z ignored).  Returns the number of values (and
+ * copies them when out holds at least n), 0 when absent.  No counterpart in
+ * the reference (which reads the same tables from its HDF5 file). */
+long xmi_tables_copy(xmi_hdf5FPtr hdf5FPtr, int Z, int which, double *out, long n);
+
 /* Binds the input's layers to the loaded tables.  Replaces
  * xmi_update_input_from_hdf5 (include/xmi_data.h:93, src/xmi_data_f.F90:98-162). */
 int xmi_update_input_from_hdf5(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr);
@@ -283,6 +298,9 @@ int xmi_provider_atom(int Z, void *xrl_atom_out);  /* copies the provider's per-
 double xmi_provider_cs(int kind, int Z, double E); /* 0 total, 1 photo, 2 Rayleigh, 3 Compton */
 double xmi_provider_dcs(int kind, int Z, double E, double theta); /* 0 Rayleigh, 1 Compton, cm2/g/sr */
 int xmi_provider_shell_cross_sections(int Z, int cascade_type, double E, double *P9);
+/* 0: fluorescence yield of shell index (0 K .. 8 M5); 1: radiative rate and
+ * 2: energy (keV) of line index (xmi_line_name numbering) */
+double xmi_provider_line_data(int Z, int which, int index);
 
 /* Number of fluorescence lines (1..line_last, line_last <= 219) of element Z
  * that the detection-weight estimator evaluates: energy >= 1 keV and a
@@ -308,6 +326,38 @@ typedef struct _xmi_escape_ratios {
   double *compton_escape_output_energies;
   char *xmi_input_string;
 } xmi_escape_ratios;
+
+/* Options of the escape-ratio Monte Carlo (include/xmi_detector.h:42-51);
+ * defaults 1990 input energies from 1 keV by 0.1 keV, 1999 Compton output
+ * energies from 0.1 keV by 0.1 keV, 500000 photons per input energy. */
+typedef struct _xmi_escape_ratios_options {
+  long n_input_energies;
+  long n_compton_output_energies;
+  long n_photons;
+  double input_energy_min;
+  double input_energy_delta;
+  double compton_output_energy_min;
+  double compton_output_energy_delta;
+} xmi_escape_ratios_options;
+
+/* include/xmi_detector.h:53 */
+xmi_escape_ratios_options xmi_get_default_escape_ratios_options(void);
+
+/* Escape ratios of the input's detector crystal on the GPU.  Replaces
+ * xmi_escape_ratios_calculation (include/xmi_detector.h:57,
+ * src/xmi_detector.c:91-141, src/xmi_main.F90:5473-5801): the crystal layers
+ * are the sample, hit at normal incidence by a pencil beam of each input
+ * energy; each photon's one forced interaction is followed by the analogue
+ * test of whether its secondary leaves the crystal.  Fluorescence ratios per
+ * (element, line KL1..L3P3, input energy) and Compton ratios per (input,
+ * output energy), normalised by the weight of the photons that interacted.
+ * hdf5_file NULL takes the tables from the built-in provider.  *escape_ratios
+ * is NULL on failure (xmi_msim_last_error holds the reason). */
+void xmi_escape_ratios_calculation(xmi_input *input, xmi_escape_ratios **escape_ratios, char *input_string,
+                                   char *hdf5_file, xmi_main_options *options, xmi_escape_ratios_options ero);
+
+/* include/xmi_detector.h:67 */
+void xmi_free_escape_ratios(xmi_escape_ratios *escape_ratios);
 
 /* Detector response of every interaction order: absorption in the window and
  * crystal, escape peaks (when options->use_escape_peaks and escape_ratios is

@@ -67,6 +67,10 @@ def load():
     lib.xmi_oracle_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     lib.xmi_oracle_set_hits_per_single.argtypes = [C.c_long]
     lib.xmi_oracle_free.argtypes = [C.c_void_p]
+    lib.xmi_oracle_escape_ratios.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_double, C.c_double,
+                                             C.c_long, C.c_double, C.c_double, C.c_long, C.c_uint64, C.c_int,
+                                             P(C.c_double), P(C.c_double)]
+    lib.xmi_oracle_escape_ratios.restype = C.c_int
     _lib = lib
     return lib
 
@@ -98,6 +102,29 @@ def run(prepared, seed: int, rng_mode: int = RNG_PER_HISTORY, n_threads: int = 0
     for p in (ch, bh, vr):
         lib.xmi_oracle_free(p)
     return channels, var_red, brute, st
+
+
+def escape_ratios(prepared, n_photons: int, n_input_energies: int, input_energy_min: float,
+                  input_energy_delta: float, n_compton_output_energies: int, compton_output_energy_min: float,
+                  compton_output_energy_delta: float, seed: int, n_threads: int = 0):
+    """Escape-ratio run restated on the CPU (prepared: the escape input, see
+    api.escape_input, with escape options); returns (fluo [n_el][109][nE],
+    compton [nE][nOut])."""
+    lib = load()
+    n_el = len(prepared.inp.unique_Z())
+    fluo = np.zeros(n_el * 109 * n_input_energies)
+    comp = np.zeros(n_input_energies * n_compton_output_energies)
+    ok = lib.xmi_oracle_escape_ratios(prepared.inputF, prepared.hdf5F, C.byref(prepared.options), n_input_energies,
+                                      input_energy_min, input_energy_delta, n_compton_output_energies,
+                                      compton_output_energy_min, compton_output_energy_delta, n_photons, seed,
+                                      n_threads or (os.cpu_count() or 1),
+                                      fluo.ctypes.data_as(C.POINTER(C.c_double)),
+                                      comp.ctypes.data_as(C.POINTER(C.c_double)))
+    if not ok:
+        raise RuntimeError("oracle escape-ratio run failed")
+    fluo = fluo.reshape(n_input_energies, 109, n_el).transpose(2, 1, 0).copy()
+    comp = comp.reshape(n_compton_output_energies, n_input_energies).T.copy()
+    return fluo, comp
 
 
 def solid_angle(prepared, hits: int, seed: int, n_threads: int = 0):

@@ -216,6 +216,7 @@ typedef struct {
   long hits_per_single;
   uint64_t seed;
   int rng_mode;
+  int escape;                 /* escape-ratio run: no estimator, analogue exit test */
 } orc_ctx;
 
 typedef struct {
@@ -236,6 +237,8 @@ typedef struct {
   int current_element_index;  /* 0-based in layer */
   int hist[MAX_INT_TRAJ + 1][3];
   int energy_changed;
+  int escaped;                /* escape-ratio run: left the crystal after its interaction */
+  double weight_escape;       /* escape-ratio run: weight after the forced step */
   orc_rng *tr;                /* transport stream */
   orc_rng *vr;                /* detection-weight stream */
 } orc_photon;
@@ -588,7 +591,10 @@ static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
   }
   /* xmi_fluorescence_yield_check_varred_optim: draws a number it never uses */
   (void)orc_uniform(p->tr);
-  p->weight *= z->FluorYieldsCorr[shell];
+  /* xmi_fluorescence_yield_check_varred_optim (src/xmi_variance_reduction.F90:
+   * 728-750): an escape-ratio run weights the escape tally, not the photon */
+  if (c->escape) p->weight_escape *= z->FluorYieldsCorr[shell];
+  else p->weight *= z->FluorYieldsCorr[shell];
   orc_coster_kronig(a, p->tr, &shell);
   /* xmi_fluorescence_line_check */
   r = orc_uniform(p->tr);
@@ -789,6 +795,33 @@ static void reseed_interaction(const orc_ctx *c, orc_photon *p, orc_rng *tr, orc
   p->vr = vr;
 }
 
+/* the analogue step of src/xmi_main.F90:1274-1407 as the escape-ratio run
+ * takes it after the one interaction: 1 when the photon crosses every layer
+ * on its way out without interacting (interactionR above the probability of
+ * an interaction on the way) */
+static int orc_escape_test(const orc_ctx *c, const orc_photon *p, double interactionR) {
+  const xmi_inputF *in = c->in;
+  int dir = dot3(p->dirv, in->geometry.n_sample_orientation) > 0.0 ? 1 : -1;
+  int last = dir > 0 ? in->n_layers : 1;
+  double lp[3] = {p->coords[0], p->coords[1], p->coords[2]};
+  double max_random_layer = 0.0, blbs = 1.0;
+  for (int i = p->current_layer; dir > 0 ? i <= last : i >= last; i += dir) {
+    double pp[3] = {0.0, 0.0, dir == 1 ? in->layers[i - 1].Z_coord_end : in->layers[i - 1].Z_coord_begin};
+    double ip[3];
+    if (!intersection_plane_line(pp, in->geometry.n_sample_orientation, lp, p->dirv, ip)) return 0;
+    double dist = distance3(lp, ip);
+    double temp_prod = -1.0 * dist * in->layers[i - 1].density * p->mus[i - 1];
+    double tempexp = exp(temp_prod);
+    max_random_layer = max_random_layer - blbs * expm1(temp_prod);
+    if (interactionR <= max_random_layer) return 0;
+    lp[0] = ip[0];
+    lp[1] = ip[1];
+    lp[2] = ip[2];
+    blbs = blbs * tempexp;
+  }
+  return 1;
+}
+
 static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, uint64_t hid, orc_rng *tr, orc_rng *vr) {
   const xmi_inputF *in = c->in;
   for (;;) {
@@ -807,7 +840,10 @@ static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, ui
     }
     reseed_interaction(c, p, tr, vr, hid, p->n_interactions + 1);
     double interactionR = orc_uniform(p->tr);
-    if (p->n_interactions == c->n_int) break;
+    if (p->n_interactions == c->n_int) {
+      if (c->escape) p->escaped = orc_escape_test(c, p, interactionR);
+      break;
+    }
     double distances[64];
     double lp[3] = {p->coords[0], p->coords[1], p->coords[2]};
     for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
@@ -824,6 +860,7 @@ static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, ui
       Pabs += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
     double Pabs2 = -1.0 * expm1(-1.0 * Pabs);
     p->weight *= Pabs2;
+    if (c->escape) p->weight_escape = p->weight;
     double negln = -1.0 * log1p(-1.0 * interactionR * Pabs2);
     int my_index = p->current_layer;
     double my_sum = 0.0;
@@ -845,7 +882,7 @@ static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, ui
     if (p->n_interactions == c->n_int) break;
     p->n_interactions++;
     t->interactions++;
-    orc_variance_reduction(c, p, t);
+    if (!c->escape) orc_variance_reduction(c, p, t);
 
     /* atom selection */
     const xmi_layerF *L = &in->layers[p->current_layer - 1];
@@ -1419,4 +1456,108 @@ void xmi_oracle_detector_convolute(xmi_inputFPtr inputFPtr, const double *noconv
   }
   free(temp);
   free(R);
+}
+
+/* ------------------------------------------------------------------------ */
+/* escape ratios of a detector crystal, src/xmi_main.F90:5473-5801: the
+ * escape input (crystal layers as the sample, prepared by the caller as
+ * src/xmi_detector.c:91-141 does) hit by n_photons photons of each input
+ * energy; every photon has one forced interaction, then the analogue exit
+ * test.  Per-history Philox streams as the engine's (source stream: the slit
+ * draws, then the polarisation azimuth).  fluo[(u + nZ (line-1 + 109 k))] and
+ * compton[k + nE o] receive the ratios (Fortran orders of the reference). */
+int xmi_oracle_escape_ratios(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, long nE,
+                             double E0, double dE, long nOut, double out0, double dOut, long n_photons, uint64_t seed,
+                             int n_threads, double *fluo, double *compton) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const xmi_hdf5F *h = (const xmi_hdf5F *)hdf5FPtr;
+  if (!in || !h || !options || nE < 1 || nOut < 1 || n_photons < 1 || !fluo || !compton) return 0;
+  if (n_threads < 1) n_threads = 1;
+  orc_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.in = in;
+  c.h = h;
+  c.opt = options;
+  c.nZ = h->n_Z;
+  c.n_int = 1;
+  c.line_last = options->use_M_lines ? XRL_M5P5 : XRL_L3Q1;
+  c.seed = seed;
+  c.rng_mode = XMI_ORACLE_RNG_PER_HISTORY;
+  c.escape = 1;
+  c.cascade_type = XRL_CASCADE_NONE;
+  if (in->n_layers > 64) return 0;
+  c.atoms = (const xrl_atom **)malloc(sizeof(xrl_atom *) * c.nZ);
+  c.line_energies = (double *)calloc((size_t)c.nZ * XMI_NLINES_VR, sizeof(double));
+  for (int i = 0; i < c.nZ; i++) {
+    c.atoms[i] = xrl_atom_get(h->Zs[i].Z);
+    for (int l = 1; l <= XMI_NLINES_VR; l++) c.line_energies[(size_t)i * XMI_NLINES_VR + l - 1] = c.atoms[i]->line_energy[l];
+  }
+  c.precalc_mu = (double *)calloc((size_t)in->n_layers * c.nZ * XMI_NLINES_VR, sizeof(double));
+  for (int k = 0; k < in->n_layers; k++)
+    for (int i = 0; i < c.nZ; i++)
+      for (int l = 0; l < XMI_NLINES_VR; l++) {
+        double E = c.line_energies[(size_t)i * XMI_NLINES_VR + l];
+        c.precalc_mu[((size_t)k * c.nZ + i) * XMI_NLINES_VR + l] = E > 0.0 ? layer_mu(&c, &in->layers[k], E) : 0.0;
+      }
+  const size_t nf = (size_t)c.nZ * 109 * nE, nc = (size_t)nE * nOut;
+  double *tf = (double *)calloc(nf, sizeof(double)), *tc = (double *)calloc(nc, sizeof(double));
+  double *interacted = (double *)calloc(nE, sizeof(double));
+  int ok = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic, 1)
+  for (long k = 0; k < nE; k++) {
+    orc_rng src_rng, tr_rng, vr_rng;
+    orc_photon p;
+    orc_tally t;
+    memset(&t, 0, sizeof(t));
+    const double E = E0 + k * dE;
+    for (long j = 0; j < n_photons; j++) {
+      uint64_t hid = (uint64_t)k * (uint64_t)n_photons + (uint64_t)j;
+      orc_rng_init(&src_rng, seed, 0u, hid);
+      memset(&p, 0, sizeof(p));
+      p.energy = E;
+      mu_calc_all(&c, E, p.mus);
+      orc_coords_dir(&c, &src_rng, 0.0, 0.0, 0.0, 0.0, &p);
+      /* polarisation azimuth uniform in the x-y plane, orthogonalised */
+      double te = orc_uniform(&src_rng) * ORC_PI * 2.0;
+      p.elecv[0] = cos(te);
+      p.elecv[1] = sin(te);
+      p.elecv[2] = 0.0;
+      double cosalfa = dot3(p.elecv, p.dirv);
+      double c_alfa = acos(cosalfa);
+      double c_ae = 1.0 / sin(c_alfa);
+      double c_be = -c_ae * cosalfa;
+      for (int i = 0; i < 3; i++) p.elecv[i] = c_ae * p.elecv[i] + c_be * p.dirv[i];
+      p.weight = 1.0;
+      if (!orc_shift_first_layer(&c, &p)) {
+        ok = 0;
+        continue;
+      }
+      if (!orc_simulate_photon(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
+      if (p.n_interactions < 1) continue;
+      interacted[k] += p.weight;
+      if (!p.escaped) continue;
+      if (p.last_interaction == XMI_COMPTON_INTERACTION) {
+        int idx = (int)((p.energy - out0) / dOut) + 1;
+        if (idx >= 1 && idx <= nOut) tc[(size_t)k + (size_t)nE * (idx - 1)] += p.weight;
+      } else if (p.last_interaction == XMI_PHOTOELECTRIC_INTERACTION) {
+        int code = p.hist[1][0];
+        if (code < 0 && -code <= 109) {
+          int u = p.hist[1][2] - 1;
+          tf[(size_t)u + (size_t)c.nZ * ((size_t)(-code - 1) + 109 * (size_t)k)] += p.weight_escape;
+        }
+      }
+    }
+  }
+  for (long k = 0; k < nE; k++) {
+    double n = interacted[k] > 0.0 ? interacted[k] : 1.0;
+    for (size_t r = 0; r < (size_t)c.nZ * 109; r++) fluo[r + (size_t)c.nZ * 109 * k] = tf[r + (size_t)c.nZ * 109 * k] / n;
+    for (long o = 0; o < nOut; o++) compton[k + nE * o] = tc[k + nE * o] / n;
+  }
+  free(tf);
+  free(tc);
+  free(interacted);
+  free(c.atoms);
+  free(c.line_energies);
+  free(c.precalc_mu);
+  return ok;
 }

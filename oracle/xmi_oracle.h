@@ -58,6 +58,12 @@ void xmi_oracle_detector_convolute(xmi_inputFPtr inputFPtr, const double *noconv
 
 /* small numerical helpers exposed for unit tests */
 int xmi_oracle_poly_solve_quadratic(double a, double b, double c, double *r1, double *r2);
+
+/* escape ratios of a crystal (the escape input prepared by the caller),
+ * src/xmi_main.F90:5473-5801; outputs in the reference's Fortran orders */
+int xmi_oracle_escape_ratios(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, long nE,
+                             double E0, double dE, long nOut, double out0, double dOut, long n_photons, uint64_t seed,
+                             int n_threads, double *fluo, double *compton);
 double xmi_oracle_energy_from_q(double init_energy, double Q, double theta);
 double xmi_oracle_q_from_energy(double old_energy, double new_energy, double theta);
 double xmi_oracle_ran_trap(double x1, double x2, double y1, double y2, double u);

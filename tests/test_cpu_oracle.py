@@ -230,3 +230,39 @@ def test_oracle_single_solid_angle_off_axis(orc, coll):
         exact = _disk_solid_angle(R, 0.5 * math.cos(0.3), 0.5 * math.sin(0.3), None)
         assert 0.85 * exact < om < 0.95 * exact
     prep.close()
+
+
+def test_escape_ratios_closed_form(orc):
+    """Escape-ratio restatement (src/xmi_main.F90:5473-5801): for a thick
+    crystal at normal incidence the escape ratio of a K line is
+    tau_K/mu(E) * omega_K * R_line * (1 - ln(1 + x)/x)/2 with x = mu(E)/mu(E_line)
+    (isotropic emission from an exponential depth distribution, semi-infinite
+    slab); no K escape below the K edge."""
+    import ctypes as C
+    import math
+    lib = api.load_library()
+    Zc = 32
+    inp = api.Input(layers=[api.Layer([26], [1.0], 7.9, 0.5)], discrete=[api.DiscreteLine(20.0, 1e9, 0.0)],
+                    crystal=[api.Layer([Zc], [1.0], 5.32, 5.0)])
+    o = api.default_options(use_M_lines=0, use_cascade_auger=0, use_cascade_radiative=0, use_variance_reduction=0,
+                            escape_ratios_mode=1)
+    prep = api.Prepared(api.escape_input(inp), options=o)
+    try:
+        for E in (20.0, 40.0):
+            n = 300_000
+            fluo, _ = orc.escape_ratios(prep, n, 1, E, 0.1, 10, 0.1, 0.1, seed=11)
+            P9 = (C.c_double * 9)()
+            lib.xmi_provider_shell_cross_sections(Zc, 1, E, P9)
+            mu_i = lib.xmi_provider_cs(0, Zc, E)
+            wK = lib.xmi_provider_line_data(Zc, 0, 0)
+            for line in (2, 3):  # KL2, KL3
+                R = lib.xmi_provider_line_data(Zc, 1, line)
+                x = mu_i / lib.xmi_provider_cs(0, Zc, lib.xmi_provider_line_data(Zc, 2, line))
+                want = P9[0] / mu_i * wK * R * 0.5 * (1.0 - math.log1p(x) / x)
+                # four standard deviations of the escaped count, plus 0.5 % for the finite slab
+                tol = 4.0 / math.sqrt(want * n) + 0.005
+                assert abs(fluo[0, line - 1, 0] / want - 1.0) < tol, (E, line, fluo[0, line - 1, 0], want)
+        fluo, _ = orc.escape_ratios(prep, 20_000, 1, 10.0, 0.1, 10, 0.1, 0.1, seed=12)  # below the Ge K edge
+        assert np.all(fluo[0, :29, 0] == 0.0)
+    finally:
+        prep.close()

@@ -159,3 +159,40 @@ def test_no_gpu_is_a_loud_error(lib):
     prep = api.Prepared(workloads.single_layer_steel(n_photons_line=10))
     with pytest.raises(RuntimeError):
         prep.compute_solid_angle(hits_per_single=10)
+
+
+def test_host_table_generator_properties(lib):
+    """Host generator (the one the GPU generator is compared with): inverse
+    CDFs are monotone with the documented end points, and the angle inverse
+    CDF of an element agrees with an independent inversion of its DCS."""
+    lib.xmi_tables_set_device(0)
+    try:
+        T = api.SharedTables([14, 82])
+        for Z in (14, 82):
+            for which in (0, 1):
+                t = T.table(Z, which).reshape(400, 2000)
+                assert np.all(np.diff(t, axis=1) >= 0.0)
+                assert np.all(t[:, 0] == 0.0) and np.all(t[:, -1] == math.pi)
+            cp = T.table(Z, 2)
+            assert cp[0] == 0.0 and cp[-1] == 100.0 and np.all(np.diff(cp) >= 0.0)
+            qs = T.table(Z, 3).reshape(-1, 10001)
+            assert np.all(qs[:, 0] == 0.0) and np.all(qs[:, -1] == 100.0)
+        phi = T.table(14, 4).reshape(200, 2000)
+        assert np.all(phi[:, 0] == 0.0) and np.allclose(phi[:, -1], 2.0 * math.pi)
+        # the azimuth CDF (phi - t sin 2 phi) / 2 pi at the table's phi values
+        t = np.linspace(0.0, 0.5, 200)[:, None]
+        r = np.linspace(0.0, 1.0, 2000)[None, :]
+        assert np.max(np.abs((phi - t * np.sin(2.0 * phi)) / (2.0 * math.pi) - r)) < 1e-12
+        # Rayleigh angle ICDF of Si at 20 keV (energy row 39.9..: pick the row
+        # nearest 20 keV) against a direct trapezoid inversion in numpy
+        E = np.linspace(0.1, 200.0, 400)
+        j = int(np.argmin(np.abs(E - 20.0)))
+        th = np.linspace(0.0, math.pi, 10000)
+        f = np.array([lib.xmi_provider_dcs(0, 14, E[j], x) for x in th]) * np.sin(th)
+        cdf = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * (th[1] - th[0]))])
+        want = np.interp(r[0] * cdf[-1], cdf, th)
+        got = T.table(14, 0).reshape(400, 2000)[j]
+        assert np.max(np.abs(got[1:-1] - want[1:-1])) < 1e-6
+        T.close()
+    finally:
+        lib.xmi_tables_set_device(-1)

@@ -2,6 +2,7 @@
 deterministic part against the oracle, escape peaks against the escaped
 spectrum built by hand, Poisson noise and pile-up against their statistics."""
 import ctypes as C
+import dataclasses
 import math
 
 import numpy as np
@@ -103,3 +104,45 @@ def test_pileup_sum_peak_ratio():
     p = math.exp(-inp.pulse_width * 1e6)
     assert summ / main == pytest.approx((1 - p), rel=0.05)
     prep.close()
+
+
+def test_escape_ratios_match_oracle():
+    """xmi_escape_ratios_calculation on the GPU (which builds the crystal's
+    escape input itself) against the oracle's restatement run on the escape
+    input built as src/xmi_detector.c:98-116 does, with the same streams:
+    every fluorescence and Compton ratio to 1e-9; then the ratios feed the
+    detector response's escape peaks."""
+    import oracle
+    inp = api.Input(layers=[api.Layer([26, 28], [0.7, 0.3], 7.9, 0.5)], discrete=[api.DiscreteLine(20.0, 1e9, 0.0)],
+                    crystal=[api.Layer([32], [1.0], 5.32, 0.5)], n_photons_line=20000)
+    args = dict(n_photons=20000, n_input_energies=40, input_energy_min=5.0, input_energy_delta=0.75,
+                n_compton_output_energies=400, compton_output_energy_min=0.1, compton_output_energy_delta=0.1)
+    Z, fluo, comp, handle = api.escape_ratios(inp, seed=31, **args)
+    try:
+        assert Z == [32]
+        o = api.default_options(use_M_lines=0, use_cascade_auger=0, use_cascade_radiative=0,
+                                use_variance_reduction=0, escape_ratios_mode=1)
+        prep = api.Prepared(api.escape_input(inp), options=o)
+        f_ref, c_ref = oracle.escape_ratios(prep, args["n_photons"], args["n_input_energies"],
+                                            args["input_energy_min"], args["input_energy_delta"],
+                                            args["n_compton_output_energies"], args["compton_output_energy_min"],
+                                            args["compton_output_energy_delta"], seed=31)
+        prep.close()
+        assert fluo.shape == f_ref.shape and comp.shape == c_ref.shape
+        assert fluo.max() > 0.05 and comp.max() > 0.0
+        np.testing.assert_allclose(fluo, f_ref, rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(comp, c_ref, rtol=1e-9, atol=1e-15)
+        # the computed ratios make escape peaks in the detector response
+        prep = api.Prepared(dataclasses.replace(inp, n_photons_line=100000))
+        prep.compute_solid_angle(hits_per_single=1000, seed=2)
+        ch, vr, _, _ = prep.run(seed=5)
+        conv_esc = api.detector_response(prep, ch, vr.copy(), escape_ratios=handle.contents)
+        conv = api.detector_response(prep, ch, vr.copy())
+        prep.close()
+        # Fe K-alpha (6.40 keV) loses to its Ge K-alpha escape at 6.40 - 9.89 < 0: no escape;
+        # the 20 keV Rayleigh/Compton peak has its Ge escape near 20 - 9.9 = 10.1 keV
+        e = inp.gain * np.arange(inp.nchannels)
+        win = (e > 9.6) & (e < 10.6)
+        assert conv_esc[-1][win].sum() > conv[-1][win].sum()
+    finally:
+        api.free_escape_ratios(handle)

@@ -112,6 +112,8 @@ EXPORTED_SYMBOLS = [
     "xmi_free", "xmi_set_seed", "xmi_msim_get_stats", "xmi_msim_last_error", "xmi_msim_set_profiling",
     "xmi_msim_get_layer_events", "xmi_provider_active_lines", "xmi_detector_convolute_all",
     "xmi_detector_convolute_all_custom", "xmi_surrogate_create", "xmi_surrogate_train_step", "xmi_main_msim_batch",
+    "xmi_tables_set_device", "xmi_tables_copy", "xmi_escape_ratios_calculation", "xmi_free_escape_ratios",
+    "xmi_get_default_escape_ratios_options",
 ]
 
 _lib = None
@@ -138,6 +140,15 @@ def load_library(path: str = LIB_PATH):
     lib.xmi_init_from_hdf5.restype = C.c_int
     lib.xmi_init_from_provider.argtypes = [C.c_void_p, P(C.c_void_p), P(xmi_main_options)]
     lib.xmi_init_from_provider.restype = C.c_int
+    lib.xmi_tables_set_device.argtypes = [C.c_int]
+    lib.xmi_tables_set_device.restype = None
+    lib.xmi_tables_copy.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), C.c_long]
+    lib.xmi_tables_copy.restype = C.c_long
+    lib.xmi_escape_ratios_calculation.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p, C.c_void_p,
+                                                  xmi_escape_ratios_options]
+    lib.xmi_escape_ratios_calculation.restype = None
+    lib.xmi_free_escape_ratios.argtypes = [C.c_void_p]
+    lib.xmi_get_default_escape_ratios_options.restype = xmi_escape_ratios_options
     lib.xmi_update_input_from_hdf5.argtypes = [C.c_void_p, C.c_void_p]
     lib.xmi_update_input_from_hdf5.restype = C.c_int
     lib.xmi_free_hdf5_F.argtypes = [P(C.c_void_p)]
@@ -157,8 +168,24 @@ def load_library(path: str = LIB_PATH):
     lib.xmi_msim_get_layer_events.restype = C.c_int
     lib.xmi_provider_active_lines.argtypes = [C.c_int, C.c_int]
     lib.xmi_provider_active_lines.restype = C.c_int
+    lib.xmi_provider_dcs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+    lib.xmi_provider_dcs.restype = C.c_double
+    lib.xmi_provider_cs.argtypes = [C.c_int, C.c_int, C.c_double]
+    lib.xmi_provider_cs.restype = C.c_double
+    lib.xmi_provider_line_data.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.xmi_provider_line_data.restype = C.c_double
+    lib.xmi_provider_shell_cross_sections.argtypes = [C.c_int, C.c_int, C.c_double, P(C.c_double)]
+    lib.xmi_provider_shell_cross_sections.restype = C.c_int
     _lib = lib
     return lib
+
+
+def get_stats() -> "xmi_msim_stats":
+    """Timing and counters of the last engine call on this thread."""
+    lib = load_library()
+    st = xmi_msim_stats()
+    lib.xmi_msim_get_stats(C.byref(st))
+    return st
 
 
 def last_error() -> str:
@@ -412,6 +439,57 @@ class xmi_escape_ratios(C.Structure):
                 ("compton_escape_output_energies", C.POINTER(C.c_double)), ("xmi_input_string", C.c_char_p)]
 
 
+class xmi_escape_ratios_options(C.Structure):
+    _fields_ = [("n_input_energies", C.c_long), ("n_compton_output_energies", C.c_long), ("n_photons", C.c_long),
+                ("input_energy_min", C.c_double), ("input_energy_delta", C.c_double),
+                ("compton_output_energy_min", C.c_double), ("compton_output_energy_delta", C.c_double)]
+
+
+def escape_input(inp: Input) -> Input:
+    """The input of an escape-ratio run (src/xmi_detector.c:98-116): the
+    detector crystal as the sample (reference layer 1), source 1 cm away
+    behind a 1 um slit, sample normal along z, one interaction per photon."""
+    import dataclasses
+    return dataclasses.replace(inp, layers=list(inp.crystal), reference_layer=1, d_sample_source=1.0,
+                               d_source_slit=1.0, slit_size_x=0.0001, slit_size_y=0.0001,
+                               n_sample_orientation=(0.0, 0.0, 1.0), n_interactions_trajectory=1,
+                               n_photons_line=100000)
+
+
+def escape_ratios(inp: Input, n_photons: int = 500000, n_input_energies: int = 1990, input_energy_min: float = 1.0,
+                  input_energy_delta: float = 0.1, n_compton_output_energies: int = 1999,
+                  compton_output_energy_min: float = 0.1, compton_output_energy_delta: float = 0.1,
+                  seed: Optional[int] = None, hdf5_file: Optional[str] = None):
+    """xmi_escape_ratios_calculation of the input's detector crystal on the
+    GPU.  Returns (Z list, fluo [n_el][109][nE] as the reference's Fortran
+    array indexed (element, line, energy), compton [nE][nOut] indexed
+    (input, output energy), handle) -- pass the handle to detector_response
+    and release it with free_escape_ratios."""
+    lib = load_library()
+    ero = xmi_escape_ratios_options(n_input_energies, n_compton_output_energies, n_photons, input_energy_min,
+                                    input_energy_delta, compton_output_energy_min, compton_output_energy_delta)
+    o = default_options()
+    if seed is not None:
+        lib.xmi_set_seed(seed)
+    cin = CInput(inp)
+    out = C.POINTER(xmi_escape_ratios)()
+    lib.xmi_escape_ratios_calculation(C.byref(cin.input), C.byref(out), None,
+                                      hdf5_file.encode() if hdf5_file else None, C.byref(o), ero)
+    if not out:
+        raise RuntimeError("xmi_escape_ratios_calculation failed: " + last_error())
+    r = out.contents
+    n_el, nE, nOut = r.n_elements, r.n_fluo_input_energies, r.n_compton_output_energies
+    Z = [r.Z[k] for k in range(n_el)]
+    fluo = np.ctypeslib.as_array(r.fluo_escape_ratios, shape=(nE, 109, n_el)).transpose(2, 1, 0).copy()
+    compton = np.ctypeslib.as_array(r.compton_escape_ratios, shape=(nOut, nE)).T.copy()
+    return Z, fluo, compton, out
+
+
+def free_escape_ratios(handle):
+    if handle:
+        load_library().xmi_free_escape_ratios(handle)
+
+
 def detector_response(prep: "Prepared", channels: np.ndarray, var_red: Optional[np.ndarray] = None,
                       escape_ratios=None, zero_interaction: bool = False, options: Optional[xmi_main_options] = None):
     """xmi_detector_convolute_all on the spectra of every order (channels
@@ -484,6 +562,18 @@ class SharedTables:
         self._holder = Prepared(Input(layers=[Layer(Zs, [1.0 / len(Zs)] * len(Zs), 1.0, 0.1)],
                                       discrete=[DiscreteLine(20.0, 1e9)], n_photons_line=10), hdf5_file=hdf5_file)
         self.hdf5F = self._holder.hdf5F
+
+    def table(self, Z: int, which: int) -> np.ndarray:
+        """One table as numpy (which: 0/1 Rayleigh/Compton angle inverse CDF
+        [400][2000], 2 total Compton-profile inverse CDF, 3 partial-profile
+        inversions [shells][10001], 4 azimuth inverse CDF [200][2000])."""
+        lib = load_library()
+        n = lib.xmi_tables_copy(self.hdf5F, Z, which, None, 0)
+        if n <= 0:
+            raise KeyError((Z, which))
+        out = np.zeros(n, np.float64)
+        lib.xmi_tables_copy(self.hdf5F, Z, which, out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        return out
 
     def close(self):
         if self._holder is not None:

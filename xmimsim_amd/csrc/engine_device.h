@@ -157,6 +157,7 @@ struct EngineParams {
   int *lay_list;                /* [n_layers][cap] */
   int *lay_woff;                /* [n_layers][n_waves]: per-wave counts, then exclusive offsets */
   int *lay_total;               /* [n_layers] entries of each list */
+  int *lay_segbase;             /* [16][32] segment totals, then offsets of the per-wave scan */
   int n_waves;                  /* row length of lay_woff: waves of a one-thread-per-slot launch over cap */
   int cap;
 };

@@ -142,7 +142,7 @@ struct Engine {
   DevBuf fl_info, fl_wf, fl_mr, lbin_g;
   DevBuf lay_rc, lay_edge, lay_pb, lay_kstart;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
-  DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, cone_acc;
+  DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, lay_segbase, cone_acc, esc_tally;
 };
 
 static Engine g_engine;
@@ -341,14 +341,25 @@ typedef unsigned __int128 u128;
 static inline u128 fx_get(const unsigned long long *b) { return ((u128)b[1] << 64) | (u128)b[0]; }
 static inline double fx_to_double(u128 v, int shift) { return ldexp((double)v, -shift); }
 
+/* an escape-ratio run (src/xmi_main.F90:5473-5801): photons of nE input
+ * energies, n_photons each, through the detector crystal; results in the
+ * reference's Fortran orders */
+struct EscapeRun {
+  int nE = 0, nOut = 0;
+  long n_photons = 0;
+  double E0 = 0.0, dE = 0.0, out0 = 0.0, dOut = 0.0;
+  std::vector<double> fluo;    /* (element, line 1..109, input energy), element fastest */
+  std::vector<double> compton; /* (input energy, output energy), input energy fastest */
+};
+
 static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
                     double **channelsPtr, xmi_main_options *options, double **brute_historyPtr,
-                    double **var_red_historyPtr, xmi_solid_angle *solid_angles) {
+                    double **var_red_historyPtr, xmi_solid_angle *solid_angles, EscapeRun *esc = nullptr) {
   auto t_wall0 = std::chrono::steady_clock::now();
   memset(&g_stats, 0, sizeof(g_stats));
   xmi_inputF *in = (xmi_inputF *)inputFPtr;
   xmi_hdf5F *h = (xmi_hdf5F *)hdf5FPtr;
-  if (!in || !h || !options || !channelsPtr || !brute_historyPtr || !var_red_historyPtr) {
+  if (!in || !h || !options || (!esc && (!channelsPtr || !brute_historyPtr || !var_red_historyPtr))) {
     xmi_set_error("xmi_main_msim: NULL argument");
     return 0;
   }
@@ -356,7 +367,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     xmi_set_error("xmi_main_msim: xmi_init_input was not called");
     return 0;
   }
-  if (!options->use_variance_reduction) {
+  if (!esc && !options->use_variance_reduction) {
     xmi_set_error("xmi_main_msim: the brute-force path (use_variance_reduction=0) is not built on the GPU");
     return 0;
   }
@@ -364,7 +375,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     xmi_set_error("xmi_main_msim: use_advanced_compton is not supported");
     return 0;
   }
-  if (!solid_angles || !solid_angles->solid_angles) {
+  if (!esc && (!solid_angles || !solid_angles->solid_angles)) {
     xmi_set_error("xmi_main_msim: variance reduction needs a solid-angle grid");
     return 0;
   }
@@ -376,7 +387,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   if (!engine_init(E)) return 0;
   DeviceTables *T = get_tables(h);
   if (!T) return 0;
-  if (!ensure_grid(E, solid_angles)) return 0;
+  if (!esc && !ensure_grid(E, solid_angles)) return 0;
   hipStream_t st = E.stream;
   const int nZ = T->nZ;
   const int n_int = in->general.n_interactions_trajectory;
@@ -607,6 +618,27 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     src.push_back(d);
     for (int l = 0; l < NL; l++) src_mus.push_back(layer_mu_host(&in->layers[l], e->energy));
   }
+  if (esc) {
+    /* escape-ratio run: one line per input energy, weight 1, random
+     * polarisation azimuth (src/xmi_main.F90:5640-5690) */
+    src.clear();
+    src_mus.clear();
+    for (int k = 0; k < esc->nE; k++) {
+      SourceDesc d;
+      memset(&d, 0, sizeof(d));
+      d.kind = 1;
+      d.valid = 1;
+      d.dist = 0;
+      d.pol = 1;
+      d.energy = esc->E0 + k * esc->dE;
+      d.count = esc->n_photons;
+      d.weight = 1.0;
+      d.hid0 = (uint64_t)k * (uint64_t)esc->n_photons;
+      src.push_back(d);
+      for (int l = 0; l < NL; l++) src_mus.push_back(layer_mu_host(&in->layers[l], d.energy));
+    }
+    total_weight = (double)esc->n_photons; /* the largest tally: one energy's photons */
+  }
   /* history ids are laid out with the full per-source budget; batches only run
    * the first `count` of each source */
   std::vector<double> exc_flat;
@@ -670,6 +702,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(G.lay_list.ensure((size_t)NL * cap * sizeof(int)));
     HIP_TRY(G.lay_woff.ensure((size_t)NL * step_waves * sizeof(int)));
     HIP_TRY(G.lay_total.ensure((size_t)ENG_MAX_LAYERS * sizeof(int)));
+    HIP_TRY(G.lay_segbase.ensure((size_t)ENG_MAX_LAYERS * 32 * sizeof(int)));
     /* off-grid cone accumulators: zero here, and cleared again by the
      * kernel after each photon's estimate */
     {
@@ -756,15 +789,35 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.phi_Tinv = T->phi_Tinv;
     P.phi_R0 = T->phi_R0;
     P.phi_Rinv = T->phi_Rinv;
-    P.sa_grid = G.sa_grid.as<double>();
-    P.sa_r = G.sa_r.as<double>();
-    P.sa_t = G.sa_t.as<double>();
-    P.sa_nr = G.sa_nr;
-    P.sa_nt = G.sa_nt;
-    P.sa_r0 = solid_angles->grid_dims_r_vals[0];
-    P.sa_rinv = uniform_inv(solid_angles->grid_dims_r_vals, (int)solid_angles->grid_dims_r_n);
-    P.sa_t0 = solid_angles->grid_dims_theta_vals[0];
-    P.sa_tinv = uniform_inv(solid_angles->grid_dims_theta_vals, (int)solid_angles->grid_dims_theta_n);
+    if (!esc) {
+      P.sa_grid = G.sa_grid.as<double>();
+      P.sa_r = G.sa_r.as<double>();
+      P.sa_t = G.sa_t.as<double>();
+      P.sa_nr = G.sa_nr;
+      P.sa_nt = G.sa_nt;
+      P.sa_r0 = solid_angles->grid_dims_r_vals[0];
+      P.sa_rinv = uniform_inv(solid_angles->grid_dims_r_vals, (int)solid_angles->grid_dims_r_n);
+      P.sa_t0 = solid_angles->grid_dims_theta_vals[0];
+      P.sa_tinv = uniform_inv(solid_angles->grid_dims_theta_vals, (int)solid_angles->grid_dims_theta_n);
+    }
+    /* escape tallies: fluorescence, Compton, normalisation */
+    EscParams Q;
+    memset(&Q, 0, sizeof(Q));
+    size_t esc_words = 0;
+    if (esc) {
+      const size_t nf = (size_t)esc->nE * ENG_ESC_LINES * nZ, nc = (size_t)esc->nOut * esc->nE;
+      esc_words = 2 * (nf + nc + (size_t)esc->nE);
+      HIP_TRY(G.esc_tally.ensure(esc_words * sizeof(unsigned long long)));
+      HIP_TRY(hipMemsetAsync(G.esc_tally.p, 0, esc_words * sizeof(unsigned long long), st));
+      Q.fluo = G.esc_tally.as<unsigned long long>();
+      Q.compton = Q.fluo + 2 * nf;
+      Q.interacted = Q.compton + 2 * nc;
+      Q.n_photons = esc->n_photons;
+      Q.nE = esc->nE;
+      Q.nOut = esc->nOut;
+      Q.out0 = esc->out0;
+      Q.dOut = esc->dOut;
+    }
     unsigned long long *tw = G.tallies.as<unsigned long long>();
     P.t_ch = tw;
     P.t_line = tw + n_ch2;
@@ -774,6 +827,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.lay_list = G.lay_list.as<int>();
     P.lay_woff = G.lay_woff.as<int>();
     P.lay_total = G.lay_total.as<int>();
+    P.lay_segbase = G.lay_segbase.as<int>();
     P.cone_acc = G.cone_acc.as<unsigned long long>();
 
     PhotonSoA S;
@@ -833,6 +887,10 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       for (int64_t done = 0; done < total_hist; done += cap) {
         int count = (int)((total_hist - done) < cap ? (total_hist - done) : cap);
         HIP_TRY(eng_launch_source(&P, &S, &ST, (uint64_t)done, count, st));
+        if (esc) {
+          HIP_TRY(eng_launch_escape(&P, &S, &V, &Q, count, st));
+          continue;
+        }
         int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
         for (int n = 1; n <= n_int; n++)
           HIP_TRY(eng_launch_order(&P, &S, &V, n, count, n_cu, st,
@@ -881,6 +939,29 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       goto fail;
     }
 
+    if (esc) {
+      /* ratios: escaped weight over the weight of the photons that
+       * interacted, per input energy (src/xmi_main.F90:5773-5776) */
+      std::vector<unsigned long long> et(esc_words);
+      HIP_TRY(hipMemcpy(et.data(), G.esc_tally.p, esc_words * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      const size_t nf = (size_t)esc->nE * ENG_ESC_LINES * nZ, nc = (size_t)esc->nOut * esc->nE;
+      const unsigned long long *tf = et.data(), *tc = tf + 2 * nf, *ti = tc + 2 * nc;
+      esc->fluo.assign(nf, 0.0);
+      esc->compton.assign(nc, 0.0);
+      for (int k = 0; k < esc->nE; k++) {
+        const double norm = fx_to_double(fx_get(&ti[(size_t)k * 2]), shift);
+        if (!(norm > 0.0)) continue;
+        for (int line = 0; line < ENG_ESC_LINES; line++)
+          for (int u = 0; u < nZ; u++)
+            esc->fluo[(size_t)u + (size_t)nZ * (line + (size_t)ENG_ESC_LINES * k)] =
+                fx_to_double(fx_get(&tf[(((size_t)k * ENG_ESC_LINES + line) * nZ + u) * 2]), shift) / norm;
+        for (int o = 0; o < esc->nOut; o++)
+          esc->compton[(size_t)k + (size_t)esc->nE * o] =
+              fx_to_double(fx_get(&tc[((size_t)o * esc->nE + k) * 2]), shift) / norm;
+      }
+      ok = 1;
+      goto fail;
+    }
     /* exact sums: lines into their channels, prefix over orders */
     const unsigned long long *tch = th.data(), *tln = th.data() + n_ch2, *trc = th.data() + n_ch2 + n_line2;
     std::vector<u128> order_ch((size_t)n_int * nch);
@@ -933,6 +1014,11 @@ fail:
     free(bf_out);
     return 0;
   }
+  if (esc) {
+    g_stats.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall0).count();
+    return 1;
+  }
   *channelsPtr = ch_out;
   *var_red_historyPtr = vr_out;
   *brute_historyPtr = bf_out;
@@ -944,6 +1030,118 @@ fail:
 /* ------------------------------------------------------------------------ */
 /* solid-angle grid (xmi_solid_angle_inputs_f, src/xmi_solid_angle_f.F90:62-301,
  * then the per-cell estimate of :432-710 on the GPU) */
+
+/* ------------------------------------------------------------------------ */
+/* escape ratios of the detector crystal (src/xmi_detector.c:91-141 builds
+ * the input, src/xmi_main.F90:5473-5801 runs it): the crystal as a sample
+ * at normal incidence of a pencil beam, one forced interaction per photon */
+
+extern "C" xmi_escape_ratios_options xmi_get_default_escape_ratios_options(void) {
+  xmi_escape_ratios_options o;
+  o.n_input_energies = 1990;
+  o.n_compton_output_energies = 1999;
+  o.n_photons = 500000;
+  o.input_energy_min = 1.0;
+  o.input_energy_delta = 0.1;
+  o.compton_output_energy_min = 0.1;
+  o.compton_output_energy_delta = 0.1;
+  return o;
+}
+
+extern "C" void xmi_free_escape_ratios(xmi_escape_ratios *r) {
+  if (!r) return;
+  free(r->Z);
+  free(r->fluo_escape_ratios);
+  free(r->fluo_escape_input_energies); /* shared with compton_escape_input_energies */
+  free(r->compton_escape_ratios);
+  free(r->compton_escape_output_energies);
+  free(r->xmi_input_string);
+  free(r);
+}
+
+extern "C" void xmi_escape_ratios_calculation(xmi_input *input, xmi_escape_ratios **escape_ratios, char *input_string,
+                                              char *hdf5_file, xmi_main_options *options,
+                                              xmi_escape_ratios_options ero) {
+  if (escape_ratios) *escape_ratios = nullptr;
+  if (!input || !escape_ratios || !options || !input->general || !input->geometry || !input->detector ||
+      input->detector->n_crystal_layers < 1 || ero.n_input_energies < 1 || ero.n_compton_output_energies < 1 ||
+      ero.n_photons < 1) {
+    xmi_set_error("xmi_escape_ratios_calculation: invalid argument");
+    return;
+  }
+  /* the escape input: crystal layers as the composition, reference layer 1,
+   * source 1 cm from the sample behind a 1 um slit, sample normal along z */
+  xmi_general gen = *input->general;
+  gen.n_interactions_trajectory = 1;
+  gen.n_photons_line = 100000;
+  xmi_composition comp;
+  comp.n_layers = input->detector->n_crystal_layers;
+  comp.layers = input->detector->crystal_layers;
+  comp.reference_layer = 1;
+  xmi_geometry geo = *input->geometry;
+  geo.d_sample_source = 1.0;
+  geo.d_source_slit = 1.0;
+  geo.slit_size_x = 0.0001;
+  geo.slit_size_y = 0.0001;
+  geo.n_sample_orientation[0] = 0.0;
+  geo.n_sample_orientation[1] = 0.0;
+  geo.n_sample_orientation[2] = 1.0;
+  xmi_input esc_in = *input;
+  esc_in.general = &gen;
+  esc_in.composition = &comp;
+  esc_in.geometry = &geo;
+  xmi_main_options opts = *options;
+  opts.use_M_lines = 0;
+  opts.use_cascade_auger = 0;
+  opts.use_cascade_radiative = 0;
+  opts.use_variance_reduction = 0;
+  opts.use_sum_peaks = 0;
+  opts.escape_ratios_mode = 1;
+  opts.use_advanced_compton = 0;
+
+  xmi_inputFPtr inF = nullptr;
+  xmi_hdf5FPtr h5 = nullptr;
+  EscapeRun run;
+  run.nE = (int)ero.n_input_energies;
+  run.nOut = (int)ero.n_compton_output_energies;
+  run.n_photons = ero.n_photons;
+  run.E0 = ero.input_energy_min;
+  run.dE = ero.input_energy_delta;
+  run.out0 = ero.compton_output_energy_min;
+  run.dOut = ero.compton_output_energy_delta;
+  int ok = xmi_input_C2F(&esc_in, &inF) && xmi_init_input(&inF) &&
+           (hdf5_file ? xmi_init_from_hdf5(hdf5_file, inF, &h5, &opts) : xmi_init_from_provider(inF, &h5, &opts)) &&
+           xmi_update_input_from_hdf5(inF, h5);
+  if (ok) {
+    std::lock_guard<std::mutex> guard(g_engine.lock);
+    ok = msim_run(g_engine, inF, h5, 1, nullptr, &opts, nullptr, nullptr, nullptr, &run);
+  }
+  xmi_escape_ratios *r = nullptr;
+  if (ok) {
+    const xmi_hdf5F *h = (const xmi_hdf5F *)h5;
+    const int nZ = h->n_Z;
+    r = (xmi_escape_ratios *)calloc(1, sizeof(xmi_escape_ratios));
+    r->n_elements = nZ;
+    r->n_fluo_input_energies = run.nE;
+    r->n_compton_input_energies = run.nE;
+    r->n_compton_output_energies = run.nOut;
+    r->Z = (int *)malloc(sizeof(int) * nZ);
+    for (int u = 0; u < nZ; u++) r->Z[u] = h->Zs[u].Z;
+    r->fluo_escape_ratios = (double *)malloc(sizeof(double) * run.fluo.size());
+    memcpy(r->fluo_escape_ratios, run.fluo.data(), sizeof(double) * run.fluo.size());
+    r->compton_escape_ratios = (double *)malloc(sizeof(double) * run.compton.size());
+    memcpy(r->compton_escape_ratios, run.compton.data(), sizeof(double) * run.compton.size());
+    r->fluo_escape_input_energies = (double *)malloc(sizeof(double) * run.nE);
+    for (int k = 0; k < run.nE; k++) r->fluo_escape_input_energies[k] = run.E0 + k * run.dE;
+    r->compton_escape_input_energies = r->fluo_escape_input_energies;
+    r->compton_escape_output_energies = (double *)malloc(sizeof(double) * run.nOut);
+    for (int k = 0; k < run.nOut; k++) r->compton_escape_output_energies[k] = run.out0 + k * run.dOut;
+    r->xmi_input_string = input_string ? strdup(input_string) : nullptr;
+  }
+  if (h5) xmi_free_hdf5_F(&h5);
+  if (inF) xmi_free_input_F(&inF);
+  *escape_ratios = r;
+}
 
 extern "C" int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
                              xmi_main_options *options, double **brute_historyPtr, double **var_red_historyPtr,

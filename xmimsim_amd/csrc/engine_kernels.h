@@ -42,7 +42,9 @@ struct SourceDesc {
   int kind;               /* 0 = continuous interval, 1 = discrete line */
   int valid;
   int dist;               /* XmiEnergyDiscreteDistribution for lines */
-  int pad;
+  int pol;                /* line polarisation: 0 from the intensities, 1 uniform
+                             azimuth in the x-y plane (escape-ratio runs,
+                             src/xmi_main.F90:5672-5687) */
   long count;             /* histories of this source */
   uint64_t hid0;          /* first history id */
   uint64_t run0;          /* position of its first history in the run order
@@ -64,9 +66,23 @@ struct SourceTable {
   const xrl_atom *exc_atoms;  /* [95] indexed by Z */
 };
 
+/* escape-ratio runs (src/xmi_main.F90:5473-5801): fixed-point tallies of
+ * the photons that leave the crystal after their one forced interaction */
+#define ENG_ESC_LINES 109 /* K and L lines: KL1 (1) .. L3P3 (109) */
+struct EscParams {
+  unsigned long long *fluo;       /* [nE][ENG_ESC_LINES][nZ][2] */
+  unsigned long long *compton;    /* [nOut][nE][2] */
+  unsigned long long *interacted; /* [nE][2] */
+  long n_photons;                 /* per input energy: history id / n_photons = energy index */
+  int nE, nOut;
+  double out0, dOut;              /* Compton output energy grid */
+};
+
 #ifdef __cplusplus
 extern "C" {
 #endif
+hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, const EscParams *Q,
+                             int count, hipStream_t st);
 hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv, const double *tv, int n_r, int n_t,
                            long hits, uint64_t seed, hipStream_t st);
 hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,

@@ -28,6 +28,10 @@
 
 #include "engine_kernels.h"
 
+/* segments of the per-wave layer-count scan (k_layer_scan, d_layer_list) */
+#define ENG_SCAN_SEG 1024
+#define ENG_SCAN_NSEG 32
+
 /* ------------------------------------------------------------------------ */
 /* physics helpers on the device */
 
@@ -359,11 +363,21 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   dirv[2] = 1.0;
   d_normalize(dirv);
 
-  int horizontal;
-  if (d.kind == 0) horizontal = phx_uniform(&s) <= hor;
-  else horizontal = (double)(j + 1) <= d.hor_ver_ratio;
-
-  double ev[3] = {horizontal ? 0.0 : 1.0, horizontal ? 1.0 : 0.0, 0.0};
+  double ev[3];
+  if (d.pol == 1) {
+    /* escape-ratio runs: a random polarisation azimuth in the x-y plane */
+    const double te = phx_uniform(&s) * ENG_PI * 2.0;
+    ev[0] = cos(te);
+    ev[1] = sin(te);
+    ev[2] = 0.0;
+  } else {
+    int horizontal;
+    if (d.kind == 0) horizontal = phx_uniform(&s) <= hor;
+    else horizontal = (double)(j + 1) <= d.hor_ver_ratio;
+    ev[0] = horizontal ? 0.0 : 1.0;
+    ev[1] = horizontal ? 1.0 : 0.0;
+    ev[2] = 0.0;
+  }
   double cosalfa = d_dot(ev, dirv);
   if (fabs(cosalfa) > 1.0) {
     atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
@@ -482,7 +496,8 @@ __device__ __forceinline__ void d_layer_list(const EngineParams &P, int layer, i
   for (int L = 1; L <= P.n_layers; L++) {
     const unsigned long long m = __ballot(layer == L);
     if (layer == L) {
-      const int pos = P.lay_woff[(size_t)(L - 1) * P.n_waves + w] + __popcll(m & below);
+      const int pos = P.lay_woff[(size_t)(L - 1) * P.n_waves + w] +
+                      P.lay_segbase[(L - 1) * ENG_SCAN_NSEG + (int)(w / ENG_SCAN_SEG)] + __popcll(m & below);
       if ((unsigned int)pos < (unsigned int)P.cap) P.lay_list[(size_t)(L - 1) * P.cap + pos] = idx;
       else atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull); /* counts of k_step and k_vr_geom disagree */
     }
@@ -609,22 +624,18 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step(EngineParams P, PhotonSoA S,
   d_layer_count(P, ev_layer);
 }
 
-/* exclusive scan of one layer's per-wave counts (workgroup l - 1 scans layer
- * l), in place, and the layer's total; nw = waves of the k_step launch, at
- * most ENG_SCAN_MAX (32 per thread: the thread's counts are loaded at once
- * and summed, then one wave scan and one pass over the 16 wave totals) */
-#define ENG_SCAN_MAX (1024 * 32)
-__global__ void __launch_bounds__(1024) k_layer_scan(EngineParams P, int nw) {
-  __shared__ int wsum[16];
-  int *row = P.lay_woff + (size_t)blockIdx.x * P.n_waves;
-  const int b = threadIdx.x * 32, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int v[32], s = 0;
-#pragma unroll
-  for (int j = 0; j < 32; j++) {
-    v[j] = b + j < nw ? row[b + j] : 0;
-    s += v[j];
-  }
-  int incl = s;
+/* exclusive scan of the per-wave counts in segments of ENG_SCAN_SEG waves:
+ * workgroup (segment, layer) scans its segment in place (coalesced, one
+ * count per thread) and stores the segment's total; k_layer_base then turns
+ * the totals into segment offsets and each layer's total.  nw = waves of the
+ * k_step launch, at most ENG_SCAN_SEG * ENG_SCAN_NSEG. */
+__global__ void __launch_bounds__(ENG_SCAN_SEG) k_layer_scan(EngineParams P, int nw) {
+  __shared__ int wsum[ENG_SCAN_SEG / 64];
+  const int seg = blockIdx.x, L0 = blockIdx.y;
+  int *row = P.lay_woff + (size_t)L0 * P.n_waves;
+  const int k = seg * ENG_SCAN_SEG + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int v = k < nw ? row[k] : 0;
+  int incl = v;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
     const int t = __shfl_up(incl, off);
@@ -633,14 +644,23 @@ __global__ void __launch_bounds__(1024) k_layer_scan(EngineParams P, int nw) {
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
   int base = 0;
-  for (int k = 0; k < w; k++) base += wsum[k];
-  int run = base + incl - s;
-#pragma unroll
-  for (int j = 0; j < 32; j++) {
-    if (b + j < nw) row[b + j] = run;
-    run += v[j];
+  for (int j = 0; j < w; j++) base += wsum[j];
+  if (k < nw) row[k] = base + incl - v;
+  if (threadIdx.x == ENG_SCAN_SEG - 1) P.lay_segbase[L0 * ENG_SCAN_NSEG + seg] = base + incl;
+}
+
+/* segment totals -> segment offsets, and each layer's total (one lane per
+ * layer) */
+__global__ void k_layer_base(EngineParams P, int nseg) {
+  const int L0 = threadIdx.x;
+  if (L0 >= P.n_layers) return;
+  int run = 0;
+  for (int s = 0; s < nseg; s++) {
+    const int t = P.lay_segbase[L0 * ENG_SCAN_NSEG + s];
+    P.lay_segbase[L0 * ENG_SCAN_NSEG + s] = run;
+    run += t;
   }
-  if (threadIdx.x == 1023) P.lay_total[blockIdx.x] = base + incl;
+  P.lay_total[L0] = run;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1489,6 +1509,106 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S
 }
 
 /* ------------------------------------------------------------------------ */
+/* escape-ratio runs (src/xmi_main.F90:5473-5801 with the analogue step of
+ * :1274-1407): after its forced first interaction a photon leaves the crystal
+ * when the analogue step's random number (the first draw of the next order's
+ * transport stream) exceeds the probability of interacting on the way out;
+ * escaped Compton photons are tallied by energy, fluorescence photons by
+ * element and line (weight times fluorescence yield), and the weight after
+ * the forced step of every interacted photon normalises its input energy.
+ * One thread per queue entry of the three interaction queues. */
+/* exact wave-aggregated fixed-point add: every lane with key >= 0 adds
+ * value to bin key of tally; the lanes of one key are summed as integers
+ * (three 42-bit limbs, which 64 lanes cannot overflow) and added once, so
+ * that thousands of photons of one bin do not serialise on its address and
+ * the sum does not depend on which photons share a wavefront.  Call from
+ * converged code. */
+__device__ __forceinline__ void d_fx_add_keyed(unsigned long long *tally, long key, double value, int shift) {
+  unsigned long long lo = 0ull, hi = 0ull;
+  if (key >= 0) fx_split(value, shift, &lo, &hi);
+  const unsigned long long M42 = (1ull << 42) - 1ull;
+  const unsigned long long l0 = lo & M42, l1 = (lo >> 42) | ((hi & ((1ull << 20) - 1ull)) << 22), l2 = hi >> 20;
+  unsigned long long pending = __ballot(key >= 0);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const long k = __shfl(key, leader);
+    const int mine = key == k;
+    unsigned long long a0 = mine ? l0 : 0ull, a1 = mine ? l1 : 0ull, a2 = mine ? l2 : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+      a0 += __shfl_xor(a0, off);
+      a1 += __shfl_xor(a1, off);
+      a2 += __shfl_xor(a2, off);
+    }
+    if (__lane_id() == leader) {
+      const unsigned __int128 t = (unsigned __int128)a0 + ((unsigned __int128)a1 << 42) + ((unsigned __int128)a2 << 84);
+      fx_merge_global(&tally[(size_t)k * 2], (unsigned long long)t, (unsigned long long)(t >> 64));
+    }
+    pending &= ~__ballot(mine);
+  }
+}
+
+__global__ void __launch_bounds__(ENG_BLOCK) k_escape(EngineParams P, PhotonSoA S, VrSoA V, EscParams Q) {
+  const unsigned int per = (unsigned int)ENG_SPREAD * (unsigned int)P.qreg;
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int type = (int)(q / per);
+  const int i = type <= 2 ? d_dequeue(P, type, q - (unsigned int)type * per) : -1;
+  const int shift = P.tally_shift;
+  long key_n = -1, key_c = -1, key_f = -1;
+  double w_n = 0.0, w = 0.0;
+  if (i >= 0) {
+    const int iE = (int)(S.hid[i] / (uint64_t)Q.n_photons);
+    if (iE >= 0 && iE < Q.nE) {
+      /* the photon's weight after the forced step (V.pesc) normalises;
+       * k_photo has multiplied S.weight by the fluorescence yield, which an
+       * escape-ratio run applies to the escape tally only
+       * (src/xmi_variance_reduction.F90:742-746) */
+      key_n = iE;
+      w_n = V.pesc[i];
+      w = S.weight[i];
+      const double E = S.energy[i];
+      const int cur = S.layer[i];
+      int escaped = E >= 1.0 && cur >= 1; /* below the energy threshold: stays inside */
+      if (escaped) {
+        phx_stream s;
+        phx_stream_init(&s, P.seed, P.sample, 2u, PHX_SUB_TRANSPORT, S.hid[i]);
+        const double R = phx_uniform(&s);
+        const double dirv[3] = {S.dx[i], S.dy[i], S.dz[i]};
+        const int up = d_dot(dirv, P.n_sample) > 0.0;
+        double lp[3] = {S.x[i], S.y[i], S.z[i]};
+        double max_r = 0.0, blbs = 1.0;
+        const size_t cap = (size_t)P.cap;
+        for (int l = cur - 1; up ? l < P.n_layers : l >= 0; l += up ? 1 : -1) {
+          double pp[3] = {0.0, 0.0, up ? P.layer_ze[l] : P.layer_zb[l]}, ip[3];
+          d_intersect(pp, P.n_sample, lp, dirv, ip);
+          const double dist = d_distance(lp, ip);
+          const double temp_prod = -1.0 * dist * P.layer_density[l] * S.mus[(size_t)l * cap + i];
+          const double tempexp = exp(temp_prod);
+          max_r = max_r - blbs * expm1(temp_prod);
+          if (R <= max_r) { /* interacts on the way out: inside */
+            escaped = 0;
+            break;
+          }
+          lp[0] = ip[0];
+          lp[1] = ip[1];
+          lp[2] = ip[2];
+          blbs = blbs * tempexp;
+        }
+      }
+      if (escaped && type == 1) {
+        const int idx = (int)((E - Q.out0) / Q.dOut) + 1;
+        if (idx >= 1 && idx <= Q.nOut) key_c = (long)(idx - 1) * Q.nE + iE;
+      } else if (escaped && type == 2) {
+        const int line = S.prev_line[i], u = S.prev_u[i];
+        if (line >= 1 && line <= ENG_ESC_LINES && u >= 0) key_f = ((long)iE * ENG_ESC_LINES + (line - 1)) * P.nZ + u;
+      }
+    }
+  }
+  d_fx_add_keyed(Q.interacted, key_n, w_n, shift);
+  d_fx_add_keyed(Q.compton, key_c, w, shift);
+  d_fx_add_keyed(Q.fluo, key_f, w, shift);
+}
+
+/* ------------------------------------------------------------------------ */
 /* host-side launch wrappers */
 
 static size_t lds_if_fits(size_t words) {
@@ -1579,6 +1699,32 @@ hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const So
   return hipGetLastError();
 }
 
+/* one escape-ratio batch: the forced first interaction, the weight after it
+ * kept in V.pesc (the weight of the escape tallies), the interaction, the
+ * escape test and tallies */
+hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, const EscParams *Q,
+                             int count, hipStream_t st) {
+  int blocks = (count + ENG_BLOCK - 1) / ENG_BLOCK;
+  hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
+  {
+    const int nl = P->n_layers;
+    if (nl <= 1) hipLaunchKernelGGL((k_step<1>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
+    else if (nl <= 2) hipLaunchKernelGGL((k_step<2>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
+    else if (nl <= 4) hipLaunchKernelGGL((k_step<4>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
+    else if (nl <= 8) hipLaunchKernelGGL((k_step<8>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
+    else hipLaunchKernelGGL((k_step<16>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
+  }
+  hipError_t e = hipMemcpyAsync(V->pesc, S->weight, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
+  const int qblocks = (int)(((size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
+  hipLaunchKernelGGL(k_rayleigh, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
+  hipLaunchKernelGGL(k_compton, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
+  hipLaunchKernelGGL(k_photo, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
+  const int eblocks = (int)((3 * (size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
+  hipLaunchKernelGGL(k_escape, dim3(eblocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, *Q);
+  return hipGetLastError();
+}
+
 hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
                             hipStream_t st, hipEvent_t *ev /* ENG_ORDER_EVENTS events or NULL */) {
   int blocks = (count + ENG_BLOCK - 1) / ENG_BLOCK;
@@ -1592,7 +1738,11 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
     else if (nl <= 8) hipLaunchKernelGGL((k_step<8>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
     else hipLaunchKernelGGL((k_step<16>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
   }
-  hipLaunchKernelGGL(k_layer_scan, dim3(P->n_layers), dim3(1024), 0, st, *P, blocks * (ENG_BLOCK / 64));
+  {
+    const int nw = blocks * (ENG_BLOCK / 64), nseg = (nw + ENG_SCAN_SEG - 1) / ENG_SCAN_SEG;
+    hipLaunchKernelGGL(k_layer_scan, dim3(nseg, P->n_layers), dim3(ENG_SCAN_SEG), 0, st, *P, nw);
+    hipLaunchKernelGGL(k_layer_base, dim3(1), dim3(64), 0, st, *P, nseg);
+  }
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
   const int qblocks = (int)(((size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);

@@ -43,64 +43,18 @@ static const char *shell_group_names[9] = {"K shell",  "L1 shell", "L2 shell", "
                                            "M2 shell", "M3 shell", "M4 shell", "M5 shell"};
 
 /* ------------------------------------------------------------------------ */
-/* angle ICDFs (src/xmi_data_f.F90:996-1073) */
+/* angle ICDFs (src/xmi_data_f.F90:996-1073), one row per energy */
 static void theta_icdf(const xrl_atom *a, int compton, const double *energies, int n_e, const double *rs,
                        int n_r, double *out /* [n_e][n_r] */) {
 #pragma omp parallel for schedule(dynamic, 4)
-  for (int j = 0; j < n_e; j++) {
-    double *cdf = (double *)malloc(sizeof(double) * NTHETA_INT);
-    const double h = M_PI / (NTHETA_INT - 1);
-    double prev = 0.0;
-    cdf[0] = 0.0;
-    for (int k = 1; k < NTHETA_INT; k++) {
-      double th = k * h;
-      double f = (compton ? xrl_dcs_compt(a, energies[j], th) : xrl_dcs_rayl(a, energies[j], th)) * sin(th);
-      double fprev = prev;
-      cdf[k] = cdf[k - 1] + 0.5 * (f + fprev) * h;
-      prev = f;
-    }
-    double total = cdf[NTHETA_INT - 1];
-    double *row = out + (size_t)j * n_r;
-    int k = 1;
-    for (int m = 0; m < n_r; m++) {
-      double target = rs[m] * total;
-      while (k < NTHETA_INT - 1 && cdf[k] < target) k++;
-      double c0 = cdf[k - 1], c1 = cdf[k];
-      double fr = (c1 > c0) ? (target - c0) / (c1 - c0) : 0.0;
-      if (fr < 0.0) fr = 0.0;
-      if (fr > 1.0) fr = 1.0;
-      row[m] = ((k - 1) + fr) * h;
-    }
-    row[0] = 0.0;
-    row[n_r - 1] = M_PI;
-    free(cdf);
-  }
+  for (int j = 0; j < n_e; j++)
+    xrl_theta_icdf_row(a, compton, energies[j], rs, n_r, NTHETA_INT, out + (size_t)j * n_r, 1);
 }
 
-/* Compton profile CDF of a hydrogenic 1s-shaped profile: int_0^u (1+t^2)^-3 dt */
-static double G3(double u) {
-  double d = 1.0 + u * u;
-  return u / (4.0 * d * d) + 3.0 * u / (8.0 * d) + 0.375 * atan(u);
-}
-
-static double total_profile_cdf(const xrl_atom *a, double pz) {
-  double s = 0.0;
-  for (int k = 0; k < XRL_NSHELLS; k++)
-    if (a->occupancy[k] > 0.0) s += a->occupancy[k] * G3(pz / a->profile_p[k]);
-  return s;
-}
-
-static double bisect_G3(double target, double p, double lo, double hi) {
-  /* solve G3(q/p) = target for q in [lo,hi] */
-  for (int it = 0; it < 100; it++) {
-    double mid = 0.5 * (lo + hi);
-    if (G3(mid / p) < target) lo = mid;
-    else hi = mid;
-  }
-  return 0.5 * (lo + hi);
-}
-
-int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, xmi_hdf5_ZF *out) {
+/* heavy = 0 leaves the angle inverse CDFs, the total-profile inverse CDF and
+ * the partial-profile inversions (cp_Qs_inv, except its ends) to the GPU
+ * generator (xmi_tables_fill_gpu); the arrays are allocated either way */
+int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, int heavy, xmi_hdf5_ZF *out) {
   const xrl_atom *a = xrl_atom_get(Z);
   if (!a) {
     xmi_set_error("no physics data for Z=%d", Z);
@@ -116,8 +70,10 @@ int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, xm
   for (int i = 0; i < XMI_DB_NR; i++) out->RandomNumbers[i] = i / (double)(XMI_DB_NR - 1);
   out->RayleighTheta_ICDF = (double *)malloc(sizeof(double) * XMI_DB_NE * XMI_DB_NR);
   out->ComptonTheta_ICDF = (double *)malloc(sizeof(double) * XMI_DB_NE * XMI_DB_NR);
-  theta_icdf(a, 0, out->Energies, XMI_DB_NE, out->RandomNumbers, XMI_DB_NR, out->RayleighTheta_ICDF);
-  theta_icdf(a, 1, out->Energies, XMI_DB_NE, out->RandomNumbers, XMI_DB_NR, out->ComptonTheta_ICDF);
+  if (heavy) {
+    theta_icdf(a, 0, out->Energies, XMI_DB_NE, out->RandomNumbers, XMI_DB_NR, out->RayleighTheta_ICDF);
+    theta_icdf(a, 1, out->Energies, XMI_DB_NE, out->RandomNumbers, XMI_DB_NR, out->ComptonTheta_ICDF);
+  }
 
   /* interaction probabilities (src/xmi_data_f.F90:1084-1125) */
   {
@@ -174,27 +130,20 @@ int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, xm
       out->cp_partial_cdf_inv[i] = 0.5 * i / (double)(N - 1);
       out->cp_random_numbers[i] = i / (double)(N - 1);
     }
-    double cmax = total_profile_cdf(a, MAXPZ);
-    for (int i = 0; i < N; i++) {
-      double target = out->cp_random_numbers[i] * cmax;
-      double lo = 0.0, hi = MAXPZ;
-      for (int it = 0; it < 100; it++) {
-        double mid = 0.5 * (lo + hi);
-        if (total_profile_cdf(a, mid) < target) lo = mid;
-        else hi = mid;
-      }
-      out->cp_total_icdf[i] = 0.5 * (lo + hi);
-    }
+    double cmax = xrl_total_profile_cdf(a, MAXPZ);
+    if (heavy)
+      for (int i = 0; i < N; i++)
+        out->cp_total_icdf[i] = xrl_bisect_total_profile(a, out->cp_random_numbers[i] * cmax, 0.0, MAXPZ);
     out->cp_total_icdf[0] = 0.0;
     out->cp_total_icdf[N - 1] = MAXPZ;
     for (int k = 0; k < nsh; k++) {
       int s = idx[k];
       double p = a->profile_p[s] > 0.0 ? a->profile_p[s] : 1.0;
-      double gmax = G3(MAXPZ / p);
+      double gmax = xrl_profile_G3(MAXPZ / p);
       for (int i = 0; i < N; i++) {
-        out->cp_partial_cdf[(size_t)k * N + i] = 0.5 * G3(out->cp_Qs[i] / p) / gmax;
+        out->cp_partial_cdf[(size_t)k * N + i] = 0.5 * xrl_profile_G3(out->cp_Qs[i] / p) / gmax;
         double target = out->cp_partial_cdf_inv[i] / 0.5 * gmax;
-        out->cp_Qs_inv[(size_t)k * N + i] = (i == N - 1) ? MAXPZ : bisect_G3(target, p, 0.0, MAXPZ);
+        out->cp_Qs_inv[(size_t)k * N + i] = (i == N - 1) ? MAXPZ : heavy ? xrl_bisect_G3(target, p, 0.0, MAXPZ) : 0.0;
       }
       out->cp_Qs_inv[(size_t)k * N] = 0.0;
     }
@@ -221,7 +170,7 @@ int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, xm
 }
 
 /* azimuth ICDF, CDF(phi) = (phi - t sin(2 phi)) / 2pi (src/xmi_data_f.F90:1483-1549) */
-int xmi_tables_build_phi(xmi_hdf5F *h) {
+int xmi_tables_build_phi(xmi_hdf5F *h, int heavy) {
   h->phi_n_theta = XMI_DB_NTHETA2;
   h->phi_n_r = XMI_DB_NR;
   h->Thetas = (double *)malloc(sizeof(double) * XMI_DB_NTHETA2);
@@ -229,18 +178,11 @@ int xmi_tables_build_phi(xmi_hdf5F *h) {
   h->Phi_ICDF = (double *)malloc(sizeof(double) * XMI_DB_NTHETA2 * XMI_DB_NR);
   for (int i = 0; i < XMI_DB_NTHETA2; i++) h->Thetas[i] = 0.5 * i / (double)(XMI_DB_NTHETA2 - 1);
   for (int i = 0; i < XMI_DB_NR; i++) h->RandomNumbers[i] = i / (double)(XMI_DB_NR - 1);
+  if (heavy)
+    for (int i = 0; i < XMI_DB_NTHETA2; i++)
+      for (int k = 0; k < XMI_DB_NR; k++)
+        h->Phi_ICDF[(size_t)i * XMI_DB_NR + k] = xrl_bisect_phi(h->Thetas[i], h->RandomNumbers[k] * 2.0 * M_PI);
   for (int i = 0; i < XMI_DB_NTHETA2; i++) {
-    double t = h->Thetas[i];
-    for (int k = 0; k < XMI_DB_NR; k++) {
-      double target = h->RandomNumbers[k] * 2.0 * M_PI;
-      double lo = 0.0, hi = 2.0 * M_PI;
-      for (int it = 0; it < 100; it++) {
-        double mid = 0.5 * (lo + hi);
-        if (mid - t * sin(2.0 * mid) < target) lo = mid;
-        else hi = mid;
-      }
-      h->Phi_ICDF[(size_t)i * XMI_DB_NR + k] = 0.5 * (lo + hi);
-    }
     h->Phi_ICDF[(size_t)i * XMI_DB_NR] = 0.0;
     h->Phi_ICDF[(size_t)i * XMI_DB_NR + XMI_DB_NR - 1] = 2.0 * M_PI;
   }
@@ -298,22 +240,57 @@ int xmi_init_from_provider(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *hdf5FPtr, xmi_
   xmi_inputF *in = (xmi_inputF *)inputFPtr;
   int uniq[XMI_MAXZ + 1];
   int n = xmi_unique_Z(in, uniq);
+  /* the heavy loops run on the GPU when one is usable */
+  const int dev = xmi_tables_use_device();
+  if (dev < 0) {
+    xmi_set_error("table generator: no usable GPU");
+    return 0;
+  }
+  xrl_atoms_preload(uniq, n);
   xmi_hdf5F *h = (xmi_hdf5F *)calloc(1, sizeof(xmi_hdf5F));
   h->cascade_type = cascade_type_of(options);
   h->use_M_lines = options->use_M_lines;
-  xmi_tables_build_phi(h);
+  xmi_tables_build_phi(h, !dev);
   h->n_Z = n;
   h->Zs = (xmi_hdf5_ZF *)calloc(n, sizeof(xmi_hdf5_ZF));
   for (int i = 0; i < n; i++) {
-    if (!xmi_tables_build_Z(uniq[i], uniq, n, h->cascade_type, &h->Zs[i])) {
+    if (!xmi_tables_build_Z(uniq[i], uniq, n, h->cascade_type, !dev, &h->Zs[i])) {
       xmi_tables_free(h);
       return 0;
     }
     h->Zs[i].Zindex = i + 1;
     h->uniqZ[uniq[i]] = i + 1;
   }
+  if (dev && !xmi_tables_fill_gpu(h->Zs, n, h)) {
+    xmi_tables_free(h);
+    return 0;
+  }
   *hdf5FPtr = h;
   return 1;
+}
+
+long xmi_tables_copy(xmi_hdf5FPtr hdf5FPtr, int Z, int which, double *out, long n) {
+  const xmi_hdf5F *h = (const xmi_hdf5F *)hdf5FPtr;
+  if (!h) return 0;
+  const double *src = NULL;
+  long len = 0;
+  if (which == 4) {
+    src = h->Phi_ICDF;
+    len = (long)h->phi_n_theta * h->phi_n_r;
+  } else {
+    if (Z < 1 || Z > XMI_MAXZ || h->uniqZ[Z] == 0) return 0;
+    const xmi_hdf5_ZF *z = &h->Zs[h->uniqZ[Z] - 1];
+    switch (which) {
+      case 0: src = z->RayleighTheta_ICDF; len = (long)z->n_e * z->n_r; break;
+      case 1: src = z->ComptonTheta_ICDF; len = (long)z->n_e * z->n_r; break;
+      case 2: src = z->cp_total_icdf; len = z->cp_len; break;
+      case 3: src = z->cp_Qs_inv; len = (long)z->cp_nshells * z->cp_len; break;
+      default: return 0;
+    }
+  }
+  if (!src) return 0;
+  if (out && n >= len) memcpy(out, src, sizeof(double) * len);
+  return len;
 }
 
 int xmi_update_input_from_hdf5(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr) {
@@ -382,10 +359,17 @@ int xmi_db(char *filename, int *Zs, int nZ) {
     H5Sclose(sp);
     H5Gclose(root);
   }
-  int ok = 1;
+  const int dev = xmi_tables_use_device();
+  int ok = dev >= 0;
+  if (!ok) xmi_set_error("table generator: no usable GPU");
   for (int i = 0; i < nZ && ok; i++) {
     xmi_hdf5_ZF z;
-    if (!xmi_tables_build_Z(sorted[i], NULL, 0, XRL_CASCADE_FULL, &z)) {
+    if (!xmi_tables_build_Z(sorted[i], NULL, 0, XRL_CASCADE_FULL, !dev, &z)) {
+      ok = 0;
+      break;
+    }
+    if (dev && !xmi_tables_fill_gpu(&z, 1, NULL)) {
+      free_Z(&z);
       ok = 0;
       break;
     }
@@ -465,7 +449,8 @@ int xmi_db(char *filename, int *Zs, int nZ) {
   if (ok) {
     xmi_hdf5F ph;
     memset(&ph, 0, sizeof(ph));
-    xmi_tables_build_phi(&ph);
+    xmi_tables_build_phi(&ph, !dev);
+    if (dev && !xmi_tables_fill_gpu(NULL, 0, &ph)) ok = 0;
     hid_t g = H5Gcreate2(file, "Phi", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
     /* on disk [r][theta] */
     double *tr = (double *)malloc(sizeof(double) * XMI_DB_NTHETA2 * XMI_DB_NR);
@@ -529,6 +514,7 @@ int xmi_init_from_hdf5(char *hdf5_file, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr *h
   }
   int uniq[XMI_MAXZ + 1];
   int n = xmi_unique_Z(in, uniq);
+  xrl_atoms_preload(uniq, n);
   xmi_hdf5F *h = (xmi_hdf5F *)calloc(1, sizeof(xmi_hdf5F));
   h->cascade_type = cascade_type_of(options);
   h->use_M_lines = options->use_M_lines;
@@ -705,6 +691,19 @@ double xmi_provider_dcs(int kind, int Z, double E, double theta) {
   const xrl_atom *a = xrl_atom_get(Z);
   if (!a) return -1.0;
   return kind == 0 ? xrl_dcs_rayl(a, E, theta) : xrl_dcs_compt(a, E, theta);
+}
+
+/* which: 0 fluorescence yield of shell `index` (0 = K .. 8 = M5), 1 radiative
+ * rate of line `index`, 2 energy of line `index` (keV) */
+double xmi_provider_line_data(int Z, int which, int index) {
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a) return 0.0;
+  switch (which) {
+    case 0: return index >= 0 && index < 9 ? a->fluor_yield[index] : 0.0;
+    case 1: return index >= 1 && index <= XRL_NLINES ? a->rad_rate[index] : 0.0;
+    case 2: return index >= 1 && index <= XRL_NLINES ? a->line_energy[index] : 0.0;
+    default: return 0.0;
+  }
 }
 
 int xmi_provider_shell_cross_sections(int Z, int cascade_type, double E, double *P9) {

@@ -116,8 +116,11 @@ typedef struct xmi_hdf5F {
 } xmi_hdf5F;
 
 /* table builders (xmi_db.c) */
-int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, xmi_hdf5_ZF *out);
-int xmi_tables_build_phi(xmi_hdf5F *h);
+int xmi_tables_build_Z(int Z, const int *uniqZ, int n_uniq, int cascade_type, int heavy, xmi_hdf5_ZF *out);
+int xmi_tables_build_phi(xmi_hdf5F *h, int heavy);
+/* the heavy loops on the GPU (tables_gpu.hip) */
+int xmi_tables_fill_gpu(xmi_hdf5_ZF *zs, int n, xmi_hdf5F *h);
+int xmi_tables_use_device(void);
 void xmi_tables_free(xmi_hdf5F *h);
 
 /* geometry helpers (xmi_input.c) */

@@ -376,14 +376,28 @@ static pthread_mutex_t g_atoms_lock = PTHREAD_MUTEX_INITIALIZER;
 const xrl_atom *xrl_atom_get(int Z) {
   if (Z < 1 || Z > 94) return NULL;
   pthread_mutex_lock(&g_atoms_lock);
-  if (!g_atoms[Z]) {
-    xrl_atom *a = (xrl_atom *)malloc(sizeof(xrl_atom));
-    if (a && xrl_atom_init(a, Z)) g_atoms[Z] = a;
-    else free(a);
-  }
   const xrl_atom *r = g_atoms[Z];
   pthread_mutex_unlock(&g_atoms_lock);
+  if (r) return r;
+  /* built outside the lock, so that different elements build in parallel;
+   * the first record published wins */
+  xrl_atom *a = (xrl_atom *)malloc(sizeof(xrl_atom));
+  if (!a || !xrl_atom_init(a, Z)) {
+    free(a);
+    return NULL;
+  }
+  pthread_mutex_lock(&g_atoms_lock);
+  if (!g_atoms[Z]) g_atoms[Z] = a;
+  else free(a);
+  r = g_atoms[Z];
+  pthread_mutex_unlock(&g_atoms_lock);
   return r;
+}
+
+/* builds the records of Zs[0..n-1] on all host threads */
+void xrl_atoms_preload(const int *Zs, int n) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < n; i++) (void)xrl_atom_get(Zs[i]);
 }
 
 /* corrected fluorescence yields: probability that a primary vacancy in shell s

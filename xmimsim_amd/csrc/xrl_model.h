@@ -304,6 +304,90 @@ XRL_FN double xrl_compton_profile(const xrl_atom *a, double pz) {
   return sum;
 }
 
+/* CDF of the hydrogenic profile shape: int_0^u (1 + t^2)^-3 dt */
+XRL_FN double xrl_profile_G3(double u) {
+  double d = 1.0 + u * u;
+  return u / (4.0 * d * d) + 3.0 * u / (8.0 * d) + 0.375 * atan(u);
+}
+
+/* unnormalised CDF of the total Compton profile on [0, pz] */
+XRL_FN double xrl_total_profile_cdf(const xrl_atom *a, double pz) {
+  double s = 0.0;
+  for (int k = 0; k < XRL_NSHELLS; k++)
+    if (a->occupancy[k] > 0.0) s += a->occupancy[k] * xrl_profile_G3(pz / a->profile_p[k]);
+  return s;
+}
+
+/* q in [lo, hi] with G3(q / p) = target, by 100 bisection steps */
+XRL_FN double xrl_bisect_G3(double target, double p, double lo, double hi) {
+  for (int it = 0; it < 100; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (xrl_profile_G3(mid / p) < target) lo = mid;
+    else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+/* the 100-step bisection of the total profile's inverse CDF at target */
+XRL_FN double xrl_bisect_total_profile(const xrl_atom *a, double target, double lo, double hi) {
+  for (int it = 0; it < 100; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (xrl_total_profile_cdf(a, mid) < target) lo = mid;
+    else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+/* azimuth inverse CDF of CDF(phi) = (phi - t sin 2 phi) / 2 pi at target * 2 pi,
+ * by 100 bisection steps on [0, 2 pi] */
+XRL_FN double xrl_bisect_phi(double t, double target) {
+  double lo = 0.0, hi = 2.0 * 3.14159265358979323846;
+  for (int it = 0; it < 100; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (mid - t * sin(2.0 * mid) < target) lo = mid;
+    else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+/* the angle inverse CDF of one (element, energy): cumulative trapezoid sum
+ * of DCS(theta) sin(theta) on n_int points of [0, pi], inverted linearly at
+ * rs[0..n_r-1]; computed in two passes (total, then the walk), so that no
+ * CDF array is stored; row[0] = 0 and row[n_r-1] = pi */
+XRL_FN void xrl_theta_icdf_row(const xrl_atom *a, int compton, double E, const double *rs, int n_r, int n_int,
+                               double *row, long row_stride) {
+  const double h = 3.14159265358979323846 / (n_int - 1);
+  double total = 0.0, prev = 0.0;
+  for (int k = 1; k < n_int; k++) {
+    double th = k * h;
+    double f = (compton ? xrl_dcs_compt(a, E, th) : xrl_dcs_rayl(a, E, th)) * sin(th);
+    total = total + 0.5 * (f + prev) * h;
+    prev = f;
+  }
+  /* walk: c0 = cdf[k - 1], c1 = cdf[k], advancing k while cdf[k] < target */
+  int k = 1;
+  double c0 = 0.0, f0 = 0.0;
+  double f1 = (compton ? xrl_dcs_compt(a, E, h) : xrl_dcs_rayl(a, E, h)) * sin(h);
+  double c1 = c0 + 0.5 * (f1 + f0) * h;
+  for (int m = 0; m < n_r; m++) {
+    double target = rs[m] * total;
+    while (k < n_int - 1 && c1 < target) {
+      k++;
+      c0 = c1;
+      f0 = f1;
+      double th = k * h;
+      f1 = (compton ? xrl_dcs_compt(a, E, th) : xrl_dcs_rayl(a, E, th)) * sin(th);
+      c1 = c0 + 0.5 * (f1 + f0) * h;
+    }
+    double fr = (c1 > c0) ? (target - c0) / (c1 - c0) : 0.0;
+    if (fr < 0.0) fr = 0.0;
+    if (fr > 1.0) fr = 1.0;
+    row[(long)m * row_stride] = ((k - 1) + fr) * h;
+  }
+  row[0] = 0.0;
+  row[(long)(n_r - 1) * row_stride] = 3.14159265358979323846;
+}
+
 /* vacancy-shell -> first/last line, lines are 1-based |xraylib line| */
 XRL_FN void xrl_shell_lines(int shell, int *first, int *last) {
   const int f[9] = {XRL_KL1, XRL_L1_FIRST, XRL_L2_FIRST, XRL_L3_FIRST, XRL_M1_FIRST,
@@ -412,6 +496,7 @@ XRL_FN void xrl_shell_cross_sections(const xrl_atom *a, int cascade_type, double
 int xrl_atom_init(xrl_atom *a, int Z);
 /* process-wide cache of all 94 elements; never freed */
 const xrl_atom *xrl_atom_get(int Z);
+void xrl_atoms_preload(const int *Zs, int n);
 /* xraylib line names, index 1..383 (0 = ""), 384 = "Rayleigh", 385 = "Compton" */
 const char *xrl_line_name(int line);
 /* donor/vacancy shells of a line (-1 for composite lines such as KO, L1N67) */
