@@ -10,7 +10,10 @@
  *     k_step        forced interaction (src/xmi_main.F90:1414-1518), atom and
  *                   interaction-type selection (:1558-1652); appends the
  *                   photon to one of three queues (Rayleigh, Compton, photo)
- *     k_varred      detection-weight estimator of every photon at order n
+ *                   and counts each wave's events per layer
+ *     k_layer_scan, k_layer_base   offsets of the per-layer event lists
+ *     k_vr_geom, k_vr_cone, k_vr_scatter, k_vr_lines
+ *                   detection-weight estimator of every photon at order n
  *                   (src/xmi_variance_reduction.F90:29-726, 949-1101)
  *     k_rayleigh    src/xmi_main.F90:1986-2101
  *     k_compton     src/xmi_main.F90:2103-2229, 4985-5069
