@@ -554,6 +554,10 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     }
   }
   P.fl_off[NL] = (int)fl_info.size();
+  /* one zero entry past the end: the walk reads entry t + 1 ahead of use */
+  fl_info.push_back(make_int4(0, 0, 0, 0));
+  fl_wf.push_back(0.0);
+  for (int l2 = 0; l2 < fl_stride; l2++) fl_mr.push_back(0.0);
 
   /* sources: continuous intervals, then discrete lines */
   std::vector<SourceDesc> src;

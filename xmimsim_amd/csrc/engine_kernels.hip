@@ -680,9 +680,9 @@ __global__ void k_layer_base(EngineParams P, int nseg) {
  *   k_vr_scatter  one lane per photon: Rayleigh and Compton terms of every
  *                 element of the photon's layer; the Compton Doppler draws
  *                 continue the photon's estimator stream in element order
- *   k_vr_lines    one lane per (element, fluorescence line) entry of a layer;
- *                 the wavefront walks its photons one at a time, reading each
- *                 photon's scalars from the lane that loaded it (readlane)
+ *   k_vr_lines    a few photons per lane; the wavefront walks the layer's
+ *                 (element, fluorescence line) entries with the line data in
+ *                 scalar registers and sums each line's terms over its photons
  *
  * Path lengths are kept in crossing order (k = 0: the photon's own layer) so
  * that register arrays have compile-time indices only (NL bounds the layer
@@ -943,12 +943,12 @@ struct LayerChunks {
   long long c_begin, c_end;
 };
 
-__device__ __forceinline__ bool d_chunk_next(const EngineParams &P, LayerChunks &w, long long c) {
+__device__ __forceinline__ bool d_chunk_next(const EngineParams &P, LayerChunks &w, long long c, int csz = 64) {
   while (c >= w.c_end && w.L < P.n_layers) {
     w.L++;
     w.tot = P.lay_total[w.L - 1];
     w.c_begin = w.c_end;
-    w.c_end += (w.tot + 63) / 64;
+    w.c_end += (w.tot + csz - 1) / csz;
   }
   return c < w.c_end;
 }
@@ -1120,8 +1120,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-/* lines phase: one lane per photon of a 64-photon chunk (all in layer L);
- * the wavefront walks the layer's (element, line) entries.  Per (line t,
+/* lines phase: KP photons per lane, a chunk of 64 KP photons (all in layer
+ * L); the wavefront walks the layer's (element, line) entries.  Per (line t,
  * photon) term
  *   wf_t * P_shell(t, photon) * g * exp(-sum_l mu_rho(t, l) * d_l)
  * with g = Omega / 4 pi * weight / mu(E) and d_l the photon's path in layer l
@@ -1130,9 +1130,10 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
  * interaction (or a photon not coming from fluorescence) uses
  * E^-p * shell_C[cascade][edges above E][shell], a fluorescence photon the
  * precalculated cascade table (src/xmi_variance_reduction.F90:540-580).  The
- * terms of a line are summed over the chunk's lanes in a fixed butterfly
- * order (deterministic) and added once to the line's fixed-point bin. */
-template <int NL, bool use_lds>
+ * terms of a line are summed over each lane's photons, then over the lanes
+ * in a fixed DPP order (deterministic), and added once to the line's
+ * fixed-point bin: one wavefront sum per line and 64 KP photons. */
+template <int NL, int KP, bool use_lds>
 __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): compact line bins [n_lbins][2] */
   extern __shared__ unsigned long long lds[];
@@ -1154,78 +1155,119 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
   const long long wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
                   waves = (gridDim.x * blockDim.x) >> 6;
   LayerChunks lc = {0, 0, 0, 0};
-  for (long long c = wave; d_chunk_next(P, lc, c); c += waves) {
+  for (long long c = wave; d_chunk_next(P, lc, c, 64 * KP); c += waves) {
     const int L = lc.L;
-    const int e = (int)((c - lc.c_begin) * 64) + __lane_id();
-    const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
-    const int info = e < lc.tot ? V.info[i] : 0;
-    const int active = info & VR_ACTIVE;
-    if (__ballot(active) == 0ull) continue;
-    double E = 2.0, g = 0.0, pe = 0.0;
-    int fo = 1, prev_line = 0, prev_u = 0;
+    /* the lane's KP photons: list entries e0, e0 + 64, ... of layer L */
+    const int e0 = (int)((c - lc.c_begin) * (64 * KP)) + __lane_id();
+    double E[KP], pe[KP], g[KP], x[KP];
     /* path length per layer, by layer index (0 where not crossed) */
-    double dl[NL];
+    double dl[KP][NL];
+    int act[KP], fo[KP], pl[KP], pu[KP], kE[KP];
+    int any_act = 0;
 #pragma unroll
-    for (int l = 0; l < NL; l++) dl[l] = 0.0;
-    if (active) {
-      E = S.energy[i];
-      pe = xrl_energy(E).pe;
-      g = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(L - 1) * cap + i];
-      prev_line = S.prev_line[i];
-      prev_u = S.prev_u[i];
-      fo = !(n > 1 && prev_line > 0);
-      const int nc = info >> VR_NC_SHIFT, dir = (info & VR_UP) ? 1 : -1;
+    for (int k = 0; k < KP; k++) {
+      const int e = e0 + 64 * k;
+      const int i = e < lc.tot ? P.lay_list[(size_t)(L - 1) * cap + e] : 0;
+      const int info = e < lc.tot ? V.info[i] : 0;
+      act[k] = info & VR_ACTIVE;
+      any_act |= act[k];
+      E[k] = 2.0;
+      pe[k] = g[k] = x[k] = 0.0;
+      fo[k] = 1;
+      pl[k] = pu[k] = kE[k] = 0;
 #pragma unroll
-      for (int k = 0; k < NL; k++) {
-        const double d = k < nc ? V.dist[(size_t)k * cap + i] : 0.0;
-        const int l0 = L - 1 + k * dir;
+      for (int l = 0; l < NL; l++) dl[k][l] = 0.0;
+      if (act[k]) {
+        E[k] = S.energy[i];
+        pe[k] = xrl_energy(E[k]).pe;
+        g[k] = V.omega[i] / 4.0 / ENG_PI * S.weight[i] / S.mus[(size_t)(L - 1) * cap + i];
+        pl[k] = S.prev_line[i];
+        pu[k] = S.prev_u[i];
+        fo[k] = !(n > 1 && pl[k] > 0);
+        const int nc = info >> VR_NC_SHIFT, dir = (info & VR_UP) ? 1 : -1;
 #pragma unroll
-        for (int l = 0; l < NL; l++)
-          if (k < nc && l == l0) dl[l] = d;
+        for (int q = 0; q < NL; q++) {
+          const double d = q < nc ? V.dist[(size_t)q * cap + i] : 0.0;
+          const int l0 = L - 1 + q * dir;
+#pragma unroll
+          for (int l = 0; l < NL; l++)
+            if (q < nc && l == l0) dl[k][l] = d;
+        }
       }
     }
+    if (__ballot(any_act) == 0ull) continue;
     const int t0 = __builtin_amdgcn_readfirstlane(P.fl_off[L - 1]), t1 = __builtin_amdgcn_readfirstlane(P.fl_off[L]);
-    int u_cur = -1, z_cur = -1, kE = 0, any = 0;
-    double Ps = 0.0;
+    int u_cur = -1, z_cur = -1, any = 0;
     /* lane j holds the chunk's total of line t0 + 64 g + j of the current
      * group g of 64 lines; the group is added to the tallies in one pass */
     double lsum = 0.0;
     const auto *fl_info = d_const((const int *)P.fl_info);
     const auto *fl_wf = d_const(P.fl_wf);
     const auto *fl_mr = d_const(P.fl_mr);
+    /* the next entry's scalars are loaded one iteration ahead (the tables
+     * end with a zero entry), so that their latency overlaps this entry's
+     * arithmetic */
+    int nx_u = fl_info[4 * t0], nx_z = fl_info[4 * t0 + 2];
+    double nx_wf = fl_wf[t0], nx_mr[NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) nx_mr[l] = fl_mr[(size_t)t0 * NL + l];
     for (int t = t0; t < t1; t++) {
       /* {element, active line, shell, compact bin} */
-      const int4 it = make_int4(fl_info[4 * t], fl_info[4 * t + 1], fl_info[4 * t + 2], fl_info[4 * t + 3]);
-      if (it.x != u_cur) {
-        /* edges of the element above the photon's energy */
-        const auto *a = d_const(&P.atoms[it.x]);
-        kE = 0;
+      const int2 it = make_int2(nx_u, nx_z);
+      const double wf = nx_wf;
+      double mr[NL];
 #pragma unroll
-        for (int q = 0; q < 9; q++) kE += a->edge[q] > E ? 1 : 0;
+      for (int l = 0; l < NL; l++) mr[l] = nx_mr[l];
+      nx_u = fl_info[4 * (t + 1)];
+      nx_z = fl_info[4 * (t + 1) + 2];
+      nx_wf = fl_wf[t + 1];
+#pragma unroll
+      for (int l = 0; l < NL; l++) nx_mr[l] = fl_mr[(size_t)(t + 1) * NL + l];
+      if (it.x != u_cur) {
+        /* edges of the element above each photon's energy */
+        const auto *a = d_const(&P.atoms[it.x]);
+#pragma unroll
+        for (int k = 0; k < KP; k++) {
+          kE[k] = 0;
+#pragma unroll
+          for (int q = 0; q < 9; q++) kE[k] += a->edge[q] > E[k] ? 1 : 0;
+        }
         u_cur = it.x;
         z_cur = -1;
       }
-      if (it.z != z_cur) {
-        z_cur = it.z;
-        double v;
-        if (fo) {
-          v = (!P.use_M_lines && z_cur >= 4) ? 0.0 : pe * P.atoms[u_cur].shell_C[P.cascade - 1][kE][z_cur];
-        } else {
-          v = z_cur <= smax ? P.precalc_xrf[(((size_t)u_cur * 9 + z_cur) * P.nZ + prev_u) * XMI_NLINES_VR_DEV +
-                                            (prev_line - 1)]
-                            : 0.0;
+      if (it.y != z_cur) {
+        z_cur = it.y;
+        int nz = 0;
+#pragma unroll
+        for (int k = 0; k < KP; k++) {
+          double v;
+          if (fo[k]) {
+            v = (!P.use_M_lines && z_cur >= 4) ? 0.0 : pe[k] * P.atoms[u_cur].shell_C[P.cascade - 1][kE[k]][z_cur];
+          } else {
+            v = z_cur <= smax ? P.precalc_xrf[(((size_t)u_cur * 9 + z_cur) * P.nZ + pu[k]) * XMI_NLINES_VR_DEV +
+                                              (pl[k] - 1)]
+                              : 0.0;
+          }
+          /* shell factor times the photon's weight factor; 0 for lanes
+           * without a photon, so that their terms vanish */
+          x[k] = act[k] ? v * g[k] : 0.0;
+          nz |= x[k] != 0.0;
         }
-        Ps = active ? v : 0.0;
-        any = __ballot(Ps != 0.0) != 0ull;
+        any = __ballot(nz) != 0ull;
       }
       const int j = (t - t0) & 63;
       if (any) {
-        /* attenuation rows are padded to NL entries (zeros past the stack) */
-        double m2 = 0.0;
+        /* attenuation rows are padded to NL entries (zeros past the stack);
+         * the lane's photons are summed before the one wavefront sum */
+        double acc = 0.0;
 #pragma unroll
-        for (int l = 0; l < NL; l++) m2 += fl_mr[(size_t)t * NL + l] * dl[l];
-        double term = Ps != 0.0 ? fl_wf[t] * Ps * g * d_exp_neg(-m2, exp_tab) : 0.0;
-        term = d_wave_total(term);
+        for (int k = 0; k < KP; k++) {
+          double m2 = 0.0;
+#pragma unroll
+          for (int l = 0; l < NL; l++) m2 += mr[l] * dl[k][l];
+          acc = fma(x[k], d_exp_neg(-m2, exp_tab), acc);
+        }
+        const double term = d_wave_total(acc * wf);
         if (__lane_id() == j) lsum = term;
       }
       if (j == 63 || t == t1 - 1) {
@@ -1640,16 +1682,19 @@ static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks)
   return vb < 1 ? 1 : vb;
 }
 
+/* photons per lane of k_vr_lines: each line's wavefront sum then serves
+ * 64 KP photons; fewer where the per-photon path lengths take more registers */
 template <int NL>
 static void launch_lines(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
                          hipStream_t st) {
+  constexpr int KP = NL <= 4 ? 4 : (NL <= 8 ? 2 : 1);
   const size_t ll = lines_lds_bytes(P);
   if (ll) {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, true>, ll, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
+    int vb = resident_blocks((const void *)k_vr_lines<NL, KP, true>, ll, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, KP, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
   } else {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, false>, 0, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+    int vb = resident_blocks((const void *)k_vr_lines<NL, KP, false>, 0, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_lines<NL, KP, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   }
 }
 
