@@ -401,6 +401,99 @@ int xmi_surrogate_predict(xmi_surrogate *s, const float *X, int batch, float *Y)
 int xmi_surrogate_get_params(const xmi_surrogate *s, float *out);
 int xmi_surrogate_set_params(xmi_surrogate *s, const float *in);
 
+/* ---- XMSI / XMSO files: include/xmi_xml.h, include/xmi_data_structs.h ----
+ * The input file (.xmsi) and the output file (.xmso) of xml/xmimsim-1.0.dtd.
+ * Errors: the functions return NULL / 0 and, when error is not NULL and
+ * *error is NULL, set *error to a malloc'ed xmi_error (release with
+ * xmi_error_free).  Its leading members have the layout of GLib's GError
+ * (domain, code, message), so that callers written against the reference's
+ * GError ** can read error->message unchanged. */
+
+typedef struct _xmi_error {
+  uint32_t domain;
+  int code;
+  char *message;
+} xmi_error;
+
+void xmi_error_free(xmi_error *error);
+
+/* include/xmi_data_structs.h:335-366 */
+typedef struct _xmi_counts {
+  double counts;
+  int interaction_number;
+} xmi_counts;
+
+typedef struct _xmi_fluorescence_line {
+  char *line_type;
+  double energy;
+  double total_counts;
+  int n_interactions;
+  xmi_counts *interactions;
+} xmi_fluorescence_line;
+
+typedef struct _xmi_fluorescence_line_counts {
+  int atomic_number;
+  double total_counts;
+  int n_lines;
+  xmi_fluorescence_line *lines;
+} xmi_fluorescence_line_counts;
+
+/* include/xmi_data_structs.h:386-404 */
+typedef struct _xmi_output {
+  float version;
+  char *inputfile;
+  char *outputfile;
+  xmi_input *input;
+  xmi_fluorescence_line_counts *brute_force_history;
+  xmi_fluorescence_line_counts *var_red_history;
+  int nbrute_force_history;
+  int nvar_red_history;
+  double **channels_conv;   /* [ninteractions + 1][nchannels] */
+  double **channels_unconv; /* [ninteractions + 1][nchannels] */
+  int ninteractions;
+  int use_zero_interactions;
+} xmi_output;
+
+/* include/xmi_data_structs.h:508-515 */
+typedef enum {
+  XMI_INPUT_GENERAL = 1,
+  XMI_INPUT_COMPOSITION = 2,
+  XMI_INPUT_GEOMETRY = 4,
+  XMI_INPUT_EXCITATION = 8,
+  XMI_INPUT_ABSORBERS = 16,
+  XMI_INPUT_DETECTOR = 32
+} XmiInputFlags;
+
+/* src/xmi_xml.c:1286 and :1814.  Weights are normalised and sorted by Z,
+ * zero weights dropped, discrete energies given twice merged; the input must
+ * pass xmi_input_validate.  Release with xmi_input_free. */
+xmi_input *xmi_input_read_from_xml_file(const char *xmsifile, xmi_error **error);
+xmi_input *xmi_input_read_from_xml_string(const char *xmsistring, xmi_error **error);
+/* src/xmi_xml.c:1404 and :1363 (the string is malloc'ed; release with xmi_free) */
+int xmi_input_write_to_xml_file(xmi_input *input, const char *xmsifile, xmi_error **error);
+int xmi_input_write_to_xml_string(xmi_input *input, char **xmlstring, xmi_error **error);
+/* src/xmi_data_structs.c: 0 when valid, else the OR of the failing sections */
+int xmi_input_validate(xmi_input *input);
+/* frees an xmi_input built by this library's readers (every member malloc'ed) */
+void xmi_input_free(xmi_input *input);
+
+/* src/xmi_data_structs.c:1371-1519: the output record of one simulation.
+ * brute_history and var_red_history are double[100][385][n_int] (the
+ * layout xmi_main_msim returns; var_red_history may be NULL),
+ * channels_conv[n_int + 1][nchannels] (xmi_detector_convolute_all) and
+ * channels_unconv double[n_int + 1][nchannels] (xmi_main_msim).  Line
+ * energies are the physics provider's.  The input is copied.  Release with
+ * xmi_output_free. */
+xmi_output *xmi_output_raw2struct(xmi_input *input, double *brute_history, double *var_red_history,
+                                  double **channels_conv, double *channels_unconv, char *inputfile,
+                                  int use_zero_interactions);
+/* src/xmi_xml.c:1638 (with the preview graphs) and :2033 */
+int xmi_output_write_to_xml_file(xmi_output *output, const char *xmsofile, xmi_error **error);
+xmi_output *xmi_output_read_from_xml_file(const char *xmsofile, xmi_error **error);
+void xmi_output_free(xmi_output *output);
+/* element symbol of Z (1..103), NULL outside; xraylib's AtomicNumberToSymbol */
+const char *xmi_element_symbol(int Z);
+
 #ifdef __cplusplus
 }
 #endif

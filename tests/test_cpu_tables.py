@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol(lib):
     header = open(os.path.join(ROOT, "include", "xmimsim_amd.h")).read()
     header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)                 # comments
-    header = re.sub(r"typedef\s+(struct|enum)[^{]*\{.*?\}[^;]*;", "", header, flags=re.S)  # types
+    header = re.sub(r"typedef\s+(struct|enum)[^{;]*\{.*?\}[^;]*;", "", header, flags=re.S)  # types
     declared = set(re.findall(r"\b(xmi_\w+)\s*\(", header))
     declared |= set(re.findall(r"^\s*extern\s+[\w\s\*]+?\b(xmi_\w+)\s*;", header, re.M))
     declared -= {"xmi_inputFPtr", "xmi_hdf5FPtr"}

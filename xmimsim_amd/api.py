@@ -104,6 +104,35 @@ class xmi_msim_stats(C.Structure):
                 ("vr_scatter_ms", C.c_double), ("vr_lines_ms", C.c_double)]
 
 
+class xmi_error(C.Structure):
+    _fields_ = [("domain", C.c_uint32), ("code", C.c_int), ("message", C.c_char_p)]
+
+
+class xmi_counts(C.Structure):
+    _fields_ = [("counts", C.c_double), ("interaction_number", C.c_int)]
+
+
+class xmi_fluorescence_line(C.Structure):
+    _fields_ = [("line_type", C.c_char_p), ("energy", C.c_double), ("total_counts", C.c_double),
+                ("n_interactions", C.c_int), ("interactions", C.POINTER(xmi_counts))]
+
+
+class xmi_fluorescence_line_counts(C.Structure):
+    _fields_ = [("atomic_number", C.c_int), ("total_counts", C.c_double), ("n_lines", C.c_int),
+                ("lines", C.POINTER(xmi_fluorescence_line))]
+
+
+class xmi_output(C.Structure):
+    _fields_ = [("version", C.c_float), ("inputfile", C.c_char_p), ("outputfile", C.c_char_p),
+                ("input", C.POINTER(xmi_input)),
+                ("brute_force_history", C.POINTER(xmi_fluorescence_line_counts)),
+                ("var_red_history", C.POINTER(xmi_fluorescence_line_counts)),
+                ("nbrute_force_history", C.c_int), ("nvar_red_history", C.c_int),
+                ("channels_conv", C.POINTER(C.POINTER(C.c_double))),
+                ("channels_unconv", C.POINTER(C.POINTER(C.c_double))), ("ninteractions", C.c_int),
+                ("use_zero_interactions", C.c_int)]
+
+
 # exported symbols of include/xmimsim_amd.h
 EXPORTED_SYMBOLS = [
     "xmi_main_options_new", "xmi_input_C2F", "xmi_init_input", "xmi_free_input_F", "xmi_db",
@@ -113,7 +142,10 @@ EXPORTED_SYMBOLS = [
     "xmi_msim_get_layer_events", "xmi_provider_active_lines", "xmi_detector_convolute_all",
     "xmi_detector_convolute_all_custom", "xmi_surrogate_create", "xmi_surrogate_train_step", "xmi_main_msim_batch",
     "xmi_tables_set_device", "xmi_tables_copy", "xmi_escape_ratios_calculation", "xmi_free_escape_ratios",
-    "xmi_get_default_escape_ratios_options",
+    "xmi_get_default_escape_ratios_options", "xmi_input_read_from_xml_file", "xmi_input_read_from_xml_string",
+    "xmi_input_write_to_xml_file", "xmi_input_write_to_xml_string", "xmi_input_validate", "xmi_input_free",
+    "xmi_output_raw2struct", "xmi_output_write_to_xml_file", "xmi_output_read_from_xml_file", "xmi_output_free",
+    "xmi_error_free", "xmi_element_symbol",
 ]
 
 _lib = None
@@ -176,6 +208,34 @@ def load_library(path: str = LIB_PATH):
     lib.xmi_provider_line_data.restype = C.c_double
     lib.xmi_provider_shell_cross_sections.argtypes = [C.c_int, C.c_int, C.c_double, P(C.c_double)]
     lib.xmi_provider_shell_cross_sections.restype = C.c_int
+    PE = P(P(xmi_error))
+    lib.xmi_input_read_from_xml_file.argtypes = [C.c_char_p, PE]
+    lib.xmi_input_read_from_xml_file.restype = P(xmi_input)
+    lib.xmi_input_read_from_xml_string.argtypes = [C.c_char_p, PE]
+    lib.xmi_input_read_from_xml_string.restype = P(xmi_input)
+    lib.xmi_input_write_to_xml_file.argtypes = [P(xmi_input), C.c_char_p, PE]
+    lib.xmi_input_write_to_xml_file.restype = C.c_int
+    lib.xmi_input_write_to_xml_string.argtypes = [P(xmi_input), P(C.c_void_p), PE]
+    lib.xmi_input_write_to_xml_string.restype = C.c_int
+    lib.xmi_input_validate.argtypes = [P(xmi_input)]
+    lib.xmi_input_validate.restype = C.c_int
+    lib.xmi_input_free.argtypes = [P(xmi_input)]
+    lib.xmi_input_free.restype = None
+    lib.xmi_output_raw2struct.argtypes = [P(xmi_input), P(C.c_double), P(C.c_double), P(P(C.c_double)),
+                                          P(C.c_double), C.c_char_p, C.c_int]
+    lib.xmi_output_raw2struct.restype = P(xmi_output)
+    lib.xmi_output_write_to_xml_file.argtypes = [P(xmi_output), C.c_char_p, PE]
+    lib.xmi_output_write_to_xml_file.restype = C.c_int
+    lib.xmi_output_read_from_xml_file.argtypes = [C.c_char_p, PE]
+    lib.xmi_output_read_from_xml_file.restype = P(xmi_output)
+    lib.xmi_output_free.argtypes = [P(xmi_output)]
+    lib.xmi_output_free.restype = None
+    lib.xmi_error_free.argtypes = [P(xmi_error)]
+    lib.xmi_error_free.restype = None
+    lib.xmi_line_name.argtypes = [C.c_int]
+    lib.xmi_line_name.restype = C.c_char_p
+    lib.xmi_element_symbol.argtypes = [C.c_int]
+    lib.xmi_element_symbol.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -234,6 +294,8 @@ class Input:
     (src/xmi_data_structs.c:551-654)."""
     layers: List[Layer]
     reference_layer: int = 1
+    outputfile: str = ""
+    comments: str = ""
     discrete: List[DiscreteLine] = field(default_factory=list)
     continuous: List[ContinuousPoint] = field(default_factory=list)
     n_photons_interval: int = 10000
@@ -283,8 +345,8 @@ class CInput:
             self._keep.append(arr)
             return len(ls), arr
 
-        self.general = xmi_general(8.1, b"", inp.n_photons_interval, inp.n_photons_line,
-                                   inp.n_interactions_trajectory, b"")
+        self.general = xmi_general(8.1, inp.outputfile.encode(), inp.n_photons_interval, inp.n_photons_line,
+                                   inp.n_interactions_trajectory, inp.comments.encode())
         nl, la = layers(inp.layers)
         self.composition = xmi_composition(nl, la, inp.reference_layer)
         g = xmi_geometry()
