@@ -216,10 +216,26 @@ void xmi_free_hdf5_F(xmi_hdf5FPtr *hdf5FPtr);
  * xmi_hits_per_single trials per cell).  Drop-in for the OpenCL plugin entry
  * xmi_solid_angle_calculation_cl (src/xmi_solid_angle_cl.c:120-439), which
  * itself replaces xmi_solid_angle_calculation_f (src/xmi_solid_angle_f.F90:303).
- * *solid_angle is malloc'ed; release with xmi_free_solid_angle. */
+ * *solid_angle is malloc'ed; release with xmi_free_solid_angle.  The grid
+ * keeps input_string (the XMSI text of the input, malloc'ed by the caller, as
+ * the reference's wrapper hands the plugin a copy) as its xmi_input_string
+ * and frees it with the grid; NULL is allowed. */
 int xmi_solid_angle_calculation_cl(xmi_inputFPtr inputFPtr, xmi_solid_angle **solid_angle,
                                    char *input_string, xmi_main_options *xmo);
 void xmi_free_solid_angle(xmi_solid_angle *solid_angle);
+
+/* HDF5 cache of solid-angle grids (include/xmi_solid_angle.h:41-50,
+ * src/xmi_solid_angle.c:193-821), in the reference's file layout.  A stored
+ * grid needs its xmi_input_string.  xmi_find_solid_angle_match sets *rv to
+ * the newest stored grid whose input matches A (or NULL) and returns 1; a
+ * file without the kind/version attributes, or older than 7.2, is recreated
+ * empty.  xmi_get_solid_angle_file: *filePtr NULL selects
+ * $XDG_DATA_HOME (or ~/.local/share)/XMI-MSIM/xmimsim-solid-angles.h5. */
+int xmi_create_empty_solid_angle_hdf5_file(char *hdf5_file);
+int xmi_update_solid_angle_hdf5_file(char *hdf5_file, xmi_solid_angle *solid_angle);
+int xmi_check_solid_angle_match(xmi_input *input_in, xmi_input *input_h5);
+int xmi_find_solid_angle_match(char *hdf5_file, xmi_input *A, xmi_solid_angle **rv, xmi_main_options *options);
+int xmi_get_solid_angle_file(char **filePtr, int create_file);
 
 /* Trials per solid-angle cell; the reference's global hits_per_single
  * (src/xmi_solid_angle_f.F90:43-44), default 5000. */
@@ -358,6 +374,16 @@ void xmi_escape_ratios_calculation(xmi_input *input, xmi_escape_ratios **escape_
 
 /* include/xmi_detector.h:67 */
 void xmi_free_escape_ratios(xmi_escape_ratios *escape_ratios);
+
+/* HDF5 cache of escape ratios (include/xmi_detector.h:59-73,
+ * src/xmi_detector.c:40-618), in the reference's file layout; same
+ * conventions as the solid-angle cache (xmimsim-escape-ratios.h5).  A stored
+ * result matches an input whose detector crystal has the same layers. */
+int xmi_create_empty_escape_ratios_hdf5_file(char *hdf5_file);
+int xmi_update_escape_ratios_hdf5_file(char *hdf5_file, xmi_escape_ratios *escape_ratios);
+int xmi_check_escape_ratios_match(xmi_input *input_in, xmi_input *input_h5);
+int xmi_find_escape_ratios_match(char *hdf5_file, xmi_input *A, xmi_escape_ratios **rv, xmi_main_options *options);
+int xmi_get_escape_ratios_file(char **filePtr, int create_file);
 
 /* Detector response of every interaction order: absorption in the window and
  * crystal, escape peaks (when options->use_escape_peaks and escape_ratios is

@@ -145,7 +145,10 @@ EXPORTED_SYMBOLS = [
     "xmi_get_default_escape_ratios_options", "xmi_input_read_from_xml_file", "xmi_input_read_from_xml_string",
     "xmi_input_write_to_xml_file", "xmi_input_write_to_xml_string", "xmi_input_validate", "xmi_input_free",
     "xmi_output_raw2struct", "xmi_output_write_to_xml_file", "xmi_output_read_from_xml_file", "xmi_output_free",
-    "xmi_error_free", "xmi_element_symbol",
+    "xmi_error_free", "xmi_element_symbol", "xmi_create_empty_solid_angle_hdf5_file",
+    "xmi_update_solid_angle_hdf5_file", "xmi_check_solid_angle_match", "xmi_find_solid_angle_match",
+    "xmi_get_solid_angle_file", "xmi_create_empty_escape_ratios_hdf5_file", "xmi_update_escape_ratios_hdf5_file",
+    "xmi_check_escape_ratios_match", "xmi_find_escape_ratios_match", "xmi_get_escape_ratios_file",
 ]
 
 _lib = None

@@ -1264,6 +1264,7 @@ extern "C" void xmi_free_solid_angle(xmi_solid_angle *sa) {
   free(sa->solid_angles);
   free(sa->grid_dims_r_vals);
   free(sa->grid_dims_theta_vals);
+  free(sa->xmi_input_string);
   free(sa);
 }
 

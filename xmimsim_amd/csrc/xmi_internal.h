@@ -17,6 +17,7 @@ extern "C" {
 #define XMI_MAXZ 94
 #define XMI_NLINES_VR 219          /* KL1..M5P5 */
 #define XMI_HIST_ROWS 385          /* 383 lines + Rayleigh + Compton */
+#define XMI_ESC_LINES 109          /* escape-ratio lines KL1..L3P3 */
 #define XMI_ENERGY_THRESHOLD 1.0   /* keV, src/xmi_aux_f.F90:672 */
 #define XMI_ENERGY_MAX 200.0       /* keV, src/xmi_aux_f.F90:673 */
 
