@@ -337,6 +337,9 @@ int xrl_atom_init(xrl_atom *a, int Z) {
   /* form factor / incoherent scattering function scales */
   a->ff_x0 = 0.485 * cbrt((double)Z);
   a->sf_x0 = 0.35 * cbrt((double)Z);
+  a->inv_ff_x0 = 1.0 / a->ff_x0;
+  a->inv_sf_x0 = 1.0 / a->sf_x0;
+  a->barn_per_A = XRL_AVOG_BARN / a->A;
 
   /* Compton profile momenta from binding energies (hydrogenic: E = p^2/2) */
   for (int s = 0; s < XRL_NSHELLS; s++) {

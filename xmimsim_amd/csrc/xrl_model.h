@@ -128,6 +128,8 @@ typedef struct xrl_atom {
   double cs_rayl[XRL_NCS];               /* cm2/g on the ln(E) grid, linear interpolation */
   double cs_compt[XRL_NCS];
   double cs_rc[XRL_NCS];                 /* cs_rayl + cs_compt */
+  double inv_ff_x0, inv_sf_x0;           /* 1 / ff_x0, 1 / sf_x0 */
+  double barn_per_A;                     /* N_A * 1e-24 / A: barn/atom to cm2/g */
 } xrl_atom;
 
 /* per-energy quantities shared by every element evaluated at that energy */
@@ -226,13 +228,13 @@ XRL_FN double xrl_moment_transf(double E, double theta) {
 }
 
 XRL_FN double xrl_ff_rayl(const xrl_atom *a, double x) {
-  double u = x / a->ff_x0;
+  double u = x * a->inv_ff_x0;
   double d = 1.0 + 3.0 * u * u;
   return (double)a->Z / (d * d);
 }
 
 XRL_FN double xrl_sf_compt(const xrl_atom *a, double x) {
-  double u = x / a->sf_x0;
+  double u = x * a->inv_sf_x0;
   double d = 1.0 + u * u;
   return (double)a->Z * (1.0 - 1.0 / (d * d));
 }
@@ -241,14 +243,14 @@ XRL_FN double xrl_sf_compt(const xrl_atom *a, double x) {
 XRL_FN double xrl_dcs_rayl(const xrl_atom *a, double E, double theta) {
   double c = cos(theta);
   double F = xrl_ff_rayl(a, xrl_moment_transf(E, theta));
-  return XRL_RE2 * 0.5 * (1.0 + c * c) * F * F * XRL_AVOG_BARN / a->A;
+  return XRL_RE2 * 0.5 * (1.0 + c * c) * F * F * a->barn_per_A;
 }
 
 XRL_FN double xrl_dcs_compt(const xrl_atom *a, double E, double theta) {
   double c = cos(theta);
   double k = 1.0 / (1.0 + E / XRL_MEC2 * (1.0 - c));
   double kn = XRL_RE2 * 0.5 * k * k * (k + 1.0 / k - (1.0 - c * c));
-  return kn * xrl_sf_compt(a, xrl_moment_transf(E, theta)) * XRL_AVOG_BARN / a->A;
+  return kn * xrl_sf_compt(a, xrl_moment_transf(E, theta)) * a->barn_per_A;
 }
 
 /* linearly polarised differential cross sections (DCSP_*), cm2/g/sr.  The
@@ -272,11 +274,11 @@ XRL_FN xrl_dcsp_ctx xrl_dcsp_setup(double E, double theta, double phi) {
 
 XRL_FN double xrl_dcsp_rayl_c(const xrl_atom *a, const xrl_dcsp_ctx *d) {
   double F = xrl_ff_rayl(a, d->x);
-  return d->thomson * F * F * XRL_AVOG_BARN / a->A;
+  return d->thomson * F * F * a->barn_per_A;
 }
 
 XRL_FN double xrl_dcsp_compt_c(const xrl_atom *a, const xrl_dcsp_ctx *d) {
-  return d->kn * xrl_sf_compt(a, d->x) * XRL_AVOG_BARN / a->A;
+  return d->kn * xrl_sf_compt(a, d->x) * a->barn_per_A;
 }
 
 XRL_FN double xrl_dcsp_rayl(const xrl_atom *a, double E, double theta, double phi) {

@@ -30,7 +30,8 @@ class XrlAtom(C.Structure):
                 ("auger_KM", C.c_double * 5), ("auger_LM", C.c_double * 15), ("ff_x0", C.c_double),
                 ("sf_x0", C.c_double), ("profile_p", C.c_double * NSHELLS), ("photo_Bsum", C.c_double * 10),
                 ("shell_C", C.c_double * 360), ("cs_rayl", C.c_double * NCS), ("cs_compt", C.c_double * NCS),
-                ("cs_rc", C.c_double * NCS)]
+                ("cs_rc", C.c_double * NCS), ("inv_ff_x0", C.c_double), ("inv_sf_x0", C.c_double),
+                ("barn_per_A", C.c_double)]
 
 
 class HDF5Z(C.Structure):
