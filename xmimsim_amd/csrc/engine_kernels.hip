@@ -15,9 +15,9 @@
  *     k_vr_geom, k_vr_cone, k_vr_scatter, k_vr_lines
  *                   detection-weight estimator of every photon at order n
  *                   (src/xmi_variance_reduction.F90:29-726, 949-1101)
- *     k_rayleigh    src/xmi_main.F90:1986-2101
- *     k_compton     src/xmi_main.F90:2103-2229, 4985-5069
- *     k_photo       src/xmi_main.F90:2231-2411, 5184-5437
+ *     k_interact    one launch over the three interaction queues:
+ *                   Rayleigh src/xmi_main.F90:1986-2101, Compton :2103-2229
+ *                   and 4985-5069, photo-absorption :2231-2411, 5184-5437
  *
  * Random numbers: Philox streams per history (philox.h); the transport stream
  * of order n is consumed as: step, atom, type, then the interaction kernel
@@ -1293,9 +1293,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
 /* ------------------------------------------------------------------------ */
 /* interactions */
 
-__global__ void __launch_bounds__(ENG_BLOCK) k_rayleigh(EngineParams P, PhotonSoA S, int n) {
-  const int i = d_dequeue(P, 0, blockIdx.x * blockDim.x + threadIdx.x);
-  if (i < 0) return;
+__device__ __forceinline__ void d_rayleigh_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -1330,9 +1328,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_rayleigh(EngineParams P, PhotonSo
   (void)cap;
 }
 
-__global__ void __launch_bounds__(ENG_BLOCK) k_compton(EngineParams P, PhotonSoA S, int n) {
-  const int i = d_dequeue(P, 1, blockIdx.x * blockDim.x + threadIdx.x);
-  if (i < 0) return;
+__device__ __forceinline__ void d_compton_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -1483,9 +1479,7 @@ __device__ int d_coster_kronig(const xrl_atom *a, phx_stream *s, int shell) {
   return shell;
 }
 
-__global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S, int n) {
-  const int i = d_dequeue(P, 2, blockIdx.x * blockDim.x + threadIdx.x);
-  if (i < 0) return;
+__device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -1553,6 +1547,48 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_photo(EngineParams P, PhotonSoA S
   S.prev_u[i] = u;
 }
 
+/* one launch serves the three interaction queues: thread q takes entry q of
+ * their concatenation (Rayleigh, Compton, photo; each queue's ENG_SPREAD
+ * sub-queues in order), so the grid's empty waves are only those beyond the
+ * photons that interact at this order, and a wavefront straddles two queues
+ * at two places at most.  Each photon is one thread's, with its own random
+ * stream, so results do not depend on the assignment. */
+__global__ void __launch_bounds__(ENG_BLOCK) k_interact(EngineParams P, PhotonSoA S, int n) {
+  constexpr int NS = 3 * ENG_SPREAD;
+  __shared__ unsigned int off[NS + 1];
+  if (threadIdx.x < 64) {
+    /* exclusive prefix sums of the 3 x 64 sub-queue lengths */
+    const int lane = threadIdx.x;
+    unsigned int base = 0;
+    for (int t = 0; t < 3; t++) {
+      const unsigned int c = (unsigned int)P.counters[(size_t)t * ENG_SPREAD + lane];
+      unsigned int x = c;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const unsigned int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      off[t * ENG_SPREAD + lane] = base + x - c;
+      base += __shfl(x, 63);
+    }
+    if (lane == 0) off[NS] = base;
+  }
+  __syncthreads();
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= off[NS]) return;
+  int lo = 0, hi = NS;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const int t = lo / ENG_SPREAD;
+  const int i = P.queue[(size_t)lo * P.qreg + (q - off[lo])];
+  if (t == 0) d_rayleigh_one(P, S, n, i);
+  else if (t == 1) d_compton_one(P, S, n, i);
+  else d_photo_one(P, S, n, i);
+}
+
 /* ------------------------------------------------------------------------ */
 /* escape-ratio runs (src/xmi_main.F90:5473-5801 with the analogue step of
  * :1274-1407): after its forced first interaction a photon leaves the crystal
@@ -1604,7 +1640,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_escape(EngineParams P, PhotonSoA 
     const int iE = (int)(S.hid[i] / (uint64_t)Q.n_photons);
     if (iE >= 0 && iE < Q.nE) {
       /* the photon's weight after the forced step (V.pesc) normalises;
-       * k_photo has multiplied S.weight by the fluorescence yield, which an
+       * the photo-absorption step has multiplied S.weight by the
+       * fluorescence yield, which an
        * escape-ratio run applies to the escape tally only
        * (src/xmi_variance_reduction.F90:742-746) */
       key_n = iE;
@@ -1764,10 +1801,7 @@ hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const Vr
   }
   hipError_t e = hipMemcpyAsync(V->pesc, S->weight, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  const int qblocks = (int)(((size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
-  hipLaunchKernelGGL(k_rayleigh, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
-  hipLaunchKernelGGL(k_compton, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
-  hipLaunchKernelGGL(k_photo, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
+  hipLaunchKernelGGL(k_interact, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
   const int eblocks = (int)((3 * (size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
   hipLaunchKernelGGL(k_escape, dim3(eblocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, *Q);
   return hipGetLastError();
@@ -1793,10 +1827,7 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
   }
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
-  const int qblocks = (int)(((size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
-  hipLaunchKernelGGL(k_rayleigh, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
-  hipLaunchKernelGGL(k_compton, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
-  hipLaunchKernelGGL(k_photo, dim3(qblocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  hipLaunchKernelGGL(k_interact, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, n);
   if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
