@@ -953,6 +953,45 @@ __device__ __forceinline__ bool d_chunk_next(const EngineParams &P, LayerChunks 
   return c < w.c_end;
 }
 
+/* 32-bit words of a lane's Philox stream, two blocks deep: the current
+ * block (a, na words left) and the next one (b, when hasb).  Refills are
+ * wave-collective: when an active lane is about to need a block it has not
+ * computed, every active lane without a next block computes its next one.
+ * Lanes empty their blocks at different trials, so a refill per lane would
+ * run the Philox rounds at nearly every trial of a wavefront; this way a
+ * wavefront runs them about once per four trials.  The words come out in
+ * the stream's order (blocks ctr, ctr + 1, ...; each block's words first to
+ * last), as phx_word gives them. */
+struct WaveWords {
+  uint32_t a[4], b[4];
+  uint32_t ctr[4], key[2];
+  int na, hasb;
+};
+
+__device__ __forceinline__ uint32_t d_ww_word(WaveWords &w) {
+  if (__ballot(w.na == 0 && !w.hasb)) {
+    if (!w.hasb) {
+      phx_4x32_10(w.ctr, w.key, w.b);
+      w.ctr[0]++;
+      w.hasb = 1;
+    }
+  }
+  if (w.na == 0) {
+    w.a[0] = w.b[0];
+    w.a[1] = w.b[1];
+    w.a[2] = w.b[2];
+    w.a[3] = w.b[3];
+    w.na = 4;
+    w.hasb = 0;
+  }
+  const uint32_t x = w.a[0];
+  w.a[0] = w.a[1];
+  w.a[1] = w.a[2];
+  w.a[2] = w.a[3];
+  w.na--;
+  return x;
+}
+
 template <int NL, bool use_lds>
 __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): channels [nch][2], Rayleigh/Compton rows
@@ -997,7 +1036,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
     double rd[NR];
     xrl_dcsp_ctx dc = {0.0, 0.0, 0.0};
     int dir = 1, nc = 0, ch_r = -1;
-    phx_wstream s = {};
+    WaveWords s = {};
 #pragma unroll
     for (int k = 0; k < NR; k++) rd[k] = 0.0;
     if (mine) {
@@ -1024,7 +1063,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         s.ctr[1] = s0.ctr[1];
         s.ctr[2] = s0.ctr[2];
         s.ctr[3] = s0.ctr[3];
-        s.avail = 0;
+        s.na = 0;
+        s.hasb = 0;
       }
       dc = xrl_dcsp_setup(E, theta, phi);
       const double sth2 = sin(theta / 2.0);
@@ -1057,7 +1097,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         int it = 0, failed = 0;
         for (;;) {
           /* one 32-bit word per trial: 31 bits of random number, a sign bit */
-          const uint32_t wd = phx_word(&s);
+          const uint32_t wd = d_ww_word(s);
           const double r = (double)(wd >> 1) * 0x1.0p-31;
           const int pos = (int)(r * inv_drn);
           if (pos == P.cp_len - 2) continue;
