@@ -100,6 +100,53 @@ def test_same_streams_random_samples(orc, case):
     _assert_same(gpu, ref, f"random {case}")
 
 
+@pytest.mark.parametrize("n_layers", [6, 11])
+def test_same_streams_many_layers(orc, n_layers):
+    """The kernels' layer bounds 8 and 16 (two photons per lane in the
+    lines estimator at 8, one at 16; path records read from memory in the
+    scatter estimator beyond 4 layers), with the reference layer inside the
+    stack so that photons cross layers in both directions."""
+    rng = np.random.default_rng(300 + n_layers)
+    inp = workloads.random_sample(rng, n_layers=(n_layers, n_layers), n_elements=(2, 5),
+                                  thickness=(1e-3, 5e-2), density=(1.0, 8.0))
+    inp.reference_layer = 2
+    inp.n_photons_line = 2000
+    print(f"{n_layers} layers, elements {[len(l.Z) for l in inp.layers]}", flush=True)
+    prep = _prep(inp)
+    gpu = prep.run(seed=40 + n_layers)
+    ref = orc.run(prep, seed=40 + n_layers, rng_mode=orc.RNG_PER_HISTORY)
+    assert gpu[3].histories == ref[3].histories == 2000
+    _assert_same(gpu, ref, f"{n_layers} layers")
+
+
+@pytest.mark.parametrize("case", ["one_order", "six_orders", "one_photon", "odd_channels", "many_elements"])
+def test_same_streams_edge_cases(orc, case):
+    """One interaction per trajectory and six; a budget of one photon (a
+    batch of one slot); 1000 channels at another gain and zero; a layer of
+    30 elements (the estimator's element loops past 32 rows)."""
+    rng = np.random.default_rng(77)
+    inp = workloads.random_sample(rng, n_layers=(2, 2), n_elements=(3, 6))
+    inp.n_photons_line = 3000
+    if case == "one_order":
+        inp.n_interactions_trajectory = 1
+    elif case == "six_orders":
+        inp.n_interactions_trajectory = 6
+    elif case == "one_photon":
+        inp.n_photons_line = 1
+    elif case == "odd_channels":
+        inp.nchannels, inp.gain, inp.zero = 1000, 0.037, 0.12
+    elif case == "many_elements":
+        inp = workloads.random_sample(rng, n_layers=(1, 1), n_elements=(30, 30))
+        inp.n_photons_line = 3000
+    print(case, flush=True)
+    prep = _prep(inp)
+    gpu = prep.run(seed=61)
+    ref = orc.run(prep, seed=61, rng_mode=orc.RNG_PER_HISTORY)
+    assert gpu[3].histories == ref[3].histories == inp.n_photons_line
+    assert gpu[0].shape == (inp.n_interactions_trajectory + 1, inp.nchannels)
+    _assert_same(gpu, ref, case)
+
+
 def test_bit_reproducible():
     prep = _prep(workloads.single_layer_steel(n_photons_line=30000))
     a = prep.run(seed=5)

@@ -45,6 +45,16 @@ struct PhotonSoA {
 
 /* per-photon records of the estimator at the current order, written by the
  * geometry kernel and read by the cone, scattering and line kernels */
+/* one ln(E) cell of a layer's attenuation table: the Rayleigh + Compton
+ * value at the cell's start and its increment to the next cell, and, when no
+ * absorption edge of the layer lies in the cell, the photo-absorption
+ * coefficient of every energy in it (edge_free; else K, the number of edges
+ * above the cell, starts the edge walk) */
+struct LayCell {
+  double T0, dT, pb;
+  int K, edge_free;
+};
+
 struct VrSoA {
   double *theta, *phi;          /* scattering angle to the detector point, azimuth to the polarisation */
   double *omega;                /* solid angle of the detector */
@@ -93,6 +103,7 @@ struct EngineParams {
   const double *lay_edge;       /* [lay_eoff[l] .. lay_eoff[l+1]) */
   const double *lay_pb;         /* [lay_eoff[l] + l .. lay_eoff[l+1] + l] */
   const int *lay_kstart;        /* [n_layers][XRL_NCS] */
+  const struct LayCell *lay_cell; /* [n_layers][XRL_NCS] */
   int lay_eoff[ENG_MAX_LAYERS + 1];
   /* per element */
   const xrl_atom *atoms;        /* [nZ] */

@@ -140,7 +140,7 @@ struct Engine {
   /* per-call buffers (grow only) */
   DevBuf el_Z, el_w, el_u, precalc_mu, act_n, act_line, act_shell, act_factor, act_E, act_ch, act_murho;
   DevBuf fl_info, fl_wf, fl_mr, lbin_g;
-  DevBuf lay_rc, lay_edge, lay_pb, lay_kstart;
+  DevBuf lay_rc, lay_edge, lay_pb, lay_kstart, lay_cell;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, lay_segbase, cone_acc, esc_tally;
 };
@@ -471,6 +471,27 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     }
   }
   P.lay_eoff[NL] = (int)lay_edge.size();
+  /* the same tables per cell, for one record load per lookup */
+  std::vector<LayCell> lay_cell((size_t)NL * XRL_NCS);
+  {
+    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
+    for (int l = 0; l < NL; l++) {
+      const int eo = P.lay_eoff[l], ne = P.lay_eoff[l + 1] - eo;
+      for (int c = 0; c < XRL_NCS; c++) {
+        LayCell &r = lay_cell[(size_t)l * XRL_NCS + c];
+        const double *T = &lay_rc[(size_t)l * XRL_NCS];
+        r.T0 = T[c];
+        r.dT = c + 1 < XRL_NCS ? T[c + 1] - T[c] : 0.0;
+        r.K = lay_kstart[(size_t)l * XRL_NCS + c];
+        /* every energy of the cell lies above the next edge (with a margin
+         * far beyond the rounding of the cell index); the end cells also
+         * take the energies beyond the grid, clamped, and keep the walk */
+        const double e_lo = exp(XRL_CS_LNE0 + (double)c * h) * (1.0 - 1e-9);
+        r.edge_free = c > 0 && c < XRL_NCS - 2 && (r.K >= ne || lay_edge[eo + r.K] < e_lo);
+        r.pb = lay_pb[eo + l + r.K];
+      }
+    }
+  }
   /* the same lookup on the host (as d_layer_mu_tab on the device) */
   auto layer_mu_tab = [&](int l, double E) -> double {
     const xrl_ectx c = xrl_energy(E);
@@ -734,6 +755,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size()));
     HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size()));
     HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size()));
+    HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size()));
     HIP_TRY(upload(G.src_desc, src.data(), src.size()));
     HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size()));
     HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size()));
@@ -784,6 +806,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.lay_edge = G.lay_edge.as<double>();
     P.lay_pb = G.lay_pb.as<double>();
     P.lay_kstart = G.lay_kstart.as<int>();
+    P.lay_cell = G.lay_cell.as<LayCell>();
     P.phi_icdf = T->phi.as<double>();
     P.phi_T = T->phiT.as<double>();
     P.phi_R = T->phiR.as<double>();

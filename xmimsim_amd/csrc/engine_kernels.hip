@@ -43,11 +43,16 @@
  * the layer sums are formed once on the host */
 __device__ __forceinline__ double d_layer_mu_tab(const EngineParams &P, int layer0, const xrl_ectx *c) {
   if (!(c->E > 0.0)) return 0.0;
-  const double *T = P.lay_rc + (size_t)layer0 * XRL_NCS;
-  const int eo = P.lay_eoff[layer0], ne = P.lay_eoff[layer0 + 1] - eo;
-  int K = P.lay_kstart[(size_t)layer0 * XRL_NCS + c->ci];
-  while (K < ne && P.lay_edge[eo + K] > c->E) K++;
-  return c->pe * P.lay_pb[eo + layer0 + K] + (T[c->ci] + c->cf * (T[c->ci + 1] - T[c->ci]));
+  const LayCell r = P.lay_cell[(size_t)layer0 * XRL_NCS + c->ci];
+  double pb = r.pb;
+  if (!r.edge_free) {
+    /* an edge inside the cell: count the edges above E */
+    const int eo = P.lay_eoff[layer0], ne = P.lay_eoff[layer0 + 1] - eo;
+    int K = r.K;
+    while (K < ne && P.lay_edge[eo + K] > c->E) K++;
+    pb = P.lay_pb[eo + layer0 + K];
+  }
+  return c->pe * pb + (r.T0 + c->cf * r.dT);
 }
 
 __device__ __forceinline__ void d_mu_all(const EngineParams &P, double E, double *mus, size_t stride, size_t idx) {
