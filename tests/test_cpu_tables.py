@@ -196,3 +196,13 @@ def test_host_table_generator_properties(lib):
         T.close()
     finally:
         lib.xmi_tables_set_device(-1)
+
+
+def test_bench_refuses_several_gpus():
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
