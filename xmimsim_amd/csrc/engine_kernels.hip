@@ -1792,7 +1792,10 @@ static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const
                                 int n_cu, int blocks, hipStream_t st, hipEvent_t *ev) {
   hipLaunchKernelGGL((k_vr_geom<NL>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
   if (ev) (void)hipEventRecord(ev[0], st);
-  hipLaunchKernelGGL(k_vr_cone, dim3(n_cu * 4), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+  /* one resident round: the trial segments are of equal size, so a second,
+   * partial round of workgroups would only add a tail */
+  const int cb = resident_blocks((const void *)k_vr_cone, 0, n_cu, n_cu * 4);
+  hipLaunchKernelGGL(k_vr_cone, dim3(cb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   if (ev) (void)hipEventRecord(ev[1], st);
   const size_t ls = scatter_lds_bytes(P);
   if (ls) {
