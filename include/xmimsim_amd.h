@@ -294,6 +294,7 @@ typedef struct _xmi_msim_stats {
   double vr_cone_ms;
   double vr_scatter_ms;
   double vr_lines_ms;
+  int64_t grid_uploads;       /* solid-angle grid copies to the device (0 when the resident copy was current) */
 } xmi_msim_stats;
 void xmi_msim_get_stats(xmi_msim_stats *out);
 

@@ -71,6 +71,20 @@ def load():
                                              C.c_long, C.c_double, C.c_double, C.c_long, C.c_uint64, C.c_int,
                                              P(C.c_double), P(C.c_double)]
     lib.xmi_oracle_escape_ratios.restype = C.c_int
+    D = P(C.c_double)
+    lib.xmi_oracle_probe_scatter.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, D, D,
+                                             C.c_long, C.c_uint64, D]
+    lib.xmi_oracle_probe_scatter.restype = C.c_int
+    lib.xmi_oracle_probe_fluorescence.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                                  C.c_long, C.c_uint64, P(C.c_int), D, D]
+    lib.xmi_oracle_probe_fluorescence.restype = C.c_int
+    lib.xmi_oracle_probe_rotate.argtypes = [D, D, C.c_double, C.c_double, D]
+    lib.xmi_oracle_probe_phi0.argtypes = [D, D]
+    lib.xmi_oracle_probe_phi0.restype = C.c_double
+    lib.xmi_oracle_probe_source.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_long, C.c_uint64, D]
+    lib.xmi_oracle_probe_source.restype = C.c_int
+    lib.xmi_oracle_input_derived.argtypes = [C.c_void_p, D]
+    lib.xmi_oracle_set_plant.argtypes = [C.c_int]
     _lib = lib
     return lib
 
@@ -157,3 +171,96 @@ def detector_convolute(prepared, noconv):
     load().xmi_oracle_detector_convolute(prepared.inputF, x.ctypes.data_as(C.POINTER(C.c_double)),
                                          out.ctypes.data_as(C.POINTER(C.c_double)))
     return out
+
+
+# ---- probes of single restated routines (pinning tests) --------------------
+PLANTS = {"none": 0, "rayleigh_energy": 1, "phi0_sign": 2, "compton_full_angle": 3, "doppler_one_sided": 4,
+          "ck_l1_swap": 5, "line_shift": 6, "elecv_stale": 7, "dirv_phi_sign": 8, "slit_full": 9, "gauss_xy": 10,
+          "compton_t_rayleigh": 11, "shell_next": 12}
+
+
+class planted:
+    """Context manager: runs the oracle with one planted mistake."""
+
+    def __init__(self, name: str):
+        self.id = PLANTS[name]
+
+    def __enter__(self):
+        load().xmi_oracle_set_plant(self.id)
+        return self
+
+    def __exit__(self, *exc):
+        load().xmi_oracle_set_plant(0)
+        return False
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def probe_scatter(prepared, kind: str, element_index: int, energy: float, dirv, elecv, n: int, seed: int):
+    """Rayleigh ("rayl") or Compton ("compt") interaction n times from one
+    state; returns (new dirv [n,3], new elecv [n,3], energy [n])."""
+    d = np.ascontiguousarray(dirv, np.float64)
+    e = np.ascontiguousarray(elecv, np.float64)
+    out = np.zeros((n, 7))
+    ok = load().xmi_oracle_probe_scatter(prepared.inputF, prepared.hdf5F, C.byref(prepared.options),
+                                         {"rayl": 1, "compt": 2}[kind], element_index, energy, _d(d), _d(e), n, seed,
+                                         _d(out))
+    if not ok:
+        raise RuntimeError("probe failed")
+    return out[:, :3], out[:, 3:6], out[:, 6]
+
+
+def probe_fluorescence(prepared, element_index: int, energy: float, n: int, seed: int, directions: bool = False):
+    """Photo-absorption + fluorescence n times; returns (line [n] (0 = none),
+    weight factor [n], and with directions: dirv [n,3], elecv [n,3])."""
+    line = np.zeros(n, np.int32)
+    w = np.zeros(n)
+    dirs = np.zeros((n, 6)) if directions else None
+    ok = load().xmi_oracle_probe_fluorescence(prepared.inputF, prepared.hdf5F, C.byref(prepared.options),
+                                              element_index, energy, n, seed,
+                                              line.ctypes.data_as(C.POINTER(C.c_int)), _d(w),
+                                              _d(dirs) if directions else None)
+    if not ok:
+        raise RuntimeError("probe failed")
+    if directions:
+        return line, w, dirs[:, :3], dirs[:, 3:]
+    return line, w
+
+
+def probe_rotate(dirv, elecv, theta: float, phi: float):
+    d = np.ascontiguousarray(dirv, np.float64)
+    e = np.ascontiguousarray(elecv, np.float64)
+    out = np.zeros(6)
+    load().xmi_oracle_probe_rotate(_d(d), _d(e), theta, phi, _d(out))
+    return out[:3], out[3:]
+
+
+def probe_phi0(dirv, elecv) -> float:
+    d = np.ascontiguousarray(dirv, np.float64)
+    e = np.ascontiguousarray(elecv, np.float64)
+    return load().xmi_oracle_probe_phi0(_d(d), _d(e))
+
+
+def probe_source(prepared, which: int, n: int, seed: int):
+    """Primary photons of discrete line `which` (>= 0) or continuous interval
+    -which-1; returns (entry point [n,3], dirv, elecv, energy [n])."""
+    out = np.zeros((n, 10))
+    ok = load().xmi_oracle_probe_source(prepared.inputF, prepared.hdf5F, C.byref(prepared.options), which, n, seed,
+                                        _d(out))
+    if not ok:
+        raise RuntimeError("probe failed")
+    return out[:, :3], out[:, 3:6], out[:, 6:9], out[:, 9]
+
+
+def input_derived(prepared):
+    """Derived fields of xmi_init_input as a dict."""
+    nl = len(prepared.inp.layers)
+    out = np.zeros(35 + 3 * nl)
+    load().xmi_oracle_input_derived(prepared.inputF, _d(out))
+    return {"detector_radius": out[0], "collimator_present": out[1], "collimator_radius": out[2],
+            "half_apex": out[3], "vertex": out[4:7], "n_sample_orientation_det": out[7:10],
+            "n_sample_orientation": out[10:13], "n_detector_orientation": out[13:16],
+            "new": out[16:25].reshape(3, 3), "inverse": out[25:34].reshape(3, 3), "detector_solid_angle": out[34],
+            "thickness_along_Z": out[35::3], "Z_coord_begin": out[36::3], "Z_coord_end": out[37::3]}

@@ -60,6 +60,10 @@ static const double ORC_PI = 3.14159265358979323846;
 
 static xmi_oracle_stats g_stats;
 static long g_hits_per_single = 5000;
+/* a mistake planted on purpose (XMI_ORACLE_PLANT_*), for tests that show a
+ * check rejects it; 0 in every other use */
+static int g_plant = XMI_ORACLE_PLANT_NONE;
+void xmi_oracle_set_plant(int id) { g_plant = id; }
 
 void xmi_oracle_set_hits_per_single(long hits) { g_hits_per_single = hits; }
 
@@ -396,6 +400,7 @@ static void orc_update_dirv(orc_photon *p, double theta_i, double phi_i) {
   double cp = cos(p->phi), sp = sin(p->phi), ct = cos(p->theta), st = sin(p->theta);
   double T[3][3] = {{ct * cp, -sp, st * cp}, {ct * sp, cp, st * sp}, {-st, 0.0, ct}};
   double ts = sin(theta_i);
+  if (g_plant == XMI_ORACLE_PLANT_DIRV_PHI_SIGN) phi_new = -phi_new;
   double v[3] = {ts * cos(phi_new), ts * sin(phi_new), cos(theta_i)};
   matvec3(T, v, p->dirv);
   normalize3(p->dirv);
@@ -406,6 +411,7 @@ static void orc_update_dirv(orc_photon *p, double theta_i, double phi_i) {
 }
 
 static void orc_update_elecv(orc_photon *p) {
+  if (g_plant == XMI_ORACLE_PLANT_ELECV_STALE) return;
   double cosalfa = dot3(p->dirv, p->elecv);
   double c_alfa = acos(cosalfa);
   double sinalfa = sin(c_alfa);
@@ -422,6 +428,7 @@ static double orc_phi0(const orc_photon *p) {
   if (fabs(cosphi0) > 1.0) cosphi0 = cosphi0 > 0 ? 1.0 : -1.0;
   double phi0 = acos(cosphi0);
   if (sinphi0 > 0.0) phi0 = -phi0;
+  if (g_plant == XMI_ORACLE_PLANT_PHI0_SIGN) phi0 = -phi0;
   return phi0;
 }
 
@@ -433,7 +440,8 @@ static const xmi_hdf5_ZF *photon_hdf5(const orc_ctx *c, const orc_photon *p) {
 static int orc_rayleigh(const orc_ctx *c, orc_photon *p) {
   const xmi_hdf5_ZF *z = photon_hdf5(c, p);
   double r = orc_uniform(p->tr);
-  double theta_i = bilinear(z->RayleighTheta_ICDF, z->Energies, z->n_e, z->RandomNumbers, z->n_r, p->energy, r);
+  double e_lookup = g_plant == XMI_ORACLE_PLANT_RAYLEIGH_ENERGY ? 1.15 * p->energy : p->energy;
+  double theta_i = bilinear(z->RayleighTheta_ICDF, z->Energies, z->n_e, z->RandomNumbers, z->n_r, e_lookup, r);
   double tt = sin(theta_i);
   tt = tt * tt;
   tt = tt / (4.0 - 2.0 * tt);
@@ -461,6 +469,7 @@ static int orc_compton(const orc_ctx *c, orc_photon *p) {
   double tt = sin(theta_i);
   tt = tt * tt;
   tt = tt / (K0K + (1.0 / K0K) - tt) / 2.0;
+  if (g_plant == XMI_ORACLE_PLANT_COMPTON_T_RAYLEIGH) tt = sin(theta_i) * sin(theta_i) / (4.0 - 2.0 * sin(theta_i) * sin(theta_i));
   double phi_i = bilinear(c->h->Phi_ICDF, c->h->Thetas, c->h->phi_n_theta, c->h->RandomNumbers, c->h->phi_n_r, tt,
                           orc_uniform(p->tr));
   double phi0 = orc_phi0(p);
@@ -470,10 +479,11 @@ static int orc_compton(const orc_ctx *c, orc_photon *p) {
     double energy = p->energy * 1000.0;
     double c_lamb0 = cc / energy;
     double sth2 = sin(theta_i / 2.0);
+    if (g_plant == XMI_ORACLE_PLANT_COMPTON_FULL_ANGLE) sth2 = sin(theta_i);
     for (;;) {
       double r = orc_uniform(p->tr);
       double pz = doppler_pz(z, r);
-      if (orc_uniform(p->tr) < 0.5) pz = -pz;
+      if (orc_uniform(p->tr) < 0.5 && g_plant != XMI_ORACLE_PLANT_DOPPLER_ONE_SIDED) pz = -pz;
       double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
       double c_lamb = c_lamb0 + dlamb;
       energy = cc / c_lamb / 1000.0;
@@ -502,6 +512,13 @@ static int orc_compton(const orc_ctx *c, orc_photon *p) {
 /* src/xmi_main.F90:5184-5323 */
 static void orc_coster_kronig(const xrl_atom *a, orc_rng *rng, int *shell) {
   const double *f = a->ck;
+  double fp[XRL_NCK];
+  if (g_plant == XMI_ORACLE_PLANT_CK_L1_SWAP) {
+    memcpy(fp, a->ck, sizeof(fp));
+    fp[XRL_FL12] = a->ck[XRL_FL13];
+    fp[XRL_FL13] = a->ck[XRL_FL12];
+    f = fp;
+  }
   if (*shell == 1 || *shell == 2) {
     for (;;) {
       double r = orc_uniform(rng);
@@ -579,7 +596,8 @@ static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
   int max_shell = c->opt->use_M_lines ? XRL_M5_SHELL : XRL_L3_SHELL;
   int shell, found = 0;
   for (shell = 0; shell <= max_shell; shell++) {
-    sumz += xrl_cs_photo_partial(a, shell, p->energy) / photo_total;
+    int s_cs = g_plant == XMI_ORACLE_PLANT_SHELL_NEXT ? shell + 1 : shell;
+    sumz += xrl_cs_photo_partial(a, s_cs, p->energy) / photo_total;
     if (r < sumz) {
       found = 1;
       break;
@@ -603,7 +621,7 @@ static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
   xrl_shell_lines(shell, &first, &last);
   found = 0;
   for (int l = first; l <= last; l++) {
-    sumz += a->rad_rate[l];
+    sumz += a->rad_rate[g_plant == XMI_ORACLE_PLANT_LINE_SHIFT && l < last ? l + 1 : l];
     if (r < sumz) {
       found = 1;
       line = l;
@@ -935,14 +953,15 @@ static void orc_coords_dir(const orc_ctx *c, orc_rng *rng, double sx, double sxp
   const xmi_geometry *g = &c->in->geometry;
   double x1, y1;
   if (fabs(sx * sy) < 1.0e-20) {
-    double x1_max = atan(g->slit_size_x / g->d_source_slit / 2.0);
+    double half = g_plant == XMI_ORACLE_PLANT_SLIT_FULL ? 1.0 : 2.0;
+    double x1_max = atan(g->slit_size_x / g->d_source_slit / half);
     double y1_max = atan(g->slit_size_y / g->d_source_slit / 2.0);
     x1 = x1_max * (-1.0 + 2.0 * orc_uniform(rng));
     y1 = y1_max * (-1.0 + 2.0 * orc_uniform(rng));
     p->coords[0] = p->coords[1] = p->coords[2] = 0.0;
   } else {
     x1 = orc_gauss(rng, sxp);
-    y1 = orc_gauss(rng, syp);
+    y1 = orc_gauss(rng, g_plant == XMI_ORACLE_PLANT_GAUSS_XY ? sxp : syp);
     p->coords[0] = orc_gauss(rng, sx) - g->d_source_slit * sin(x1);
     p->coords[1] = orc_gauss(rng, sy) - g->d_source_slit * sin(y1);
     p->coords[2] = 0.0;
@@ -1004,6 +1023,89 @@ static double exc_correction(const orc_ctx *c, double E) {
   return corr;
 }
 
+/* one primary photon of the continuous interval [e0, e1]: energy from the
+ * trapezoid, weight, direction, polarisation, entry point
+ * (src/xmi_main.F90:319-430); -1 on failure */
+static int orc_source_continuous(const orc_ctx *c, const xmi_energy_continuous *e0, const xmi_energy_continuous *e1,
+                                 long n_photons, orc_rng *src, orc_photon *p) {
+  double x1 = e0->energy, x2 = e1->energy;
+  double y1 = e0->vertical_intensity + e0->horizontal_intensity;
+  double y2 = e1->vertical_intensity + e1->horizontal_intensity;
+  double total_intensity = (y1 + y2) * (x2 - x1) / 2.0;
+  p->energy = xmi_oracle_ran_trap(x1, x2, y1, y2, orc_uniform(src));
+  if (p->energy < 0.0) return -1;
+  double hor = interpolate_simple(x1, e0->horizontal_intensity, x2, e1->horizontal_intensity, p->energy) /
+               interpolate_simple(x1, y1, x2, y2, p->energy);
+  double exc_corr = exc_correction(c, p->energy);
+  mu_calc_all(c, p->energy, p->mus);
+  double weight = total_intensity * exc_corr / (double)n_photons;
+  orc_coords_dir(c, src, e0->sigma_x, e0->sigma_xp, e0->sigma_y, e0->sigma_yp, p);
+  int horizontal = orc_uniform(src) <= hor;
+  p->weight = weight;
+  if (!orc_polarise(p, horizontal) || !orc_shift_first_layer(c, p)) return -1;
+  return 1;
+}
+
+/* photon jj (0-based) of discrete line d: energy (fixed, Gaussian or
+ * Lorentzian), direction, polarisation (the first hor_ver_ratio photons
+ * horizontal), entry point (src/xmi_main.F90:579-716); 0 when the energy
+ * falls outside (E_thr, E_max], -1 on failure */
+static int orc_source_discrete(const orc_ctx *c, const xmi_energy_discrete *d, const double *initial_mus,
+                               double weight, long jj, double hor_ver_ratio, orc_rng *src, orc_photon *p) {
+  if (d->distribution_type == XMI_ENERGY_DISCRETE_DISTRIBUTION_GAUSSIAN) {
+    p->energy = orc_gauss(src, d->scale_parameter) + d->energy;
+    if (p->energy <= XMI_ENERGY_THRESHOLD || p->energy > XMI_ENERGY_MAX) return 0;
+    mu_calc_all(c, p->energy, p->mus);
+  } else if (d->distribution_type == XMI_ENERGY_DISCRETE_DISTRIBUTION_LORENTZIAN) {
+    p->energy = d->scale_parameter * tan(ORC_PI * (orc_uniform(src) - 0.5)) + d->energy;
+    if (p->energy <= XMI_ENERGY_THRESHOLD || p->energy > XMI_ENERGY_MAX) return 0;
+    mu_calc_all(c, p->energy, p->mus);
+  } else {
+    p->energy = d->energy;
+    memcpy(p->mus, initial_mus, sizeof(double) * c->in->n_layers);
+  }
+  orc_coords_dir(c, src, d->sigma_x, d->sigma_xp, d->sigma_y, d->sigma_yp, p);
+  int horizontal = (double)(jj + 1) <= hor_ver_ratio;
+  p->weight = weight;
+  if (!orc_polarise(p, horizontal) || !orc_shift_first_layer(c, p)) return -1;
+  return 1;
+}
+
+static int cascade_of_options(const xmi_main_options *o) {
+  if (!o->use_cascade_auger && !o->use_cascade_radiative) return XRL_CASCADE_NONE;
+  if (o->use_cascade_auger && !o->use_cascade_radiative) return XRL_CASCADE_NONRADIATIVE;
+  if (!o->use_cascade_auger && o->use_cascade_radiative) return XRL_CASCADE_RADIATIVE;
+  return XRL_CASCADE_FULL;
+}
+
+/* per-element records, line energies and the attenuation of every layer at
+ * every line energy (the reference's LineEnergies and precalc_mu_cs,
+ * src/xmi_main.F90:218-237) */
+static void orc_ctx_tables(orc_ctx *c) {
+  const xmi_inputF *in = c->in;
+  const xmi_hdf5F *h = c->h;
+  c->atoms = (const xrl_atom **)malloc(sizeof(xrl_atom *) * c->nZ);
+  c->line_energies = (double *)calloc((size_t)c->nZ * XMI_NLINES_VR, sizeof(double));
+  for (int i = 0; i < c->nZ; i++) {
+    c->atoms[i] = xrl_atom_get(h->Zs[i].Z);
+    for (int l = 1; l <= XMI_NLINES_VR; l++)
+      c->line_energies[(size_t)i * XMI_NLINES_VR + l - 1] = c->atoms[i]->line_energy[l];
+  }
+  c->precalc_mu = (double *)calloc((size_t)in->n_layers * c->nZ * XMI_NLINES_VR, sizeof(double));
+  for (int k = 0; k < in->n_layers; k++)
+    for (int i = 0; i < c->nZ; i++)
+      for (int l = 0; l < XMI_NLINES_VR; l++) {
+        double E = c->line_energies[(size_t)i * XMI_NLINES_VR + l];
+        c->precalc_mu[((size_t)k * c->nZ + i) * XMI_NLINES_VR + l] = E > 0.0 ? layer_mu(c, &in->layers[k], E) : 0.0;
+      }
+}
+
+static void orc_ctx_free_tables(orc_ctx *c) {
+  free(c->atoms);
+  free(c->line_energies);
+  free(c->precalc_mu);
+}
+
 /* ------------------------------------------------------------------------ */
 
 int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
@@ -1036,24 +1138,9 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
   c.hits_per_single = g_hits_per_single;
   c.seed = seed;
   c.rng_mode = rng_mode;
-  if (!options->use_cascade_auger && !options->use_cascade_radiative) c.cascade_type = XRL_CASCADE_NONE;
-  else if (options->use_cascade_auger && !options->use_cascade_radiative) c.cascade_type = XRL_CASCADE_NONRADIATIVE;
-  else if (!options->use_cascade_auger && options->use_cascade_radiative) c.cascade_type = XRL_CASCADE_RADIATIVE;
-  else c.cascade_type = XRL_CASCADE_FULL;
+  c.cascade_type = cascade_of_options(options);
   if (c.n_int > MAX_INT_TRAJ || in->n_layers > 64) return 0;
-  c.atoms = (const xrl_atom **)malloc(sizeof(xrl_atom *) * c.nZ);
-  c.line_energies = (double *)calloc((size_t)c.nZ * XMI_NLINES_VR, sizeof(double));
-  for (int i = 0; i < c.nZ; i++) {
-    c.atoms[i] = xrl_atom_get(h->Zs[i].Z);
-    for (int l = 1; l <= XMI_NLINES_VR; l++) c.line_energies[(size_t)i * XMI_NLINES_VR + l - 1] = c.atoms[i]->line_energy[l];
-  }
-  c.precalc_mu = (double *)calloc((size_t)in->n_layers * c.nZ * XMI_NLINES_VR, sizeof(double));
-  for (int k = 0; k < in->n_layers; k++)
-    for (int i = 0; i < c.nZ; i++)
-      for (int l = 0; l < XMI_NLINES_VR; l++) {
-        double E = c.line_energies[(size_t)i * XMI_NLINES_VR + l];
-        c.precalc_mu[((size_t)k * c.nZ + i) * XMI_NLINES_VR + l] = E > 0.0 ? layer_mu(&c, &in->layers[k], E) : 0.0;
-      }
+  orc_ctx_tables(&c);
 
   size_t nchan_tot = (size_t)(c.n_int + 1) * c.nch;
   size_t nhist = (size_t)100 * XMI_HIST_ROWS * c.n_int;
@@ -1097,11 +1184,9 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
 
     for (int i = 0; i + 1 < n_cont && ok; i++) {
       const xmi_energy_continuous *e0 = &in->continuous[i], *e1 = &in->continuous[i + 1];
-      double x1 = e0->energy, x2 = e1->energy;
       double y1 = e0->vertical_intensity + e0->horizontal_intensity;
       double y2 = e1->vertical_intensity + e1->horizontal_intensity;
-      double total_intensity = (y1 + y2) * (x2 - x1) / 2.0;
-      if (total_intensity == 0.0) continue;
+      if ((y1 + y2) * (e1->energy - e0->energy) / 2.0 == 0.0) continue;
       for (long j = 0; j < cont_n; j++) {
         uint64_t hid = (uint64_t)i * (uint64_t)nint_ph + (uint64_t)(cont_start + j);
         orc_rng *src = &thread_rng;
@@ -1111,20 +1196,7 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
         }
         memset(&p, 0, sizeof(p));
         p.tr = p.vr = &thread_rng;
-        p.energy = xmi_oracle_ran_trap(x1, x2, y1, y2, orc_uniform(src));
-        if (p.energy < 0.0) {
-          ok = 0;
-          break;
-        }
-        double hor = interpolate_simple(x1, e0->horizontal_intensity, x2, e1->horizontal_intensity, p.energy) /
-                     interpolate_simple(x1, y1, x2, y2, p.energy);
-        double exc_corr = exc_correction(&c, p.energy);
-        mu_calc_all(&c, p.energy, p.mus);
-        double weight = total_intensity * exc_corr / (double)nint_ph;
-        orc_coords_dir(&c, src, e0->sigma_x, e0->sigma_xp, e0->sigma_y, e0->sigma_yp, &p);
-        int horizontal = orc_uniform(src) <= hor;
-        p.weight = weight;
-        if (!orc_polarise(&p, horizontal) || !orc_shift_first_layer(&c, &p)) {
+        if (orc_source_continuous(&c, e0, e1, nint_ph, src, &p) < 0) {
           ok = 0;
           break;
         }
@@ -1153,22 +1225,9 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
         }
         memset(&p, 0, sizeof(p));
         p.tr = p.vr = &thread_rng;
-        if (d->distribution_type == XMI_ENERGY_DISCRETE_DISTRIBUTION_GAUSSIAN) {
-          p.energy = orc_gauss(src, d->scale_parameter) + d->energy;
-          if (p.energy <= XMI_ENERGY_THRESHOLD || p.energy > XMI_ENERGY_MAX) continue;
-          mu_calc_all(&c, p.energy, p.mus);
-        } else if (d->distribution_type == XMI_ENERGY_DISCRETE_DISTRIBUTION_LORENTZIAN) {
-          p.energy = d->scale_parameter * tan(ORC_PI * (orc_uniform(src) - 0.5)) + d->energy;
-          if (p.energy <= XMI_ENERGY_THRESHOLD || p.energy > XMI_ENERGY_MAX) continue;
-          mu_calc_all(&c, p.energy, p.mus);
-        } else {
-          p.energy = d->energy;
-          memcpy(p.mus, initial_mus, sizeof(double) * in->n_layers);
-        }
-        orc_coords_dir(&c, src, d->sigma_x, d->sigma_xp, d->sigma_y, d->sigma_yp, &p);
-        int horizontal = (double)(jj + 1) <= hor_ver_ratio;
-        p.weight = weight;
-        if (!orc_polarise(&p, horizontal) || !orc_shift_first_layer(&c, &p)) {
+        int rs = orc_source_discrete(&c, d, initial_mus, weight, jj, hor_ver_ratio, src, &p);
+        if (rs == 0) continue;
+        if (rs < 0) {
           ok = 0;
           break;
         }
@@ -1197,9 +1256,7 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
   for (size_t k = 0; k < nchan_tot; k++) ch_out[k] = channels[k] * live;
   free(channels);
   free(var_red);
-  free(c.atoms);
-  free(c.line_energies);
-  free(c.precalc_mu);
+  orc_ctx_free_tables(&c);
   g_stats.histories = histories;
   g_stats.interactions = interactions;
   g_stats.vr_calls = vr_calls;
@@ -1222,7 +1279,7 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
 
 static double depth_quantile(const xmi_inputF *in, double energy, double R) {
   int nl = in->n_layers;
-  double mu[64], my_sum = 0.0;
+  double mu[64] = {0.0}, my_sum = 0.0;
   for (int i = 0; i < nl; i++) {
     mu[i] = 0.0;
     for (int j = 0; j < in->layers[i].n_elements; j++)
@@ -1338,7 +1395,8 @@ double xmi_oracle_fpm_first_order(xmi_inputFPtr inputFPtr, int Z, int line, int 
   normalize3(to_det);
   double sin_alpha = fabs(dot3(beam, n));
   double sin_beta = fabs(dot3(to_det, n));
-  double th = atan(in->detector_radius / D);
+  /* detector radius sqrt(A/pi) from the input geometry (src/xmi_main.F90:1769) */
+  double th = atan(sqrt(in->geometry.area_detector / ORC_PI) / D);
   double Omega = 2.0 * ORC_PI * (1.0 - cos(th));
   double G = Omega / 4.0 / ORC_PI / sin_alpha;
   double P[9];
@@ -1377,7 +1435,7 @@ double xmi_oracle_scatter_first_order(xmi_inputFPtr inputFPtr, int Z, int kind) 
   normalize3(to_det);
   double sin_alpha = fabs(dot3(beam, n));
   double sin_beta = fabs(dot3(to_det, n));
-  double Omega = 2.0 * ORC_PI * (1.0 - cos(atan(in->detector_radius / D)));
+  double Omega = 2.0 * ORC_PI * (1.0 - cos(atan(sqrt(in->geometry.area_detector / ORC_PI) / D)));
   double cth = dot3(beam, to_det);
   double theta = acos(cth);
   double proj[3];
@@ -1560,4 +1618,198 @@ int xmi_oracle_escape_ratios(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi
   free(c.line_energies);
   free(c.precalc_mu);
   return ok;
+}
+
+/* ------------------------------------------------------------------------ */
+/* probes: one restated routine run n times from a given state, each run on
+ * its own stream (history i of substream 14), for the tests that pin the
+ * routines against closed forms and sampled distributions */
+
+static void probe_ctx(orc_ctx *c, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options) {
+  memset(c, 0, sizeof(*c));
+  c->in = (const xmi_inputF *)inputFPtr;
+  c->h = (const xmi_hdf5F *)hdf5FPtr;
+  c->opt = options;
+  c->nZ = c->h->n_Z;
+  c->n_int = c->in->general.n_interactions_trajectory;
+  c->nch = c->in->detector.nchannels;
+  c->line_last = options->use_M_lines ? XRL_M5P5 : XRL_L3Q1;
+  c->hits_per_single = g_hits_per_single;
+  c->cascade_type = cascade_of_options(options);
+  orc_ctx_tables(c);
+}
+
+/* a photon in the middle of layer 1 about to interact with element
+ * element_index of that layer */
+static void probe_photon(const orc_ctx *c, orc_photon *p, int element_index, double energy, const double *dirv,
+                         const double *elecv) {
+  const xmi_layerF *L = &c->in->layers[0];
+  memset(p, 0, sizeof(*p));
+  p->coords[2] = 0.5 * (L->Z_coord_begin + L->Z_coord_end);
+  for (int k = 0; k < 3; k++) {
+    p->dirv[k] = dirv[k];
+    p->elecv[k] = elecv[k];
+  }
+  p->theta = acos(p->dirv[2]);
+  p->phi = atan2(p->dirv[1], p->dirv[0]);
+  if (p->phi < 0.0) p->phi += 2.0 * ORC_PI;
+  p->energy = energy;
+  p->weight = 1.0;
+  p->current_layer = 1;
+  p->current_element_index = element_index;
+  p->current_element = L->Z[element_index];
+  p->n_interactions = 1;
+  mu_calc_all(c, energy, p->mus);
+}
+
+int xmi_oracle_probe_scatter(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, int kind,
+                             int element_index, double energy, const double *dirv, const double *elecv, long n,
+                             uint64_t seed, double *out) {
+  orc_ctx c;
+  probe_ctx(&c, inputFPtr, hdf5FPtr, options);
+  if (element_index < 0 || element_index >= c.in->layers[0].n_elements) {
+    orc_ctx_free_tables(&c);
+    return 0;
+  }
+  orc_photon p;
+  orc_rng r;
+  for (long i = 0; i < n; i++) {
+    probe_photon(&c, &p, element_index, energy, dirv, elecv);
+    orc_rng_init(&r, seed, 14u, (uint64_t)i);
+    p.tr = p.vr = &r;
+    if (kind == XMI_RAYLEIGH_INTERACTION) orc_rayleigh(&c, &p);
+    else orc_compton(&c, &p);
+    double *o = out + 7 * i;
+    for (int k = 0; k < 3; k++) {
+      o[k] = p.dirv[k];
+      o[3 + k] = p.elecv[k];
+    }
+    o[6] = p.energy;
+  }
+  orc_ctx_free_tables(&c);
+  return 1;
+}
+
+int xmi_oracle_probe_fluorescence(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options,
+                                  int element_index, double energy, long n, uint64_t seed, int *line,
+                                  double *weight, double *dir) {
+  orc_ctx c;
+  probe_ctx(&c, inputFPtr, hdf5FPtr, options);
+  if (element_index < 0 || element_index >= c.in->layers[0].n_elements) {
+    orc_ctx_free_tables(&c);
+    return 0;
+  }
+  const double d0[3] = {0.0, 0.0, 1.0}, e0[3] = {0.0, 1.0, 0.0};
+  orc_photon p;
+  orc_rng r;
+  for (long i = 0; i < n; i++) {
+    probe_photon(&c, &p, element_index, energy, d0, e0);
+    orc_rng_init(&r, seed, 14u, (uint64_t)i);
+    p.tr = p.vr = &r;
+    orc_fluorescence(&c, &p);
+    line[i] = p.energy > 0.0 ? -p.hist[p.n_interactions][0] : 0;
+    weight[i] = p.weight;
+    if (dir)
+      for (int k = 0; k < 3; k++) {
+        dir[6 * i + k] = p.dirv[k];
+        dir[6 * i + 3 + k] = p.elecv[k];
+      }
+  }
+  orc_ctx_free_tables(&c);
+  return 1;
+}
+
+void xmi_oracle_probe_rotate(const double *dirv, const double *elecv, double theta, double phi, double *out) {
+  orc_photon p;
+  memset(&p, 0, sizeof(p));
+  for (int k = 0; k < 3; k++) {
+    p.dirv[k] = dirv[k];
+    p.elecv[k] = elecv[k];
+  }
+  p.theta = acos(p.dirv[2]);
+  p.phi = atan2(p.dirv[1], p.dirv[0]);
+  if (p.phi < 0.0) p.phi += 2.0 * ORC_PI;
+  orc_update_dirv(&p, theta, phi);
+  orc_update_elecv(&p);
+  for (int k = 0; k < 3; k++) {
+    out[k] = p.dirv[k];
+    out[3 + k] = p.elecv[k];
+  }
+}
+
+double xmi_oracle_probe_phi0(const double *dirv, const double *elecv) {
+  orc_photon p;
+  memset(&p, 0, sizeof(p));
+  for (int k = 0; k < 3; k++) {
+    p.dirv[k] = dirv[k];
+    p.elecv[k] = elecv[k];
+  }
+  p.theta = acos(p.dirv[2]);
+  p.phi = atan2(p.dirv[1], p.dirv[0]);
+  if (p.phi < 0.0) p.phi += 2.0 * ORC_PI;
+  return orc_phi0(&p);
+}
+
+int xmi_oracle_probe_source(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, int which,
+                            long n, uint64_t seed, double *out) {
+  orc_ctx c;
+  probe_ctx(&c, inputFPtr, hdf5FPtr, options);
+  const xmi_inputF *in = c.in;
+  orc_photon p;
+  orc_rng src;
+  double initial_mus[64];
+  int ok = 1;
+  const xmi_energy_discrete *d = NULL;
+  double hor_ver_ratio = 0.0;
+  if (which >= 0) {
+    if (which >= in->n_discrete) ok = 0;
+    else {
+      d = &in->discrete[which];
+      hor_ver_ratio = d->horizontal_intensity * (double)n / (d->vertical_intensity + d->horizontal_intensity);
+      mu_calc_all(&c, d->energy, initial_mus);
+    }
+  } else if (-which >= in->n_continuous) {
+    ok = 0;
+  }
+  for (long i = 0; i < n && ok; i++) {
+    memset(&p, 0, sizeof(p));
+    orc_rng_init(&src, seed, 0u, (uint64_t)i);
+    int rs = d ? orc_source_discrete(&c, d, initial_mus, 1.0, i, hor_ver_ratio, &src, &p)
+               : orc_source_continuous(&c, &in->continuous[-which - 1], &in->continuous[-which], n, &src, &p);
+    double *o = out + 10 * i;
+    if (rs < 0) ok = 0;
+    for (int k = 0; k < 3; k++) {
+      o[k] = p.coords[k];
+      o[3 + k] = p.dirv[k];
+      o[6 + k] = p.elecv[k];
+    }
+    o[9] = rs > 0 ? p.energy : NAN;
+  }
+  orc_ctx_free_tables(&c);
+  return ok;
+}
+
+/* derived fields of an initialised input (xmi_init_input), for tests */
+void xmi_oracle_input_derived(xmi_inputFPtr inputFPtr, double *out) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  out[0] = in->detector_radius;
+  out[1] = in->collimator_present;
+  out[2] = in->collimator_radius;
+  out[3] = in->half_apex;
+  for (int k = 0; k < 3; k++) {
+    out[4 + k] = in->vertex[k];
+    out[7 + k] = in->n_sample_orientation_det[k];
+    out[10 + k] = in->geometry.n_sample_orientation[k];
+    out[13 + k] = in->geometry.n_detector_orientation[k];
+    for (int j = 0; j < 3; j++) {
+      out[16 + 3 * k + j] = in->n_detector_orientation_new[k][j];
+      out[25 + 3 * k + j] = in->n_detector_orientation_inverse[k][j];
+    }
+  }
+  out[34] = in->detector_solid_angle;
+  for (int l = 0; l < in->n_layers; l++) {
+    out[35 + 3 * l] = in->layers[l].thickness_along_Z;
+    out[36 + 3 * l] = in->layers[l].Z_coord_begin;
+    out[37 + 3 * l] = in->layers[l].Z_coord_end;
+  }
 }

@@ -71,6 +71,54 @@ void xmi_oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 void xmi_oracle_free(void *p);
 
+/* ---- probes of single restated routines (tests only) --------------------
+ * Each runs one routine n times from the given state, run i on the Philox
+ * stream (seed, substream 14, history i).  The photon sits in the middle of
+ * layer 1 and interacts with element element_index of that layer. */
+
+/* Rayleigh (kind 1, src/xmi_main.F90:1986-2101) or Compton (kind 2,
+ * :2103-2229 with :4985-5069): out[7i..7i+6] = new dirv, new elecv, energy */
+int xmi_oracle_probe_scatter(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, int kind,
+                             int element_index, double energy, const double *dirv, const double *elecv, long n,
+                             uint64_t seed, double *out);
+/* photo-absorption + fluorescence (:2231-2411, :5184-5437): emitted line
+ * (0 when none), weight factor, and optionally dir[6i..] = dirv, elecv */
+int xmi_oracle_probe_fluorescence(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options,
+                                  int element_index, double energy, long n, uint64_t seed, int *line,
+                                  double *weight, double *dir);
+/* xmi_update_photon_dirv then xmi_update_photon_elecv (:5071-5182): out = dirv, elecv */
+void xmi_oracle_probe_rotate(const double *dirv, const double *elecv, double theta, double phi, double *out);
+/* azimuth of the polarisation in the photon's frame (:2034-2060) */
+double xmi_oracle_probe_phi0(const double *dirv, const double *elecv);
+/* primary photons of discrete line `which` (>= 0) or continuous interval
+ * -which-1 (< 0), history i on stream (seed, 0, i) as in the engine:
+ * out[10i..] = entry point, dirv, elecv, energy (NaN when outside the window) */
+int xmi_oracle_probe_source(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, int which,
+                            long n, uint64_t seed, double *out);
+/* derived fields after xmi_init_input: out[0..34] detector radius, collimator
+ * present/radius, half apex, vertex[3], n_sample_orientation_det[3],
+ * n_sample_orientation[3], n_detector_orientation[3], new[3][3], inverse[3][3]
+ * (row-major), detector solid angle; then per layer thickness along z,
+ * z begin, z end */
+void xmi_oracle_input_derived(xmi_inputFPtr inputFPtr, double *out);
+
+/* planted mistakes: a test sets one, shows that its check rejects the
+ * result, and resets it to XMI_ORACLE_PLANT_NONE */
+#define XMI_ORACLE_PLANT_NONE 0
+#define XMI_ORACLE_PLANT_RAYLEIGH_ENERGY 1     /* Rayleigh angle looked up at 1.15 E */
+#define XMI_ORACLE_PLANT_PHI0_SIGN 2           /* polarisation azimuth with the wrong sign */
+#define XMI_ORACLE_PLANT_COMPTON_FULL_ANGLE 3  /* Doppler shift with sin(theta) for sin(theta/2) */
+#define XMI_ORACLE_PLANT_DOPPLER_ONE_SIDED 4   /* momentum sign never flipped */
+#define XMI_ORACLE_PLANT_CK_L1_SWAP 5          /* FL12 and FL13 exchanged */
+#define XMI_ORACLE_PLANT_LINE_SHIFT 6          /* radiative rate of the next line */
+#define XMI_ORACLE_PLANT_ELECV_STALE 7         /* polarisation not re-orthogonalised */
+#define XMI_ORACLE_PLANT_DIRV_PHI_SIGN 8       /* rotation by -phi */
+#define XMI_ORACLE_PLANT_SLIT_FULL 9           /* slit half-angle not halved */
+#define XMI_ORACLE_PLANT_GAUSS_XY 10           /* x divergence used for y */
+#define XMI_ORACLE_PLANT_COMPTON_T_RAYLEIGH 11 /* Compton azimuth with the Rayleigh parameter */
+#define XMI_ORACLE_PLANT_SHELL_NEXT 12         /* shell drawn with the next shell's cross section */
+void xmi_oracle_set_plant(int id);
+
 /* trials of the Monte Carlo solid-angle fallback inside the estimator
  * (the reference's hits_per_single), default 5000 */
 void xmi_oracle_set_hits_per_single(long hits);

@@ -1,0 +1,542 @@
+"""Pins of the oracle's transport routines against what the mathematics
+fixes, each with a planted mistake (set in the oracle, on the CPU) that the
+same check rejects.
+
+Routines and the reference code they restate:
+  rotation and polarisation frame  src/xmi_main.F90:5071-5182, 2034-2060
+  Rayleigh angles                  src/xmi_main.F90:1986-2101
+  Compton angles and energy        src/xmi_main.F90:2103-2229, 4985-5069
+  shell, Coster-Kronig, line       src/xmi_main.F90:2231-2411, 5184-5437,
+                                   src/xmi_variance_reduction.F90:728-750
+  corrected fluorescence yields    src/xmi_data_f.F90:1243-1305
+  source direction and beam        src/xmi_main.F90:957-1186, 579-716
+  xmi_init_input                   src/xmi_main.F90:1741-1918
+  Compton escape ratios            src/xmi_main.F90:5640-5760
+
+The distributions are computed here by quadrature of the provider's
+differential cross sections and Compton profiles (the physics the tables
+are built from), by a Markov-chain form of the vacancy transfers, or by
+closed forms; none repeats the oracle's formulas."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from xmimsim_amd import api
+from xmimsim_amd import lines as L
+
+MEC2 = 510.998910          # keV
+HC = 12.39842              # keV * Angstrom
+LAMBDA_C = 0.0242631       # Compton wavelength, Angstrom
+ALPHA = 1.0 / 137.035999   # fine-structure constant: atomic unit of momentum / (m c)
+
+
+def _prep(Z, energy=20.0, **kw):
+    inp = api.Input(layers=[api.Layer([Z], [1.0], 2.0, 0.1)], discrete=[api.DiscreteLine(energy, 1e9, 0.0)],
+                    n_photons_line=10, **kw)
+    return api.Prepared(inp)
+
+
+@pytest.fixture(scope="module")
+def prep_fe():
+    p = _prep(26)
+    yield p
+    p.close()
+
+
+def _unit(v):
+    v = np.asarray(v, float)
+    return v / np.linalg.norm(v)
+
+
+def _perp(d, v):
+    v = np.asarray(v, float) - np.dot(v, d) * d
+    return v / np.linalg.norm(v)
+
+
+# a direction off every axis and a polarisation at a generic azimuth in its frame
+D0 = _unit([0.3, -0.4, 0.866])
+E0 = _perp(D0, [0.2, 0.9, 0.1])
+
+
+def _frame(d):
+    """e_theta, e_phi of the spherical frame at direction d"""
+    th, ph = math.acos(d[2]), math.atan2(d[1], d[0])
+    e_t = np.array([math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th)])
+    e_p = np.array([-math.sin(ph), math.cos(ph), 0.0])
+    return e_t, e_p
+
+
+def _rel_angles(d, e, dn):
+    """polar angle of the new directions dn about d and their azimuth measured
+    from the polarisation e towards d x e"""
+    b = np.cross(d, e)
+    cth = np.clip(dn @ d, -1.0, 1.0)
+    phi = np.arctan2(dn @ b, dn @ e)
+    return np.arccos(cth), np.mod(phi, 2 * np.pi)
+
+
+def _theta_cdf(kind, Z, E):
+    """CDF of the scattering angle: quadrature of 2 pi sin(t) dsigma/dOmega"""
+    t = np.linspace(0.0, math.pi, 20001)
+    f = np.array([L.dcs(kind, Z, E, x) for x in t]) * np.sin(t)
+    c = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(t))])
+    c /= c[-1]
+    return lambda x: np.interp(x, t, c)
+
+
+def _phi_pit(phi, t):
+    """probability-integral transform of the azimuth whose CDF is
+    (phi - t sin 2phi) / 2 pi, i.e. density (1 - 2 t cos 2phi) / 2 pi"""
+    return (phi - t * np.sin(2 * phi)) / (2 * math.pi)
+
+
+def _phi_moment_z(phi, t):
+    """that density has E[cos 2phi] = -t and E[cos^2 2phi] = 1/2: the
+    standardised sum of cos 2phi + t"""
+    return np.sum(np.cos(2 * phi) + t) / math.sqrt(np.sum(0.5 - t * t))
+
+
+# ---------------------------------------------------------------- rotation
+
+
+def test_rotation_frame_closed_form(orc):
+    rng = np.random.default_rng(5)
+    cases = []
+    for _ in range(200):
+        d = _unit(rng.normal(size=3))
+        e = _perp(d, rng.normal(size=3))
+        cases.append((d, e, rng.uniform(0.01, math.pi - 0.01), rng.uniform(0, 2 * math.pi)))
+    cases.append((np.array([0.0, 0.0, 1.0]), np.array([0.0, 1.0, 0.0]), 0.7, 1.1))
+
+    def check():
+        bad = 0
+        for d, e, th, ph in cases:
+            dn, en = orc.probe_rotate(d, e, th, ph)
+            e_t, e_p = _frame(d)
+            az = math.atan2(dn @ e_p, dn @ e_t) % (2 * math.pi)
+            proj = e - (e @ dn) * dn
+            ok = (abs(np.linalg.norm(dn) - 1) < 1e-12 and abs(dn @ d - math.cos(th)) < 1e-10
+                  and min(abs(az - ph), 2 * math.pi - abs(az - ph)) < 1e-8
+                  and abs(en @ dn) < 1e-10 and abs(np.linalg.norm(en) - 1) < 1e-12
+                  and np.linalg.norm(np.cross(en, proj)) < 1e-9 and en @ e > 0)
+            bad += not ok
+        return bad
+
+    assert check() == 0
+    for plant in ("dirv_phi_sign", "elecv_stale"):
+        with orc.planted(plant):
+            assert check() > 50, plant
+
+
+def test_polarisation_azimuth_closed_form(orc):
+    """phi0 is the azimuth of the polarisation in the photon's frame, so a
+    rotation by (theta, phi0 + phi) leaves the new direction at azimuth phi
+    from the polarisation"""
+    rng = np.random.default_rng(6)
+
+    def check():
+        bad = 0
+        for _ in range(200):
+            d = _unit(rng.normal(size=3))
+            e = _perp(d, rng.normal(size=3))
+            e_t, e_p = _frame(d)
+            phi0 = orc.probe_phi0(d, e)
+            bad += abs(math.cos(phi0) - e @ e_t) > 1e-9 or abs(math.sin(phi0) - e @ e_p) > 1e-9
+            th, ph = rng.uniform(0.1, 3.0), rng.uniform(0, 2 * math.pi)
+            dn, _ = orc.probe_rotate(d, e, th, phi0 + ph)
+            t, az = _rel_angles(d, e, dn[None, :])
+            bad += abs(t[0] - th) > 1e-8 or min(abs(az[0] - ph), 2 * math.pi - abs(az[0] - ph)) > 1e-8
+        return bad
+
+    assert check() == 0
+    with orc.planted("phi0_sign"):
+        assert check() > 100
+
+
+# ---------------------------------------------------------------- Rayleigh
+
+
+def test_rayleigh_angles_follow_the_polarised_cross_section(orc, prep_fe):
+    """theta ~ 2 pi sin(theta) DCS_Rayl(E, theta) (quadrature of the
+    provider's cross section); phi given theta ~ 1 - sin^2 theta cos^2 phi
+    measured from the polarisation (Thomson), i.e. CDF (phi - t sin 2phi)/2pi
+    with t = s / (4 - 2 s), s = sin^2 theta"""
+    E, n = 20.0, 200_000
+    cdf = _theta_cdf("rayl", 26, E)
+
+    def sample():
+        dn, en, energy = orc.probe_scatter(prep_fe, "rayl", 0, E, D0, E0, n, seed=3)
+        assert np.all(energy == E)
+        assert np.allclose(np.einsum("ij,ij->i", dn, en), 0.0, atol=1e-10)
+        return _rel_angles(D0, E0, dn)
+
+    th, ph = sample()
+    assert stats.kstest(th, cdf).pvalue > 1e-3
+    s = np.sin(th) ** 2
+    assert stats.kstest(_phi_pit(ph, s / (4 - 2 * s)), "uniform").pvalue > 1e-3
+    assert abs(_phi_moment_z(ph, s / (4 - 2 * s))) < 4
+    with orc.planted("rayleigh_energy"):
+        th, _ = sample()
+        assert stats.kstest(th, cdf).pvalue < 1e-6
+    with orc.planted("phi0_sign"):
+        th, ph = sample()
+        s = np.sin(th) ** 2
+        assert stats.kstest(_phi_pit(ph, s / (4 - 2 * s)), "uniform").pvalue < 1e-6
+
+
+# ---------------------------------------------------------------- Compton
+
+
+def _two_sided_profile_cdf(Z):
+    """CDF on (-inf, inf) of the total Compton profile: the sum over subshells
+    of occupancy * 8/(3 pi p) (1 + (q/p)^2)^-3, integrated in closed form"""
+    a = L.atom(Z)
+    occ, p = a["occupancy"], a["profile_p"]
+    m = occ > 0
+
+    def G(u):  # int_0^u (1 + t^2)^-3 dt
+        return u / (4 * (1 + u * u) ** 2) + 3 * u / (8 * (1 + u * u)) + 3 * np.arctan(u) / 8
+
+    def F(q):
+        q = np.asarray(q, float)
+        half = sum(o * 8 / (3 * math.pi * pp) * pp * G(np.abs(q) / pp) for o, pp in zip(occ[m], p[m]))
+        return 0.5 + np.sign(q) * half / occ[m].sum()
+    return F
+
+
+def _compton_pz(E, E1, th):
+    """projected momentum (atomic units) from the first-order Doppler relation
+    lambda' - lambda = lambda_C (1 - cos theta) - 2 lambda sin(theta/2) alpha p_z"""
+    lam, lam1 = HC / E, HC / E1
+    return (LAMBDA_C * (1 - np.cos(th)) - (lam1 - lam)) / (2 * lam * np.sin(th / 2) * ALPHA)
+
+
+def test_compton_angles_follow_the_polarised_klein_nishina(orc):
+    """theta ~ 2 pi sin(theta) DCS_Compt(E, theta); phi given theta ~
+    K + 1/K - 2 sin^2 theta cos^2 phi (polarised Klein-Nishina), whose CDF is
+    (phi - t sin 2phi)/2pi with t = s / (2 (K + 1/K - s)) and
+    1/K = 1 + E (1 - cos theta) / m c^2.  At 150 keV the Rayleigh parameter
+    s / (4 - 2 s) differs enough to be rejected."""
+    E, n = 150.0, 200_000
+    prep = _prep(26, energy=E)
+    try:
+        cdf = _theta_cdf("compt", 26, E)
+
+        def sample():
+            dn, en, _ = orc.probe_scatter(prep, "compt", 0, E, D0, E0, n, seed=4)
+            return _rel_angles(D0, E0, dn)
+
+        def t_of(th):
+            s = np.sin(th) ** 2
+            k = 1 + E * (1 - np.cos(th)) / MEC2
+            return s / (2 * (k + 1 / k - s))
+
+        def pit(th, ph):
+            return _phi_pit(ph, t_of(th))
+
+        th, ph = sample()
+        assert stats.kstest(th, cdf).pvalue > 1e-3
+        assert stats.kstest(pit(th, ph), "uniform").pvalue > 1e-3
+        assert abs(_phi_moment_z(ph, t_of(th))) < 4
+        with orc.planted("compton_t_rayleigh"):
+            th, ph = sample()
+            assert abs(_phi_moment_z(ph, t_of(th))) > 6
+        with orc.planted("phi0_sign"):
+            assert stats.kstest(pit(*sample()), "uniform").pvalue < 1e-6
+    finally:
+        prep.close()
+
+
+def test_compton_energy_follows_the_profile(orc, prep_fe):
+    """The scattered energy carries a projected momentum p_z (recovered with
+    the textbook first-order Doppler relation) distributed as the total
+    Compton profile J(p_z) of both signs, truncated where E' would exceed E
+    (the reference redraws those): given theta, F_J(p_z) / F_J(p_max(theta))
+    is uniform.  At p_z = 0 the energy is the Compton line E/(1 + E/mc2 (1-cos))."""
+    E, n = 20.0, 200_000
+    F = _two_sided_profile_cdf(26)
+
+    def pit():
+        dn, _, E1 = orc.probe_scatter(prep_fe, "compt", 0, E, D0, E0, n, seed=8)
+        th = np.arccos(np.clip(dn @ D0, -1, 1))
+        keep = th > 0.05            # the Doppler term vanishes in the forward direction
+        th, E1 = th[keep], E1[keep]
+        assert np.all(E1 <= E)
+        pz = _compton_pz(E, E1, th)
+        pmax = _compton_pz(E, np.full_like(E1, E), th)
+        return F(pz) / F(pmax), pz, th, E1
+
+    u, pz, th, E1 = pit()
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    # the energy at the profile's median momentum is the Compton line
+    sel = np.abs(pz) < 0.005
+    assert sel.sum() > 100
+    line = E / (1 + E / MEC2 * (1 - np.cos(th[sel])))
+    assert np.allclose(E1[sel], line, rtol=1e-4)
+    with orc.planted("doppler_one_sided"):
+        assert stats.kstest(pit()[0], "uniform").pvalue < 1e-6
+    with orc.planted("compton_full_angle"):
+        assert stats.kstest(pit()[0], "uniform").pvalue < 1e-6
+
+
+# ---------------------------------------------------------------- fluorescence
+
+
+CK_EDGES = {  # (from, to): index in the provider's Coster-Kronig array (xraylib order)
+    (1, 2): 0, (1, 3): 1, (2, 3): 3, (4, 5): 4, (4, 6): 5, (4, 7): 6, (4, 8): 7,
+    (5, 6): 8, (5, 7): 9, (5, 8): 10, (6, 7): 11, (6, 8): 12, (7, 8): 13}
+
+
+def _vacancy_chain(a):
+    """Markov chain of the Coster-Kronig transfers over the shells K..M5:
+    N = (I - C)^-1 counts the visits of each shell, N diag(1 - rowsum C) the
+    shell a vacancy ends in"""
+    C = np.zeros((9, 9))
+    for (i, j), k in CK_EDGES.items():
+        C[i, j] = a["ck"][k]
+    N = np.linalg.inv(np.eye(9) - C)
+    return N, N * (1 - C.sum(axis=1))[None, :]
+
+
+@pytest.mark.parametrize("Z", [26, 56, 79])
+def test_corrected_fluorescence_yields(orc, Z):
+    """FluorYieldsCorr[s] = sum over shells s' of (visits of s' from s) * FY[s']"""
+    prep = _prep(Z)
+    try:
+        fyc = L.tables_of(prep)["FluorYieldsCorr"]
+        a = L.atom(Z)
+        N, _ = _vacancy_chain(a)
+        assert np.allclose(fyc, N @ a["fluor_yield"], rtol=1e-12, atol=1e-15)
+    finally:
+        prep.close()
+
+
+def _line_probabilities(Z, E, max_shell=8):
+    """P(line) per history and the expected weight factor per line: shell by
+    the partial photo cross sections tau_s / tau, end shell by the vacancy
+    chain, line by the radiative rates of the end shell; weight FYcorr of
+    the first shell"""
+    a = L.atom(Z)
+    tau = np.array([a["photo_B"][s] * E ** -2.75 if 0 < a["edge"][s] <= E else 0.0 for s in range(9)])
+    P = tau / L.cs_photo_total(Z, E)
+    P[max_shell + 1:] = 0.0
+    N, R = _vacancy_chain(a)
+    fyc = N @ a["fluor_yield"]
+    end = P @ R                 # probability of ending in each shell
+    wend = (P * fyc) @ R        # the same weighted by the first shell's FYcorr
+    prob = np.zeros(384)
+    wprob = np.zeros(384)
+    for s, (f, l) in enumerate(L.SHELL_LINES):
+        rr = a["rad_rate"][f:l + 1]
+        prob[f:l + 1] = end[s] * rr
+        wprob[f:l + 1] = wend[s] * rr
+    prob[0] = 1 - prob.sum()
+    return prob, wprob
+
+
+@pytest.mark.parametrize("Z,E", [(26, 20.0), (56, 40.0), (79, 90.0)])
+def test_fluorescence_lines_and_weights(orc, Z, E):
+    n = 400_000
+    prep = _prep(Z, energy=E)
+    try:
+        want, wwant = _line_probabilities(Z, E)
+
+        def chi2_p(line):
+            cnt = np.bincount(line, minlength=384)[:384]
+            exp = want * n
+            m = exp > 5
+            chi2 = ((cnt[m] - exp[m]) ** 2 / exp[m]).sum() + ((cnt[~m].sum() - exp[~m].sum()) ** 2 /
+                                                              max(exp[~m].sum(), 1e-300)) * (exp[~m].sum() > 5)
+            return stats.chi2.sf(chi2, m.sum())
+
+        line, w = orc.probe_fluorescence(prep, 0, E, n, seed=9)
+        assert np.all(line[line > 0] <= 219)
+        assert chi2_p(line) > 1e-3
+        # weight factor summed per line against its expectation (4 sigma of the count)
+        wsum = np.bincount(line, weights=w, minlength=384)[:384]
+        big = want > 2e-3
+        big[0] = False
+        sig = np.sqrt(want[big] * n) * wwant[big] / want[big]
+        assert np.all(np.abs(wsum[big] - wwant[big] * n) < 4 * sig + 1e-9)
+        for plant in ("line_shift", "shell_next") + (("ck_l1_swap",) if Z != 26 else ()):
+            with orc.planted(plant):
+                assert chi2_p(orc.probe_fluorescence(prep, 0, E, n, seed=9)[0]) < 1e-6, plant
+    finally:
+        prep.close()
+
+
+def test_fluorescence_is_isotropic(orc, prep_fe):
+    n = 200_000
+    line, _, d, e = orc.probe_fluorescence(prep_fe, 0, 20.0, n, seed=10, directions=True)
+    d, e = d[line > 0], e[line > 0]
+    m = len(d)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-12)
+    assert np.all(np.abs(d.mean(axis=0)) < 4 / math.sqrt(3 * m))
+    # E[d_i^2] = 1/3, E[d_i^4] = 1/5 on the sphere
+    assert np.all(np.abs((d ** 2).mean(axis=0) - 1 / 3) < 4 * math.sqrt((1 / 5 - 1 / 9) / m))
+    assert np.allclose(np.einsum("ij,ij->i", d, e), 0.0, atol=1e-10)
+
+
+# ---------------------------------------------------------------- source
+
+
+def test_slit_directions_and_entry_points(orc):
+    sx, sy, dss = 0.012, 0.004, 80.0
+    inp = api.Input(layers=[api.Layer([26], [1.0], 2.0, 0.1)], discrete=[api.DiscreteLine(20.0, 3e8, 1e8)],
+                    n_photons_line=10, slit_size_x=sx, slit_size_y=sy, d_source_slit=dss,
+                    n_sample_orientation=(0.1, -0.6, 0.8))
+    prep = api.Prepared(inp)
+    n = 100_000
+    try:
+        def angles():
+            pt, d, e, en = orc.probe_source(prep, 0, n, seed=12)
+            return pt, d, e, en, np.arctan(d[:, 0] / d[:, 2]), np.arctan(d[:, 1] / d[:, 2])
+
+        pt, d, e, en, ax, ay = angles()
+        xm, ym = math.atan(sx / dss / 2), math.atan(sy / dss / 2)
+        assert stats.kstest(ax, "uniform", args=(-xm, 2 * xm)).pvalue > 1e-3
+        assert stats.kstest(ay, "uniform", args=(-ym, 2 * ym)).pvalue > 1e-3
+        assert np.all(en == 20.0)
+        # entry point on the sample's first plane, on the ray from the source
+        der = oracle_derived(orc, prep)
+        nrm = der["n_sample_orientation"]
+        zb = der["Z_coord_begin"][0]
+        assert np.allclose((pt - np.array([0, 0, zb])) @ nrm, 0.0, atol=1e-9)
+        assert np.allclose(np.cross(pt, d), 0.0, atol=1e-9)
+        # polarisation: perpendicular to the direction; the first 3/4 of the
+        # photons horizontal (electric vector close to y), the rest close to x
+        assert np.allclose(np.einsum("ij,ij->i", d, e), 0.0, atol=1e-12)
+        k = int(n * 0.75)
+        assert np.all(np.abs(e[:k, 1]) > 0.999) and np.all(np.abs(e[k:, 0]) > 0.999)
+        with orc.planted("slit_full"):
+            ax = angles()[4]
+            assert stats.kstest(ax, "uniform", args=(-xm, 2 * xm)).pvalue < 1e-6
+    finally:
+        prep.close()
+
+
+def oracle_derived(orc, prep):
+    return orc.input_derived(prep)
+
+
+def test_gaussian_beam_moments_and_line_shapes(orc):
+    sx, sxp, sy, syp, dss = 0.01, 1e-3, 0.03, 4e-3, 100.0
+    lines = [api.DiscreteLine(20.0, 1e9, 0.0, sx, sxp, sy, syp),
+             api.DiscreteLine(30.0, 1e9, 0.0, distribution_type=1, scale_parameter=0.5),   # Gaussian
+             api.DiscreteLine(30.0, 1e9, 0.0, distribution_type=2, scale_parameter=0.2)]   # Lorentzian
+    inp = api.Input(layers=[api.Layer([26], [1.0], 2.0, 0.1)], discrete=lines, n_photons_line=10,
+                    d_source_slit=dss, n_sample_orientation=(0.0, 0.0, 1.0))
+    prep = api.Prepared(inp)
+    n = 200_000
+    try:
+        def moments():
+            pt, d, _, _ = orc.probe_source(prep, 0, n, seed=13)
+            ax, ay = np.arctan(d[:, 0] / d[:, 2]), np.arctan(d[:, 1] / d[:, 2])
+            p0 = pt - (pt[:, 2] / d[:, 2])[:, None] * d          # back to the source plane z = 0
+            return ax.std(), ay.std(), p0[:, 0].std(), p0[:, 1].std()
+
+        # x0 = N(0, sx) - D sin(x1), x1 ~ N(0, sxp): var = sx^2 + D^2 (1 - exp(-2 sxp^2)) / 2
+        want = (sxp, syp, math.sqrt(sx ** 2 + dss ** 2 * (1 - math.exp(-2 * sxp ** 2)) / 2),
+                math.sqrt(sy ** 2 + dss ** 2 * (1 - math.exp(-2 * syp ** 2)) / 2))
+        got = moments()
+        assert np.allclose(got, want, rtol=0.015), (got, want)
+        with orc.planted("gauss_xy"):
+            got = moments()
+            assert abs(got[1] / want[1] - 1) > 0.5
+        _, _, _, eg = orc.probe_source(prep, 1, n, seed=14)
+        assert abs(eg.mean() - 30.0) < 4 * 0.5 / math.sqrt(n) and abs(eg.std() / 0.5 - 1) < 0.01
+        _, _, _, el = orc.probe_source(prep, 2, n, seed=15)
+        el = el[np.isfinite(el)]
+        q1, q2, q3 = np.percentile(el, [25, 50, 75])
+        assert abs(q2 - 30.0) < 0.01 and abs((q3 - q1) / 0.4 - 1) < 0.02   # Cauchy: IQR = 2 gamma
+    finally:
+        prep.close()
+
+
+# ---------------------------------------------------------------- xmi_init_input
+
+
+def test_init_input_derived_geometry(orc):
+    """Every field xmi_init_input derives, against closed forms: detector
+    radius sqrt(A/pi); conical collimator radius, half apex and vertex;
+    right-handed detector frame whose x axis is the detector normal and whose
+    y axis is e_y x x; its inverse; on-axis solid angle 2 pi (1 - cos atan(R/D));
+    sample normal (z >= 0) in detector coordinates; layers stacked along z
+    from the reference layer, which starts at the sample-source distance"""
+    n_s = np.array([0.2, -0.5, -0.7])             # flipped to z >= 0 by xmi_init_input
+    n_d = np.array([0.1, -1.0, 0.2])
+    pdet = np.array([0.5, 2.0, 95.0])
+    layers = [api.Layer([8], [1.0], 0.001, 0.3), api.Layer([26], [1.0], 7.9, 0.05), api.Layer([14], [1.0], 2.3, 0.2)]
+    inp = api.Input(layers=layers, reference_layer=2, discrete=[api.DiscreteLine(20.0, 1e9)], n_photons_line=10,
+                    n_sample_orientation=tuple(n_s), n_detector_orientation=tuple(n_d), p_detector_window=tuple(pdet),
+                    area_detector=0.5, collimator_height=0.4, collimator_diameter=0.3, d_sample_source=90.0)
+    prep = api.Prepared(inp)
+    try:
+        g = orc.input_derived(prep)
+        R = math.sqrt(0.5 / math.pi)
+        assert g["detector_radius"] == pytest.approx(R, rel=1e-15)
+        assert g["collimator_present"] == 1 and g["collimator_radius"] == pytest.approx(0.15)
+        ha = math.atan((R - 0.15) / 0.4)
+        assert g["half_apex"] == pytest.approx(ha, rel=1e-14)
+        assert np.allclose(g["vertex"], [R / math.tan(ha), 0, 0], rtol=1e-14)
+        x = _unit(n_d)
+        y = _unit(np.cross([0, 1, 0], x))
+        M = np.column_stack([x, y, np.cross(x, y)])
+        assert np.allclose(g["new"], M, atol=1e-14)
+        assert np.linalg.det(g["new"]) == pytest.approx(1.0, abs=1e-12)
+        assert np.allclose(g["inverse"] @ g["new"], np.eye(3), atol=1e-12)
+        ns = -_unit(n_s)
+        assert np.allclose(g["n_sample_orientation"], ns, atol=1e-15)
+        assert np.allclose(g["n_sample_orientation_det"], M.T @ ns, atol=1e-12)
+        D = np.linalg.norm(pdet - [0, 0, 90.0])
+        assert g["detector_solid_angle"] == pytest.approx(2 * math.pi * (1 - math.cos(math.atan(R / D))), rel=1e-13)
+        tz = np.array([0.3, 0.05, 0.2]) / abs(ns[2])
+        assert np.allclose(g["thickness_along_Z"], tz, rtol=1e-14)
+        zb = 90.0 + np.array([-tz[0], 0.0, tz[1]])
+        assert np.allclose(g["Z_coord_begin"], zb, rtol=1e-14)
+        assert np.allclose(g["Z_coord_end"], zb + tz, rtol=1e-14)
+        # a plausible misreading (stacking from the first layer) is rejected
+        assert not np.allclose(g["Z_coord_begin"], 90.0 + np.concatenate([[0.0], np.cumsum(tz)[:-1]]))
+    finally:
+        prep.close()
+
+
+# ---------------------------------------------------------------- Compton escape
+
+
+def test_compton_escape_ratio_closed_form(orc):
+    """Escape-ratio run (one forced interaction, analogue exit): the Compton
+    escape ratio summed over output energies is sigma_C/mu times the escape
+    probability averaged over the Compton angular distribution and the
+    truncated exponential depth law of the forced interaction; normal
+    incidence, slab of thickness t, exit attenuation at the pz = 0 energy"""
+    Zc, rho, t = 14, 2.33, 0.02
+    inp = api.Input(layers=[api.Layer([26], [1.0], 7.9, 0.5)], discrete=[api.DiscreteLine(20.0, 1e9, 0.0)],
+                    crystal=[api.Layer([Zc], [1.0], rho, t)])
+    o = api.default_options(use_M_lines=0, use_cascade_auger=0, use_cascade_radiative=0, use_variance_reduction=0,
+                            escape_ratios_mode=1)
+    prep = api.Prepared(api.escape_input(inp), options=o)
+    try:
+        for E in (15.0, 30.0):
+            n = 200_000
+            _, comp = orc.escape_ratios(prep, n, 1, E, 0.1, 600, 0.05, 0.05, seed=21)
+            got = comp[0].sum()
+            mu = L.cs_total(Zc, E) * rho
+            th = np.linspace(1e-4, math.pi - 1e-4, 4001)
+            w = np.array([L.dcs("compt", Zc, E, x) for x in th]) * np.sin(th)
+            E1 = E / (1 + E / MEC2 * (1 - np.cos(th)))
+            mu1 = np.array([L.cs_total(Zc, x) for x in E1]) * rho
+            c = np.cos(th)
+            # depth z with density mu e^{-mu z} / (1 - e^{-mu t}) on [0, t]; exit path (t - z)/c forward, z/|c| back
+            a = mu1 / np.abs(c)
+            norm = 1 - math.exp(-mu * t)
+            fwd = mu * (math.exp(-mu * t) - np.exp(-a * t)) / (a - mu) / norm
+            bwd = mu * -np.expm1(-(a + mu) * t) / (a + mu) / norm
+            pesc = np.where(c > 0, fwd, bwd)
+            want = L.cs("compt", Zc, E) / L.cs_total(Zc, E) * np.trapezoid(w * pesc, th) / np.trapezoid(w, th)
+            sig = math.sqrt(want * (1 - want) / n)
+            assert abs(got - want) < 4 * sig + 0.01 * want, (E, got, want)
+    finally:
+        prep.close()

@@ -41,10 +41,35 @@ def test_struct_layouts_match_reference_c_types():
     assert C.sizeof(api.xmi_solid_angle) == 6 * 8
 
 
-def test_prepare_and_geometry(lib):
-    prep = api.Prepared(workloads.single_layer_steel(n_photons_line=10))
-    # detector radius sqrt(A/pi), on-axis solid angle 2 pi (1 - cos atan(R/D))
-    assert prep.inputF
+def test_prepare_and_geometry(lib, orc):
+    """xmi_init_input on the steel input: detector radius sqrt(A/pi) and the
+    on-axis solid angle 2 pi (1 - cos atan(R/D)) with D the distance from the
+    sample point (0, 0, d_sample_source) to the window (the general case,
+    with collimator, tilted sample and a middle reference layer, is pinned in
+    test_cpu_oracle_pins.py)"""
+    inp = workloads.single_layer_steel(n_photons_line=10)
+    prep = api.Prepared(inp)
+    g = orc.input_derived(prep)
+    R = math.sqrt(inp.area_detector / math.pi)
+    assert g["detector_radius"] == pytest.approx(R, rel=1e-15)
+    D = math.dist(inp.p_detector_window, (0.0, 0.0, inp.d_sample_source))
+    assert g["detector_solid_angle"] == pytest.approx(2 * math.pi * (1 - math.cos(math.atan(R / D))), rel=1e-13)
+    assert g["collimator_present"] == 0
+    assert g["Z_coord_begin"][0] == pytest.approx(inp.d_sample_source)
+    assert g["Z_coord_end"][0] == pytest.approx(inp.d_sample_source + 0.5 / math.sqrt(0.5), rel=1e-12)
+    prep.close()
+
+
+def test_interaction_order_bound_is_an_error(lib):
+    """The random-stream counter keeps the interaction order in 8 bits:
+    more than 255 orders per trajectory is refused (before any device work)"""
+    prep = api.Prepared(workloads.single_layer_steel(n_photons_line=10, n_interactions=256))
+    grid = api.xmi_solid_angle()
+    with pytest.raises(RuntimeError, match="n_interactions_trajectory must be in 1..255"):
+        prep.solid_angle = C.pointer(grid)
+        prep._sa_owner = "test"
+        prep.run(seed=1)
+    prep.solid_angle = None
     prep.close()
 
 

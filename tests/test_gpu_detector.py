@@ -106,6 +106,22 @@ def test_pileup_sum_peak_ratio():
     prep.close()
 
 
+def test_pileup_train_limit_is_an_error():
+    """Pulse trains longer than 100 pulses stop the reference
+    (src/xmi_detector_f.F90:152-156); here the call fails with that reason
+    and returns no spectra."""
+    inp = workloads.single_layer_steel(n_photons_line=10)
+    inp.pulse_width = 1e-4   # 100 mean gaps at 1e6 counts/s: every train overflows
+    inp.det_absorbers = []
+    inp.crystal = []
+    prep = api.Prepared(inp)
+    ch = np.zeros((2, inp.nchannels))
+    ch[1, 200] = 1e6
+    with pytest.raises(RuntimeError, match="pulse train maximum"):
+        api.detector_response(prep, ch, options=_opts(sum_peaks=1))
+    prep.close()
+
+
 def test_escape_ratios_match_oracle():
     """xmi_escape_ratios_calculation on the GPU (which builds the crystal's
     escape input itself) against the oracle's restatement run on the escape

@@ -154,6 +154,20 @@ def test_bit_reproducible():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+def test_grid_edit_forces_upload():
+    """The device copy of the solid-angle grid is reused only while every
+    cell and both axes are unchanged: an in-place edit of one cell is
+    uploaded again and changes nothing else"""
+    prep = _prep(workloads.single_layer_steel(n_photons_line=2000))
+    assert prep.run(seed=5)[3].grid_uploads == 0     # the grid computed on the device is current
+    assert prep.run(seed=5)[3].grid_uploads == 0
+    sa = prep.solid_angle.contents
+    sa.solid_angles[1] = sa.solid_angles[1] * (1.0 + 1e-12)
+    assert prep.run(seed=5)[3].grid_uploads == 1
+    assert prep.run(seed=5)[3].grid_uploads == 0
+    prep.close()
+
+
 def test_live_time_and_mpi_hosts_scaling():
     inp = workloads.single_layer_steel(n_photons_line=4000)
     prep = _prep(inp)

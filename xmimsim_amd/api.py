@@ -101,7 +101,8 @@ class xmi_msim_stats(C.Structure):
                 ("launches", C.c_int64), ("vr_kernel_ms", C.c_double), ("vr_events", C.c_int64),
                 ("step_kernel_ms", C.c_double), ("interaction_kernel_ms", C.c_double),
                 ("solid_angle_fallbacks", C.c_int64), ("vr_geom_ms", C.c_double), ("vr_cone_ms", C.c_double),
-                ("vr_scatter_ms", C.c_double), ("vr_lines_ms", C.c_double)]
+                ("vr_scatter_ms", C.c_double), ("vr_lines_ms", C.c_double),
+                ("grid_uploads", C.c_int64)]
 
 
 class xmi_error(C.Structure):
@@ -577,11 +578,12 @@ def detector_response(prep: "Prepared", channels: np.ndarray, var_red: Optional[
     lib.xmi_detector_convolute_all(prep.inputF, noconv, conv, None, vr_ptr, C.byref(opts),
                                    C.byref(escape_ratios) if escape_ratios is not None else None, n1 - 1,
                                    1 if zero_interaction else 0)
+    if not all(conv[k] for k in range(n1)):
+        raise RuntimeError("xmi_detector_convolute_all failed: " + last_error())
     out = np.zeros((n1, nch))
     for k in range(n1):
-        if conv[k]:
-            out[k] = np.ctypeslib.as_array(conv[k], shape=(nch,))
-            lib.xmi_free(conv[k])
+        out[k] = np.ctypeslib.as_array(conv[k], shape=(nch,))
+        lib.xmi_free(conv[k])
     return out
 
 

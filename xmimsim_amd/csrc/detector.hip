@@ -279,6 +279,7 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
   const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
   const int nch = in->detector.nchannels;
   const int n_int = in->general.n_interactions_trajectory;
+  for (int n = 0; n <= n_interactions_all; n++) channels_conv[n] = nullptr;
   DetResp d;
   d.zero = in->detector.zero;
   d.gain = in->detector.gain;
@@ -295,6 +296,7 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
   unsigned long long *dhist = nullptr;
   unsigned int *dover = nullptr;
   hipStream_t st = nullptr;
+  int failed = 1; /* cleared once every order is convoluted */
   const int nb = (nch + 255) / 256;
   DET_TRY(hipStreamCreate(&st));
   DET_TRY(hipMalloc(&dsrc, nch * sizeof(double)));
@@ -331,8 +333,16 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
                            seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(n + 1)), dhist, dover);
         DET_TRY(hipGetLastError());
         std::vector<unsigned long long> h(nch);
+        unsigned int over = 0;
         DET_TRY(hipMemcpyAsync(h.data(), dhist, nch * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        DET_TRY(hipMemcpyAsync(&over, dover, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         DET_TRY(hipStreamSynchronize(st));
+        if (over) { /* src/xmi_detector_f.F90:152-156 stops the program here */
+          xmi_set_error("detector response: pulse train maximum (100 pulses) reached %u times; adjust the pulse "
+                        "width or disable the pile-up generation", over);
+          failed = 1;
+          goto done;
+        }
         for (int i = 0; i < nch; i++) temp[i] = (double)h[i];
       }
     }
@@ -348,6 +358,7 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
     DET_TRY(hipStreamSynchronize(st));
   }
   if (!zero_interaction) channels_conv[0] = (double *)calloc(nch, sizeof(double));
+  failed = 0;
   /* net-line histories: detector absorption at the line energies */
   for (double *hist : {var_red_history, brute_history}) {
     if (!hist) continue;
@@ -369,6 +380,11 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
     }
   }
 done:
+  if (failed) /* no partial result: every order's pointer is NULL after a failure */
+    for (int n = 0; n <= n_interactions_all; n++) {
+      free(channels_conv[n]);
+      channels_conv[n] = nullptr;
+    }
   if (dsrc) (void)hipFree(dsrc);
   if (dnorm) (void)hipFree(dnorm);
   if (dconv) (void)hipFree(dconv);

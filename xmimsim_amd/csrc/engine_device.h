@@ -25,6 +25,9 @@
 #include "xrl_model.h"
 
 #define ENG_MAX_LAYERS 16
+/* interaction orders per trajectory: the Philox counter word of a history
+ * stream keeps the order in 8 bits (philox.h) */
+#define ENG_MAX_INTERACTIONS 255
 #define ENG_MAX_ACT 96          /* active fluorescence lines per element */
 #define ENG_IP_ACC 4096         /* interaction-probability accelerator cells */
 #define ENG_PI 3.14159265358979323846

@@ -270,23 +270,36 @@ extern "C" void xmi_engine_forget_tables(const void *h) {
     }
 }
 
-static uint64_t sampled_hash(const xmi_solid_angle *sa) {
-  uint64_t hsh = 1469598103934665603ull;
-  auto mix = [&](const void *p, size_t n) {
-    const unsigned char *c = (const unsigned char *)p;
-    for (size_t i = 0; i < n; i++) hsh = (hsh ^ c[i]) * 1099511628211ull;
+/* hash of every cell and both axes of a grid: a grid edited in place, or a new
+ * grid at a reused address, is uploaded again unless its contents are the same
+ * (four independent multiply-xorshift chains over the 64-bit words) */
+static uint64_t grid_hash(const xmi_solid_angle *sa) {
+  uint64_t h[4] = {0x9E3779B97F4A7C15ull, 0xC2B2AE3D27D4EB4Full, 0x165667B19E3779F9ull, 0x27D4EB2F165667C5ull};
+  auto mix = [&](const double *p, size_t n) {
+    size_t i = 0;
+    for (; i + 4 <= n; i += 4)
+      for (int k = 0; k < 4; k++) {
+        uint64_t w;
+        memcpy(&w, p + i + k, 8);
+        h[k] = (h[k] ^ w) * 0xff51afd7ed558ccdull;
+        h[k] ^= h[k] >> 29;
+      }
+    for (; i < n; i++) {
+      uint64_t w;
+      memcpy(&w, p + i, 8);
+      h[0] = (h[0] ^ w) * 0xff51afd7ed558ccdull;
+      h[0] ^= h[0] >> 29;
+    }
   };
   size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
-  mix(sa->grid_dims_r_vals, sizeof(double) * sa->grid_dims_r_n);
-  mix(sa->grid_dims_theta_vals, sizeof(double) * sa->grid_dims_theta_n);
-  size_t step = tot / 8192 + 1;
-  for (size_t i = 0; i < tot; i += step) mix(&sa->solid_angles[i], sizeof(double));
-  mix(&sa->solid_angles[tot - 1], sizeof(double));
-  return hsh;
+  mix(sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n);
+  mix(sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n);
+  mix(sa->solid_angles, tot);
+  return (h[0] ^ (h[1] * 3) ^ (h[2] * 5) ^ (h[3] * 7)) + tot;
 }
 
 static int ensure_grid(Engine &E, const xmi_solid_angle *sa) {
-  uint64_t hsh = sampled_hash(sa);
+  uint64_t hsh = grid_hash(sa);
   if (E.sa_key == sa && E.sa_hash == hsh) return 1;
   size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
   if (upload(E.sa_grid, sa->solid_angles, tot) != hipSuccess ||
@@ -299,6 +312,7 @@ static int ensure_grid(Engine &E, const xmi_solid_angle *sa) {
   E.sa_hash = hsh;
   E.sa_nr = (int)sa->grid_dims_r_n;
   E.sa_nt = (int)sa->grid_dims_theta_n;
+  g_stats.grid_uploads++;
   return 1;
 }
 
@@ -369,6 +383,13 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   }
   if (!esc && !options->use_variance_reduction) {
     xmi_set_error("xmi_main_msim: the brute-force path (use_variance_reduction=0) is not built on the GPU");
+    return 0;
+  }
+  /* the Philox counter holds the interaction order in 8 bits (philox.h):
+   * order n and n + 256 would share their random streams */
+  if (in->general.n_interactions_trajectory < 1 || in->general.n_interactions_trajectory > ENG_MAX_INTERACTIONS) {
+    xmi_set_error("xmi_main_msim: n_interactions_trajectory must be in 1..%d, got %ld", ENG_MAX_INTERACTIONS,
+                  (long)in->general.n_interactions_trajectory);
     return 0;
   }
   if (options->use_advanced_compton) {
@@ -1364,7 +1385,7 @@ extern "C" int xmi_solid_angle_calculation_cl(xmi_inputFPtr inputFPtr, xmi_solid
   }
   /* the device copy is the grid just returned: remember it */
   g_engine.sa_key = sa;
-  g_engine.sa_hash = sampled_hash(sa);
+  g_engine.sa_hash = grid_hash(sa);
   g_engine.sa_nr = (int)n;
   g_engine.sa_nt = (int)n;
   *out = sa;
