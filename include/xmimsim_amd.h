@@ -405,6 +405,20 @@ void xmi_detector_convolute_all_custom(xmi_inputFPtr inputFPtr, double **channel
                                        xmi_escape_ratios *escape_ratios, int n_interactions_all,
                                        int zero_interaction);
 
+/* The same detector response without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (harnesses that forbid hipMalloc, tight serving loops).  The caller owns
+ * the outputs: channels_conv[0..n_interactions_all] must each point to
+ * nchannels doubles.  Device work runs on the null stream in a file-scope
+ * device array of the library (one per device, at most 65536 channels:
+ * 2.6 MB); calls that overlap on one device share it and race.  Returns 1 on
+ * success, 0 with xmi_msim_last_error set (more channels than the maximum,
+ * a NULL output, a device error, a pile-up train over 100 pulses). */
+int xmi_detector_convolute_all_noalloc(xmi_inputFPtr inputFPtr, double **channels_noconv, double **channels_conv,
+                                       double *brute_history, double *var_red_history, xmi_main_options *options,
+                                       xmi_escape_ratios *escape_ratios, int n_interactions_all,
+                                       int zero_interaction);
+
 /* ---- net-line surrogate ------------------------------------------------
  * No counterpart in the reference.  An ensemble of multilayer perceptrons
  * (dims[0] inputs, GELU hidden layers, linear outputs) trained on the GPU

@@ -587,6 +587,33 @@ def detector_response(prep: "Prepared", channels: np.ndarray, var_red: Optional[
     return out
 
 
+def detector_response_noalloc(prep: "Prepared", channels: np.ndarray, var_red: Optional[np.ndarray] = None,
+                              escape_ratios=None, zero_interaction: bool = False,
+                              options: Optional[xmi_main_options] = None):
+    """xmi_detector_convolute_all_noalloc: the same response as
+    detector_response, with caller-owned outputs and no device allocation."""
+    lib = load_library()
+    n1, nch = channels.shape
+    ch = [np.ascontiguousarray(channels[k], np.float64) for k in range(n1)]
+    out = np.zeros((n1, nch))
+    noconv = (C.POINTER(C.c_double) * n1)(*[c.ctypes.data_as(C.POINTER(C.c_double)) for c in ch])
+    conv = (C.POINTER(C.c_double) * n1)(*[out[k].ctypes.data_as(C.POINTER(C.c_double)) for k in range(n1)])
+    vr_ptr = None
+    if var_red is not None:
+        assert var_red.flags["C_CONTIGUOUS"] and var_red.dtype == np.float64
+        vr_ptr = var_red.ctypes.data_as(C.POINTER(C.c_double))
+    opts = options or prep.options
+    f = lib.xmi_detector_convolute_all_noalloc
+    f.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),
+                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(xmi_main_options),
+                  C.POINTER(xmi_escape_ratios), C.c_int, C.c_int]
+    f.restype = C.c_int
+    if not f(prep.inputF, noconv, conv, None, vr_ptr, C.byref(opts),
+             C.byref(escape_ratios) if escape_ratios is not None else None, n1 - 1, 1 if zero_interaction else 0):
+        raise RuntimeError("xmi_detector_convolute_all_noalloc failed: " + last_error())
+    return out
+
+
 def run_batch(preps: Sequence["Prepared"], tables: "SharedTables", seed: int, n_streams: int = 4,
               options: Optional[xmi_main_options] = None):
     """xmi_main_msim_batch over inputs bound to `tables`, each with a

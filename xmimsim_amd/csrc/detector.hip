@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/xmimsim_amd.h"
@@ -266,20 +267,33 @@ static void det_escape(std::vector<double> &ch, const xmi_inputF *in, const xmi_
     hipError_t e_ = (x);                                                    \
     if (e_ != hipSuccess) {                                                 \
       xmi_set_error("detector response: %s: %s", #x, hipGetErrorString(e_)); \
-      goto done;                                                            \
+      return 0;                                                             \
     }                                                                       \
   } while (0)
 
 extern "C" uint64_t xmi_engine_current_seed(const xmi_main_options *options);
 
-extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **channels_noconv, double **channels_conv,
-                                           double *brute_history, double *var_red_history, xmi_main_options *options,
-                                           xmi_escape_ratios *escape_ratios, int n_interactions_all,
-                                           int zero_interaction) {
-  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+/* clears n words (a kernel rather than a memset, so that a caller may
+ * capture the launch sequence in a graph) */
+__global__ void k_det_clear(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+/* device buffers of one convolution, nch entries each (over: one word) */
+struct DetBufs {
+  double *src, *norm, *conv, *cdf;
+  unsigned long long *hist, *over;
+};
+
+/* the response of every order into channels_conv[n] (nch doubles each,
+ * owned by the caller), src/xmi_detector_f.F90:219-580; 1 on success */
+static int det_convolute(const xmi_inputF *in, double **channels_noconv, double **channels_conv,
+                         double *brute_history, double *var_red_history, const xmi_main_options *options,
+                         const xmi_escape_ratios *escape_ratios, int n_interactions_all, int zero_interaction,
+                         const DetBufs &b, hipStream_t st) {
   const int nch = in->detector.nchannels;
   const int n_int = in->general.n_interactions_trajectory;
-  for (int n = 0; n <= n_interactions_all; n++) channels_conv[n] = nullptr;
   DetResp d;
   d.zero = in->detector.zero;
   d.gain = in->detector.gain;
@@ -292,20 +306,8 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
   const uint64_t seed = xmi_engine_current_seed(options);
   std::vector<double> eff(nch);
   for (int i = 0; i < nch; i++) eff[i] = det_efficiency(in, i * d.gain + d.zero);
-  double *dsrc = nullptr, *dnorm = nullptr, *dconv = nullptr, *dcdf = nullptr;
-  unsigned long long *dhist = nullptr;
-  unsigned int *dover = nullptr;
-  hipStream_t st = nullptr;
-  int failed = 1; /* cleared once every order is convoluted */
   const int nb = (nch + 255) / 256;
-  DET_TRY(hipStreamCreate(&st));
-  DET_TRY(hipMalloc(&dsrc, nch * sizeof(double)));
-  DET_TRY(hipMalloc(&dnorm, nch * sizeof(double)));
-  DET_TRY(hipMalloc(&dconv, nch * sizeof(double)));
-  DET_TRY(hipMalloc(&dcdf, nch * sizeof(double)));
-  DET_TRY(hipMalloc(&dhist, nch * sizeof(unsigned long long)));
-  DET_TRY(hipMalloc(&dover, sizeof(unsigned int)));
-  hipLaunchKernelGGL(k_det_norm, dim3(nb), dim3(256), 0, st, d, dnorm);
+  hipLaunchKernelGGL(k_det_norm, dim3(nb), dim3(256), 0, st, d, b.norm);
   DET_TRY(hipGetLastError());
   for (int n = zero_interaction ? 0 : 1; n <= n_interactions_all; n++) {
     std::vector<double> temp(channels_noconv[n], channels_noconv[n] + nch);
@@ -323,42 +325,39 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
           cdf[i] = acc / Nt;
         }
         cdf[nch - 1] = 1.0;
-        DET_TRY(hipMemcpyAsync(dcdf, cdf.data(), nch * sizeof(double), hipMemcpyHostToDevice, st));
-        DET_TRY(hipMemsetAsync(dhist, 0, nch * sizeof(unsigned long long), st));
-        DET_TRY(hipMemsetAsync(dover, 0, sizeof(unsigned int), st));
+        DET_TRY(hipMemcpyAsync(b.cdf, cdf.data(), nch * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_det_clear, dim3(nb), dim3(256), 0, st, b.hist, nch);
+        hipLaunchKernelGGL(k_det_clear, dim3(1), dim3(64), 0, st, b.over, 1);
         const long threads = 65536;
         const long each = (Nt_long + threads - 1) / threads;
-        hipLaunchKernelGGL(k_det_pileup, dim3((unsigned)(threads / 256)), dim3(256), 0, st, nch, dcdf,
+        hipLaunchKernelGGL(k_det_pileup, dim3((unsigned)(threads / 256)), dim3(256), 0, st, nch, b.cdf,
                            in->detector.live_time / Nt, in->detector.pulse_width, d.zero, d.gain, each, Nt_long,
-                           seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(n + 1)), dhist, dover);
+                           seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(n + 1)), b.hist, (unsigned int *)b.over);
         DET_TRY(hipGetLastError());
         std::vector<unsigned long long> h(nch);
         unsigned int over = 0;
-        DET_TRY(hipMemcpyAsync(h.data(), dhist, nch * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        DET_TRY(hipMemcpyAsync(&over, dover, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        DET_TRY(hipMemcpyAsync(h.data(), b.hist, nch * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        DET_TRY(hipMemcpyAsync(&over, b.over, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         DET_TRY(hipStreamSynchronize(st));
         if (over) { /* src/xmi_detector_f.F90:152-156 stops the program here */
           xmi_set_error("detector response: pulse train maximum (100 pulses) reached %u times; adjust the pulse "
                         "width or disable the pile-up generation", over);
-          failed = 1;
-          goto done;
+          return 0;
         }
         for (int i = 0; i < nch; i++) temp[i] = (double)h[i];
       }
     }
-    DET_TRY(hipMemcpyAsync(dsrc, temp.data(), nch * sizeof(double), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_det_conv, dim3(nb), dim3(256), 0, st, d, dsrc, dnorm, dconv);
+    DET_TRY(hipMemcpyAsync(b.src, temp.data(), nch * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_det_conv, dim3(nb), dim3(256), 0, st, d, b.src, b.norm, b.conv);
     DET_TRY(hipGetLastError());
     if (options->use_poisson) {
-      hipLaunchKernelGGL(k_det_poisson, dim3(nb), dim3(256), 0, st, nch, dconv, seed, (uint32_t)n);
+      hipLaunchKernelGGL(k_det_poisson, dim3(nb), dim3(256), 0, st, nch, b.conv, seed, (uint32_t)n);
       DET_TRY(hipGetLastError());
     }
-    channels_conv[n] = (double *)malloc(nch * sizeof(double));
-    DET_TRY(hipMemcpyAsync(channels_conv[n], dconv, nch * sizeof(double), hipMemcpyDeviceToHost, st));
+    DET_TRY(hipMemcpyAsync(channels_conv[n], b.conv, nch * sizeof(double), hipMemcpyDeviceToHost, st));
     DET_TRY(hipStreamSynchronize(st));
   }
-  if (!zero_interaction) channels_conv[0] = (double *)calloc(nch, sizeof(double));
-  failed = 0;
+  if (!zero_interaction) memset(channels_conv[0], 0, nch * sizeof(double));
   /* net-line histories: detector absorption at the line energies */
   for (double *hist : {var_red_history, brute_history}) {
     if (!hist) continue;
@@ -379,19 +378,106 @@ extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **cha
       }
     }
   }
-done:
-  if (failed) /* no partial result: every order's pointer is NULL after a failure */
+  return 1;
+}
+
+extern "C" void xmi_detector_convolute_all(xmi_inputFPtr inputFPtr, double **channels_noconv, double **channels_conv,
+                                           double *brute_history, double *var_red_history, xmi_main_options *options,
+                                           xmi_escape_ratios *escape_ratios, int n_interactions_all,
+                                           int zero_interaction) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  const size_t nch = (size_t)in->detector.nchannels;
+  for (int n = 0; n <= n_interactions_all; n++) channels_conv[n] = nullptr;
+  DetBufs b;
+  void *block = nullptr;
+  hipStream_t st = nullptr;
+  int ok = 0;
+  hipError_t e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipMalloc(&block, nch * 5 * sizeof(double) + 256);
+  if (e != hipSuccess) {
+    xmi_set_error("detector response: device allocation: %s", hipGetErrorString(e));
+  } else {
+    b.src = (double *)block;
+    b.norm = b.src + nch;
+    b.conv = b.norm + nch;
+    b.cdf = b.conv + nch;
+    b.hist = (unsigned long long *)(b.cdf + nch);
+    b.over = b.hist + nch;
+    for (int n = 0; n <= n_interactions_all; n++) channels_conv[n] = (double *)malloc(nch * sizeof(double));
+    ok = det_convolute(in, channels_noconv, channels_conv, brute_history, var_red_history, options, escape_ratios,
+                       n_interactions_all, zero_interaction, b, st);
+  }
+  if (!ok) /* no partial result: every order's pointer is NULL after a failure */
     for (int n = 0; n <= n_interactions_all; n++) {
       free(channels_conv[n]);
       channels_conv[n] = nullptr;
     }
-  if (dsrc) (void)hipFree(dsrc);
-  if (dnorm) (void)hipFree(dnorm);
-  if (dconv) (void)hipFree(dconv);
-  if (dcdf) (void)hipFree(dcdf);
-  if (dhist) (void)hipFree(dhist);
-  if (dover) (void)hipFree(dover);
+  if (block) (void)hipFree(block);
   if (st) (void)hipStreamDestroy(st);
+}
+
+/* ---- the allocation-free variant ------------------------------------------
+ * The buffers live in this file-scope array; each device holds its own copy,
+ * found once per device through hipGetSymbolAddress.  Calls that overlap on
+ * one device share the array and race. */
+#define DET_NOALLOC_MAX_CHANNELS 65536
+static __device__ __align__(256) unsigned char scratch[(size_t)DET_NOALLOC_MAX_CHANNELS * 5 * sizeof(double) + 256];
+
+#define DET_MAX_DEVICES 64
+static void *g_scratch_ptr[DET_MAX_DEVICES];
+static std::mutex g_scratch_mutex;
+
+extern "C" int xmi_detector_convolute_all_noalloc(xmi_inputFPtr inputFPtr, double **channels_noconv,
+                                                  double **channels_conv, double *brute_history,
+                                                  double *var_red_history, xmi_main_options *options,
+                                                  xmi_escape_ratios *escape_ratios, int n_interactions_all,
+                                                  int zero_interaction) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  if (!in || !channels_noconv || !channels_conv || !options) {
+    xmi_set_error("xmi_detector_convolute_all_noalloc: NULL argument");
+    return 0;
+  }
+  const int nch = in->detector.nchannels;
+  if (nch < 1 || nch > DET_NOALLOC_MAX_CHANNELS) {
+    xmi_set_error("xmi_detector_convolute_all_noalloc: %d channels; the allocation-free response holds at most %d",
+                  nch, DET_NOALLOC_MAX_CHANNELS);
+    return 0;
+  }
+  for (int n = 0; n <= n_interactions_all; n++)
+    if (!channels_conv[n]) {
+      xmi_set_error("xmi_detector_convolute_all_noalloc: channels_conv[%d] is NULL (the caller owns the outputs)", n);
+      return 0;
+    }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= DET_MAX_DEVICES) {
+    xmi_set_error("xmi_detector_convolute_all_noalloc: no usable device (%s)", hipGetErrorString(e));
+    return 0;
+  }
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        xmi_set_error("xmi_detector_convolute_all_noalloc: scratch address: %s", hipGetErrorString(e));
+        return 0;
+      }
+      g_scratch_ptr[dev] = p;
+    }
+    base = g_scratch_ptr[dev];
+  }
+  const size_t n = (size_t)nch;
+  DetBufs b;
+  b.src = (double *)base;
+  b.norm = b.src + n;
+  b.conv = b.norm + n;
+  b.cdf = b.conv + n;
+  b.hist = (unsigned long long *)(b.cdf + n);
+  b.over = b.hist + n;
+  return det_convolute(in, channels_noconv, channels_conv, brute_history, var_red_history, options, escape_ratios,
+                       n_interactions_all, zero_interaction, b, (hipStream_t)0);
 }
 
 /* the name under which the reference loads a custom detector response
