@@ -265,11 +265,14 @@ int xmi_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_host
  * solid-angle grid (one grid may serve every sample of one geometry).
  * Outputs per sample as xmi_main_msim's (channels[k], var_red_history[k],
  * malloc'ed); done_ms[k], when given, receives the sample's completion time in
- * ms after the start.  Each sample's result equals xmi_main_msim's for the
- * same seed.  Returns 1 on success, 0 (no outputs) on failure. */
+ * ms after the start.  seeds[k], when seeds is given, is sample k's seed;
+ * otherwise every sample uses the seed xmi_main_msim would.  Each sample's
+ * result equals xmi_main_msim's for the same seed.  Afterwards xmi_msim_get_stats on the calling thread holds the
+ * batch's totals (histories, detection-weight events, wall time).  Returns 1
+ * on success, 0 (no outputs) on failure. */
 int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
                         xmi_main_options *options, xmi_solid_angle **solid_angles, double **channels,
-                        double **var_red_history, int n_streams, double *done_ms);
+                        double **var_red_history, int n_streams, double *done_ms, const uint64_t *seeds);
 
 /* Releases memory returned by this library (src/xmi_aux.c:217). */
 void xmi_free(void *ptr);

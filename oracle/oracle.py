@@ -24,6 +24,19 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+class OracleWork(C.Structure):
+    _fields_ = [("events", C.c_int64), ("elements", C.c_int64), ("element_layers", C.c_int64),
+                ("element2_layers", C.c_int64), ("doppler_trials", C.c_int64), ("lines", C.c_int64),
+                ("line_layers", C.c_int64)]
+
+
+def work() -> dict:
+    """Estimator work counters of the last run() (sums over scored events)."""
+    w = OracleWork()
+    load().xmi_oracle_get_work(C.byref(w))
+    return {name: getattr(w, name) for name, _ in OracleWork._fields_}
+
+
 class OracleStats(C.Structure):
     _fields_ = [("histories", C.c_int64), ("interactions", C.c_int64), ("vr_calls", C.c_int64),
                 ("solid_angle_fallbacks", C.c_int64), ("seconds", C.c_double)]
@@ -43,6 +56,7 @@ def load():
                                          P(api.xmi_solid_angle), C.c_int, C.c_uint64, C.c_int]
     lib.xmi_oracle_main_msim.restype = C.c_int
     lib.xmi_oracle_get_stats.argtypes = [P(OracleStats)]
+    lib.xmi_oracle_get_work.argtypes = [P(OracleWork)]
     lib.xmi_oracle_solid_angle_calculation.argtypes = [C.c_void_p, P(P(api.xmi_solid_angle)), C.c_long, C.c_uint64,
                                                        C.c_int]
     lib.xmi_oracle_solid_angle_calculation.restype = C.c_int
@@ -84,6 +98,9 @@ def load():
     lib.xmi_oracle_probe_source.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_long, C.c_uint64, D]
     lib.xmi_oracle_probe_source.restype = C.c_int
     lib.xmi_oracle_input_derived.argtypes = [C.c_void_p, D]
+    lib.xmi_oracle_probe_doppler.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                             C.c_long, C.c_uint64, D]
+    lib.xmi_oracle_probe_doppler.restype = C.c_int
     lib.xmi_oracle_set_plant.argtypes = [C.c_int]
     _lib = lib
     return lib
@@ -176,7 +193,7 @@ def detector_convolute(prepared, noconv):
 # ---- probes of single restated routines (pinning tests) --------------------
 PLANTS = {"none": 0, "rayleigh_energy": 1, "phi0_sign": 2, "compton_full_angle": 3, "doppler_one_sided": 4,
           "ck_l1_swap": 5, "line_shift": 6, "elecv_stale": 7, "dirv_phi_sign": 8, "slit_full": 9, "gauss_xy": 10,
-          "compton_t_rayleigh": 11, "shell_next": 12}
+          "compton_t_rayleigh": 11, "shell_next": 12, "qimax_ignored": 13, "doppler_no_negative": 14}
 
 
 class planted:
@@ -264,3 +281,12 @@ def input_derived(prepared):
             "n_sample_orientation": out[10:13], "n_detector_orientation": out[13:16],
             "new": out[16:25].reshape(3, 3), "inverse": out[25:34].reshape(3, 3), "detector_solid_angle": out[34],
             "thickness_along_Z": out[35::3], "Z_coord_begin": out[36::3], "Z_coord_end": out[37::3]}
+
+
+def probe_doppler(prepared, element_index: int, energy: float, theta: float, n: int, seed: int):
+    """The estimator's Doppler-broadened Compton energies at (energy, theta)."""
+    out = np.zeros(n)
+    if not load().xmi_oracle_probe_doppler(prepared.inputF, prepared.hdf5F, C.byref(prepared.options), element_index,
+                                           energy, theta, n, seed, _d(out)):
+        raise RuntimeError("probe failed")
+    return out

@@ -59,6 +59,8 @@ static const double ORC_MEC2 = 9.10938188e-31 * 2.99792458e8 * 2.99792458e8 / 1.
 static const double ORC_PI = 3.14159265358979323846;
 
 static xmi_oracle_stats g_stats;
+static xmi_oracle_work g_work;
+void xmi_oracle_get_work(xmi_oracle_work *w) { *w = g_work; }
 static long g_hits_per_single = 5000;
 /* a mistake planted on purpose (XMI_ORACLE_PLANT_*), for tests that show a
  * check rejects it; 0 in every other use */
@@ -212,6 +214,7 @@ typedef struct {
   const xrl_atom **atoms;     /* per hdf5 index */
   double *line_energies;      /* [nZ][219] by hdf5 index */
   double *precalc_mu;         /* [n_layers][nZ][219] */
+  double *cp_cdfg;            /* [nZ][XMI_CP_GRID] profile CDF on the momentum grid */
   int nZ;
   int n_int;
   int nch;
@@ -227,6 +230,7 @@ typedef struct {
   double *channels;           /* [(n_int+1)][nch] */
   double *var_red;            /* [100][385][n_int] Fortran-equivalent indexed as [Z-1][row-1][n-1] */
   int64_t interactions, vr_calls, fallbacks;
+  xmi_oracle_work work;       /* estimator work counters */
 } orc_tally;
 
 typedef struct {
@@ -461,6 +465,134 @@ static double doppler_pz(const xmi_hdf5_ZF *z, double r) {
   return interpolate_simple(rn[pos], z->cp_total_icdf[pos], rn[pos + 1], z->cp_total_icdf[pos + 1], r);
 }
 
+/* the profile CDF at momentum pz >= 0 from its grid (xmi_profile_cdf_grid),
+ * clamped to the table's last point */
+static double orc_profile_cdf(const double *g, double top, double pz) {
+  if (!(pz > 0.0)) return 0.0;
+  const double x = pz * ((double)(XMI_CP_GRID - 1) / top);
+  if (x >= (double)(XMI_CP_GRID - 1)) return g[XMI_CP_GRID - 1];
+  const int j = (int)x;
+  return g[j] + (g[j + 1] - g[j]) * (x - (double)j);
+}
+
+/* the Doppler-broadened energy of the estimator's Compton term
+ * (xmi_compton_varred2, src/xmi_variance_reduction.F90:949-1008) from one
+ * 32-bit word w.  The reference redraws the momentum until E' <= E; the
+ * accepted momenta are pz <= A0/A1 and, where E' turns negative, pz >
+ * (c_lamb0 + A0)/A1.  Here they are drawn directly from that part of the
+ * profile: the same distribution without the rejection loop.  r ranges over
+ * [0, r_end), the inverse table without its last interval, which the
+ * reference also leaves out. */
+static double orc_doppler_energy(const xmi_hdf5_ZF *z, const double *g, double E, double theta, uint32_t w) {
+  const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
+  const double c_lamb0 = cc / (E * 1000.0);
+  const double sth2 = sin(theta / 2.0);
+  const double A0 = c0 * sth2 * sth2, A1 = c1 * c_lamb0 * sth2;
+  const double r_end = z->cp_random_numbers[z->cp_len - 2];
+  double f_max = r_end, f_neg = r_end;
+  if (A1 > 0.0) {
+    f_max = orc_profile_cdf(g, z->cp_total_icdf[z->cp_len - 2], A0 / A1);
+    f_neg = orc_profile_cdf(g, z->cp_total_icdf[z->cp_len - 2], (c_lamb0 + A0) / A1);
+  }
+  if (g_plant == XMI_ORACLE_PLANT_DOPPLER_NO_NEGATIVE) f_neg = r_end;
+  const double m_pos = f_max + (r_end - f_neg);
+  const double u = (double)w * 0x1.0p-32 * (r_end + m_pos);
+  double r, sign;
+  if (u < r_end) {
+    r = u;
+    sign = -1.0;
+  } else {
+    const double v = u - r_end;
+    r = v < f_max ? v : v - f_max + f_neg;
+    sign = 1.0;
+  }
+  const double drn = z->cp_random_numbers[1] - z->cp_random_numbers[0];
+  int pos = (int)(r / drn);
+  if (pos > z->cp_len - 3) pos = z->cp_len - 3;
+  const double pz = sign * interpolate_simple(z->cp_random_numbers[pos], z->cp_total_icdf[pos],
+                                              z->cp_random_numbers[pos + 1], z->cp_total_icdf[pos + 1], r);
+  return cc / (c_lamb0 + (A0 - A1 * pz)) / 1000.0;
+}
+
+/* ---- the advanced Compton model (use_advanced_compton) ---------------- */
+
+/* xmi_get_qimax, src/xmi_aux_f.F90:1951-1977: the largest projected momentum
+ * an electron of binding energy Ii can carry for scattering by theta */
+static double orc_qimax(double energy, double Ii, double theta) {
+  if (Ii != 0.0 && energy < Ii) return 0.0;
+  double EminIi = energy - Ii, ct = cos(theta);
+  double Q = 137.0 * (EminIi * energy * (1.0 - ct) / ORC_MEC2 - Ii);
+  return Q / sqrt(EminIi * EminIi + energy * energy - 2.0 * EminIi * energy * ct);
+}
+
+/* each occupied subshell's electron count and partial-profile CDF up to its
+ * Qimax (src/xmi_main.F90:4806-4863, src/xmi_variance_reduction.F90:781-825);
+ * returns the sum of count * CDF */
+static double orc_adv_cdfs(const xmi_hdf5_ZF *z, const xrl_atom *a, double energy, double theta, double *ec,
+                           double *cdfs) {
+  const double dq = z->cp_Qs[1] - z->cp_Qs[0];
+  double sum = 0.0;
+  for (int k = 0; k < z->cp_nshells; k++) {
+    const int sh = z->cp_shell_indices[k];
+    const double *pc = z->cp_partial_cdf + (size_t)k * z->cp_len;
+    ec[k] = a->occupancy[sh];
+    double q = orc_qimax(energy, a->edge[sh], theta);
+    if (g_plant == XMI_ORACLE_PLANT_QIMAX_IGNORED) q = 1000.0;
+    if (q < -100.0) {
+      cdfs[k] = 0.0;
+    } else if (q > 100.0) {
+      cdfs[k] = 1.0;
+    } else {
+      double qp = fabs(q);
+      int pos = (int)(qp / dq);
+      if (pos > z->cp_len - 2) pos = z->cp_len - 2; /* |Q| = 100 exactly: the reference reads one past the table */
+      double v = interpolate_simple(z->cp_Qs[pos], pc[pos], z->cp_Qs[pos + 1], pc[pos + 1], qp);
+      cdfs[k] = q < 0.0 ? 1.0 - (0.5 + v) : 0.5 + v;
+    }
+    sum += ec[k] * cdfs[k];
+  }
+  return sum;
+}
+
+/* projected momentum of subshell k at a value cdf of its two-sided partial
+ * profile (src/xmi_main.F90:4898-4960): the half-profile inverse table, by
+ * the uniform grid of the CDF values */
+static double orc_adv_q(const xmi_hdf5_ZF *z, int k, double cdf) {
+  const double *inv = z->cp_partial_cdf_inv, *qi = z->cp_Qs_inv + (size_t)k * z->cp_len;
+  const double cp = cdf < 0.5 ? 0.5 - cdf : cdf - 0.5;
+  int pos = (int)(cp / (inv[1] - inv[0]));
+  if (pos > z->cp_len - 2) pos = z->cp_len - 2; /* cdf of exactly 0 or 1: the reference stops here */
+  const double Q = interpolate_simple(inv[pos], qi[pos], inv[pos + 1], qi[pos + 1], cp);
+  return cdf < 0.5 ? -Q : Q;
+}
+
+/* xmi_update_photon_energy_compton (src/xmi_main.F90:4785-4983): the
+ * subshell by count * CDF(Qimax), then Q from that subshell's profile below
+ * its Qimax, then the energy from Q; 0 kills the photon */
+static void orc_compton_energy_advanced(const orc_ctx *c, orc_photon *p, const xmi_hdf5_ZF *z, double theta_i) {
+  const xrl_atom *a = xrl_atom_get(p->current_element);
+  double ec[XRL_NSHELLS], cdfs[XRL_NSHELLS];
+  const double sum = orc_adv_cdfs(z, a, p->energy, theta_i, ec, cdfs);
+  if (sum == 0.0) {
+    p->energy = 0.0;
+    return;
+  }
+  const double r = orc_uniform(p->tr);
+  double ts = 0.0;
+  int k = 0, last = 0;
+  for (; k < z->cp_nshells; k++) {
+    if (ec[k] * cdfs[k] > 0.0) last = k;
+    ts += ec[k] * cdfs[k] / sum;
+    if (r <= ts) break;
+  }
+  if (k == z->cp_nshells) k = last; /* rounding left the running sum just below r */
+  const double Q = orc_adv_q(z, k, orc_uniform(p->tr) * cdfs[k]);
+  double e = xmi_oracle_energy_from_q(p->energy, Q, theta_i);
+  p->energy = e > 0.0 ? e : 0.0; /* no root with Q's sign: the reference stops, here the photon ends */
+  p->energy_changed = 0;
+  if (p->energy > 0.0) mu_calc_all(c, p->energy, p->mus);
+}
+
 static int orc_compton(const orc_ctx *c, orc_photon *p) {
   const xmi_hdf5_ZF *z = photon_hdf5(c, p);
   double theta_i = bilinear(z->ComptonTheta_ICDF, z->Energies, z->n_e, z->RandomNumbers, z->n_r, p->energy,
@@ -473,8 +605,9 @@ static int orc_compton(const orc_ctx *c, orc_photon *p) {
   double phi_i = bilinear(c->h->Phi_ICDF, c->h->Thetas, c->h->phi_n_theta, c->h->RandomNumbers, c->h->phi_n_r, tt,
                           orc_uniform(p->tr));
   double phi0 = orc_phi0(p);
-  /* xmi_update_photon_energy_compton2 */
-  {
+  if (c->opt->use_advanced_compton) {
+    orc_compton_energy_advanced(c, p, z, theta_i);
+  } else { /* xmi_update_photon_energy_compton2 */
     const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
     double energy = p->energy * 1000.0;
     double c_lamb0 = cc / energy;
@@ -722,6 +855,17 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
   const xmi_layerF *L = &in->layers[p->current_layer - 1];
   int n_ia = p->n_interactions;
   double mu_cur = p->mus[p->current_layer - 1];
+  /* work counters: layers on the escape path (those the exit loop visits) */
+  int crossed = 0;
+  for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir) crossed++;
+  t->work.events++;
+  t->work.elements += L->n_elements;
+  t->work.element_layers += (int64_t)L->n_elements * crossed;
+  t->work.element2_layers += (int64_t)L->n_elements * L->n_elements * crossed;
+  /* the Compton draws continue the stream from the block after the
+   * geometry and cone draws, nothing buffered (as the GPU's words do) */
+  rng->has_pending = 0;
+  rng->n_words = 0;
   for (int i = 0; i < L->n_elements; i++) {
     int Zi = L->Z[i];
     int hidx = L->hdf5_index[i];
@@ -733,42 +877,37 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
     double tw = Pconv * Pdir * Pesc_rayl * p->weight;
     tally_history(c, t, Zi, 384, n_ia, tw);
     tally_channel(c, t, n_ia, p->energy, tw);
-    /* Compton, xmi_compton_varred2 with N_COMPTON_VARRED = 1 */
-    {
-      const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
-      double energy = p->energy * 1000.0;
-      double c_lamb0 = cc / energy;
-      double sth2 = sin(theta / 2.0);
-      int it = 0, failed = 0;
-      int nrn = z->cp_len;
-      for (;;) {
-        /* one 32-bit word per trial: 31 bits of random number, a sign bit */
-        const uint32_t w = orc_word(rng);
-        double r = (double)(w >> 1) * 0x1.0p-31;
-        int pos = (int)(r / (z->cp_random_numbers[1] - z->cp_random_numbers[0]));
-        if (pos == nrn - 2) continue; /* last interval excluded */
-        double pz = interpolate_simple(z->cp_random_numbers[pos], z->cp_total_icdf[pos], z->cp_random_numbers[pos + 1],
-                                       z->cp_total_icdf[pos + 1], r);
-        if (w & 1u) pz = -pz;
-        double dlamb = c0 * sth2 * sth2 - c1 * c_lamb0 * sth2 * pz;
-        energy = cc / (c_lamb0 + dlamb) / 1000.0;
-        if (energy <= p->energy) break; /* also accepts negative energies, as the reference does */
-        if (it == 100) {
-          failed = 1; /* the reference stops with an error here */
-          break;
-        }
-        it++;
-      }
-      if (!failed) {
+    if (c->opt->use_advanced_compton) {
+      /* xmi_compton_varred (src/xmi_variance_reduction.F90:752-947): every
+       * subshell with its weight count * CDF(Qimax) / sum, one draw each */
+      double ec[XRL_NSHELLS], cdfs[XRL_NSHELLS];
+      const double csum = orc_adv_cdfs(z, a, p->energy, theta, ec, cdfs);
+      for (int k = 0; csum != 0.0 && k < z->cp_nshells; k++) {
+        const double sw = ec[k] * cdfs[k] / csum;
+        if (sw == 0.0) continue;
+        const double Q = orc_adv_q(z, k, orc_uniform(rng) * cdfs[k]);
+        double energy = xmi_oracle_energy_from_q(p->energy, Q, theta);
+        t->work.doppler_trials++;
+        if (!(energy > 0.0)) continue;
         double m2 = 0.0;
         for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
           m2 += layer_mu(c, &in->layers[j - 1], energy) * in->layers[j - 1].density * distances[j - 1];
-        double Pesc_comp = exp(-m2);
-        double Pdir_c = omega * xrl_dcsp_compt(a, p->energy, theta, phi);
-        double twc = Pconv * Pdir_c * Pesc_comp * p->weight / 1.0;
+        double twc = Pconv * (omega * xrl_dcsp_compt(a, p->energy, theta, phi)) * exp(-m2) * p->weight * sw;
         tally_history(c, t, Zi, 385, n_ia, twc);
         tally_channel(c, t, n_ia, energy, twc);
       }
+    } else { /* Compton, xmi_compton_varred2 with N_COMPTON_VARRED = 1 */
+      const uint32_t w = orc_word(rng);
+      t->work.doppler_trials++;
+      const double energy = orc_doppler_energy(z, c->cp_cdfg + (size_t)hidx * XMI_CP_GRID, p->energy, theta, w);
+      double m2 = 0.0;
+      for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
+        m2 += layer_mu(c, &in->layers[j - 1], energy) * in->layers[j - 1].density * distances[j - 1];
+      double Pesc_comp = exp(-m2);
+      double Pdir_c = omega * xrl_dcsp_compt(a, p->energy, theta, phi);
+      double twc = Pconv * Pdir_c * Pesc_comp * p->weight / 1.0;
+      tally_history(c, t, Zi, 385, n_ia, twc);
+      tally_channel(c, t, n_ia, energy, twc);
     }
     /* fluorescence */
     double P[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -788,6 +927,8 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
       if (ef < XMI_ENERGY_THRESHOLD) continue;
       int s = xrl_line_shell(line);
       if (P[s] == 0.0) continue;
+      t->work.lines++;
+      t->work.line_layers += crossed;
       double Pc = L->weight[i] * P[s] * a->fluor_yield[s] * a->rad_rate[line] / mu_cur;
       double m2 = 0.0;
       for (int j = p->current_layer; step_do_dir > 0 ? j <= step_do_max : j >= step_do_max; j += step_do_dir)
@@ -1091,6 +1232,10 @@ static void orc_ctx_tables(orc_ctx *c) {
     for (int l = 1; l <= XMI_NLINES_VR; l++)
       c->line_energies[(size_t)i * XMI_NLINES_VR + l - 1] = c->atoms[i]->line_energy[l];
   }
+  c->cp_cdfg = (double *)malloc(sizeof(double) * (size_t)c->nZ * XMI_CP_GRID);
+  for (int i = 0; i < c->nZ; i++)
+    xmi_profile_cdf_grid(h->Zs[i].cp_random_numbers, h->Zs[i].cp_total_icdf, h->Zs[i].cp_len, XMI_CP_GRID,
+                         c->cp_cdfg + (size_t)i * XMI_CP_GRID);
   c->precalc_mu = (double *)calloc((size_t)in->n_layers * c->nZ * XMI_NLINES_VR, sizeof(double));
   for (int k = 0; k < in->n_layers; k++)
     for (int i = 0; i < c->nZ; i++)
@@ -1101,6 +1246,7 @@ static void orc_ctx_tables(orc_ctx *c) {
 }
 
 static void orc_ctx_free_tables(orc_ctx *c) {
+  free(c->cp_cdfg);
   free(c->atoms);
   free(c->line_energies);
   free(c->precalc_mu);
@@ -1116,10 +1262,6 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
   if (!in || !h || !options || !channelsPtr || !brute_historyPtr || !var_red_historyPtr) return 0;
   if (!options->use_variance_reduction || !solid_angles) {
     fprintf(stderr, "xmi_oracle_main_msim: only the variance-reduction path is restated\n");
-    return 0;
-  }
-  if (options->use_advanced_compton) {
-    fprintf(stderr, "xmi_oracle_main_msim: advanced Compton is not restated\n");
     return 0;
   }
   if (n_mpi_hosts < 1) n_mpi_hosts = 1;
@@ -1148,6 +1290,7 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
   double *var_red = (double *)calloc(nhist, sizeof(double));
   int64_t histories = 0, interactions = 0, vr_calls = 0, fallbacks = 0;
   int ok = 1;
+  memset(&g_work, 0, sizeof(g_work));
 
   /* valid continuous intervals and history numbering: continuous intervals
    * first, then discrete lines */
@@ -1240,6 +1383,13 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
     fallbacks += t.fallbacks;
 #pragma omp critical
     {
+      g_work.events += t.work.events;
+      g_work.elements += t.work.elements;
+      g_work.element_layers += t.work.element_layers;
+      g_work.element2_layers += t.work.element2_layers;
+      g_work.doppler_trials += t.work.doppler_trials;
+      g_work.lines += t.work.lines;
+      g_work.line_layers += t.work.line_layers;
       for (size_t k = 0; k < nchan_tot; k++) channels[k] += t.channels[k];
       for (size_t k = 0; k < nhist; k++) var_red[k] += t.var_red[k];
     }
@@ -1812,4 +1962,23 @@ void xmi_oracle_input_derived(xmi_inputFPtr inputFPtr, double *out) {
     out[36 + 3 * l] = in->layers[l].Z_coord_begin;
     out[37 + 3 * l] = in->layers[l].Z_coord_end;
   }
+}
+
+/* the estimator's Doppler-broadened Compton energy of element element_index
+ * of layer 1 at (E, theta), n times from consecutive words of one stream */
+int xmi_oracle_probe_doppler(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options,
+                             int element_index, double energy, double theta, long n, uint64_t seed, double *out) {
+  orc_ctx c;
+  probe_ctx(&c, inputFPtr, hdf5FPtr, options);
+  if (element_index < 0 || element_index >= c.in->layers[0].n_elements) {
+    orc_ctx_free_tables(&c);
+    return 0;
+  }
+  const int hidx = c.in->layers[0].hdf5_index[element_index];
+  orc_rng r;
+  orc_rng_init(&r, seed, 15u, 0);
+  for (long i = 0; i < n; i++)
+    out[i] = orc_doppler_energy(&c.h->Zs[hidx], c.cp_cdfg + (size_t)hidx * XMI_CP_GRID, energy, theta, orc_word(&r));
+  orc_ctx_free_tables(&c);
+  return 1;
 }

@@ -35,6 +35,20 @@ typedef struct {
 } xmi_oracle_stats;
 void xmi_oracle_get_stats(xmi_oracle_stats *s);
 
+/* work of the detection-weight estimator in the last xmi_oracle_main_msim
+ * call, summed over the events that reach its element loop: the inputs of the
+ * algorithmic byte and flop accounting of the GPU kernels */
+typedef struct {
+  int64_t events;           /* estimator evaluations scored */
+  int64_t elements;         /* sum of E, the elements of the event's layer */
+  int64_t element_layers;   /* sum of E * L, L = layers on the escape path */
+  int64_t element2_layers;  /* sum of E * E * L (attenuation at the Compton energy) */
+  int64_t doppler_trials;   /* Doppler words drawn by the Compton terms */
+  int64_t lines;            /* fluorescence lines evaluated (energy above threshold, shell open) */
+  int64_t line_layers;      /* sum of L over those lines */
+} xmi_oracle_work;
+void xmi_oracle_get_work(xmi_oracle_work *w);
+
 /* solid-angle grid restated (src/xmi_solid_angle_f.F90:62-710), one Philox
  * stream per cell; *out malloc'ed (free with xmi_oracle_free_solid_angle). */
 int xmi_oracle_solid_angle_calculation(xmi_inputFPtr inputFPtr, xmi_solid_angle **out, long hits_per_single,
@@ -95,6 +109,11 @@ double xmi_oracle_probe_phi0(const double *dirv, const double *elecv);
  * out[10i..] = entry point, dirv, elecv, energy (NaN when outside the window) */
 int xmi_oracle_probe_source(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options, int which,
                             long n, uint64_t seed, double *out);
+/* the estimator's Doppler-broadened Compton energy (its truncated-profile
+ * draw, one word each) of element element_index of layer 1 at (energy,
+ * theta), n draws from one stream */
+int xmi_oracle_probe_doppler(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi_main_options *options,
+                             int element_index, double energy, double theta, long n, uint64_t seed, double *out);
 /* derived fields after xmi_init_input: out[0..34] detector radius, collimator
  * present/radius, half apex, vertex[3], n_sample_orientation_det[3],
  * n_sample_orientation[3], n_detector_orientation[3], new[3][3], inverse[3][3]
@@ -117,6 +136,8 @@ void xmi_oracle_input_derived(xmi_inputFPtr inputFPtr, double *out);
 #define XMI_ORACLE_PLANT_GAUSS_XY 10           /* x divergence used for y */
 #define XMI_ORACLE_PLANT_COMPTON_T_RAYLEIGH 11 /* Compton azimuth with the Rayleigh parameter */
 #define XMI_ORACLE_PLANT_SHELL_NEXT 12         /* shell drawn with the next shell's cross section */
+#define XMI_ORACLE_PLANT_QIMAX_IGNORED 13      /* advanced Compton: subshell profiles not truncated at Qimax */
+#define XMI_ORACLE_PLANT_DOPPLER_NO_NEGATIVE 14 /* estimator: momenta of negative E' not drawn */
 void xmi_oracle_set_plant(int id);
 
 /* trials of the Monte Carlo solid-angle fallback inside the estimator

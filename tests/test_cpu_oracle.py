@@ -89,9 +89,9 @@ def test_trapezoid_sampler_cdf(orc):
     assert np.all(np.diff(xs) > 0) and xs[0] >= 1.0 and xs[-1] <= 1.1
 
 
-def _prepared(inp, hits=64, seed=3):
+def _prepared(inp, hits=64, seed=3, options=None):
     import oracle
-    prep = api.Prepared(inp)
+    prep = api.Prepared(inp, options=options)
     oracle.solid_angle(prep, hits, seed, n_threads=0)
     return prep
 
@@ -170,6 +170,14 @@ def test_oracle_first_order_scatter_matches_closed_form(orc):
             mc = vr[Z - 1, row, 0]
             ref = orc.scatter_first_order(prep, Z, kind)
             assert mc == pytest.approx(ref, rel=tol), (Z, kind, mc, ref)
+    # the advanced Compton model spreads the same differential cross section
+    # over the subshells (weights summing to one): the same first-order row
+    prep_a = _prepared(inp, hits=64, options=api.default_options(use_advanced_compton=1))
+    _, vr_a, _, _ = orc.run(prep_a, seed=11, rng_mode=orc.RNG_PER_HISTORY)
+    for Z in (8, 16, 20):
+        ref = orc.scatter_first_order(prep_a, Z, "compt")
+        assert vr_a[Z - 1, 384, 0] == pytest.approx(ref, rel=0.02), (Z, vr_a[Z - 1, 384, 0], ref)
+    prep_a.close()
     # a fully horizontally polarised beam does not Rayleigh-scatter towards a
     # detector on the polarisation axis (dipole pattern)
     inp.discrete = [api.DiscreteLine(10.0, 1e12, 0.0)]

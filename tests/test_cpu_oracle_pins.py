@@ -540,3 +540,93 @@ def test_compton_escape_ratio_closed_form(orc):
             assert abs(got - want) < 4 * sig + 0.01 * want, (E, got, want)
     finally:
         prep.close()
+
+
+def test_advanced_compton_energy_follows_the_subshell_profiles(orc):
+    """use_advanced_compton (src/xmi_main.F90:4785-4983): the subshell is
+    drawn with weight occupancy * CDF_k(Qimax_k) and Q from its profile below
+    Qimax_k, where Qimax_k is the momentum at which the scattered photon
+    would carry E - I_k (I_k the subshell's binding energy).  Q recovered from
+    (E, E', theta) with the exact Compton relation then has, given theta, the
+    CDF sum_k n_k min(C_k(Q), C_k(Qimax_k)) / sum_k n_k C_k(Qimax_k), with C_k
+    the two-sided CDF of the hydrogenic subshell profile (closed form)."""
+    E, n = 20.0, 100_000
+    opts = api.default_options(use_advanced_compton=1)
+    inp = api.Input(layers=[api.Layer([26], [1.0], 2.0, 0.1)], discrete=[api.DiscreteLine(E, 1e9, 0.0)],
+                    n_photons_line=10)
+    prep = api.Prepared(inp, options=opts)
+    a = L.atom(26)
+    occ, pk, edge = a["occupancy"], a["profile_p"], a["edge"]
+    sh = np.nonzero(occ > 0)[0]
+
+    def G(u):
+        return u / (4 * (1 + u * u) ** 2) + 3 * u / (8 * (1 + u * u)) + 3 * np.arctan(u) / 8
+
+    def C(k, q):                  # two-sided CDF of subshell k on the tables' range |Q| <= 100
+        return 0.5 + np.sign(q) * G(np.minimum(np.abs(q), 100.0) / pk[k]) / (2 * G(100.0 / pk[k]))
+
+    def q_of(E0, E1, th):          # exact relation, atomic units (the reference's 137)
+        c = np.cos(th)
+        return 137.0 * (E1 - E0 + (1 - c) * E0 * E1 / MEC2) / np.sqrt(E0 ** 2 + E1 ** 2 - 2 * E0 * E1 * c)
+
+    def pit():
+        dn, _, E1 = orc.probe_scatter(prep, "compt", 0, E, D0, E0, n, seed=31)
+        th = np.arccos(np.clip(dn @ D0, -1, 1))
+        keep = (th > 0.05) & (E1 > 0)
+        th, E1 = th[keep], E1[keep]
+        q = q_of(E, E1, th)
+        num = np.zeros_like(q)
+        den = np.zeros_like(q)
+        for k in sh:
+            qmax = np.where(edge[k] > E, 0.0, q_of(E, E - edge[k], th))
+            cm = C(k, qmax)
+            num += occ[k] * np.minimum(C(k, q), cm)
+            den += occ[k] * cm
+        return num / den
+
+    assert stats.kstest(pit(), "uniform").pvalue > 1e-3
+    with orc.planted("qimax_ignored"):
+        assert stats.kstest(pit(), "uniform").pvalue < 1e-6
+    prep.close()
+
+
+@pytest.mark.parametrize("Z,E,theta", [(26, 20.0, 1.6), (82, 40.0, 2.8), (8, 10.0, 0.4)])
+def test_estimator_doppler_draw_equals_the_reference_rejection(orc, Z, E, theta):
+    """The estimator draws the Compton momentum from the accepted part of the
+    profile directly; the reference (src/xmi_variance_reduction.F90:949-1008)
+    redraws r (uniform, the inverse table's last interval excluded) and a
+    sign until E' <= E.  That rejection loop, run here in numpy on the same
+    inverse table, gives the same energy distribution (KS); Pb at 40 keV and
+    backward angles includes the momenta whose energy turns negative, which
+    the reference's test accepts too."""
+    prep = _prep(Z, energy=E)
+    try:
+        got = orc.probe_doppler(prep, 0, E, theta, 200_000, seed=4)
+        import ctypes as C
+        h = C.cast(prep.hdf5F, C.POINTER(L.HDF5)).contents
+        z = h.Zs[0]
+        rn = np.ctypeslib.as_array(z.cp_random_numbers, (z.cp_len,)).copy()
+        icdf = np.ctypeslib.as_array(z.cp_total_icdf, (z.cp_len,)).copy()
+        cc, c0, c1 = float(np.float32(1.2399e-6)), float(np.float32(4.85e-12)), float(np.float32(1.456e-2))
+        rng = np.random.default_rng(12)
+        lam0 = cc / (E * 1000.0)
+        s2 = math.sin(theta / 2)
+        want = []
+        while len(want) < 200_000:
+            r = rng.random(400_000)
+            pos = (r / (rn[1] - rn[0])).astype(int)
+            ok = pos != len(rn) - 2
+            r, pos = r[ok], pos[ok]
+            pz = icdf[pos] + (icdf[pos + 1] - icdf[pos]) * (r - rn[pos]) / (rn[pos + 1] - rn[pos])
+            pz = np.where(rng.random(len(pz)) < 0.5, -pz, pz)
+            e1 = cc / (lam0 + c0 * s2 * s2 - c1 * lam0 * s2 * pz) / 1000.0
+            want.extend(e1[e1 <= E][:200_000 - len(want)])
+        want = np.array(want)
+        assert stats.ks_2samp(got, want).pvalue > 1e-3
+        if Z == 82:
+            pg, pw = (got < 0).mean(), (want < 0).mean()
+            assert pg > 0 and abs(pg - pw) < 5 * math.sqrt(pw / len(want)) + 1e-5, (pg, pw)
+            with orc.planted("doppler_no_negative"):
+                assert (orc.probe_doppler(prep, 0, E, theta, 200_000, seed=4) < 0).mean() == 0.0
+    finally:
+        prep.close()

@@ -69,6 +69,30 @@ def test_same_streams_three_layers_continuum(orc):
     _assert_same(gpu, ref, "kramers")
 
 
+@pytest.mark.parametrize("which", ["steel", "kramers", "random"])
+def test_same_streams_advanced_compton(orc, which):
+    """use_advanced_compton (src/xmi_main.F90:4785-4983,
+    src/xmi_variance_reduction.F90:752-947): subshell-resolved Compton
+    energies in the transport and in the estimator, same streams, 1e-9"""
+    if which == "steel":
+        inp = workloads.single_layer_steel(n_photons_line=6000)
+    elif which == "kramers":
+        inp = workloads.three_layer_kramers(n_photons_interval=15, n_photons_line=1000)
+    else:
+        inp = workloads.random_sample(np.random.default_rng(321), n_layers=(2, 3), n_elements=(3, 8))
+        inp.n_photons_line = 4000
+    print(which, flush=True)
+    prep = _prep(inp, options=api.default_options(use_advanced_compton=1))
+    gpu = prep.run(seed=19)
+    ref = orc.run(prep, seed=19, rng_mode=orc.RNG_PER_HISTORY)
+    assert gpu[3].histories == ref[3].histories
+    assert gpu[1][:, 384, :].sum() > 0
+    _assert_same(gpu, ref, "advanced " + which)
+    # the Compton rows differ from the default model's (other energies and weights)
+    prep_d = _prep(inp)
+    assert not np.array_equal(prep_d.run(seed=19)[1][:, 384, :], gpu[1][:, 384, :])
+
+
 @pytest.mark.parametrize("case", range(6))
 def test_same_streams_random_samples(orc, case):
     rng = np.random.default_rng(100 + case)

@@ -615,15 +615,17 @@ def detector_response_noalloc(prep: "Prepared", channels: np.ndarray, var_red: O
 
 
 def run_batch(preps: Sequence["Prepared"], tables: "SharedTables", seed: int, n_streams: int = 4,
-              options: Optional[xmi_main_options] = None):
+              options: Optional[xmi_main_options] = None, seeds: Optional[Sequence[int]] = None):
     """xmi_main_msim_batch over inputs bound to `tables`, each with a
-    solid-angle grid; returns ([(channels, var_red_history)], done_ms)."""
+    solid-angle grid (seeds: one per sample, else `seed` for all); returns
+    ([(channels, var_red_history)], done_ms) and leaves the batch totals in
+    batch_stats()."""
     lib = load_library()
     n = len(preps)
     P = C.POINTER
     lib.xmi_main_msim_batch.argtypes = [C.c_int, P(C.c_void_p), C.c_void_p, C.c_int, P(xmi_main_options),
                                         P(P(xmi_solid_angle)), P(P(C.c_double)), P(P(C.c_double)), C.c_int,
-                                        P(C.c_double)]
+                                        P(C.c_double), P(C.c_uint64)]
     lib.xmi_main_msim_batch.restype = C.c_int
     inputs = (C.c_void_p * n)(*[p.inputF.value for p in preps])
     sas = (P(xmi_solid_angle) * n)(*[p.solid_angle for p in preps])
@@ -632,7 +634,8 @@ def run_batch(preps: Sequence["Prepared"], tables: "SharedTables", seed: int, n_
     done = (C.c_double * n)()
     opts = options or preps[0].options
     lib.xmi_set_seed(seed)
-    if not lib.xmi_main_msim_batch(n, inputs, tables.hdf5F, 1, C.byref(opts), sas, ch, vr, n_streams, done):
+    sd = (C.c_uint64 * n)(*[int(x) for x in seeds]) if seeds is not None else None
+    if not lib.xmi_main_msim_batch(n, inputs, tables.hdf5F, 1, C.byref(opts), sas, ch, vr, n_streams, done, sd):
         raise RuntimeError("xmi_main_msim_batch failed: " + last_error())
     out = []
     for k, p in enumerate(preps):
@@ -643,6 +646,13 @@ def run_batch(preps: Sequence["Prepared"], tables: "SharedTables", seed: int, n_
         lib.xmi_free(vr[k])
         out.append((c, v))
     return out, np.array(done)
+
+
+def batch_stats() -> "xmi_msim_stats":
+    """Totals of the last xmi_main_msim_batch call on this thread."""
+    st = xmi_msim_stats()
+    load_library().xmi_msim_get_stats(C.byref(st))
+    return st
 
 
 class SharedTables:

@@ -126,6 +126,16 @@ struct EngineParams {
   const double *cp_icdf;        /* [nZ][cp_len] */
   const double *cp_rn;          /* [cp_len] shared */
   int cp_len;
+  const double *cp_cdfg;        /* [nZ][XMI_CP_GRID] profile CDF on a uniform momentum grid */
+  const double *cp_gscale;      /* [nZ] grid points per unit momentum */
+  /* the advanced Compton model (use_advanced_compton): the occupied
+   * subshells of element u are g = cp_sh_off[u] .. cp_sh_off[u+1]-1, with
+   * the provider subshell cp_sh_idx[g], the partial-profile CDF
+   * cp_pcdf[g][cp_len] on the Q grid cp_Qs and its inverse cp_qinv[g][cp_len]
+   * on the CDF grid cp_inv (both grids uniform and shared) */
+  int adv_compton;
+  const int *cp_sh_off, *cp_sh_idx;
+  const double *cp_pcdf, *cp_qinv, *cp_Qs, *cp_inv;
   const double *precalc_xrf;    /* [nZ][9][nZ][219] */
   const double *line_E;         /* [nZ][219] */
   const double *precalc_mu;     /* [n_layers][nZ][219] */

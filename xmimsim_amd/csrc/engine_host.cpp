@@ -113,7 +113,12 @@ struct DeviceTables {
   int nZ = 0;
   int Zs[XMI_MAXZ + 1];
   DevBuf atoms, ray, comp, tabE, tabR, fycorr, ipE, ipr, iprc, ipoff, ipacc, cp_icdf, cp_rn, precalc_xrf, lineE;
+  DevBuf cp_cdfg, cp_gscale;
   DevBuf phi, phiT, phiR;
+  /* the advanced Compton model's subshell profiles, uploaded on first use */
+  bool adv_ready = false;
+  bool has_precalc = true; /* every element has its cascade cross sections */
+  DevBuf cp_shoff, cp_shidx, cp_pcdf, cp_qinv, cp_Qs, cp_inv;
   int n_e = 0, n_r = 0, cp_len = 0, phi_nt = 0, phi_nr = 0;
   double tab_E0 = 0, tab_Einv = 0, tab_R0 = 0, tab_Rinv = 0, phi_T0 = 0, phi_Tinv = 0, phi_R0 = 0, phi_Rinv = 0;
   double ip_acc_E0 = 0, ip_acc_inv = 0;
@@ -189,6 +194,7 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
   t->cp_len = z0->cp_len;
   size_t plane = (size_t)t->n_e * t->n_r;
   std::vector<double> ray(plane * nZ), comp(plane * nZ), fy(9 * (size_t)nZ), cp((size_t)t->cp_len * nZ);
+  std::vector<double> cdfg((size_t)XMI_CP_GRID * nZ), gscale(nZ);
   std::vector<double> ipE, ipr, iprc;
   std::vector<int> ipoff(nZ + 1), ipacc((size_t)nZ * ENG_IP_ACC);
   std::vector<double> pxrf((size_t)nZ * 9 * nZ * XMI_NLINES_VR, 0.0), lineE((size_t)nZ * XMI_NLINES_VR, 0.0);
@@ -197,7 +203,7 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
   t->ip_acc_inv = (double)ENG_IP_ACC / (Emax - t->ip_acc_E0);
   for (int i = 0; i < nZ; i++) {
     const xmi_hdf5_ZF *z = &h->Zs[i];
-    if (z->n_e != t->n_e || z->n_r != t->n_r || z->cp_len != t->cp_len || !z->precalc_xrf_cs) {
+    if (z->n_e != t->n_e || z->n_r != t->n_r || z->cp_len != t->cp_len) {
       xmi_set_error("inconsistent table dimensions for Z=%d", z->Z);
       delete t;
       return nullptr;
@@ -206,6 +212,9 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
     memcpy(&comp[plane * i], z->ComptonTheta_ICDF, sizeof(double) * plane);
     memcpy(&fy[9 * (size_t)i], z->FluorYieldsCorr, sizeof(double) * 9);
     memcpy(&cp[(size_t)t->cp_len * i], z->cp_total_icdf, sizeof(double) * t->cp_len);
+    xmi_profile_cdf_grid(z->cp_random_numbers, z->cp_total_icdf, z->cp_len, XMI_CP_GRID,
+                         &cdfg[(size_t)XMI_CP_GRID * i]);
+    gscale[i] = (double)(XMI_CP_GRID - 1) / z->cp_total_icdf[z->cp_len - 2];
     ipoff[i] = (int)ipE.size();
     ipE.insert(ipE.end(), z->ip_energies, z->ip_energies + z->ip_len);
     ipr.insert(ipr.end(), z->ip_rayl, z->ip_rayl + z->ip_len);
@@ -217,7 +226,12 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
       while (p + 2 < z->ip_len && z->ip_energies[p + 1] < E) p++;
       ipacc[(size_t)i * ENG_IP_ACC + c] = p;
     }
-    memcpy(&pxrf[(size_t)i * 9 * nZ * XMI_NLINES_VR], z->precalc_xrf_cs, sizeof(double) * 9 * nZ * XMI_NLINES_VR);
+    /* a table file read without variance reduction holds no cascade cross
+     * sections (src/xmi_data_f.F90 reads them for that mode only) */
+    if (z->precalc_xrf_cs)
+      memcpy(&pxrf[(size_t)i * 9 * nZ * XMI_NLINES_VR], z->precalc_xrf_cs, sizeof(double) * 9 * nZ * XMI_NLINES_VR);
+    else
+      t->has_precalc = false;
     for (int l = 1; l <= XMI_NLINES_VR; l++) lineE[(size_t)i * XMI_NLINES_VR + l - 1] = atoms[i].line_energy[l];
   }
   ipoff[nZ] = (int)ipE.size();
@@ -245,6 +259,8 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
   if (e == hipSuccess) e = upload(t->ipacc, ipacc.data(), ipacc.size());
   if (e == hipSuccess) e = upload(t->cp_icdf, cp.data(), cp.size());
   if (e == hipSuccess) e = upload(t->cp_rn, z0->cp_random_numbers, (size_t)t->cp_len);
+  if (e == hipSuccess) e = upload(t->cp_cdfg, cdfg.data(), cdfg.size());
+  if (e == hipSuccess) e = upload(t->cp_gscale, gscale.data(), gscale.size());
   if (e == hipSuccess) e = upload(t->precalc_xrf, pxrf.data(), pxrf.size());
   if (e == hipSuccess) e = upload(t->lineE, lineE.data(), lineE.size());
   if (e == hipSuccess) e = upload(t->phi, h->Phi_ICDF, (size_t)h->phi_n_theta * h->phi_n_r);
@@ -257,6 +273,40 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
   }
   g_tables.push_back(t);
   return t;
+}
+
+/* the partial Compton profiles of every element, flattened over the
+ * occupied subshells (advanced Compton model only) */
+static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h) {
+  std::lock_guard<std::mutex> g(g_tables_lock);
+  if (t->adv_ready) return 1;
+  const int nZ = h->n_Z, N = t->cp_len;
+  std::vector<int> off(nZ + 1), idx;
+  std::vector<double> pcdf, qinv;
+  for (int i = 0; i < nZ; i++) {
+    const xmi_hdf5_ZF *z = &h->Zs[i];
+    off[i] = (int)idx.size();
+    if (!z->cp_partial_cdf || !z->cp_Qs_inv || !z->cp_Qs || !z->cp_partial_cdf_inv) {
+      xmi_set_error("advanced Compton model: no partial Compton profiles for Z=%d", z->Z);
+      return 0;
+    }
+    for (int k = 0; k < z->cp_nshells; k++) idx.push_back(z->cp_shell_indices[k]);
+    pcdf.insert(pcdf.end(), z->cp_partial_cdf, z->cp_partial_cdf + (size_t)z->cp_nshells * N);
+    qinv.insert(qinv.end(), z->cp_Qs_inv, z->cp_Qs_inv + (size_t)z->cp_nshells * N);
+  }
+  off[nZ] = (int)idx.size();
+  hipError_t e = upload(t->cp_shoff, off.data(), off.size());
+  if (e == hipSuccess) e = upload(t->cp_shidx, idx.data(), idx.size());
+  if (e == hipSuccess) e = upload(t->cp_pcdf, pcdf.data(), pcdf.size());
+  if (e == hipSuccess) e = upload(t->cp_qinv, qinv.data(), qinv.size());
+  if (e == hipSuccess) e = upload(t->cp_Qs, h->Zs[0].cp_Qs, (size_t)N);
+  if (e == hipSuccess) e = upload(t->cp_inv, h->Zs[0].cp_partial_cdf_inv, (size_t)N);
+  if (e != hipSuccess) {
+    xmi_set_error("advanced Compton table upload failed: %s", hipGetErrorString(e));
+    return 0;
+  }
+  t->adv_ready = true;
+  return 1;
 }
 
 /* released through xmi_hdf5F's destructor hook */
@@ -368,7 +418,8 @@ struct EscapeRun {
 
 static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
                     double **channelsPtr, xmi_main_options *options, double **brute_historyPtr,
-                    double **var_red_historyPtr, xmi_solid_angle *solid_angles, EscapeRun *esc = nullptr) {
+                    double **var_red_historyPtr, xmi_solid_angle *solid_angles, EscapeRun *esc = nullptr,
+                    const uint64_t *seed = nullptr) {
   auto t_wall0 = std::chrono::steady_clock::now();
   memset(&g_stats, 0, sizeof(g_stats));
   xmi_inputF *in = (xmi_inputF *)inputFPtr;
@@ -392,10 +443,6 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
                   (long)in->general.n_interactions_trajectory);
     return 0;
   }
-  if (options->use_advanced_compton) {
-    xmi_set_error("xmi_main_msim: use_advanced_compton is not supported");
-    return 0;
-  }
   if (!esc && (!solid_angles || !solid_angles->solid_angles)) {
     xmi_set_error("xmi_main_msim: variance reduction needs a solid-angle grid");
     return 0;
@@ -408,6 +455,12 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   if (!engine_init(E)) return 0;
   DeviceTables *T = get_tables(h);
   if (!T) return 0;
+  if (options->use_advanced_compton && !ensure_adv_tables(T, h)) return 0;
+  if (!esc && !T->has_precalc) {
+    xmi_set_error("xmi_main_msim: the tables were read without the cross sections of the variance reduction "
+                  "(xmi_init_from_hdf5 with use_variance_reduction=0)");
+    return 0;
+  }
   if (!esc && !ensure_grid(E, solid_angles)) return 0;
   hipStream_t st = E.stream;
   const int nZ = T->nZ;
@@ -431,7 +484,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   P.cascade = cascade_of(options);
   P.use_M_lines = options->use_M_lines;
   P.hits_per_single = xmi_hits_per_single;
-  P.seed = xmi_engine_current_seed(options);
+  P.seed = seed ? *seed : xmi_engine_current_seed(options);
   P.sample = 0;
   P.nZ = nZ;
 
@@ -809,6 +862,17 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.cp_icdf = T->cp_icdf.as<double>();
     P.cp_rn = T->cp_rn.as<double>();
     P.cp_len = T->cp_len;
+    P.cp_cdfg = T->cp_cdfg.as<double>();
+    P.cp_gscale = T->cp_gscale.as<double>();
+    P.adv_compton = options->use_advanced_compton ? 1 : 0;
+    if (P.adv_compton) {
+      P.cp_sh_off = T->cp_shoff.as<int>();
+      P.cp_sh_idx = T->cp_shidx.as<int>();
+      P.cp_pcdf = T->cp_pcdf.as<double>();
+      P.cp_qinv = T->cp_qinv.as<double>();
+      P.cp_Qs = T->cp_Qs.as<double>();
+      P.cp_inv = T->cp_inv.as<double>();
+    }
     P.precalc_xrf = T->precalc_xrf.as<double>();
     P.line_E = T->lineE.as<double>();
     P.precalc_mu = G.precalc_mu.as<double>();
@@ -1216,7 +1280,8 @@ static double sample_histories(const xmi_inputF *in, int n_mpi_hosts) {
 
 extern "C" int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts,
                                    xmi_main_options *options, xmi_solid_angle **solid_angles, double **channels,
-                                   double **var_red_history, int n_streams, double *done_ms) {
+                                   double **var_red_history, int n_streams, double *done_ms,
+                                   const uint64_t *seeds) {
   if (n_samples < 0 || (n_samples > 0 && (!inputs || !hdf5FPtr || !options || !solid_angles || !channels ||
                                           !var_red_history))) {
     xmi_set_error("xmi_main_msim_batch: NULL argument");
@@ -1241,6 +1306,7 @@ extern "C" int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf
            sample_histories((const xmi_inputF *)inputs[b], n_mpi_hosts);
   });
   std::atomic<int> next(0), failed(0);
+  std::atomic<int64_t> histories(0), vr_events(0);
   std::string first_error;
   std::mutex err_lock;
   const auto t0 = std::chrono::steady_clock::now();
@@ -1253,12 +1319,14 @@ extern "C" int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf
       const int k = order[q];
       double *brute = nullptr;
       if (!msim_run(E, inputs[k], hdf5FPtr, n_mpi_hosts, &channels[k], options, &brute, &var_red_history[k],
-                    solid_angles[k])) {
+                    solid_angles[k], nullptr, seeds ? &seeds[k] : nullptr)) {
         std::lock_guard<std::mutex> g(err_lock);
         if (!failed.exchange(1)) first_error = xmi_msim_last_error();
         break;
       }
       free(brute);
+      histories += g_stats.histories;
+      vr_events += g_stats.vr_events;
       if (done_ms)
         done_ms[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1275,6 +1343,11 @@ extern "C" int xmi_main_msim_batch(int n_samples, xmi_inputFPtr *inputs, xmi_hdf
     xmi_set_error("xmi_main_msim_batch: %s", first_error.c_str());
     return 0;
   }
+  /* the calling thread's statistics: totals of the batch */
+  memset(&g_stats, 0, sizeof(g_stats));
+  g_stats.histories = histories.load();
+  g_stats.vr_events = vr_events.load();
+  g_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 1;
 }
 

@@ -1002,7 +1002,95 @@ __device__ __forceinline__ uint32_t d_ww_word(WaveWords &w) {
   return x;
 }
 
-template <int NL, bool use_lds>
+/* the Compton profile's CDF at momentum pz >= 0 from its grid (the
+ * xmi_profile_cdf_grid table of the element; gs grid points per unit
+ * momentum), clamped to the grid's last point */
+__device__ __forceinline__ double d_profile_cdf(const double *g, double gs, double pz) {
+  if (!(pz > 0.0)) return 0.0;
+  const double x = pz * gs;
+  if (x >= (double)(XMI_CP_GRID - 1)) return g[XMI_CP_GRID - 1];
+  const int j = (int)x;
+  return g[j] + (g[j + 1] - g[j]) * (x - (double)j);
+}
+
+/* ---- the advanced Compton model (use_advanced_compton) ------------------
+ * src/xmi_main.F90:4785-4983 (transport) and src/xmi_variance_reduction.F90:
+ * 752-947 (estimator) with src/xmi_aux_f.F90:1951-2057 (kinematics) */
+
+/* the largest projected momentum an electron of binding energy Ii can carry
+ * for scattering by theta (xmi_get_qimax) */
+__device__ __forceinline__ double d_qimax(double energy, double Ii, double theta) {
+  if (Ii != 0.0 && energy < Ii) return 0.0;
+  const double EminIi = energy - Ii, ct = cos(theta);
+  const double Q = 137.0 * (EminIi * energy * (1.0 - ct) / ENG_MEC2 - Ii);
+  return Q / sqrt(EminIi * EminIi + energy * energy - 2.0 * EminIi * energy * ct);
+}
+
+__device__ __forceinline__ double d_q_from_energy(double E0, double E1, double theta) {
+  const double ct = cos(theta);
+  const double Q = 137.0 * (E1 - E0 + (1.0 - ct) * E0 * E1 / ENG_MEC2);
+  return Q / sqrt(E1 * E1 + E0 * E0 - 2.0 * E0 * E1 * ct);
+}
+
+/* xmi_get_energy_from_q: the root of the quadratic whose momentum has Q's
+ * sign; 0 in the reference's exceptions, -1 when no root has it (the
+ * reference stops there) */
+__device__ double d_energy_from_q(double E0, double Q, double theta) {
+  const double c = cos(theta);
+  if (fabs(c - 1.0) < 1e-8) return 0.0;
+  if (fabs(Q) < 1e-4) return E0 / (1.0 + E0 * (1.0 - c) / ENG_MEC2);
+  const double d = 1.0 + E0 / ENG_MEC2 - E0 * c / ENG_MEC2;
+  const double aq = 137.0 * 137.0 * d * d - Q * Q;
+  const double bq = -2.0 * 137.0 * 137.0 * E0 * d + 2.0 * E0 * c * Q * Q;
+  const double cq = 137.0 * 137.0 * E0 * E0 - E0 * E0 * Q * Q;
+  double E1, E2;
+  if (aq == 0.0) {
+    if (bq == 0.0) return 0.0;
+    E1 = E2 = -1.0 * cq / bq;
+  } else {
+    const double delta = bq * bq - 4.0 * aq * cq;
+    if (delta < 0.0) return 0.0;
+    if (delta == 0.0) {
+      E1 = E2 = -bq / 2.0 / aq;
+    } else {
+      const double sd = sqrt(delta);
+      const double t1 = (-bq + sd) / 2.0 / aq, t2 = (-bq - sd) / 2.0 / aq;
+      E1 = t1 < t2 ? t1 : t2;
+      E2 = t1 < t2 ? t2 : t1;
+    }
+  }
+  const double Q1 = d_q_from_energy(E0, E1, theta), Q2 = d_q_from_energy(E0, E2, theta);
+  if (Q * Q1 > 0.0) return E1;
+  if (Q * Q2 > 0.0) return E2;
+  if (fabs(E1 - E2) < 1e-10 || fabs(Q1 - Q2) < 1e-10) return E1;
+  return -1.0;
+}
+
+/* subshell g's two-sided partial-profile CDF at its Qimax */
+__device__ __forceinline__ double d_adv_cdf(const EngineParams &P, const xrl_atom *a, int g, double energy,
+                                            double theta) {
+  const double q = d_qimax(energy, a->edge[P.cp_sh_idx[g]], theta);
+  if (q < -100.0) return 0.0;
+  if (q > 100.0) return 1.0;
+  const double qp = fabs(q);
+  const double *Qs = P.cp_Qs, *pc = P.cp_pcdf + (size_t)g * P.cp_len;
+  int pos = (int)(qp / (Qs[1] - Qs[0]));
+  if (pos > P.cp_len - 2) pos = P.cp_len - 2;
+  const double v = d_interp(Qs[pos], pc[pos], Qs[pos + 1], pc[pos + 1], qp);
+  return q < 0.0 ? 1.0 - (0.5 + v) : 0.5 + v;
+}
+
+/* the projected momentum at a value cdf of subshell g's two-sided profile */
+__device__ __forceinline__ double d_adv_q(const EngineParams &P, int g, double cdf) {
+  const double *inv = P.cp_inv, *qi = P.cp_qinv + (size_t)g * P.cp_len;
+  const double cp = cdf < 0.5 ? 0.5 - cdf : cdf - 0.5;
+  int pos = (int)(cp / (inv[1] - inv[0]));
+  if (pos > P.cp_len - 2) pos = P.cp_len - 2;
+  const double Q = d_interp(inv[pos], qi[pos], inv[pos + 1], qi[pos + 1], cp);
+  return cdf < 0.5 ? -Q : Q;
+}
+
+template <int NL, bool use_lds, bool ADV>
 __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_vr_scatter(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): channels [nch][2], Rayleigh/Compton rows
    * [nZ][2][2]; flushed once per workgroup */
@@ -1039,7 +1127,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
     const int info = e < lc.tot ? V.info[i] : 0;
     const int mine = info & VR_ACTIVE;
     /* per-photon factors, shared by every element of the layer */
-    double E = 0.0, wo = 0.0, pesc = 0.0, inv_mu = 0.0, c_lamb0 = 1.0, A0 = 0.0, A1 = 0.0;
+    double E = 0.0, wo = 0.0, pesc = 0.0, inv_mu = 0.0, c_lamb0 = 1.0, A0 = 0.0, A1 = 0.0, th_e = 0.0;
+    phx_stream sd; /* advanced model: uniforms from the same stream position */
     /* rho * path in the k-th layer crossed: in registers for up to four
      * layers, read from the records beyond (register budget) */
     constexpr int NR = NL <= 4 ? NL : 1;
@@ -1075,7 +1164,12 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
         s.ctr[3] = s0.ctr[3];
         s.na = 0;
         s.hasb = 0;
+        if (ADV) {
+          sd = s0;
+          sd.ctr[0] = V.blk[i];
+        }
       }
+      th_e = theta;
       dc = xrl_dcsp_setup(E, theta, phi);
       const double sth2 = sin(theta / 2.0);
       c_lamb0 = cc / (E * 1000.0);
@@ -1100,30 +1194,67 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
       /* Compton, xmi_compton_varred2: Doppler-broadened energy from the
        * element's Compton profile, continuing the photon's estimator stream */
       double twc = 0.0;
-      if (mine) {
+      if (ADV && mine) {
+        /* every occupied subshell with weight count * CDF(Qimax) / sum, one
+         * uniform each (xmi_compton_varred) */
+        const int g0 = P.cp_sh_off[u], g1 = P.cp_sh_off[u + 1];
+        double csum = 0.0;
+        for (int g = g0; g < g1; g++) csum += a->occupancy[P.cp_sh_idx[g]] * d_adv_cdf(P, a, g, E, th_e);
+        const double dcw = Pconv * (xrl_dcsp_compt_c(a, &dc) * wo);
+        for (int g = g0; csum != 0.0 && g < g1; g++) {
+          const double cdf = d_adv_cdf(P, a, g, E, th_e);
+          const double sw = a->occupancy[P.cp_sh_idx[g]] * cdf / csum;
+          if (sw == 0.0) continue;
+          const double energy = d_energy_from_q(E, d_adv_q(P, g, phx_uniform(&sd) * cdf), th_e);
+          if (!(energy > 0.0)) continue;
+          const xrl_ectx ce = xrl_energy(energy);
+          double m2 = 0.0;
+#pragma unroll 1
+          for (int kk = 0; kk < nc; kk++) {
+            const int l0 = L - 1 + kk * dir;
+            m2 += d_layer_mu_tab(P, l0, &ce) * (P.layer_density[l0] * V.dist[(size_t)kk * cap + i]);
+          }
+          const double tw = dcw * d_exp_neg(-m2, exp_tab) * sw;
+          twc += tw;
+          const int ch = energy >= 1.0 ? (int)((energy - P.zero) * inv_gain) : -1;
+          if (ch >= 0 && ch <= P.nch - 1 && tw > 0.0) fx_add_global(&tch[2 * ch], tw, shift);
+        }
+      }
+      if (!ADV && mine) {
+        /* xmi_compton_varred2 (src/xmi_variance_reduction.F90:949-1008)
+         * redraws the momentum until E' <= E; the accepted momenta are pz <=
+         * A0/A1 and, where E' turns negative, pz > (c_lamb0 + A0)/A1.  They
+         * are drawn here directly from that part of the profile, one word
+         * each (the same distribution, and no wavefront waits on its lanes'
+         * longest rejection run); r ranges over [0, r_end), the inverse
+         * table without its last interval */
         const double *rn = P.cp_rn;
         const double *icdf = P.cp_icdf + (size_t)u * P.cp_len;
-        double energy = 0.0;
-        int it = 0, failed = 0;
-        for (;;) {
-          /* one 32-bit word per trial: 31 bits of random number, a sign bit */
-          const uint32_t wd = d_ww_word(s);
-          const double r = (double)(wd >> 1) * 0x1.0p-31;
-          const int pos = (int)(r * inv_drn);
-          if (pos == P.cp_len - 2) continue;
-          double pz = icdf[pos] + (icdf[pos + 1] - icdf[pos]) * ((r - rn[pos]) * inv_drn);
-          if (wd & 1u) pz = -pz;
-          energy = (cc / 1000.0) / (c_lamb0 + (A0 - A1 * pz));
-          /* a large positive pz can give a negative energy: accepted, as in
-           * the reference, where it then only reaches the history row */
-          if (energy <= E) break;
-          if (it == 100) {
-            failed = 1; /* the reference stops with an error here */
-            break;
-          }
-          it++;
+        const double *g = P.cp_cdfg + (size_t)u * XMI_CP_GRID;
+        const double gs = P.cp_gscale[u];
+        const double r_end = rn[P.cp_len - 2];
+        double f_max = r_end, f_neg = r_end;
+        if (A1 > 0.0) {
+          f_max = d_profile_cdf(g, gs, A0 / A1);
+          f_neg = d_profile_cdf(g, gs, (c_lamb0 + A0) / A1);
         }
-        if (!failed) {
+        const double m_pos = f_max + (r_end - f_neg);
+        const uint32_t wd = d_ww_word(s);
+        const double uu = (double)wd * 0x1.0p-32 * (r_end + m_pos);
+        double r, sgn;
+        if (uu < r_end) {
+          r = uu;
+          sgn = -1.0;
+        } else {
+          const double v = uu - r_end;
+          r = v < f_max ? v : v - f_max + f_neg;
+          sgn = 1.0;
+        }
+        int pos = (int)(r * inv_drn);
+        if (pos > P.cp_len - 3) pos = P.cp_len - 3;
+        const double pz = sgn * (icdf[pos] + (icdf[pos + 1] - icdf[pos]) * ((r - rn[pos]) * inv_drn));
+        const double energy = (cc / 1000.0) / (c_lamb0 + (A0 - A1 * pz));
+        {
           const xrl_ectx ce = xrl_energy(energy);
           double m2 = 0.0;
           if (NL <= 4) {
@@ -1140,9 +1271,6 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
           twc = Pconv * (xrl_dcsp_compt_c(a, &dc) * wo) * d_exp_neg(-m2, exp_tab);
           const int ch = energy >= 1.0 ? (int)((energy - P.zero) * inv_gain) : -1;
           if (ch >= 0 && ch <= P.nch - 1 && twc > 0.0) fx_add_global(&tch[2 * ch], twc, shift);
-        } else {
-          atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull);
-          atomicAdd(d_ctr(P, ENG_CNT_ERR_VR_DOPPLER), 1ull);
         }
       }
       /* history rows: wave totals collected in lanes, 32 elements a group */
@@ -1397,6 +1525,40 @@ __device__ __forceinline__ void d_compton_one(const EngineParams &P, const Photo
   double th = S.theta[i], ph = S.phi[i];
   double ev[3] = {S.ex[i], S.ey[i], S.ez[i]};
   double phi0 = d_phi0(th, ph, ev);
+  if (P.adv_compton) {
+    /* xmi_update_photon_energy_compton: the subshell by count * CDF(Qimax),
+     * Q from its profile below Qimax, the energy from Q; 0 ends the photon */
+    const xrl_atom *a = &P.atoms[u];
+    const int g0 = P.cp_sh_off[u], g1 = P.cp_sh_off[u + 1];
+    double csum = 0.0;
+    for (int g = g0; g < g1; g++) csum += a->occupancy[P.cp_sh_idx[g]] * d_adv_cdf(P, a, g, E, theta_i);
+    if (csum == 0.0) {
+      S.energy[i] = 0.0;
+      return;
+    }
+    const double r = phx_uniform(&s);
+    double ts = 0.0, cdf_sel = 0.0;
+    int sel = -1, last = g0;
+    for (int g = g0; g < g1; g++) {
+      const double w = a->occupancy[P.cp_sh_idx[g]] * d_adv_cdf(P, a, g, E, theta_i);
+      if (w > 0.0) last = g;
+      ts += w / csum;
+      if (r <= ts) {
+        sel = g;
+        break;
+      }
+    }
+    if (sel < 0) sel = last; /* rounding left the running sum just below r */
+    cdf_sel = d_adv_cdf(P, a, sel, E, theta_i);
+    const double e = d_energy_from_q(E, d_adv_q(P, sel, phx_uniform(&s) * cdf_sel), theta_i);
+    if (!(e > 0.0)) {
+      S.energy[i] = 0.0;
+      return;
+    }
+    E = e;
+    S.energy[i] = E;
+    d_mu_all(P, E, S.mus, cap, i);
+  } else {
   /* xmi_update_photon_energy_compton2 */
   const double cc = (double)1.2399e-6f, c0 = (double)4.85e-12f, c1 = (double)1.456e-2f;
   double energy = E * 1000.0;
@@ -1431,6 +1593,7 @@ __device__ __forceinline__ void d_compton_one(const EngineParams &P, const Photo
   S.energy[i] = E;
   d_mu_all(P, E, S.mus, cap, i);
   if (E == 0.0) return;
+  }
   double dirv[3];
   d_update_dirv(dirv, &th, &ph, theta_i, phi_i + phi0);
   d_update_elecv(dirv, ev);
@@ -1789,6 +1952,18 @@ static void launch_lines(const EngineParams *P, const PhotonSoA *S, const VrSoA 
   }
 }
 
+template <int NL, bool ADV>
+static void launch_scatter(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
+                           size_t ls, hipStream_t st) {
+  if (ls) {
+    int vb = resident_blocks((const void *)k_vr_scatter<NL, true, ADV>, ls, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_scatter<NL, true, ADV>), dim3(vb), dim3(ENG_BLOCK), ls, st, *P, *S, *V, n);
+  } else {
+    int vb = resident_blocks((const void *)k_vr_scatter<NL, false, ADV>, 0, n_cu, blocks);
+    hipLaunchKernelGGL((k_vr_scatter<NL, false, ADV>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+  }
+}
+
 template <int NL>
 static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count,
                                 int n_cu, int blocks, hipStream_t st, hipEvent_t *ev) {
@@ -1800,13 +1975,10 @@ static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const
   hipLaunchKernelGGL(k_vr_cone, dim3(cb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   if (ev) (void)hipEventRecord(ev[1], st);
   const size_t ls = scatter_lds_bytes(P);
-  if (ls) {
-    int vb = resident_blocks((const void *)k_vr_scatter<NL, true>, ls, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_scatter<NL, true>), dim3(vb), dim3(ENG_BLOCK), ls, st, *P, *S, *V, n);
-  } else {
-    int vb = resident_blocks((const void *)k_vr_scatter<NL, false>, 0, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_scatter<NL, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
-  }
+  if (P->adv_compton)
+    launch_scatter<NL, true>(P, S, V, n, n_cu, blocks, ls, st);
+  else
+    launch_scatter<NL, false>(P, S, V, n, n_cu, blocks, ls, st);
   if (ev) (void)hipEventRecord(ev[2], st);
   launch_lines<NL>(P, S, V, n, n_cu, blocks, st);
   if (ev) (void)hipEventRecord(ev[3], st);

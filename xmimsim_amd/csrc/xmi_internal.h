@@ -124,6 +124,27 @@ int xmi_tables_fill_gpu(xmi_hdf5_ZF *zs, int n, xmi_hdf5F *h);
 int xmi_tables_use_device(void);
 void xmi_tables_free(xmi_hdf5F *h);
 
+/* The CDF of the Compton-profile inverse table (rn uniform in [0, 1], icdf
+ * increasing, n entries) at m equally spaced momenta 0 .. icdf[n-2], the
+ * range the estimator draws from (the last interval excluded): out[j] =
+ * r at which the piecewise-linear icdf reaches j * icdf[n-2] / (m - 1).  The
+ * detection-weight estimator reads it to draw Doppler momenta from the
+ * truncated profile directly instead of by rejection. */
+static inline void xmi_profile_cdf_grid(const double *rn, const double *icdf, int n, int m, double *out) {
+  const double top = icdf[n - 2];
+  int k = 0;
+  for (int j = 0; j < m; j++) {
+    const double pz = top * j / (double)(m - 1);
+    while (k < n - 3 && icdf[k + 1] < pz) k++;
+    const double d = icdf[k + 1] - icdf[k];
+    double r = d > 0.0 ? rn[k] + (pz - icdf[k]) / d * (rn[k + 1] - rn[k]) : rn[k + 1];
+    if (r > rn[n - 2]) r = rn[n - 2];
+    out[j] = r;
+  }
+  out[0] = 0.0;
+  out[m - 1] = rn[n - 2];
+}
+
 /* geometry helpers (xmi_input.c) */
 void xmi_normalize3(double v[3]);
 int xmi_unique_Z(const xmi_inputF *in, int *uniqZ);  /* sorted, returns count */

@@ -306,6 +306,10 @@ XRL_FN double xrl_compton_profile(const xrl_atom *a, double pz) {
   return sum;
 }
 
+/* points of the Compton-profile CDF grid that the detection-weight estimator
+ * draws truncated profiles from (xmi_profile_cdf_grid) */
+#define XMI_CP_GRID 4096
+
 /* CDF of the hydrogenic profile shape: int_0^u (1 + t^2)^-3 dt */
 XRL_FN double xrl_profile_G3(double u) {
   double d = 1.0 + u * u;
