@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--train-samples", type=int, default=32768, help="quantify: Monte Carlo training samples")
     ap.add_argument("--train-steps", type=int, default=8000, help="quantify: Adam steps")
     ap.add_argument("--quant-samples", type=int, default=64, help="quantify: unknown samples")
+    ap.add_argument("--train-photons", type=int, default=100_000, help="quantify: photons per training sample")
+    ap.add_argument("--plain", default="pymca", choices=["pymca", "gn"],
+                    help="quantify: the plain Monte Carlo loop (the reference's, or Gauss-Newton)")
     ap.add_argument("--photons", type=float, default=1e7, help="photons per interval and per line")
     ap.add_argument("--grid-hits", type=int, default=5000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
@@ -70,33 +73,35 @@ def make_input(args):
     return workloads.three_layer_kramers(n_photons_interval=n, n_photons_line=n)
 
 
-def vr_bytes_per_event(inp, layer_events, n_trials=2.0):
+def estimator_work(workload):
+    """Per-event work of the detection-weight estimator, counted by the oracle
+    (the CPU restatement, same algorithm and random streams) on a sample of the
+    workload and frozen in profiles/estimator_work_<workload>.json by
+    oracle/count_work.py: E (elements of the event's layer), E*L and E*E*L
+    (L = layers on the escape path), Doppler trials, fluorescence lines
+    evaluated and their sum of L."""
+    path = os.path.join(ROOT, "profiles", f"estimator_work_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["per_event"]
+
+
+def vr_bytes_per_event(w):
     """Algorithmic bytes of one detection-weight evaluation (8 B per table
-    scalar needed, 8 B per tally add), summed over the layers it happens in,
-    split by estimator kernel:
+    scalar needed, 8 B per tally add), split by estimator kernel:
       geometry  32 (solid-angle bilinear lookup)
-      scatter   E*[16 (form factor) + 16*L*E (attenuation at the Compton
-                energy) + 16 (scattering function) + 16*n_trials (Doppler
-                ICDF)] + 8*2E (Rayleigh and Compton tallies)
-      lines     E*[72 (shell cross sections) + A*(16 + 8*L)] + 8*2E*A
-    E elements of the layer, L layers on the escape path, A active lines per
-    element.  Returns {kernel: total bytes} and the per-event total."""
-    total = {"geom": 0.0, "scatter": 0.0, "lines": 0.0}
-    n_ev = sum(layer_events)
-    if n_ev == 0:
-        return total, 0.0
-    active = _active_lines(inp)
-    for l, ev in enumerate(layer_events):
-        if ev == 0:
-            continue
-        layer = inp.layers[l]
-        E = len(layer.Z)
-        L = l + 1  # reflection geometry: the escape path crosses the layers above
-        A = sum(active[z] for z in layer.Z) / E
-        total["geom"] += ev * 32
-        total["scatter"] += ev * (E * (16 + 16 * L * E + 16 + 16 * n_trials) + 8 * 2 * E)
-        total["lines"] += ev * (E * (72 + A * (16 + 8 * L)) + 8 * 2 * E * A)
-    return total, sum(total.values()) / n_ev
+      scatter   per element 16 (form factor) + 16 (scattering function)
+                + 16*L*E (attenuation at the Compton energy) + 48 per Doppler
+                draw (two profile-CDF pairs bounding the accepted momenta and
+                the inverse-table pair) + 16 (Rayleigh and Compton tallies)
+      lines     per element 72 (shell cross sections); per line 16 (line
+                data) + 8*L (attenuation per crossed layer) + 16 (tallies)"""
+    E, E2L, tr = w["elements"], w["element2_layers"], w["doppler_trials"]
+    A, AL = w["lines"], w["line_layers"]
+    return {"geom": 32.0,
+            "scatter": E * (16 + 16 + 16) + 16 * E2L + 48 * tr,
+            "lines": 72 * E + A * (16 + 16) + 8 * AL}
 
 
 def _fp64_weights():
@@ -104,44 +109,27 @@ def _fp64_weights():
         return json.load(f)["weights"]
 
 
-def vr_flops_per_event(inp, layer_events, n_trials=2.0):
+def vr_flops_per_event(w):
     """Algorithmic FP64 flops of the scatter and line phases of one
     detection-weight evaluation (FMA = 2, add/mul = 1, math routines weighted
-    as in profiles/fp64_op_weights.json), summed over the layers the events
-    happen in.  E elements of the layer, L layers on the escape path, A_tot
-    active lines of the layer, n_trials Doppler trials:
+    as in profiles/fp64_op_weights.json):
       lines    per line: 2L (attenuation sum) + exp + 3 (wf * P_shell * g *
                e^-mu) + 1 (sum over photons); per event 3 + div (g)
       scatter  per event: sin, cos (theta), cos (phi), sin (theta/2) and 12 +
                div (Thomson and Klein-Nishina factors); per element:
                Rayleigh 5 div + 11 (form factor, cross section, weight,
-               channel); Compton n_trials (4 div + 11) (Doppler) + log + exp
-               + 4 (energy context) + 7L (attenuation) + exp + 4 div + 12
-               (scattering function, cross section, weight, channel)
-    Returns {kernel: total flops}."""
-    w = _fp64_weights()
-    ex, lg, dv, sn, cs = w["exp"], w["log"], w["div"], w["sin"], w["cos"]
-    active = _active_lines(inp)
-    tot = {"scatter": 0.0, "lines": 0.0}
-    for l, ev in enumerate(layer_events):
-        if ev == 0:
-            continue
-        layer = inp.layers[l]
-        E = len(layer.Z)
-        L = l + 1
-        A_tot = sum(active[z] for z in layer.Z)
-        tot["lines"] += ev * (A_tot * (2 * L + ex + 4) + 3 + dv)
-        compton = n_trials * (4 * dv + 11) + lg + ex + 4 + 7 * L + ex + 4 * dv + 12
-        tot["scatter"] += ev * (2 * sn + 2 * cs + 12 + dv + E * (5 * dv + 11 + compton))
-    return tot
-
-
-def _active_lines(inp):
-    """Fluorescence lines the estimator evaluates per element (the engine's own
-    active-line count, from the physics provider)."""
-    from xmimsim_amd import api
-    lib = api.load_library()
-    return {z: lib.xmi_provider_active_lines(z, 219) for z in inp.unique_Z()}
+               channel); Compton per Doppler draw 3 div + 20 (bounds of the
+               accepted momenta, their CDF values, the draw, the energy),
+               then log + exp + 4 (energy context) + 7L (attenuation) + exp
+               + 4 div + 12 (scattering function, cross section, weight,
+               channel)"""
+    wt = _fp64_weights()
+    ex, lg, dv, sn, cs = wt["exp"], wt["log"], wt["div"], wt["sin"], wt["cos"]
+    E, EL, tr = w["elements"], w["element_layers"], w["doppler_trials"]
+    A, AL = w["lines"], w["line_layers"]
+    return {"lines": A * (ex + 4) + 2 * AL + 3 + dv,
+            "scatter": (2 * sn + 2 * cs + 12 + dv + E * (5 * dv + 11) + tr * (3 * dv + 20)
+                        + E * (lg + 2 * ex + 4 * dv + 16) + 7 * EL)}
 
 
 def cpu_baseline(args, prep_gpu):
@@ -176,16 +164,15 @@ def cpu_baseline(args, prep_gpu):
     }
 
 
-def _measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+def _measured_traffic(workload):
+    """HBM bytes per launch of every kernel, measured with rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), from the
     committed profile summary of this workload, if there is one."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
-        t = json.load(f)
-    return t.get("bytes_per_launch")
+        return {k: v["bytes_per_launch"] for k, v in json.load(f)["kernels"].items()}
 
 
 def batch(args):
@@ -343,8 +330,9 @@ def quantify(args):
     ensemble on the GPU, then both quantification loops on the same unknowns."""
     from xmimsim_amd import quant
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
-    r = quant.run_pipeline(n_train=args.train_samples, photons_train=100_000, steps=args.train_steps,
-                           n_quant=args.quant_samples, photons_quant=1_000_000, seed=args.seed % 100000, log=log)
+    r = quant.run_pipeline(n_train=args.train_samples, photons_train=args.train_photons, steps=args.train_steps,
+                           n_quant=args.quant_samples, photons_quant=1_000_000, seed=args.seed % 100000,
+                           plain=args.plain, log=log)
     out = {
         "metric": "quantified samples/s (surrogate-screened vs plain MC)",
         "value": round(r["screened"]["samples_per_s"], 3),
@@ -354,9 +342,10 @@ def quantify(args):
         "vs_baseline": None,
         "dtype": "bf16 (surrogate) / f64 (Monte Carlo)",
         "data": "synthetic (seeded single-layer samples, 3-12 elements, Z 11-92)",
-        "config": {"workload": "quantify", "train_samples": args.train_samples, "train_photons": 100_000,
+        "config": {"workload": "quantify", "train_samples": args.train_samples, "train_photons": args.train_photons,
                    "train_steps": args.train_steps, "ensemble": 4, "quant_samples": args.quant_samples,
-                   "quant_photons": 1_000_000},
+                   "quant_photons": 1_000_000, "plain": args.plain,
+                   "train_histories": r["train_histories"]},
         "plain_mc": r["plain"],
         "screened": r["screened"],
         "speedup": round(r["speedup"], 2),
@@ -417,33 +406,44 @@ def main():
     value = hist / elapsed
 
     roofline = None
-    if prof is not None and prof["scatter"] > 0:
-        tot_bytes, per_event = vr_bytes_per_event(inp, prof["layer_events"])
-        tot_flops = vr_flops_per_event(inp, prof["layer_events"])
+    work = estimator_work(args.workload)
+    if prof is not None and prof["scatter"] > 0 and work is not None:
         launches = max(prof["launches"], 1)
+        events = int(sum(prof["layer_events"]))
+        ev_launch = events / launches
+        bpe = vr_bytes_per_event(work)
+        fpe = vr_flops_per_event(work)
         kernel_ms = {k: round(prof[k], 2) for k in ("geom", "cone", "scatter", "lines", "step", "interactions")}
-        # the dominant kernel of the step; the scatter and line kernels are
-        # bound by FP64 vector issue (their table reads hit the caches: the
-        # measured HBM traffic is far below the algorithmic bytes), so their
-        # roofline is FP64 at the vector peak
+        names = {"scatter": "k_vr_scatter", "lines": "k_vr_lines", "geom": "k_vr_geom", "cone": "k_vr_cone",
+                 "step": "k_step", "interactions": "k_interact"}
+        traffic = _measured_traffic(args.workload)
+        # the dominant kernel of the step: the scatter and line kernels are
+        # bound by FP64 vector issue and gather latency (their table reads hit
+        # the caches: the measured HBM traffic is far below the algorithmic
+        # bytes), so their roofline is FP64 at the vector peak
         dom = max(("scatter", "lines"), key=lambda k: prof[k])
-        names = {"scatter": "k_vr_scatter", "lines": "k_vr_lines"}
-        sec = prof[dom] * 1e-3
-        achieved = tot_flops[dom] / sec / 1e12
-        # "mfma" names the compute roofline of the contract; the pipe is FP64
-        # vector issue, whose peak equals the FP64 matrix peak on MI355X
-        roofline = {"bound": "mfma", "pipe": "fp64 vector (VALU); FP64 vector and matrix peaks are both 78.6 TFLOP/s",
-                    "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                    "traffic": _measured_traffic(names[dom], args.workload),
+        sec = prof[dom] * 1e-3 / launches
+        achieved = fpe[dom] * ev_launch / sec / 1e12
+        # measured HBM traffic of the four kernels with the most time per step
+        hbm = {}
+        for k in sorted(kernel_ms, key=lambda k: -prof[k])[:4]:
+            b = traffic.get(names[k])
+            t = prof[k] * 1e-3 / launches
+            hbm[names[k]] = {"avg_launch_ms": round(t * 1e3, 4), "bytes_per_launch": b,
+                             "GBps": round(b / t / 1e9, 1) if b else None,
+                             "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4) if b else None}
+        roofline = {"bound": "fp64 valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                    "traffic": traffic.get(names[dom]),
                     "kernel": names[dom] + " (detection weight: " + dom + ")",
-                    "flops_per_launch": round(tot_flops[dom] / launches),
-                    "avg_launch_ms": round(prof[dom] / launches, 4),
-                    "algorithmic_bytes_per_launch": round(tot_bytes[dom] / launches),
-                    "algorithmic_bytes_rate_GBps": round(tot_bytes[dom] / sec / 1e9, 1),
-                    "estimator_bytes_per_event": round(per_event, 1), "events": int(sum(prof["layer_events"])),
-                    "launches": int(launches), "solid_angle_fallbacks": int(prof["fallbacks"]),
-                    "kernel_ms": kernel_ms}
+                    "flops_per_launch": round(fpe[dom] * ev_launch),
+                    "avg_launch_ms": round(sec * 1e3, 4),
+                    "algorithmic_bytes_per_launch": round(bpe[dom] * ev_launch),
+                    "algorithmic_bytes_rate_GBps": round(bpe[dom] * ev_launch / sec / 1e9, 1),
+                    "events": events, "events_per_launch": round(ev_launch), "launches": int(launches),
+                    "work_per_event": {k: round(v, 4) for k, v in work.items()},
+                    "hbm_measured": hbm,
+                    "solid_angle_fallbacks": int(prof["fallbacks"]), "kernel_ms": kernel_ms}
 
     cpu = None
     if not args.no_cpu:

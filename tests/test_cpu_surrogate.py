@@ -62,3 +62,13 @@ def test_xrf_encoding():
     k = S.OUT_LINES.index((26, 3))
     assert m[k] == 1.0 and t[k] == pytest.approx(np.log10(1011.0))
     assert m.sum() == 1.0
+
+
+def test_reference_loop_step_limits():
+    """bin/xmimsim-pymca.c:698-725 step limits; src/xmi_pymca.c:1684-1751 rescaling"""
+    from xmimsim_amd import quant
+    assert [quant._max_scale(w) for w in (1e-5, 5e-3, 0.05, 0.2, 0.3, 0.45, 0.55, 0.65, 0.9)] == \
+        [100.0, 10.0, 2.5, 1.5, 1.2, 1.1, 1.05, 1.025, 1.01]
+    # weights above 1 in total: the largest and those before it absorb the excess
+    w = quant._pymca_composition(np.array([0.2, 0.7, 0.3]))
+    assert w.sum() == pytest.approx(1.0) and w[2] == 0.3

@@ -80,3 +80,33 @@ def test_batch_constraints_are_errors():
     with pytest.raises(RuntimeError):
         s.gradients(X, np.zeros((100, DIMS[-1]), np.float32), np.ones((100, DIMS[-1]), np.float32))
     s.close()
+
+
+def test_gradients_at_training_scale():
+    """The training configuration of the net-line surrogate: 85 -> 4 x 512
+    (GELU) -> 256, an ensemble of 4, batch 4096.  Loss and every parameter
+    gradient of each member against the CPU reference with the same bf16
+    operand rounding (relative 1e-2 in norm), member 0 also against exact
+    arithmetic (5e-2)."""
+    import surrogate_ref as R
+    from xmimsim_amd.surrogate import DEFAULT_DIMS, Surrogate
+    B = 4096
+    rng = np.random.default_rng(17)
+    s = Surrogate(n_nets=4, dims=DEFAULT_DIMS, max_batch=B, seed=5)
+    X = rng.normal(0, 1, (B, DEFAULT_DIMS[0])).astype(np.float32)
+    T = rng.normal(0, 1, (B, DEFAULT_DIMS[-1])).astype(np.float32)
+    M = (rng.random((B, DEFAULT_DIMS[-1])) > 0.3).astype(np.float32)
+    loss, g = s.gradients(X, T, M)
+    P = s.get_params()
+    for e in range(4):
+        print("member", e, flush=True)
+        layers = s.unpack(P[e])
+        rl, rg = R.loss_and_grads(layers, X, T, M, use_bf16=True)
+        assert loss[e] == pytest.approx(rl, rel=1e-3)
+        flat_ref = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in rg])
+        assert np.linalg.norm(g[e] - flat_ref) / np.linalg.norm(flat_ref) < 1e-2
+        if e == 0:
+            _, eg = R.loss_and_grads(layers, X, T, M, use_bf16=False)
+            flat_ex = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in eg])
+            assert np.linalg.norm(g[e] - flat_ex) / np.linalg.norm(flat_ex) < 5e-2
+    s.close()

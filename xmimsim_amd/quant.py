@@ -4,21 +4,31 @@ surrogate (xmimsim_amd.surrogate), all driven by the GPU Monte Carlo engine.
 Samples are single homogeneous layers excited by one monochromatic line, in
 the reference's default 45/45 degree geometry.  The Monte Carlo intensities
 are the var_red_history rows of xmi_main_msim summed over interaction orders.
+Every Monte Carlo evaluation of many samples goes through one
+xmi_main_msim_batch call (one seed per sample), so that the GPU serves the
+whole set at once.
 
-Quantification solves for the weight fractions of known elements from measured
-net-line intensities by damped Gauss-Newton on log10 intensities:
-  - screened: Gauss-Newton on the surrogate ensemble (finite-difference
-    Jacobians, one batched forward pass per iteration for all samples); then a
-    Monte Carlo run at the answer confirms it, refined while needed by steps
-    with the Monte Carlo residual and the surrogate Jacobian (corrected by
-    the secants of successive Monte Carlo runs); samples whose ensemble
-    spread exceeds the gate get twice the refinement steps, and samples still
-    above the tolerance finish with a few plain iterations;
-  - plain: the same Gauss-Newton loop with every function value and Jacobian
-    column from a Monte Carlo run (common random numbers across columns).
+Quantification recovers the weight fractions of known elements from measured
+net-line intensities:
+  - plain: the reference's own loop, bin/xmimsim-pymca.c:597-806 (one Monte
+    Carlo run per iteration; each element's weight scaled by the ratio of its
+    measured to simulated K-alpha or L-alpha intensity, with the reference's
+    step limits; convergence when the summed squared relative residuals of
+    the K and of the L elements are below 1e-3, at most 100 iterations), run
+    for all unconverged samples in one batch per iteration.  The samples have
+    no matrix element, so the weights are renormalised after each update;
+  - screened: damped Gauss-Newton on log10 intensities of the surrogate
+    ensemble (finite-difference Jacobians, one batched forward pass per
+    iteration for all samples); then a Monte Carlo run at every answer
+    confirms it, refined while needed by steps with the Monte Carlo residual
+    and the surrogate Jacobian (corrected by the secants of successive runs);
+    samples whose ensemble spread exceeds the gate get twice the refinement
+    steps, and samples still above the tolerance fall back to plain
+    Gauss-Newton with Monte Carlo Jacobians.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -57,7 +67,7 @@ def make_input(s: Sample, w: Optional[np.ndarray] = None, photons: int = 1_000_0
 
 
 class Engine:
-    """Monte Carlo evaluations of net lines with shared tables for Z 11..92.
+    """Monte Carlo evaluations of many samples with shared tables for Z 11..92.
 
     The solid-angle grid depends on the detector geometry only; its r range is
     chosen per sample by the reference to cover the sample's interaction
@@ -65,11 +75,13 @@ class Engine:
     range covers the deepest sample of the family (1 cm of a light, low
     density matrix at 40 keV) serves every sample."""
 
-    def __init__(self, grid_hits: int = 1000, Zs: Sequence[int] = range(S.Z_MIN, S.Z_MAX + 1), seed: int = 1):
+    def __init__(self, grid_hits: int = 1000, Zs: Sequence[int] = range(S.Z_MIN, S.Z_MAX + 1), seed: int = 1,
+                 streams: int = 4, chunk: int = 512):
         Zs = sorted(set(int(z) for z in Zs))
         self.tables = api.SharedTables(Zs)
         self.mc_runs = 0
         self.mc_histories = 0
+        self.streams, self.chunk = streams, chunk
         deep = Sample(Zs[:2], np.full(len(Zs[:2]), 1.0 / len(Zs[:2])), 1.0, 1.0, 40.0)
         self.grid = api.Prepared(make_input(deep, photons=10), tables=self.tables)
         self.grid.compute_solid_angle(hits_per_single=grid_hits, seed=seed)
@@ -78,33 +90,44 @@ class Engine:
         self.grid.close()
         self.tables.close()
 
-    def lines(self, s: Sample, w: np.ndarray, photons: int, seed: int):
-        """log10 net lines [N_OUT] and loss mask of sample s at composition w."""
-        prep = api.Prepared(make_input(s, w, photons), tables=self.tables)
-        prep.solid_angle = self.grid.solid_angle
-        prep._sa_owner = "borrowed"
-        try:
-            _, vr, _, st = prep.run(seed=seed)
-        finally:
-            prep.solid_angle = None
-            prep.close()
-        self.mc_runs += 1
-        self.mc_histories += st.histories
-        return S.targets(vr, s.Z)
+    def run(self, samples: Sequence[Sample], Ws: Sequence[np.ndarray], photons: int, seeds: Sequence[int]):
+        """var_red_history [100][385][4] of every (sample, composition), one
+        seed each, through xmi_main_msim_batch in chunks of self.chunk."""
+        out = []
+        for c in range(0, len(samples), self.chunk):
+            preps = []
+            for s, w in zip(samples[c:c + self.chunk], Ws[c:c + self.chunk]):
+                p = api.Prepared(make_input(s, w, photons), tables=self.tables)
+                p.solid_angle = self.grid.solid_angle
+                p._sa_owner = "borrowed"
+                preps.append(p)
+            try:
+                res, _ = api.run_batch(preps, self.tables, seed=0, n_streams=self.streams,
+                                       seeds=list(seeds[c:c + self.chunk]))
+                self.mc_histories += api.batch_stats().histories
+            finally:
+                for p in preps:
+                    p.solid_angle = None
+                    p.close()
+            self.mc_runs += len(preps)
+            out.extend(vr for _, vr in res)
+        return out
+
+    def lines(self, samples: Sequence[Sample], Ws: Sequence[np.ndarray], photons: int, seeds: Sequence[int]):
+        """log10 net lines [n][N_OUT] and loss masks of the samples at the
+        compositions Ws."""
+        vrs = self.run(samples, Ws, photons, seeds)
+        tm = [S.targets(vr, s.Z) for vr, s in zip(vrs, samples)]
+        return np.array([t for t, _ in tm]), np.array([m for _, m in tm])
 
 
 def generate(engine: Engine, n: int, photons: int, seed: int, n_el=(2, 30), alpha=0.5, zrange=(S.Z_MIN, S.Z_MAX)):
     """Training set: X [n][85], T [n][256], M [n][256]."""
     rng = np.random.default_rng(seed)
-    X = np.zeros((n, S.N_IN), np.float32)
-    T = np.zeros((n, S.N_OUT), np.float32)
-    M = np.zeros((n, S.N_OUT), np.float32)
-    for k in range(n):
-        s = random_sample(rng, n_el, alpha, zrange)
-        t, m = engine.lines(s, s.w, photons, seed=seed * 100003 + k)
-        X[k] = S.encode(s.Z, s.w, s.thickness, s.density, s.energy)
-        T[k], M[k] = t, m
-    return X, T, M
+    samples = [random_sample(rng, n_el, alpha, zrange) for _ in range(n)]
+    X = np.stack([S.encode(s.Z, s.w, s.thickness, s.density, s.energy) for s in samples]).astype(np.float32)
+    T, M = engine.lines(samples, [s.w for s in samples], photons, [seed * 100003 + k for k in range(n)])
+    return X, T.astype(np.float32), M.astype(np.float32)
 
 
 def train(sg: S.Surrogate, X, T, M, steps: int, batch: int, lr: float = 1e-3, lr_min: float = 1e-5, seed: int = 0,
@@ -124,22 +147,169 @@ def train(sg: S.Surrogate, X, T, M, steps: int, batch: int, lr: float = 1e-3, lr
     return np.array(hist)
 
 
+# ---------------------------------------------------------------- measurement
+
+
+@dataclass
+class Measurement:
+    """Measured net lines of one sample: the Monte Carlo intensities at the
+    true composition with Poisson counting noise.  totals [len(Z)][383]
+    (counts/s, summed over interaction orders); t, m: the surrogate's log10
+    outputs and mask (lines below min_counts left out); min_counts: the
+    counts below which a line or line group is not measured."""
+    totals: np.ndarray
+    t: np.ndarray
+    m: np.ndarray
+    min_counts: float = 0.0
+
+
+def measure(engine: Engine, samples: Sequence[Sample], photons: int, seed: int, min_counts: float = 4e4,
+            live_time: float = 1.0) -> List[Measurement]:
+    """Lines below min_counts (counting noise above 0.5 %) are left out."""
+    vrs = engine.run(samples, [s.w for s in samples], photons, [seed + k for k in range(len(samples))])
+    out = []
+    for k, (s, vr) in enumerate(zip(samples, vrs)):
+        rng = np.random.default_rng(seed + k)
+        tot = vr[np.array(s.Z) - 1, :383, :].sum(axis=2)
+        noisy = rng.poisson(tot * live_time).astype(np.float64) / live_time
+        t = np.zeros(S.N_OUT, np.float32)
+        m = np.zeros(S.N_OUT, np.float32)
+        zi = {z: i for i, z in enumerate(s.Z)}
+        for j, (z, line) in enumerate(S.OUT_LINES):
+            if z in zi and noisy[zi[z], line - 1] * live_time >= min_counts:
+                t[j] = np.log10(noisy[zi[z], line - 1])
+                m[j] = 1.0
+        out.append(Measurement(noisy, t, m, min_counts / live_time))
+    return out
+
+
+def _line_err(t, meas, mk):
+    return float(np.max(np.abs(10.0 ** (t[mk] - meas[mk]) - 1.0))) if mk.any() else 0.0
+
+
+# ---------------------------------------------------------------- the reference's loop
+
+PYMCA_MAX_ITERATIONS = 100      # include/xmi_pymca.h:29
+PYMCA_CONV_THRESHOLD = 1e-3     # include/xmi_pymca.h:30
+# xraylib line numbers of the alpha lines the reference scales by
+ALPHA_LINES = {"K": (2, 3), "L": (89, 90), "M": (215, 216)}   # KL2+KL3, L3M4+L3M5, M5N6+M5N7
+
+
+def _max_scale(w: float) -> float:
+    """Largest factor one iteration may raise a weight by,
+    bin/xmimsim-pymca.c:698-725"""
+    for lim, m in ((0.0001, 100.0), (0.01, 10.0), (0.1, 2.5), (0.25, 1.5), (0.375, 1.2), (0.50, 1.1),
+                   (0.6, 1.05), (0.7, 1.025)):
+        if w <= lim:
+            return m
+    return 1.01
+
+
+def _alpha_family(meas: Measurement, j: int) -> Optional[str]:
+    """The line family the reference scales element j by: K-alpha when it is
+    measured, else L-alpha (bin/xmimsim-pymca.c:566-583).  The reference
+    stops for an element with neither; here M-alpha (M5N6 + M5N7) serves
+    such heavy elements below their L3 edge, and an element with no measured
+    alpha line at all keeps its weight."""
+    for fam in ("K", "L", "M"):
+        if sum(meas.totals[j, l - 1] for l in ALPHA_LINES[fam]) >= max(meas.min_counts, 1e-300):
+            return fam
+    return None
+
+
+def _alpha_sim(vr: np.ndarray, Z: int, fam: str) -> float:
+    """k_sim / l_sim of bin/xmimsim-pymca.c:655-671: the two alpha rows summed
+    over k = 0..n_int with the reference's Fortran index (order k at column
+    k - 1), so that k = 0 reads the last order of the row before each line,
+    as the reference does; the detector absorption factor it applies cancels
+    against the measured side, which carries the same factor."""
+    n_int = vr.shape[2]
+    flat = vr.reshape(100, 385 * n_int)
+    tot = 0.0
+    for line in ALPHA_LINES[fam]:
+        base = (line - 1) * n_int
+        tot += float(flat[Z - 1, base - 1:base + n_int].sum())
+    return tot
+
+
+def _pymca_composition(weights: np.ndarray) -> np.ndarray:
+    """xmi_ilay_composition_pymca (src/xmi_pymca.c:1647-1810) for a layer of
+    quantified elements only (no matrix, no reference element): weights whose
+    sum exceeds 1 are rescaled, the element of the largest weight (the
+    largest measured intensity among ties) and those before it scaled so
+    that the total is 1."""
+    w = weights.copy()
+    tot = w.sum()
+    if tot > 1.0:
+        maxloc = int(np.argmax(w))
+        above = w[:maxloc + 1].sum()
+        w[:maxloc + 1] *= (1.0 - (tot - above)) / above
+    return w
+
+
+def quantify_pymca(engine: Engine, samples: List[Sample], meas: List[Measurement], photons: int, seed: int = 0,
+                   start: Optional[List[np.ndarray]] = None, max_iterations: int = PYMCA_MAX_ITERATIONS, log=None):
+    """The reference's iteration (bin/xmimsim-pymca.c:597-806) for every
+    sample, one batched Monte Carlo run per iteration over the samples not yet
+    converged.  The start is equal fractions (the reference starts from
+    pymca's concentrations when it has them, else from 1e-4).  Returns
+    (weights, info): the weights after the last update, the iterations per
+    sample, convergence flags and the net-line error of each sample's last
+    Monte Carlo run against the measured lines."""
+    n = len(samples)
+    W = [np.array(w, float) for w in start] if start is not None else [np.full(len(s.Z), 1.0 / len(s.Z))
+                                                                         for s in samples]
+    fams = [[_alpha_family(m, j) for j in range(len(s.Z))] for s, m in zip(samples, meas)]
+    iters = np.zeros(n, int)
+    conv = np.zeros(n, bool)
+    line_err = np.full(n, np.inf)
+    active = list(range(n))
+    for it in range(1, max_iterations + 1):
+        if not active:
+            break
+        vrs = engine.run([samples[i] for i in active], [W[i] for i in active], photons,
+                         [seed + 104729 * it + i for i in active])
+        still = []
+        for i, vr in zip(active, vrs):
+            s, m = samples[i], meas[i]
+            iters[i] = it
+            t, _ = S.targets(vr, s.Z)
+            line_err[i] = _line_err(t, m.t, m.m > 0)
+            sum_k = sum_l = 0.0
+            for j, z in enumerate(s.Z):
+                fam = fams[i][j]
+                if fam is None:
+                    continue
+                exp = sum(m.totals[j, l - 1] for l in ALPHA_LINES[fam])
+                sim = _alpha_sim(vr, z, fam)
+                if not sim > 0.0:
+                    continue
+                r = (exp - sim) / exp
+                if fam == "K":
+                    sum_k += r * r
+                else:
+                    sum_l += r * r
+                W[i][j] *= min(exp / sim, _max_scale(W[i][j]))
+            # the layer holds the quantified elements only: the weights are
+            # fractions of the whole layer again (the reference's matrix
+            # element takes up the difference instead)
+            W[i] = _pymca_composition(W[i])
+            W[i] /= W[i].sum()
+            if sum_k > PYMCA_CONV_THRESHOLD or sum_l > PYMCA_CONV_THRESHOLD:
+                still.append(i)
+            else:
+                conv[i] = True
+        active = still
+        if log:
+            log(f"pymca iteration {it}: {len(active)} of {n} samples not converged")
+    return W, {"iterations": iters, "converged": conv, "line_err": line_err}
+
+
+# ---------------------------------------------------------------- screened loop
+
+
 def _encode_batch(s: Sample, W: np.ndarray) -> np.ndarray:
     return np.stack([S.encode(s.Z, w, s.thickness, s.density, s.energy) for w in W])
-
-
-def measure(engine: Engine, s: Sample, photons: int, seed: int, min_counts: float = 4e4, live_time: float = 1.0):
-    """Measured net lines: the MC intensities at the true composition with
-    Poisson counting noise over `live_time` seconds; lines below min_counts
-    (counting noise above 0.5 %) are left out."""
-    t, m = engine.lines(s, s.w, photons, seed)
-    rng = np.random.default_rng(seed)
-    counts = np.where(m > 0, 10.0 ** t, 0.0) * live_time
-    noisy = rng.poisson(counts).astype(np.float64)
-    keep = (m > 0) & (noisy >= min_counts)
-    out = np.zeros_like(t)
-    out[keep] = np.log10(noisy[keep] / live_time)
-    return out.astype(np.float32), keep.astype(np.float32)
 
 
 def _gn_update(w, J, r, lam):
@@ -148,10 +318,6 @@ def _gn_update(w, J, r, lam):
     step = np.linalg.solve(A, -J.T @ r)
     w_new = w * np.exp(np.clip(step, -1.0, 1.0))
     return w_new / w_new.sum()
-
-
-def _line_err(t, meas, mk):
-    return float(np.max(np.abs(10.0 ** (t[mk] - meas[mk]) - 1.0))) if mk.any() else 0.0
 
 
 def _surrogate_jacobians(sg: S.Surrogate, samples, W, h=0.2):
@@ -177,116 +343,137 @@ def _surrogate_jacobians(sg: S.Surrogate, samples, W, h=0.2):
     return out
 
 
-def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], meas, meas_mask, photons: int,
+def _mc_gn_batch(engine: Engine, samples, meas, idx, W, photons, seed, tol, iters, calls, line_err):
+    """Gauss-Newton with Monte Carlo values and finite-difference Jacobians
+    (one seed for all columns of one iteration) for the samples idx, all
+    columns of all samples of an iteration in one batch."""
+    h = 0.05
+    active = list(idx)
+    for it in range(iters):
+        if not active:
+            break
+        jobs, where = [], {}
+        for i in active:
+            where[i] = len(jobs)
+            jobs.append((i, W[i]))
+            if it < iters - 1:
+                for j in range(len(samples[i].Z)):
+                    wp = W[i].copy()
+                    wp[j] *= np.exp(h)
+                    jobs.append((i, wp / wp.sum()))
+        T, _ = engine.lines([samples[i] for i, _ in jobs], [w for _, w in jobs], photons,
+                            [seed + 7919 * i + 101 * it for i, _ in jobs])
+        still = []
+        for i in active:
+            o = where[i]
+            mk = meas[i].m > 0
+            calls[i] += 1 + (len(samples[i].Z) if it < iters - 1 else 0)
+            line_err[i] = _line_err(T[o], meas[i].t, mk)
+            if line_err[i] <= tol or it == iters - 1:
+                continue
+            cols = [(T[o + 1 + j][mk] - T[o][mk]) / h for j in range(len(samples[i].Z))]
+            W[i] = _gn_update(W[i], np.array(cols).T, T[o][mk] - meas[i].t[mk], 1e-3)
+            still.append(i)
+        active = still
+
+
+def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], meas: List[Measurement], photons: int,
                       iters: int = 15, gate: float = 0.02, tol: float = 0.02, hybrid_iters: int = 4,
                       fallback_iters: int = 3, seed: int = 0):
-    """Gauss-Newton on the surrogate for all samples at once; then per sample
-    Monte Carlo confirmation, refined while needed by Gauss-Newton steps that
-    take the residual from the Monte Carlo run and the Jacobian from the
-    surrogate, Broyden-corrected by the Monte Carlo secants; samples gated by
-    ensemble spread get twice the refinement steps; samples still above tol
-    finish with fallback_iters plain Monte Carlo iterations.  Returns (w_hat
-    list, info dict)."""
+    """Gauss-Newton on the surrogate for all samples at once; then Monte Carlo
+    confirmation of every answer, refined while needed by Gauss-Newton steps
+    that take the residual from the Monte Carlo run and the Jacobian from the
+    surrogate, Broyden-corrected by the Monte Carlo secants (one seed per
+    sample: successive runs differ by the composition change only); samples
+    gated by ensemble spread get twice the refinement steps; samples still
+    above tol fall back to plain Gauss-Newton with Monte Carlo Jacobians.
+    Each round's Monte Carlo runs of all samples go in one batch.  Returns
+    (w_hat list, info dict)."""
     n = len(samples)
     W = [np.full(len(s.Z), 1.0 / len(s.Z)) for s in samples]
     for it in range(iters):
         jac = _surrogate_jacobians(sg, samples, W)
         for i in range(n):
-            mk = meas_mask[i] > 0
+            mk = meas[i].m > 0
             f0, J, _ = jac[i]
-            W[i] = _gn_update(W[i], J[mk], f0[mk] - meas[i][mk], 1e-3)
+            W[i] = _gn_update(W[i], J[mk], f0[mk] - meas[i].t[mk], 1e-3)
     jac = _surrogate_jacobians(sg, samples, W)
     spread = np.zeros(n)
     for i in range(n):
-        mk = meas_mask[i] > 0
+        mk = meas[i].m > 0
         lin = 10.0 ** jac[i][2][:, mk]
         spread[i] = float(np.max(lin.std(axis=0) / lin.mean(axis=0))) if mk.any() else 0.0
     gated = spread > gate
+    n_ref = np.where(gated, 2 * hybrid_iters, hybrid_iters)
     line_err = np.full(n, np.inf)
     mc_calls = np.zeros(n, int)
-    fallback = 0
-    trace = []
-    for i, s in enumerate(samples):
-        mk = meas_mask[i] > 0
-        # the Monte Carlo checks every answer; where the ensemble disagrees it
-        # also supplies the residual of each refinement step (more of them).
-        # The refinement runs use one seed (common random numbers), so that
-        # successive runs differ by the composition change only, and each
-        # run's secant corrects the surrogate Jacobian (Broyden's update).
-        n_ref = hybrid_iters * (2 if gated[i] else 1)
-        J = t_prev = u_prev = None
-        errs = []
-        for h_it in range(n_ref + 1):
-            t, _ = engine.lines(s, W[i], photons, seed=seed + 7919 * i)
-            mc_calls[i] += 1
-            line_err[i] = _line_err(t, meas[i], mk)
-            errs.append(line_err[i])
-            if line_err[i] <= tol or h_it == n_ref:
-                break
-            u = np.log(W[i])
-            if J is None:
-                J = _surrogate_jacobians(sg, [s], [W[i]])[0][1][mk]
-            else:
-                du, df = u - u_prev, t[mk] - t_prev
-                J = J + np.outer(df - J @ du, du) / max(float(du @ du), 1e-30)
-            t_prev, u_prev = t[mk], u
-            W[i] = _gn_update(W[i], J, t[mk] - meas[i][mk], 1e-3)
-        if line_err[i] > tol:
-            # a few plain Monte Carlo iterations from the refined composition
-            fallback += 1
-            W[i], line_err[i], k = _mc_gn(engine, s, meas[i], meas_mask[i], photons, seed + 7919 * i + 100, W[i], tol,
-                                          iters=fallback_iters)
-            mc_calls[i] += k
-            errs.append(line_err[i])
-        trace.append(errs)
-    return W, {"spread": spread, "gated": int(gated.sum()), "fallback": fallback, "line_err": line_err,
-               "mc_calls": mc_calls, "trace": trace}
-
-
-def _mc_gn(engine: Engine, s: Sample, meas, meas_mask, photons, seed, w0, tol, iters: int = 8):
-    """Gauss-Newton with Monte Carlo values and finite-difference Jacobians
-    (the same seed for all columns of one iteration).  Returns (w, line
-    error, Monte Carlo runs)."""
-    w = w0.copy()
-    mk = meas_mask > 0
-    h = 0.05
-    err = np.inf
-    calls = 0
-    for it in range(iters):
-        t0, _ = engine.lines(s, w, photons, seed=seed + it)
-        calls += 1
-        err = _line_err(t0, meas, mk)
-        if err <= tol or it == iters - 1:
+    confirmed = np.zeros(n, bool)
+    J = [None] * n
+    prev = [None] * n
+    active = list(range(n))
+    for r in range(int(n_ref.max()) + 1):
+        if not active:
             break
-        cols = []
-        for j in range(len(s.Z)):
-            wp = w.copy()
-            wp[j] *= np.exp(h)
-            tj, _ = engine.lines(s, wp / wp.sum(), photons, seed=seed + it)
-            calls += 1
-            cols.append((tj[mk] - t0[mk]) / h)
-        w = _gn_update(w, np.array(cols).T, t0[mk] - meas[mk], 1e-3)
-    return w, err, calls
+        T, _ = engine.lines([samples[i] for i in active], [W[i] for i in active], photons,
+                            [seed + 7919 * i for i in active])
+        still = []
+        for k, i in enumerate(active):
+            mk = meas[i].m > 0
+            t = T[k]
+            mc_calls[i] += 1
+            confirmed[i] = True
+            line_err[i] = _line_err(t, meas[i].t, mk)
+            if line_err[i] <= tol or r == n_ref[i]:
+                continue
+            u = np.log(W[i])
+            if J[i] is None:
+                J[i] = _surrogate_jacobians(sg, [samples[i]], [W[i]])[0][1][mk]
+            else:
+                t_prev, u_prev = prev[i]
+                du, df = u - u_prev, t[mk] - t_prev
+                J[i] = J[i] + np.outer(df - J[i] @ du, du) / max(float(du @ du), 1e-30)
+            prev[i] = (t[mk], u)
+            W[i] = _gn_update(W[i], J[i], t[mk] - meas[i].t[mk], 1e-3)
+            still.append(i)
+        active = still
+    fb = [i for i in range(n) if line_err[i] > tol]
+    _mc_gn_batch(engine, samples, meas, fb, W, photons, seed + 100, tol, fallback_iters, mc_calls, line_err)
+    return W, {"spread": spread, "gated": int(gated.sum()), "fallback": len(fb), "line_err": line_err,
+               "mc_calls": mc_calls, "confirmed": confirmed}
 
 
-def quantify_plain(engine: Engine, samples: List[Sample], meas, meas_mask, photons, tol=0.02, seed=0):
-    W, errs, calls = [], [], []
-    for i, s in enumerate(samples):
-        w, e, k = _mc_gn(engine, s, meas[i], meas_mask[i], photons, seed + 7919 * i + 100,
-                         np.full(len(s.Z), 1.0 / len(s.Z)), tol)
-        W.append(w)
-        errs.append(e)
-        calls.append(k)
-    return W, {"line_err": np.array(errs), "mc_calls": np.array(calls)}
+def quantify_plain_gn(engine: Engine, samples: List[Sample], meas: List[Measurement], photons: int, tol=0.02,
+                      seed=0, iters: int = 8):
+    """Gauss-Newton with Monte Carlo values and Jacobians from equal fractions."""
+    n = len(samples)
+    W = [np.full(len(s.Z), 1.0 / len(s.Z)) for s in samples]
+    calls = np.zeros(n, int)
+    line_err = np.full(n, np.inf)
+    _mc_gn_batch(engine, samples, meas, range(n), W, photons, seed + 100, tol, iters, calls, line_err)
+    return W, {"line_err": line_err, "mc_calls": calls}
+
+
+# ---------------------------------------------------------------- pipeline
+
+
+def _summary(W, samples, info, dt, mc_runs):
+    n = len(samples)
+    comp = [float(np.max(np.abs(w - s.w))) for w, s in zip(W, samples)]
+    return {"seconds": dt, "samples_per_s": n / dt, "mc_runs": mc_runs,
+            "line_err_median": float(np.median(info["line_err"])),
+            "frac_within_2pct": float(np.mean(info["line_err"] <= 0.02)),
+            "composition_err_median": float(np.median(comp)),
+            "composition_err_p90": float(np.quantile(comp, 0.9))}
 
 
 def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int = 8000, n_quant: int = 64,
                  photons_quant: int = 1_000_000, zrange=(S.Z_MIN, S.Z_MAX), n_el_train=(2, 30), n_el_quant=(3, 12),
-                 n_nets: int = 4, batch: int = 4096, seed: int = 1, log=None):
+                 n_nets: int = 4, batch: int = 4096, seed: int = 1, plain: str = "pymca", log=None):
     """Training data from the GPU Monte Carlo, training of the ensemble, and
-    quantification of n_quant unknown samples both ways.  Returns a dict of
-    timings, sample rates, Monte Carlo run counts and errors."""
-    import time
+    quantification of n_quant unknown samples both ways (plain: "pymca", the
+    reference's loop, or "gn", Gauss-Newton with Monte Carlo Jacobians).
+    Returns a dict of timings, sample rates, Monte Carlo run counts and
+    errors."""
     log = log or (lambda *a: None)
     out = {}
     t0 = time.time()
@@ -296,7 +483,7 @@ def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int 
     X, T, M = generate(eng, n_train, photons_train, seed, n_el_train, 0.5, zrange)
     out["generate_s"] = time.time() - t0
     out["train_histories"] = eng.mc_histories
-    log(f"generated {n_train} samples in {out['generate_s']:.1f} s")
+    log(f"generated {n_train} samples ({eng.mc_histories:.3g} histories) in {out['generate_s']:.1f} s")
     sg = S.Surrogate(n_nets=n_nets, max_batch=batch, seed=seed)
     t0 = time.time()
     b = min(batch, max(64, n_train // 64 * 64))
@@ -310,21 +497,23 @@ def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int 
     out["holdout_err_median"], out["holdout_err_p90"] = float(np.median(err)), float(np.quantile(err, 0.9))
     rng = np.random.default_rng(seed + 2024)
     samples = [random_sample(rng, n_el_quant, 0.5, zrange) for _ in range(n_quant)]
-    meas, mm = zip(*[measure(eng, s, photons_quant, seed=5000 + i) for i, s in enumerate(samples)])
-    meas, mm = np.array(meas), np.array(mm)
-    for name, fn in (("screened", lambda: quantify_screened(sg, eng, samples, meas, mm, photons_quant, seed=seed)),
-                     ("plain", lambda: quantify_plain(eng, samples, meas, mm, photons_quant, seed=seed))):
+    meas = measure(eng, samples, photons_quant, seed=5000)
+    plain_fn = ((lambda: quantify_pymca(eng, samples, meas, photons_quant, seed=seed, log=log)) if plain == "pymca"
+                else (lambda: quantify_plain_gn(eng, samples, meas, photons_quant, seed=seed)))
+    for name, fn in (("screened", lambda: quantify_screened(sg, eng, samples, meas, photons_quant, seed=seed)),
+                     ("plain", plain_fn)):
         c0, t0 = eng.mc_runs, time.time()
         W, info = fn()
         dt = time.time() - t0
-        out[name] = {"seconds": dt, "samples_per_s": n_quant / dt, "mc_runs": eng.mc_runs - c0,
-                     "line_err_median": float(np.median(info["line_err"])),
-                     "frac_within_2pct": float(np.mean(info["line_err"] <= 0.02)),
-                     "composition_err_median": float(np.median([np.max(np.abs(w - s.w))
-                                                               for w, s in zip(W, samples)]))}
+        out[name] = _summary(W, samples, info, dt, eng.mc_runs - c0)
         if name == "screened":
-            out[name].update({"gated": info["gated"], "fallback": info["fallback"]})
-        out[name]["mc_calls"] = [int(c) for c in info["mc_calls"]]
+            out[name].update({"gated": info["gated"], "fallback": info["fallback"],
+                              "fallback_fraction": info["fallback"] / n_quant,
+                              "mc_confirmed_fraction": float(np.mean(info["confirmed"]))})
+        elif plain == "pymca":
+            out[name].update({"method": "bin/xmimsim-pymca.c:597-806",
+                              "iterations_median": float(np.median(info["iterations"])),
+                              "converged_fraction": float(np.mean(info["converged"]))})
         log(f"{name}: {out[name]}")
     out["speedup"] = out["screened"]["samples_per_s"] / out["plain"]["samples_per_s"]
     sg.close()
