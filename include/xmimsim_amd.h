@@ -246,10 +246,17 @@ extern long xmi_hits_per_single;
  * Outputs (malloc'ed, owned by the caller, release with xmi_free):
  *   *channels        double[n_interactions+1][nchannels], cumulative over
  *                    interaction order, scaled by live_time;
- *   *brute_history   double[100][385][n_interactions] (zero: variance
- *                    reduction never produces detector hits);
+ *   *brute_history   double[100][385][n_interactions]: the weight of the
+ *                    photons that reached the detector, by element, row of
+ *                    their last interaction (line, 384 Rayleigh, 385 Compton)
+ *                    and its order; zero with variance reduction, which
+ *                    never produces detector hits;
  *   *var_red_history double[100][385][n_interactions], or NULL when variance
  *                    reduction is off.
+ * With options->use_variance_reduction = 0 the brute-force path runs
+ * (analogue transport, detector and collimator tests of every path segment,
+ * Auger and radiative cascades with their offspring photons per
+ * use_cascade_auger / use_cascade_radiative) and solid_angles may be NULL.
  * Random numbers: Philox4x32-10, one stream per photon history, keyed by the
  * seed (options->use_default_seeds selects seed 1, otherwise
  * xmi_set_seed / time-based).  Returns 1 on success, 0 on failure. */
@@ -321,6 +328,11 @@ int xmi_provider_shell_cross_sections(int Z, int cascade_type, double E, double 
 /* 0: fluorescence yield of shell index (0 K .. 8 M5); 1: radiative rate and
  * 2: energy (keV) of line index (xmi_line_name numbering) */
 double xmi_provider_line_data(int Z, int which, int index);
+/* The Auger transition trans (0 .. 153) of element Z, the equivalent of
+ * xraylib's AugerRate: writes the vacancy shell it empties and the two it
+ * fills (0 K .. 8 M5, 9 any shell beyond M5) and returns its rate within
+ * that shell's Auger decays; -1 for an invalid index. */
+double xmi_provider_auger(int Z, int trans, int *shells3);
 
 /* Number of fluorescence lines (1..line_last, line_last <= 219) of element Z
  * that the detection-weight estimator evaluates: energy >= 1 keV and a

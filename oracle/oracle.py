@@ -102,12 +102,15 @@ def load():
                                              C.c_long, C.c_uint64, D]
     lib.xmi_oracle_probe_doppler.restype = C.c_int
     lib.xmi_oracle_set_plant.argtypes = [C.c_int]
+    lib.xmi_oracle_probe_detector_hit.argtypes = [C.c_void_p, D, D, C.c_long, C.POINTER(C.c_int)]
+    lib.xmi_oracle_probe_detector_segment.argtypes = [C.c_void_p, D, D, C.c_long, C.POINTER(C.c_int)]
     _lib = lib
     return lib
 
 
 def run(prepared, seed: int, rng_mode: int = RNG_PER_HISTORY, n_threads: int = 0, n_mpi_hosts: int = 1):
-    """xmi_main_msim restated on the CPU; returns (channels, var_red, brute, stats)."""
+    """xmi_main_msim restated on the CPU; returns (channels, var_red, brute, stats);
+    var_red is None for a run without variance reduction."""
     lib = load()
     if n_threads <= 0:
         n_threads = os.cpu_count() or 1
@@ -128,7 +131,8 @@ def run(prepared, seed: int, rng_mode: int = RNG_PER_HISTORY, n_threads: int = 0
     n_int = prepared.inp.n_interactions_trajectory
     nch = prepared.inp.nchannels
     channels = np.ctypeslib.as_array(ch, shape=(n_int + 1, nch)).copy()
-    var_red = np.ctypeslib.as_array(vr, shape=(100, 385, n_int)).copy()
+    # NULL without variance reduction, as in the reference
+    var_red = np.ctypeslib.as_array(vr, shape=(100, 385, n_int)).copy() if vr else None
     brute = np.ctypeslib.as_array(bh, shape=(100, 385, n_int)).copy()
     for p in (ch, bh, vr):
         lib.xmi_oracle_free(p)
@@ -193,7 +197,8 @@ def detector_convolute(prepared, noconv):
 # ---- probes of single restated routines (pinning tests) --------------------
 PLANTS = {"none": 0, "rayleigh_energy": 1, "phi0_sign": 2, "compton_full_angle": 3, "doppler_one_sided": 4,
           "ck_l1_swap": 5, "line_shift": 6, "elecv_stale": 7, "dirv_phi_sign": 8, "slit_full": 9, "gauss_xy": 10,
-          "compton_t_rayleigh": 11, "shell_next": 12, "qimax_ignored": 13, "doppler_no_negative": 14}
+          "compton_t_rayleigh": 11, "shell_next": 12, "qimax_ignored": 13, "doppler_no_negative": 14,
+          "brute_blbs": 15, "auger_one_vacancy": 16, "radiative_shell": 17, "no_collimator": 18}
 
 
 class planted:
@@ -289,4 +294,26 @@ def probe_doppler(prepared, element_index: int, energy: float, theta: float, n: 
     if not load().xmi_oracle_probe_doppler(prepared.inputF, prepared.hdf5F, C.byref(prepared.options), element_index,
                                            energy, theta, n, seed, _d(out)):
         raise RuntimeError("probe failed")
+    return out
+
+
+def probe_detector_hit(prepared, coords, dirv):
+    """1 where a photon that left the sample at coords moving along dirv
+    reaches the detector (lab frame, arrays of shape (n, 3))."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    dirv = np.ascontiguousarray(dirv, dtype=np.float64)
+    out = np.zeros(len(coords), dtype=np.int32)
+    load().xmi_oracle_probe_detector_hit(prepared.inputF, _d(coords), _d(dirv), len(coords),
+                                         out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
+
+
+def probe_detector_segment(prepared, begin, end):
+    """Codes of the detector test of path segments begin -> end (lab frame):
+    0 none, 1 collimator, 2 detector window, 3 window from behind."""
+    begin = np.ascontiguousarray(begin, dtype=np.float64)
+    end = np.ascontiguousarray(end, dtype=np.float64)
+    out = np.zeros(len(begin), dtype=np.int32)
+    load().xmi_oracle_probe_detector_segment(prepared.inputF, _d(begin), _d(end), len(begin),
+                                             out.ctypes.data_as(C.POINTER(C.c_int)))
     return out

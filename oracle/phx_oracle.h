@@ -11,6 +11,11 @@
 #define XMI_ORACLE_PHX_H
 #include <stdint.h>
 
+/* substreams: 0 source, 1 transport of interaction n, 2 detection-weight
+ * estimator of interaction n, 3 solid-angle grid, 4 transport of interaction
+ * n of a cascade's offspring photon (brute-force runs) */
+#define PHX_SUB_OFFSPRING 4u
+
 typedef struct {
   uint32_t k0, k1;
   uint32_t c0, c1, c2, c3;

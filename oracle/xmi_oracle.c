@@ -224,16 +224,18 @@ typedef struct {
   uint64_t seed;
   int rng_mode;
   int escape;                 /* escape-ratio run: no estimator, analogue exit test */
+  int brute;                  /* use_variance_reduction = 0: analogue transport, detector hits */
 } orc_ctx;
 
 typedef struct {
   double *channels;           /* [(n_int+1)][nch] */
   double *var_red;            /* [100][385][n_int] Fortran-equivalent indexed as [Z-1][row-1][n-1] */
+  double *brute;              /* the same layout: weight of detector hits by last interaction */
   int64_t interactions, vr_calls, fallbacks;
   xmi_oracle_work work;       /* estimator work counters */
 } orc_tally;
 
-typedef struct {
+typedef struct orc_photon_s {
   double coords[3], dirv[3], elecv[3];
   double theta, phi;
   double energy, weight;
@@ -249,6 +251,12 @@ typedef struct {
   double weight_escape;       /* escape-ratio run: weight after the forced step */
   orc_rng *tr;                /* transport stream */
   orc_rng *vr;                /* detection-weight stream */
+  /* brute-force runs */
+  int detector_hit;
+  uint32_t tr_sub;            /* substream of the transport: 1, or 4 for a cascade's offspring */
+  int casc_auger, casc_radiative;
+  struct orc_photon_s *offspring; /* where a cascade puts its second photon */
+  int has_offspring;
 } orc_photon;
 
 static double layer_mu(const orc_ctx *c, const xmi_layerF *L, double E) {
@@ -718,6 +726,162 @@ static void orc_coster_kronig(const xrl_atom *a, orc_rng *rng, int *shell) {
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* cascades of the brute-force runs, src/xmi_main.F90:2413-4783 */
+
+/* xmi_fluorescence_yield_check (src/xmi_main.F90:5325-5350): 1 when the
+ * vacancy of shell fluoresces.  The reference reads the corrected yield of
+ * shells beyond M5 past the end of its nine-entry array; here they never
+ * fluoresce (the number is drawn all the same). */
+static int orc_yield_check(const xmi_hdf5_ZF *z, int shell, orc_rng *rng) {
+  double r = orc_uniform(rng);
+  double y = (shell >= 0 && shell <= 8) ? z->FluorYieldsCorr[shell] : 0.0;
+  return !(r > y);
+}
+
+/* xmi_fluorescence_line_check (src/xmi_main.F90:5352-5437): the line of a
+ * vacancy in shell by its radiative rates; the line energy, or 0 (and no
+ * line) when none is found or the shell has no lines */
+static double orc_line_check(const orc_ctx *c, int hidx, int shell, orc_rng *rng, int *line) {
+  const xrl_atom *a = c->atoms[hidx];
+  double r = orc_uniform(rng);
+  *line = 0;
+  if (shell < 0 || shell > 8) return 0.0;
+  int first, last;
+  xrl_shell_lines(shell, &first, &last);
+  double sumz = 0.0;
+  for (int l = first; l <= last; l++) {
+    sumz += a->rad_rate[l];
+    if (r < sumz) {
+      *line = l;
+      return c->line_energies[(size_t)hidx * XMI_NLINES_VR + (l - 1)];
+    }
+  }
+  return 0.0;
+}
+
+/* a cascade's photon: isotropic direction, random polarisation orthogonal
+ * to it, history entry of its line (src/xmi_main.F90:4330-4370) */
+static void orc_cascade_emit(const orc_ctx *c, orc_photon *q, double energy, int line, orc_rng *rng) {
+  q->energy = energy;
+  q->energy_changed = 0;
+  mu_calc_all(c, energy, q->mus);
+  q->detector_hit = 0;
+  q->casc_auger = q->casc_radiative = 0;
+  q->theta = acos(2.0 * orc_uniform(rng) - 1.0);
+  q->phi = 2.0 * ORC_PI * orc_uniform(rng);
+  q->dirv[0] = sin(q->theta) * cos(q->phi);
+  q->dirv[1] = sin(q->theta) * sin(q->phi);
+  q->dirv[2] = cos(q->theta);
+  q->hist[q->n_interactions][0] = -line;
+  q->hist[q->n_interactions][1] = q->current_element;
+  double r = 2.0 * ORC_PI * orc_uniform(rng);
+  q->elecv[0] = cos(r);
+  q->elecv[1] = sin(r);
+  q->elecv[2] = 0.0;
+  q->last_interaction = XMI_PHOTOELECTRIC_INTERACTION;
+  double cosalfa = dot3(q->elecv, q->dirv);
+  double c_alfa = acos(cosalfa);
+  double c_ae = 1.0 / sin(c_alfa), c_be = -c_ae * cosalfa;
+  for (int k = 0; k < 3; k++) q->elecv[k] = c_ae * q->elecv[k] + c_be * q->dirv[k];
+}
+
+/* the second photon of a cascade starts where the parent is, with its
+ * weight, layer, element and history */
+static orc_photon *orc_offspring_init(orc_photon *p) {
+  orc_photon *o = p->offspring;
+  if (!o) return NULL;
+  memcpy(o->hist, p->hist, sizeof(p->hist));
+  memcpy(o->coords, p->coords, sizeof(p->coords));
+  o->current_layer = p->current_layer;
+  o->weight = p->weight;
+  o->n_interactions = p->n_interactions;
+  o->current_element = p->current_element;
+  o->current_element_index = p->current_element_index;
+  o->detector_hit = 0;
+  o->tr_sub = PHX_SUB_OFFSPRING;
+  o->offspring = NULL;
+  o->has_offspring = 0;
+  return o;
+}
+
+/* xmi_simulate_photon_cascade_auger (src/xmi_main.F90:2413-4594): the
+ * vacancy of shell decays by an Auger transition into two vacancies; the
+ * first may fluoresce as the parent photon (which then has no cascades
+ * left), the second as an offspring photon.  M vacancies count only with
+ * use_M_lines. */
+static void orc_cascade_auger(const orc_ctx *c, orc_photon *p, int shell) {
+  int first, last;
+  if (!xrl_auger_range(shell, &first, &last)) return;
+  const xmi_layerF *L = &c->in->layers[p->current_layer - 1];
+  int hidx = L->hdf5_index[p->current_element_index];
+  const xrl_atom *a = c->atoms[hidx];
+  const xmi_hdf5_ZF *z = &c->h->Zs[hidx];
+  double r = orc_uniform(p->tr), sumz = 0.0;
+  int t, found = 0;
+  for (t = first; t <= last; t++) {
+    sumz += xrl_auger_rate(a, t);
+    if (r < sumz) {
+      found = 1;
+      break;
+    }
+  }
+  if (!found) return;
+  int s[2];
+  xrl_auger_shells(t, &s[0], &s[1]);
+  const int use_M = c->opt->use_M_lines;
+  orc_photon *o = orc_offspring_init(p);
+  for (int k = 0; k < 2; k++) {
+    const int m_shell = s[k] >= XRL_M1_SHELL && s[k] <= XRL_M5_SHELL;
+    if ((m_shell && !use_M) || !orc_yield_check(z, s[k], p->tr)) continue;
+    orc_coster_kronig(a, p->tr, &s[k]);
+    int line;
+    double e = orc_line_check(c, hidx, s[k], p->tr, &line);
+    if (e <= XMI_ENERGY_THRESHOLD) {
+      if (k == 0) p->energy = 0.0;
+      continue;
+    }
+    if (k == 0) orc_cascade_emit(c, p, e, line, p->tr);
+    else if (o && g_plant != XMI_ORACLE_PLANT_AUGER_ONE_VACANCY) {
+      orc_cascade_emit(c, o, e, line, p->tr);
+      p->has_offspring = 1;
+    }
+  }
+}
+
+/* xmi_simulate_photon_cascade_radiative (src/xmi_main.F90:4596-4783): the
+ * line the parent emitted leaves a vacancy in its other shell (K -> L or M,
+ * L -> M with use_M_lines), which may fluoresce as an offspring photon; the
+ * parent and the offspring then have no cascades left */
+static void orc_cascade_radiative(const orc_ctx *c, orc_photon *p, int shell, int line) {
+  const int use_M = c->opt->use_M_lines;
+  int sn = -1;
+  if (shell == XRL_K_SHELL) {
+    if (line >= xrl_line_KL(0) && line <= xrl_line_KM(4)) sn = line; /* KL1..KL3, KM1..KM5 -> L1..M5 */
+    if (g_plant == XMI_ORACLE_PLANT_RADIATIVE_SHELL && sn > XRL_L1_SHELL) sn--;
+  } else if (shell >= XRL_L1_SHELL && shell <= XRL_L3_SHELL && use_M) {
+    int base = xrl_line_LM(shell - 1, 0);
+    if (line >= base && line <= base + 4) sn = XRL_M1_SHELL + (line - base);
+  } else {
+    return;
+  }
+  if (sn < 0) return;
+  if (sn >= XRL_M1_SHELL && sn <= XRL_M5_SHELL && !use_M) return;
+  const xmi_layerF *L = &c->in->layers[p->current_layer - 1];
+  int hidx = L->hdf5_index[p->current_element_index];
+  const xmi_hdf5_ZF *z = &c->h->Zs[hidx];
+  if (!orc_yield_check(z, sn, p->tr)) return;
+  orc_coster_kronig(c->atoms[hidx], p->tr, &sn);
+  int ln;
+  double e = orc_line_check(c, hidx, sn, p->tr, &ln);
+  if (e <= XMI_ENERGY_THRESHOLD) return;
+  orc_photon *o = orc_offspring_init(p);
+  p->casc_auger = p->casc_radiative = 0;
+  if (!o) return;
+  orc_cascade_emit(c, o, e, ln, p->tr);
+  p->has_offspring = 1;
+}
+
 static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
   const xmi_layerF *L = &c->in->layers[p->current_layer - 1];
   int hidx = L->hdf5_index[p->current_element_index];
@@ -740,12 +904,22 @@ static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
     p->energy = 0.0;
     return 1;
   }
-  /* xmi_fluorescence_yield_check_varred_optim: draws a number it never uses */
-  (void)orc_uniform(p->tr);
-  /* xmi_fluorescence_yield_check_varred_optim (src/xmi_variance_reduction.F90:
-   * 728-750): an escape-ratio run weights the escape tally, not the photon */
-  if (c->escape) p->weight_escape *= z->FluorYieldsCorr[shell];
-  else p->weight *= z->FluorYieldsCorr[shell];
+  if (c->brute) {
+    /* xmi_fluorescence_yield_check (src/xmi_main.F90:5325-5350): no
+     * fluorescence, perhaps an Auger cascade (src/xmi_main.F90:2298-2315) */
+    if (!orc_yield_check(z, shell, p->tr)) {
+      p->energy = 0.0;
+      if (p->casc_auger) orc_cascade_auger(c, p, shell);
+      return 1;
+    }
+  } else {
+    /* xmi_fluorescence_yield_check_varred_optim (src/xmi_variance_reduction.F90:
+     * 728-750) draws a number it never uses; an escape-ratio run weights the
+     * escape tally, not the photon */
+    (void)orc_uniform(p->tr);
+    if (c->escape) p->weight_escape *= z->FluorYieldsCorr[shell];
+    else p->weight *= z->FluorYieldsCorr[shell];
+  }
   orc_coster_kronig(a, p->tr, &shell);
   /* xmi_fluorescence_line_check */
   r = orc_uniform(p->tr);
@@ -776,6 +950,7 @@ static int orc_fluorescence(const orc_ctx *c, orc_photon *p) {
   p->hist[p->n_interactions][0] = -line;
   p->hist[p->n_interactions][1] = p->current_element;
   p->hist[p->n_interactions][2] = c->h->uniqZ[p->current_element];
+  if (c->brute && p->casc_radiative) orc_cascade_radiative(c, p, shell, line);
   return 1;
 }
 
@@ -948,7 +1123,7 @@ static void orc_variance_reduction(const orc_ctx *c, orc_photon *p, orc_tally *t
 
 static void reseed_interaction(const orc_ctx *c, orc_photon *p, orc_rng *tr, orc_rng *vr, uint64_t hid, int k) {
   if (c->rng_mode != XMI_ORACLE_RNG_PER_HISTORY) return;
-  orc_rng_init(tr, c->seed, ((uint32_t)(k & 0xFF) << 4) | 1u, hid);
+  orc_rng_init(tr, c->seed, ((uint32_t)(k & 0xFF) << 4) | (p->tr_sub ? p->tr_sub : 1u), hid);
   orc_rng_init(vr, c->seed, ((uint32_t)(k & 0xFF) << 4) | 2u, hid);
   p->tr = tr;
   p->vr = vr;
@@ -981,6 +1156,203 @@ static int orc_escape_test(const orc_ctx *c, const orc_photon *p, double interac
   return 1;
 }
 
+/* ------------------------------------------------------------------------ */
+/* brute-force transport (use_variance_reduction = 0) */
+
+/* return codes of src/xmi_aux_f.F90:1622-1833 */
+enum { ORC_NO_INTERSECTION = 0, ORC_COLLIMATOR_INTERSECTION = 1, ORC_DETECTOR_INTERSECTION = 2,
+       ORC_DETECTOR_BAD_INTERSECTION = 3 };
+
+static void to_detector_frame(const xmi_inputF *in, const double *x, double *out) {
+  double d[3] = {x[0] - in->geometry.p_detector_window[0], x[1] - in->geometry.p_detector_window[1],
+                 x[2] - in->geometry.p_detector_window[2]};
+  matvec3(in->n_detector_orientation_inverse, d, out);
+}
+
+/* xmi_check_detector_intersection (src/xmi_aux_f.F90:1622-1833): does the
+ * path segment b -> e cross the detector window (in the detector frame the
+ * plane x = 0 within the detector radius) or the collimator cone below its
+ * height?  Without a collimator the reference sets the line's direction to
+ * norm(e - b), a scalar, which Fortran assigns to all three components
+ * (src/xmi_aux_f.F90:1668): the line is (L, L, L), its crossing of x = 0
+ * lies at (0, e_y - e_x, e_z - e_x) and its dot product with the plane
+ * normal is L >= 0, so every crossing within the radius is a bad one.  Kept
+ * as the reference computes it.  Where the reference stops the program (a
+ * line parallel to the plane) this returns no intersection. */
+static int orc_detector_intersection(const xmi_inputF *in, const double *b, const double *e) {
+  double bd[3], ed[3], ip[3];
+  to_detector_frame(in, b, bd);
+  to_detector_frame(in, e, ed);
+  const double pp0[3] = {0.0, 0.0, 0.0}, pn[3] = {1.0, 0.0, 0.0};
+  if (!in->collimator_present) {
+    if (bd[0] * ed[0] > 0.0) return ORC_NO_INTERSECTION;
+    double L = distance3(ed, bd);
+    double ld[3] = {L, L, L};
+    if (!intersection_plane_line(pp0, pn, ed, ld, ip)) return ORC_NO_INTERSECTION;
+    ip[0] = 0.0;
+    if (norm3(ip) <= in->detector_radius) {
+      if (dot3(ld, pn) >= 0.0) return ORC_DETECTOR_BAD_INTERSECTION;
+      return ORC_DETECTOR_INTERSECTION;
+    }
+    return ORC_NO_INTERSECTION;
+  }
+  double ld[3] = {ed[0] - bd[0], ed[1] - bd[1], ed[2] - bd[2]};
+  double t_begin = (bd[0] - ed[0]) / ld[0], t_end = (ed[0] - ed[0]) / ld[0];
+  double t_hi = t_begin > t_end ? t_begin : t_end, t_lo = t_begin < t_end ? t_begin : t_end;
+  double delta[3] = {ed[0] - in->vertex[0], ed[1] - in->vertex[1], ed[2] - in->vertex[2]};
+  double cth = cos(in->half_apex), cos2theta = cth * cth;
+  double Md[3] = {1.0 - cos2theta, -cos2theta, -cos2theta};
+  double dM[3] = {ld[0] * Md[0], ld[1] * Md[1], ld[2] * Md[2]};
+  double deltaM[3] = {delta[0] * Md[0], delta[1] * Md[1], delta[2] * Md[2]};
+  double c2 = dot3(dM, ld), c1 = dot3(dM, delta), c0 = dot3(deltaM, delta);
+  double disc = c1 * c1 - c0 * c2;
+  if (disc < 0.0) return ORC_NO_INTERSECTION;
+  int n_valid_t = 0;
+  double t1 = (-c1 + sqrt(disc)) / c2, t2 = (-c1 - sqrt(disc)) / c2;
+  double X1[3], X2[3];
+  for (int k = 0; k < 3; k++) {
+    X1[k] = ed[k] + t1 * ld[k];
+    X2[k] = ed[k] + t2 * ld[k];
+  }
+  if (-(X1[0] - in->vertex[0]) >= 0.0) n_valid_t = 1;
+  if (-(X2[0] - in->vertex[0]) >= 0.0) n_valid_t = n_valid_t == 1 ? 3 : 2;
+  const double h = in->geometry.collimator_height;
+  if (n_valid_t == 0) return ORC_NO_INTERSECTION;
+  if (n_valid_t == 3) {
+    if (t1 <= t_hi && t1 >= t_lo && X1[0] <= h) return ORC_COLLIMATOR_INTERSECTION;
+    if (t2 <= t_hi && t2 >= t_lo && X2[0] <= h) return ORC_COLLIMATOR_INTERSECTION;
+    return ORC_NO_INTERSECTION;
+  }
+  const double t = n_valid_t == 1 ? t1 : t2, Xx = n_valid_t == 1 ? X1[0] : X2[0];
+  if (t <= t_hi && t >= t_lo && Xx <= h) return ORC_COLLIMATOR_INTERSECTION;
+  if (!intersection_plane_line(pp0, pn, ed, ld, ip)) return ORC_NO_INTERSECTION;
+  ip[0] = 0.0;
+  if (norm3(ip) <= in->detector_radius && distance3(bd, ip) <= distance3(bd, ed)) {
+    if (dot3(ld, pn) >= 0.0) return ORC_DETECTOR_BAD_INTERSECTION;
+    return ORC_DETECTOR_INTERSECTION;
+  }
+  return ORC_NO_INTERSECTION;
+}
+
+/* xmi_check_photon_detector_hit (src/xmi_main.F90:1920-1984): a photon that
+ * left the sample reaches the detector when it moves against the detector
+ * normal, crosses the window plane within the detector radius and, with a
+ * collimator, crosses the collimator's top plane within its radius */
+static int orc_photon_detector_hit(const xmi_inputF *in, const orc_photon *p) {
+  if (dot3(p->dirv, in->geometry.n_detector_orientation) >= 0.0) return 0;
+  double dd[3], cd[3], ip[3];
+  matvec3(in->n_detector_orientation_inverse, p->dirv, dd);
+  to_detector_frame(in, p->coords, cd);
+  const double pp0[3] = {0.0, 0.0, 0.0}, pn[3] = {1.0, 0.0, 0.0};
+  if (!intersection_plane_line(pp0, pn, cd, dd, ip)) return 0;
+  if (norm3(ip) > in->detector_radius) return 0;
+  if (!in->collimator_present || g_plant == XMI_ORACLE_PLANT_NO_COLLIMATOR) return 1;
+  const double pc[3] = {in->geometry.collimator_height, 0.0, 0.0};
+  if (!intersection_plane_line(pc, pn, cd, dd, ip)) return 0;
+  ip[0] = 0.0;
+  return norm3(ip) <= in->collimator_radius;
+}
+
+/* the analogue step of src/xmi_main.F90:1274-1407 with the detector checks
+ * of every path segment: 1 when the photon interacts in the sample, 2 when
+ * its history ends (it left the sample, reached the detector window or hit
+ * the collimator; detector_hit set for the window and for a photon that
+ * leaves towards the detector), 0 on failure */
+static int orc_brute_step(const orc_ctx *c, orc_photon *p, double interactionR) {
+  const xmi_inputF *in = c->in;
+  int dir = dot3(p->dirv, in->geometry.n_sample_orientation) > 0.0 ? 1 : -1;
+  int last = dir > 0 ? in->n_layers : 1;
+  double max_random_layer = 0.0, min_random_layer = 0.0, blbs = 1.0;
+  for (int i = p->current_layer; dir > 0 ? i <= last : i >= last; i += dir) {
+    const xmi_layerF *L = &in->layers[i - 1];
+    double pp[3] = {0.0, 0.0, dir == 1 ? L->Z_coord_end : L->Z_coord_begin}, ip[3];
+    if (!intersection_plane_line(pp, in->geometry.n_sample_orientation, p->coords, p->dirv, ip)) return 0;
+    double dist = distance3(p->coords, ip);
+    double temp_prod = -1.0 * dist * L->density * p->mus[i - 1];
+    double tempexp = exp(temp_prod);
+    min_random_layer = max_random_layer;
+    max_random_layer = max_random_layer - blbs * expm1(temp_prod);
+    int rc;
+    if (interactionR <= max_random_layer) {
+      dist = -1.0 * log1p(-1.0 * (interactionR - min_random_layer) / blbs) / p->mus[i - 1] / L->density;
+      double from[3] = {p->coords[0], p->coords[1], p->coords[2]};
+      for (int k = 0; k < 3; k++) p->coords[k] += dist * p->dirv[k];
+      rc = orc_detector_intersection(in, from, p->coords);
+      if (rc == ORC_DETECTOR_INTERSECTION) p->detector_hit = 1;
+      if (rc != ORC_NO_INTERSECTION) return 2;
+      p->current_layer = i;
+      return 1;
+    }
+    rc = orc_detector_intersection(in, p->coords, ip);
+    if (rc == ORC_DETECTOR_INTERSECTION) p->detector_hit = 1;
+    if (rc != ORC_NO_INTERSECTION) return 2;
+    p->coords[0] = ip[0];
+    p->coords[1] = ip[1];
+    p->coords[2] = ip[2];
+    if (g_plant != XMI_ORACLE_PLANT_BRUTE_BLBS) blbs = blbs * tempexp;
+  }
+  p->detector_hit = orc_photon_detector_hit(in, p);
+  return 2;
+}
+
+/* the tallies of a photon that reached the detector (src/xmi_main.F90:
+ * 445-535): its weight into the channel of its energy for every order from
+ * its number of interactions on, and into the history row of its last
+ * interaction (line, Rayleigh or Compton) of the element it happened on */
+static void orc_brute_tally(const orc_ctx *c, orc_tally *t, const orc_photon *p) {
+  if (!p->detector_hit) return;
+  tally_channel(c, t, p->n_interactions, p->energy, p->weight);
+  int k = p->n_interactions;
+  if (k <= 0) return;
+  int code = p->hist[k][0], Z = p->hist[k][1], row = 0;
+  if (code <= -1 && code >= -383) row = -code;
+  else if (code == XMI_RAYLEIGH_INTERACTION) row = 384;
+  else if (code == XMI_COMPTON_INTERACTION) row = 385;
+  if (row) t->brute[((size_t)(Z - 1) * XMI_HIST_ROWS + (row - 1)) * c->n_int + (k - 1)] += p->weight;
+}
+
+/* the forced step of src/xmi_main.F90:1415-1520: the photon interacts in
+ * the sample with its weight multiplied by the probability of interacting
+ * on the way out; 1 on success, 0 on failure */
+static int orc_forced_step(const orc_ctx *c, orc_photon *p, double interactionR, int step_do_max, int step_do_dir) {
+  const xmi_inputF *in = c->in;
+  double distances[64];
+  double lp[3] = {p->coords[0], p->coords[1], p->coords[2]};
+  for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
+    double pp[3] = {0.0, 0.0, step_do_dir == 1 ? in->layers[i - 1].Z_coord_end : in->layers[i - 1].Z_coord_begin};
+    double ip[3];
+    if (!intersection_plane_line(pp, in->geometry.n_sample_orientation, lp, p->dirv, ip)) return 0;
+    distances[i - 1] = distance3(lp, ip);
+    lp[0] = ip[0];
+    lp[1] = ip[1];
+    lp[2] = ip[2];
+  }
+  double Pabs = 0.0;
+  for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir)
+    Pabs += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
+  double Pabs2 = -1.0 * expm1(-1.0 * Pabs);
+  p->weight *= Pabs2;
+  if (c->escape) p->weight_escape = p->weight;
+  double negln = -1.0 * log1p(-1.0 * interactionR * Pabs2);
+  int my_index = p->current_layer;
+  double my_sum = 0.0;
+  for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
+    my_sum += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
+    if (my_sum > negln) {
+      my_index = i;
+      break;
+    }
+  }
+  double mrm = p->mus[my_index - 1] * in->layers[my_index - 1].density;
+  double temp_sum = 0.0;
+  for (int i = p->current_layer; step_do_dir > 0 ? i <= my_index : i >= my_index; i += step_do_dir)
+    temp_sum += (1.0 - (p->mus[i - 1] * in->layers[i - 1].density / mrm)) * distances[i - 1];
+  temp_sum = temp_sum - 1.0 * log1p(-1.0 * interactionR * Pabs2) / mrm;
+  for (int i = 0; i < 3; i++) p->coords[i] += temp_sum * p->dirv[i];
+  p->current_layer = my_index;
+  return 1;
+}
+
 static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, uint64_t hid, orc_rng *tr, orc_rng *vr) {
   const xmi_inputF *in = c->in;
   for (;;) {
@@ -999,49 +1371,22 @@ static int orc_simulate_photon(const orc_ctx *c, orc_photon *p, orc_tally *t, ui
     }
     reseed_interaction(c, p, tr, vr, hid, p->n_interactions + 1);
     double interactionR = orc_uniform(p->tr);
-    if (p->n_interactions == c->n_int) {
-      if (c->escape) p->escaped = orc_escape_test(c, p, interactionR);
-      break;
-    }
-    double distances[64];
-    double lp[3] = {p->coords[0], p->coords[1], p->coords[2]};
-    for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
-      double pp[3] = {0.0, 0.0, step_do_dir == 1 ? in->layers[i - 1].Z_coord_end : in->layers[i - 1].Z_coord_begin};
-      double ip[3];
-      if (!intersection_plane_line(pp, in->geometry.n_sample_orientation, lp, p->dirv, ip)) return 0;
-      distances[i - 1] = distance3(lp, ip);
-      lp[0] = ip[0];
-      lp[1] = ip[1];
-      lp[2] = ip[2];
-    }
-    double Pabs = 0.0;
-    for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir)
-      Pabs += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
-    double Pabs2 = -1.0 * expm1(-1.0 * Pabs);
-    p->weight *= Pabs2;
-    if (c->escape) p->weight_escape = p->weight;
-    double negln = -1.0 * log1p(-1.0 * interactionR * Pabs2);
-    int my_index = p->current_layer;
-    double my_sum = 0.0;
-    for (int i = p->current_layer; step_do_dir > 0 ? i <= step_do_max : i >= step_do_max; i += step_do_dir) {
-      my_sum += p->mus[i - 1] * in->layers[i - 1].density * distances[i - 1];
-      if (my_sum > negln) {
-        my_index = i;
+    if (c->brute) {
+      int rs = orc_brute_step(c, p, interactionR);
+      if (rs == 0) return 0;
+      if (rs == 2) break;
+    } else {
+      if (p->n_interactions == c->n_int) {
+        if (c->escape) p->escaped = orc_escape_test(c, p, interactionR);
         break;
       }
+      if (!orc_forced_step(c, p, interactionR, step_do_max, step_do_dir)) return 0;
     }
-    double mrm = p->mus[my_index - 1] * in->layers[my_index - 1].density;
-    double temp_sum = 0.0;
-    for (int i = p->current_layer; step_do_dir > 0 ? i <= my_index : i >= my_index; i += step_do_dir)
-      temp_sum += (1.0 - (p->mus[i - 1] * in->layers[i - 1].density / mrm)) * distances[i - 1];
-    temp_sum = temp_sum - 1.0 * log1p(-1.0 * interactionR * Pabs2) / mrm;
-    for (int i = 0; i < 3; i++) p->coords[i] += temp_sum * p->dirv[i];
-    p->current_layer = my_index;
 
     if (p->n_interactions == c->n_int) break;
     p->n_interactions++;
     t->interactions++;
-    if (!c->escape) orc_variance_reduction(c, p, t);
+    if (!c->escape && !c->brute) orc_variance_reduction(c, p, t);
 
     /* atom selection */
     const xmi_layerF *L = &in->layers[p->current_layer - 1];
@@ -1254,14 +1599,47 @@ static void orc_ctx_free_tables(orc_ctx *c) {
 
 /* ------------------------------------------------------------------------ */
 
+/* a fresh primary photon; its cascades follow the options in a brute-force
+ * run (src/xmi_main.F90:2306,2394) */
+static void orc_photon_init(const orc_ctx *c, orc_photon *p, orc_photon *po, orc_rng *thread_rng) {
+  memset(p, 0, sizeof(*p));
+  p->tr = p->vr = thread_rng;
+  p->tr_sub = 1u;
+  if (c->brute) {
+    p->casc_auger = c->opt->use_cascade_auger ? 1 : 0;
+    p->casc_radiative = c->opt->use_cascade_radiative ? 1 : 0;
+    memset(po, 0, sizeof(*po));
+    po->tr = po->vr = thread_rng;
+    p->offspring = po;
+  }
+}
+
+/* one history: the primary, then the offspring of its cascade (the
+ * reference runs the offspring inside the parent's interaction; with a
+ * stream of its own the order does not change any number), each tallied
+ * when it reaches the detector (src/xmi_main.F90:435-538) */
+static int orc_history(const orc_ctx *c, orc_photon *p, orc_tally *t, uint64_t hid, orc_rng *tr, orc_rng *vr) {
+  if (!orc_simulate_photon(c, p, t, hid, tr, vr)) return 0;
+  if (!c->brute) return 1;
+  orc_brute_tally(c, t, p);
+  if (p->has_offspring) {
+    orc_photon *o = p->offspring;
+    o->tr = o->vr = p->tr;
+    if (c->rng_mode == XMI_ORACLE_RNG_PER_HISTORY) o->tr = o->vr = tr;
+    if (!orc_simulate_photon(c, o, t, hid, tr, vr)) return 0;
+    orc_brute_tally(c, t, o);
+  }
+  return 1;
+}
+
 int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_mpi_hosts, double **channelsPtr,
                          xmi_main_options *options, double **brute_historyPtr, double **var_red_historyPtr,
                          xmi_solid_angle *solid_angles, int rng_mode, uint64_t seed, int n_threads) {
   const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
   const xmi_hdf5F *h = (const xmi_hdf5F *)hdf5FPtr;
   if (!in || !h || !options || !channelsPtr || !brute_historyPtr || !var_red_historyPtr) return 0;
-  if (!options->use_variance_reduction || !solid_angles) {
-    fprintf(stderr, "xmi_oracle_main_msim: only the variance-reduction path is restated\n");
+  if (options->use_variance_reduction && !solid_angles) {
+    fprintf(stderr, "xmi_oracle_main_msim: the variance-reduction path needs the solid-angle grid\n");
     return 0;
   }
   if (n_mpi_hosts < 1) n_mpi_hosts = 1;
@@ -1281,6 +1659,7 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
   c.seed = seed;
   c.rng_mode = rng_mode;
   c.cascade_type = cascade_of_options(options);
+  c.brute = !options->use_variance_reduction;
   if (c.n_int > MAX_INT_TRAJ || in->n_layers > 64) return 0;
   orc_ctx_tables(&c);
 
@@ -1288,6 +1667,7 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
   size_t nhist = (size_t)100 * XMI_HIST_ROWS * c.n_int;
   double *channels = (double *)calloc(nchan_tot, sizeof(double));
   double *var_red = (double *)calloc(nhist, sizeof(double));
+  double *brute = (double *)calloc(nhist, sizeof(double));
   int64_t histories = 0, interactions = 0, vr_calls = 0, fallbacks = 0;
   int ok = 1;
   memset(&g_work, 0, sizeof(g_work));
@@ -1305,9 +1685,10 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
     memset(&t, 0, sizeof(t));
     t.channels = (double *)calloc(nchan_tot, sizeof(double));
     t.var_red = (double *)calloc(nhist, sizeof(double));
+    t.brute = (double *)calloc(nhist, sizeof(double));
     orc_rng thread_rng, src_rng, tr_rng, vr_rng;
     orc_rng_init(&thread_rng, seed, 0xF0000000u | (uint32_t)tid, 0);
-    orc_photon p;
+    orc_photon p, po;
 
     /* photon ranges: per-thread mode follows the reference split (thread 0
      * takes the remainder); per-history mode hands out contiguous chunks */
@@ -1337,13 +1718,12 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
           orc_rng_init(&src_rng, seed, 0u, hid);
           src = &src_rng;
         }
-        memset(&p, 0, sizeof(p));
-        p.tr = p.vr = &thread_rng;
+        orc_photon_init(&c, &p, &po, &thread_rng);
         if (orc_source_continuous(&c, e0, e1, nint_ph, src, &p) < 0) {
           ok = 0;
           break;
         }
-        if (!orc_simulate_photon(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
+        if (!orc_history(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
         histories++;
       }
     }
@@ -1366,15 +1746,14 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
           orc_rng_init(&src_rng, seed, 0u, hid);
           src = &src_rng;
         }
-        memset(&p, 0, sizeof(p));
-        p.tr = p.vr = &thread_rng;
+        orc_photon_init(&c, &p, &po, &thread_rng);
         int rs = orc_source_discrete(&c, d, initial_mus, weight, jj, hor_ver_ratio, src, &p);
         if (rs == 0) continue;
         if (rs < 0) {
           ok = 0;
           break;
         }
-        if (!orc_simulate_photon(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
+        if (!orc_history(&c, &p, &t, hid, &tr_rng, &vr_rng)) ok = 0;
         histories++;
       }
     }
@@ -1392,20 +1771,26 @@ int xmi_oracle_main_msim(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, int n_m
       g_work.line_layers += t.work.line_layers;
       for (size_t k = 0; k < nchan_tot; k++) channels[k] += t.channels[k];
       for (size_t k = 0; k < nhist; k++) var_red[k] += t.var_red[k];
+      for (size_t k = 0; k < nhist; k++) brute[k] += t.brute[k];
     }
     free(t.channels);
     free(t.var_red);
+    free(t.brute);
   }
 
   double live = in->detector.live_time;
   /* [Z][row][n] already C order; scale */
-  double *vr_out = (double *)malloc(sizeof(double) * nhist);
+  /* the variance-reduction history exists only with variance reduction
+   * (src/xmi_main.F90:934-943) */
+  double *vr_out = c.brute ? NULL : (double *)malloc(sizeof(double) * nhist);
   double *bf_out = (double *)calloc(nhist, sizeof(double));
-  for (size_t k = 0; k < nhist; k++) vr_out[k] = var_red[k] * live * n_mpi_hosts;
+  for (size_t k = 0; vr_out && k < nhist; k++) vr_out[k] = var_red[k] * live * n_mpi_hosts;
+  for (size_t k = 0; k < nhist; k++) bf_out[k] = brute[k] * live * n_mpi_hosts;
   double *ch_out = (double *)malloc(sizeof(double) * nchan_tot);
   for (size_t k = 0; k < nchan_tot; k++) ch_out[k] = channels[k] * live;
   free(channels);
   free(var_red);
+  free(brute);
   orc_ctx_free_tables(&c);
   g_stats.histories = histories;
   g_stats.interactions = interactions;
@@ -1981,4 +2366,22 @@ int xmi_oracle_probe_doppler(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi
     out[i] = orc_doppler_energy(&c.h->Zs[hidx], c.cp_cdfg + (size_t)hidx * XMI_CP_GRID, energy, theta, orc_word(&r));
   orc_ctx_free_tables(&c);
   return 1;
+}
+
+void xmi_oracle_probe_detector_hit(xmi_inputFPtr inputFPtr, const double *coords, const double *dirv, long n,
+                                   int *out) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  orc_photon p;
+  memset(&p, 0, sizeof(p));
+  for (long i = 0; i < n; i++) {
+    memcpy(p.coords, coords + 3 * i, sizeof(p.coords));
+    memcpy(p.dirv, dirv + 3 * i, sizeof(p.dirv));
+    out[i] = orc_photon_detector_hit(in, &p);
+  }
+}
+
+void xmi_oracle_probe_detector_segment(xmi_inputFPtr inputFPtr, const double *begin, const double *end, long n,
+                                       int *out) {
+  const xmi_inputF *in = (const xmi_inputF *)inputFPtr;
+  for (long i = 0; i < n; i++) out[i] = orc_detector_intersection(in, begin + 3 * i, end + 3 * i);
 }

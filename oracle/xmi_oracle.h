@@ -120,6 +120,16 @@ int xmi_oracle_probe_doppler(xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, xmi
  * (row-major), detector solid angle; then per layer thickness along z,
  * z begin, z end */
 void xmi_oracle_input_derived(xmi_inputFPtr inputFPtr, double *out);
+/* the brute-force run's detector tests on n given cases: out[i] = 1 when a
+ * photon at coords[i] moving along dirv[i] (lab frame) that has left the
+ * sample reaches the detector (src/xmi_main.F90:1920-1984) */
+void xmi_oracle_probe_detector_hit(xmi_inputFPtr inputFPtr, const double *coords, const double *dirv, long n,
+                                   int *out);
+/* out[i] = the code of src/xmi_aux_f.F90:1622-1833 for the path segment
+ * begin[i] -> end[i] (lab frame): 0 none, 1 collimator, 2 detector window,
+ * 3 window from behind */
+void xmi_oracle_probe_detector_segment(xmi_inputFPtr inputFPtr, const double *begin, const double *end, long n,
+                                       int *out);
 
 /* planted mistakes: a test sets one, shows that its check rejects the
  * result, and resets it to XMI_ORACLE_PLANT_NONE */
@@ -138,6 +148,10 @@ void xmi_oracle_input_derived(xmi_inputFPtr inputFPtr, double *out);
 #define XMI_ORACLE_PLANT_SHELL_NEXT 12         /* shell drawn with the next shell's cross section */
 #define XMI_ORACLE_PLANT_QIMAX_IGNORED 13      /* advanced Compton: subshell profiles not truncated at Qimax */
 #define XMI_ORACLE_PLANT_DOPPLER_NO_NEGATIVE 14 /* estimator: momenta of negative E' not drawn */
+#define XMI_ORACLE_PLANT_BRUTE_BLBS 15         /* analogue step: transmission of the crossed layers dropped */
+#define XMI_ORACLE_PLANT_AUGER_ONE_VACANCY 16  /* Auger cascade: second vacancy dropped */
+#define XMI_ORACLE_PLANT_RADIATIVE_SHELL 17    /* radiative cascade: K line's vacancy one shell too low */
+#define XMI_ORACLE_PLANT_NO_COLLIMATOR 18      /* detector hit test ignores the collimator */
 void xmi_oracle_set_plant(int id);
 
 /* trials of the Monte Carlo solid-angle fallback inside the estimator

@@ -94,9 +94,11 @@ int main(int argc, char **argv) {
     if (!xmi_input_write_to_xml_string(input, &s, &error)) FAIL("input string: %s", error ? error->message : "?");
     xmi_hits_per_single = hits;
     xmi_set_seed(seed_grid);
+    /* the grid keeps s and frees it with itself: the plugin entry takes the
+     * copy that the reference's wrapper xmi_solid_angle_calculation makes
+     * (src/xmi_solid_angle.c:183) */
     if (!xmi_solid_angle_calculation_cl(inputFPtr, &sa, s, options)) FAIL("grid: %s", xmi_msim_last_error());
     if (!xmi_update_solid_angle_hdf5_file(grids, sa)) FAIL("grid store: %s", xmi_msim_last_error());
-    xmi_free(s);
     printf("solid-angle grid computed and stored in %s\n", grids);
   }
 

@@ -460,9 +460,11 @@ class Prepared:
         self.solid_angle = None
 
     def run(self, seed: int, n_mpi_hosts: int = 1):
-        """xmi_main_msim on the GPU; returns (channels, var_red_history, stats)."""
+        """xmi_main_msim on the GPU; returns (channels, var_red_history,
+        brute_history, stats).  Without variance reduction (the brute-force
+        path) var_red_history is None and no solid-angle grid is needed."""
         lib = self.lib
-        if self.solid_angle is None:
+        if self.solid_angle is None and self.options.use_variance_reduction:
             raise RuntimeError("no solid-angle grid; call compute_solid_angle() first")
         lib.xmi_set_seed(seed)
         ch = C.POINTER(C.c_double)()
@@ -690,7 +692,8 @@ def _collect(lib, inp: Input, ch, bh, vr):
     n_int = inp.n_interactions_trajectory
     nch = inp.nchannels
     channels = np.ctypeslib.as_array(ch, shape=(n_int + 1, nch)).copy()
-    var_red = np.ctypeslib.as_array(vr, shape=(100, 385, n_int)).copy()
+    # NULL without variance reduction (src/xmi_main.F90:934-943)
+    var_red = np.ctypeslib.as_array(vr, shape=(100, 385, n_int)).copy() if vr else None
     brute = np.ctypeslib.as_array(bh, shape=(100, 385, n_int)).copy()
     lib.xmi_free(ch)
     lib.xmi_free(bh)

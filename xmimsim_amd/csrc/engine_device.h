@@ -44,6 +44,7 @@ struct PhotonSoA {
   int *prev_line;               /* line of the previous interaction if fluorescence, else 0 */
   int *prev_u;                  /* hdf5 index of the element that emitted it */
   uint64_t *hid;                /* history id */
+  int *cflags;                  /* brute-force runs: cascades left (bit 0 Auger, bit 1 radiative) */
 };
 
 /* per-photon records of the estimator at the current order, written by the
@@ -77,6 +78,14 @@ struct EngineParams {
   double det_inv[3][3];
   double det_radius, coll_radius, coll_height;
   int coll_present;
+  double det_vertex[3];         /* collimator cone in the detector frame: vertex */
+  double det_cos2;              /* cos^2 of its half apex */
+  /* brute-force runs (use_variance_reduction = 0): analogue transport, a
+   * photon is tallied when it reaches the detector; a cascade's offspring
+   * photon of slot i lives in slot i + off_half (0: no cascades) */
+  int brute;
+  int casc;                     /* cascades of a primary: bit 0 Auger, bit 1 radiative */
+  int off_half;
   double slit_x, slit_y, d_source_slit;
   int n_layers;
   int n_int;
@@ -170,6 +179,8 @@ struct EngineParams {
   unsigned long long *t_ch;     /* [n_int][nch][2] */
   unsigned long long *t_line;   /* [n_int][nZ][ENG_MAX_ACT][2] */
   unsigned long long *t_rc;     /* [n_int][nZ][2][2] */
+  unsigned long long *t_bch;    /* brute force: [n_int + 1][nch][2] detector hits by interactions so far */
+  unsigned long long *t_bhist;  /* brute force: [n_int][nZ][385][2] by last interaction's order, element, row */
   int tally_shift;              /* fixed point: value * 2^tally_shift */
   unsigned long long *counters; /* [slot ENG_CNT_*][ENG_SPREAD]: queue sizes, events, fallbacks, errors */
   int *queue;                   /* [ENG_NQUEUES][ENG_SPREAD][qreg] */

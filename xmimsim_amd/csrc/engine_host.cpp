@@ -388,6 +388,11 @@ static void fill_geometry(EngineParams &P, const xmi_inputF *in) {
   P.coll_radius = in->collimator_radius;
   P.coll_height = in->geometry.collimator_height;
   P.coll_present = in->collimator_present;
+  for (int k = 0; k < 3; k++) P.det_vertex[k] = in->vertex[k];
+  {
+    const double c = cos(in->half_apex);
+    P.det_cos2 = c * c;
+  }
   P.slit_x = in->geometry.slit_size_x;
   P.slit_y = in->geometry.slit_size_y;
   P.d_source_slit = in->geometry.d_source_slit;
@@ -432,10 +437,9 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     xmi_set_error("xmi_main_msim: xmi_init_input was not called");
     return 0;
   }
-  if (!esc && !options->use_variance_reduction) {
-    xmi_set_error("xmi_main_msim: the brute-force path (use_variance_reduction=0) is not built on the GPU");
-    return 0;
-  }
+  /* without variance reduction: the brute-force path, no solid angles,
+   * detector hits tallied (src/xmi_main.F90:1274-1407,445-535) */
+  const bool brute = !esc && !options->use_variance_reduction;
   /* the Philox counter holds the interaction order in 8 bits (philox.h):
    * order n and n + 256 would share their random streams */
   if (in->general.n_interactions_trajectory < 1 || in->general.n_interactions_trajectory > ENG_MAX_INTERACTIONS) {
@@ -443,7 +447,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
                   (long)in->general.n_interactions_trajectory);
     return 0;
   }
-  if (!esc && (!solid_angles || !solid_angles->solid_angles)) {
+  if (!esc && !brute && (!solid_angles || !solid_angles->solid_angles)) {
     xmi_set_error("xmi_main_msim: variance reduction needs a solid-angle grid");
     return 0;
   }
@@ -456,12 +460,12 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   DeviceTables *T = get_tables(h);
   if (!T) return 0;
   if (options->use_advanced_compton && !ensure_adv_tables(T, h)) return 0;
-  if (!esc && !T->has_precalc) {
+  if (!esc && !brute && !T->has_precalc) {
     xmi_set_error("xmi_main_msim: the tables were read without the cross sections of the variance reduction "
                   "(xmi_init_from_hdf5 with use_variance_reduction=0)");
     return 0;
   }
-  if (!esc && !ensure_grid(E, solid_angles)) return 0;
+  if (!esc && !brute && !ensure_grid(E, solid_angles)) return 0;
   hipStream_t st = E.stream;
   const int nZ = T->nZ;
   const int n_int = in->general.n_interactions_trajectory;
@@ -487,6 +491,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   P.seed = seed ? *seed : xmi_engine_current_seed(options);
   P.sample = 0;
   P.nZ = nZ;
+  P.brute = brute ? 1 : 0;
+  P.casc = brute ? ((options->use_cascade_auger ? 1 : 0) | (options->use_cascade_radiative ? 2 : 0)) : 0;
 
   /* layers */
   std::vector<int> elZ, elU;
@@ -774,27 +780,33 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     }
     const int cap_max = 1 << 21;
     int cap = (int)(total_hist < cap_max ? (total_hist > 0 ? total_hist : 1) : cap_max);
-    P.cap = cap;
-    size_t per = (size_t)cap;
+    /* a brute-force run with cascades keeps a second slot per primary for
+     * the offspring photon of its cascade */
+    const int slots = (brute && P.casc) ? 2 * cap : cap;
+    P.cap = slots;
+    P.off_half = (brute && P.casc) ? cap : 0;
+    size_t per = (size_t)slots;
     /* the kernels' layer-count bound (estimator records hold nl_bound paths) */
     const int nl_bound = NL <= 1 ? 1 : NL <= 2 ? 2 : NL <= 4 ? 4 : NL <= 8 ? 8 : 16;
-    size_t state_bytes = per * (15 * sizeof(double) + (size_t)NL * sizeof(double) + 4 * sizeof(int) + sizeof(uint64_t));
+    size_t state_bytes = per * (15 * sizeof(double) + (size_t)NL * sizeof(double) + 5 * sizeof(int) + sizeof(uint64_t));
     size_t vr_bytes = per * ((6 + (size_t)nl_bound) * sizeof(double) + 2 * sizeof(int));
     size_t n_ch2 = (size_t)n_int * nch * 2, n_line2 = (size_t)n_int * nZ * ENG_MAX_ACT * 2, n_rc2 = (size_t)n_int * nZ * 4;
-    size_t tally_words = n_ch2 + n_line2 + n_rc2;
+    /* brute force: hits by interactions so far, and by the last interaction */
+    size_t n_bch2 = brute ? (size_t)(n_int + 1) * nch * 2 : 0, n_bh2 = brute ? (size_t)n_int * nZ * 385 * 2 : 0;
+    size_t tally_words = n_ch2 + n_line2 + n_rc2 + n_bch2 + n_bh2;
     HIP_TRY(G.state.ensure(state_bytes));
     HIP_TRY(G.vr.ensure(vr_bytes));
     HIP_TRY(G.tallies.ensure(tally_words * sizeof(unsigned long long)));
     /* sub-queue size: k_step runs ceil(cap / ENG_BLOCK) workgroups, and each
      * sub-queue takes the photons of every ENG_SPREAD-th of their waves */
-    const int step_waves = ((cap + ENG_BLOCK - 1) / ENG_BLOCK) * (ENG_BLOCK / 64);
+    const int step_waves = ((slots + ENG_BLOCK - 1) / ENG_BLOCK) * (ENG_BLOCK / 64);
     P.qreg = ((step_waves + ENG_SPREAD - 1) / ENG_SPREAD) * 64;
     HIP_TRY(G.counters.ensure((size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long)));
     HIP_TRY(G.queue.ensure((size_t)ENG_NQUEUES * ENG_SPREAD * P.qreg * sizeof(int)));
     /* layer lists: a row of cap slots per layer; per-wave offsets of the
      * k_step launch (ceil(cap / ENG_BLOCK) workgroups of ENG_BLOCK / 64 waves) */
     P.n_waves = step_waves;
-    if (step_waves > 32 * 1024) {
+    if (!brute && step_waves > 32 * 1024) {
       xmi_set_error("xmi_main_msim: %d waves per batch exceed the layer scan's 32768", step_waves);
       return 0;
     }
@@ -901,7 +913,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.phi_Tinv = T->phi_Tinv;
     P.phi_R0 = T->phi_R0;
     P.phi_Rinv = T->phi_Rinv;
-    if (!esc) {
+    if (!esc && !brute) {
       P.sa_grid = G.sa_grid.as<double>();
       P.sa_r = G.sa_r.as<double>();
       P.sa_t = G.sa_t.as<double>();
@@ -934,6 +946,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.t_ch = tw;
     P.t_line = tw + n_ch2;
     P.t_rc = tw + n_ch2 + n_line2;
+    P.t_bch = brute ? tw + n_ch2 + n_line2 + n_rc2 : nullptr;
+    P.t_bhist = brute ? P.t_bch + n_bch2 : nullptr;
     P.counters = G.counters.as<unsigned long long>();
     P.queue = G.queue.as<int>();
     P.lay_list = G.lay_list.as<int>();
@@ -961,6 +975,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     S.prev_line = (int *)sp;
     sp += per * sizeof(int);
     S.prev_u = (int *)sp;
+    sp += per * sizeof(int);
+    S.cflags = (int *)sp;
 
     VrSoA V;
     char *vp = G.vr.as<char>();
@@ -1001,6 +1017,12 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         HIP_TRY(eng_launch_source(&P, &S, &ST, (uint64_t)done, count, st));
         if (esc) {
           HIP_TRY(eng_launch_escape(&P, &S, &V, &Q, count, st));
+          continue;
+        }
+        if (brute) {
+          /* order n_int + 1: the step after the last interaction, in which a
+           * photon can still reach the detector */
+          for (int n = 1; n <= n_int + 1; n++) HIP_TRY(eng_launch_order_brute(&P, &S, n, count, st));
           continue;
         }
         int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
@@ -1070,6 +1092,36 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         for (int o = 0; o < esc->nOut; o++)
           esc->compton[(size_t)k + (size_t)esc->nE * o] =
               fx_to_double(fx_get(&tc[((size_t)o * esc->nE + k) * 2]), shift) / norm;
+      }
+      ok = 1;
+      goto fail;
+    }
+    if (brute) {
+      /* channels(k, :) holds the photons that reached the detector after at
+       * most k interactions; the history rows by the order of the last */
+      const unsigned long long *tbc = th.data() + n_ch2 + n_line2 + n_rc2, *tbh = tbc + n_bch2;
+      const double live = in->detector.live_time;
+      const size_t nhist = (size_t)100 * XMI_HIST_ROWS * n_int;
+      ch_out = (double *)calloc((size_t)(n_int + 1) * nch, sizeof(double));
+      bf_out = (double *)calloc(nhist, sizeof(double));
+      if (!ch_out || !bf_out) {
+        xmi_set_error("out of host memory");
+        goto fail;
+      }
+      for (int c = 0; c < nch; c++) {
+        u128 acc = 0;
+        for (int k = 0; k <= n_int; k++) {
+          acc += fx_get(&tbc[((size_t)k * nch + c) * 2]);
+          ch_out[(size_t)k * nch + c] = fx_to_double(acc, shift) * live;
+        }
+      }
+      for (int u = 0; u < nZ; u++) {
+        const int Z = T->Zs[u];
+        for (int n = 0; n < n_int; n++)
+          for (int row = 0; row < XMI_HIST_ROWS; row++) {
+            const u128 v = fx_get(&tbh[(((size_t)n * nZ + u) * 385 + row) * 2]);
+            if (v) bf_out[((size_t)(Z - 1) * XMI_HIST_ROWS + row) * n_int + n] = fx_to_double(v, shift) * live * n_mpi_hosts;
+          }
       }
       ok = 1;
       goto fail;

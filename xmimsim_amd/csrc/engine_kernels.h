@@ -87,6 +87,9 @@ hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv
                            long hits, uint64_t seed, hipStream_t st);
 hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,
                              int count, hipStream_t st);
+/* order n of a brute-force run: the analogue step with its detector tests
+ * and tallies, then (n <= n_int) the interactions */
+hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st);
 hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
                             hipStream_t st, hipEvent_t *ev);
 #ifdef __cplusplus

@@ -320,6 +320,10 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   S.hid[i] = hid;
   S.prev_line[i] = 0;
   S.prev_u[i] = -1;
+  if (P.brute) {
+    S.cflags[i] = P.casc;
+    if (P.off_half) S.layer[i + P.off_half] = 0; /* no offspring yet */
+  }
   if (!d.valid) return 0;
   const long j = (long)(hid - d.hid0);
   phx_stream s;
@@ -515,6 +519,48 @@ __device__ __forceinline__ void d_layer_list(const EngineParams &P, int layer, i
 /* ------------------------------------------------------------------------ */
 /* forced interaction + atom and interaction-type selection */
 
+/* atom selection and interaction type of the photon in slot i (energy E,
+ * now in layer my), from its transport stream after the step's number:
+ * S.elem set, the type returned (0 Rayleigh, 1 Compton, 2 photo-absorption) */
+__device__ __forceinline__ int d_select(const EngineParams &P, const PhotonSoA &S, int i, double E, int my,
+                                        phx_stream *sp) {
+  phx_stream &s = *sp;
+  const size_t cap = (size_t)P.cap;
+  /* atom selection, src/xmi_main.F90:1558-1572 */
+  double R2 = phx_uniform(&s);
+  int off = P.layer_off[my - 1], nel = P.layer_nel[my - 1];
+  double mu_cur = S.mus[(size_t)(my - 1) * cap + i];
+  double thr = 0.0;
+  int sel = nel - 1;
+  xrl_ectx ec = xrl_energy(E);
+  for (int k = 0; k < nel; k++) {
+    thr += P.el_w[off + k] * xrl_cs_total_c(&P.atoms[P.el_u[off + k]], &ec) / mu_cur;
+    if (R2 < thr) {
+      sel = k;
+      break;
+    }
+  }
+  S.elem[i] = sel;
+  /* interaction type, src/xmi_main.F90:1591-1652 */
+  double R3 = phx_uniform(&s);
+  int u = P.el_u[off + sel];
+  const double *ie = P.ip_E + P.ip_off[u];
+  int len = P.ip_off[u + 1] - P.ip_off[u];
+  int pos;
+  if (fabs(E - ie[0]) < 1e-10) pos = 0;
+  else {
+    int cell = (int)floor((E - P.ip_acc_E0) * P.ip_acc_inv);
+    if (cell < 0) cell = 0;
+    if (cell > ENG_IP_ACC - 1) cell = ENG_IP_ACC - 1;
+    pos = P.ip_acc[(size_t)u * ENG_IP_ACC + cell];
+    while (pos + 2 < len && ie[pos + 1] < E) pos++;
+  }
+  const double *ir = P.ip_r + P.ip_off[u], *irc = P.ip_rc + P.ip_off[u];
+  double pr = d_interp(ie[pos], ir[pos], ie[pos + 1], ir[pos + 1], E);
+  double pc = d_interp(ie[pos], irc[pos], ie[pos + 1], irc[pos + 1], E);
+  return (R3 < pr) ? 0 : ((R3 < pc) ? 1 : 2);
+}
+
 /* NL bounds the layer count; path lengths and mu*rho are kept in crossing
  * order (k = 0: the photon's layer) so that register arrays are only indexed
  * by compile-time indices */
@@ -596,39 +642,7 @@ k_step(EngineParams P, PhotonSoA S, int n, int count) {
       /* an estimator event of this order (the test of k_vr_geom) */
       ev_layer = E > 1.0 ? my : 0;
 
-      /* atom selection, src/xmi_main.F90:1558-1572 */
-      double R2 = phx_uniform(&s);
-      int off = P.layer_off[my - 1], nel = P.layer_nel[my - 1];
-      double mu_cur = S.mus[(size_t)(my - 1) * cap + i];
-      double thr = 0.0;
-      int sel = nel - 1;
-      xrl_ectx ec = xrl_energy(E);
-      for (int k = 0; k < nel; k++) {
-        thr += P.el_w[off + k] * xrl_cs_total_c(&P.atoms[P.el_u[off + k]], &ec) / mu_cur;
-        if (R2 < thr) {
-          sel = k;
-          break;
-        }
-      }
-      S.elem[i] = sel;
-      /* interaction type, src/xmi_main.F90:1591-1652 */
-      double R3 = phx_uniform(&s);
-      int u = P.el_u[off + sel];
-      const double *ie = P.ip_E + P.ip_off[u];
-      int len = P.ip_off[u + 1] - P.ip_off[u];
-      int pos;
-      if (fabs(E - ie[0]) < 1e-10) pos = 0;
-      else {
-        int cell = (int)floor((E - P.ip_acc_E0) * P.ip_acc_inv);
-        if (cell < 0) cell = 0;
-        if (cell > ENG_IP_ACC - 1) cell = ENG_IP_ACC - 1;
-        pos = P.ip_acc[(size_t)u * ENG_IP_ACC + cell];
-        while (pos + 2 < len && ie[pos + 1] < E) pos++;
-      }
-      const double *ir = P.ip_r + P.ip_off[u], *irc = P.ip_rc + P.ip_off[u];
-      double pr = d_interp(ie[pos], ir[pos], ie[pos + 1], ir[pos + 1], E);
-      double pc = d_interp(ie[pos], irc[pos], ie[pos + 1], irc[pos + 1], E);
-      type = (R3 < pr) ? 0 : ((R3 < pc) ? 1 : 2);
+      type = d_select(P, S, i, E, my, &s);
     }
   }
   d_enqueue(P, type, i);
@@ -1471,12 +1485,17 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
 /* ------------------------------------------------------------------------ */
 /* interactions */
 
+/* transport substream of slot i: a cascade's offspring photon has its own */
+__device__ __forceinline__ uint32_t d_tr_sub(const EngineParams &P, int i) {
+  return (P.off_half && i >= P.off_half) ? PHX_SUB_OFFSPRING : PHX_SUB_TRANSPORT;
+}
+
 __device__ __forceinline__ void d_rayleigh_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
   phx_stream s;
-  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_TRANSPORT, S.hid[i]);
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
   double r = phx_uniform(&s);
@@ -1502,7 +1521,9 @@ __device__ __forceinline__ void d_rayleigh_one(const EngineParams &P, const Phot
   S.ez[i] = ev[2];
   S.theta[i] = th;
   S.phi[i] = ph;
-  S.prev_line[i] = 0;
+  /* brute force: the history row of the last interaction, and its element */
+  S.prev_line[i] = P.brute ? 384 : 0;
+  if (P.brute) S.prev_u[i] = u;
   (void)cap;
 }
 
@@ -1511,7 +1532,7 @@ __device__ __forceinline__ void d_compton_one(const EngineParams &P, const Photo
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
   phx_stream s;
-  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_TRANSPORT, S.hid[i]);
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
   double theta_i = d_bilinear(P.comp_icdf + (size_t)u * P.n_e * P.n_r, P.tab_E, P.n_e, P.tab_E0, P.tab_Einv, P.tab_R,
@@ -1620,7 +1641,8 @@ __device__ __forceinline__ void d_compton_one(const EngineParams &P, const Photo
   S.ez[i] = ev[2];
   S.theta[i] = th;
   S.phi[i] = ph;
-  S.prev_line[i] = 0;
+  S.prev_line[i] = P.brute ? 385 : 0;
+  if (P.brute) S.prev_u[i] = u;
 }
 
 /* Coster-Kronig vacancy shift, src/xmi_main.F90:5184-5323 */
@@ -1692,13 +1714,147 @@ __device__ int d_coster_kronig(const xrl_atom *a, phx_stream *s, int shell) {
   return shell;
 }
 
+/* ------------------------------------------------------------------------ */
+/* cascades of brute-force runs, src/xmi_main.F90:2413-4783 */
+
+/* xmi_fluorescence_yield_check (src/xmi_main.F90:5325-5350): 1 when the
+ * vacancy fluoresces.  Shells beyond M5 never do (the reference reads past
+ * the end of its nine corrected yields there); the number is drawn all the
+ * same. */
+__device__ __forceinline__ int d_yield_check(const EngineParams &P, int u, int shell, phx_stream *s) {
+  const double r = phx_uniform(s);
+  const double y = (shell >= 0 && shell <= 8) ? P.fycorr[(size_t)u * 9 + shell] : 0.0;
+  return !(r > y);
+}
+
+/* xmi_fluorescence_line_check (src/xmi_main.F90:5352-5437): the line of a
+ * vacancy by the radiative rates, its energy, or 0 and line 0 */
+__device__ double d_line_check(const EngineParams &P, const xrl_atom *a, int u, int shell, phx_stream *s,
+                               int *line) {
+  const double r = phx_uniform(s);
+  *line = 0;
+  if (shell < 0 || shell > 8) return 0.0;
+  int first, last;
+  xrl_shell_lines(shell, &first, &last);
+  double sumz = 0.0;
+  for (int l = first; l <= last; l++) {
+    sumz += a->rad_rate[l];
+    if (r < sumz) {
+      *line = l;
+      return P.line_E[(size_t)u * XMI_NLINES_VR_DEV + (l - 1)];
+    }
+  }
+  return 0.0;
+}
+
+/* a cascade's photon in slot j: energy and attenuation, isotropic direction,
+ * random polarisation made orthogonal to it, history row of its line, no
+ * cascades left (src/xmi_main.F90:4330-4370,4720-4765) */
+__device__ void d_cascade_emit(const EngineParams &P, const PhotonSoA &S, int j, double E, int line, int u,
+                               phx_stream *s) {
+  S.energy[j] = E;
+  d_mu_all(P, E, S.mus, (size_t)P.cap, j);
+  S.cflags[j] = 0;
+  const double th = acos(2.0 * phx_uniform(s) - 1.0);
+  const double ph = 2.0 * ENG_PI * phx_uniform(s);
+  const double dirv[3] = {sin(th) * cos(ph), sin(th) * sin(ph), cos(th)};
+  const double r = 2.0 * ENG_PI * phx_uniform(s);
+  double ev[3] = {cos(r), sin(r), 0.0};
+  const double cosalfa = d_dot(ev, dirv);
+  const double c_alfa = acos(cosalfa);
+  const double c_ae = 1.0 / sin(c_alfa), c_be = -c_ae * cosalfa;
+  for (int k = 0; k < 3; k++) ev[k] = c_ae * ev[k] + c_be * dirv[k];
+  S.dx[j] = dirv[0];
+  S.dy[j] = dirv[1];
+  S.dz[j] = dirv[2];
+  S.ex[j] = ev[0];
+  S.ey[j] = ev[1];
+  S.ez[j] = ev[2];
+  S.theta[j] = th;
+  S.phi[j] = ph;
+  S.prev_line[j] = line;
+  S.prev_u[j] = u;
+}
+
+/* the offspring of slot i starts at the parent's interaction point, in its
+ * layer, with its weight and history id */
+__device__ __forceinline__ int d_offspring(const EngineParams &P, const PhotonSoA &S, int i) {
+  const int j = i + P.off_half;
+  S.x[j] = S.x[i];
+  S.y[j] = S.y[i];
+  S.z[j] = S.z[i];
+  S.layer[j] = S.layer[i];
+  S.elem[j] = S.elem[i];
+  S.weight[j] = S.weight[i];
+  S.hid[j] = S.hid[i];
+  return j;
+}
+
+/* xmi_simulate_photon_cascade_auger (src/xmi_main.F90:2413-4594): the
+ * vacancy's Auger transition (the provider's rates) leaves two vacancies;
+ * the first may fluoresce as the parent photon, the second as the
+ * offspring.  M vacancies count with use_M_lines only. */
+__device__ void d_cascade_auger(const EngineParams &P, const PhotonSoA &S, int i, int u, const xrl_atom *a,
+                                int shell, phx_stream *s) {
+  int first, last;
+  if (!xrl_auger_range(shell, &first, &last)) return;
+  const double r = phx_uniform(s);
+  double sumz = 0.0;
+  int t = -1;
+  for (int k = first; k <= last; k++) {
+    sumz += xrl_auger_rate(a, k);
+    if (r < sumz) {
+      t = k;
+      break;
+    }
+  }
+  if (t < 0) return;
+  int sh[2];
+  xrl_auger_shells(t, &sh[0], &sh[1]);
+  for (int k = 0; k < 2; k++) {
+    const int m_shell = sh[k] >= XRL_M1_SHELL && sh[k] <= XRL_M5_SHELL;
+    if ((m_shell && !P.use_M_lines) || !d_yield_check(P, u, sh[k], s)) continue;
+    const int v = d_coster_kronig(a, s, sh[k]);
+    int line;
+    const double e = d_line_check(P, a, u, v, s, &line);
+    if (e <= 1.0) continue;
+    d_cascade_emit(P, S, k == 0 ? i : d_offspring(P, S, i), e, line, u, s);
+  }
+}
+
+/* xmi_simulate_photon_cascade_radiative (src/xmi_main.F90:4596-4783): the
+ * line the parent emitted leaves a vacancy in its other shell (K -> L or M;
+ * L -> M with use_M_lines), which may fluoresce as the offspring; then the
+ * parent has no cascades left either */
+__device__ void d_cascade_radiative(const EngineParams &P, const PhotonSoA &S, int i, int u, const xrl_atom *a,
+                                    int shell, int line, phx_stream *s) {
+  int sn = -1;
+  if (shell == XRL_K_SHELL) {
+    if (line >= xrl_line_KL(0) && line <= xrl_line_KM(4)) sn = line; /* KL1..KL3, KM1..KM5 -> L1..M5 */
+  } else if (shell >= XRL_L1_SHELL && shell <= XRL_L3_SHELL && P.use_M_lines) {
+    const int base = xrl_line_LM(shell - 1, 0);
+    if (line >= base && line <= base + 4) sn = XRL_M1_SHELL + (line - base);
+  } else {
+    return;
+  }
+  if (sn < 0) return;
+  if (sn >= XRL_M1_SHELL && sn <= XRL_M5_SHELL && !P.use_M_lines) return;
+  if (!d_yield_check(P, u, sn, s)) return;
+  sn = d_coster_kronig(a, s, sn);
+  int ln;
+  const double e = d_line_check(P, a, u, sn, s, &ln);
+  if (e <= 1.0) return;
+  S.cflags[i] = 0;
+  d_cascade_emit(P, S, d_offspring(P, S, i), e, ln, u, s);
+}
+
 __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
   const xrl_atom *a = &P.atoms[u];
   phx_stream s;
-  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_TRANSPORT, S.hid[i]);
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
   xrl_ectx ec = xrl_energy(E);
@@ -1719,8 +1875,17 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
     S.energy[i] = 0.0;
     return;
   }
-  (void)phx_uniform(&s); /* drawn and unused by the reference's yield weighting */
-  S.weight[i] *= P.fycorr[(size_t)u * 9 + shell];
+  const double ry = phx_uniform(&s); /* unused by the reference's yield weighting */
+  if (P.brute) {
+    /* the analogue yield, perhaps an Auger cascade (src/xmi_main.F90:2298-2315) */
+    if (ry > P.fycorr[(size_t)u * 9 + shell]) {
+      S.energy[i] = 0.0;
+      if (S.cflags[i] & 1) d_cascade_auger(P, S, i, u, a, shell, &s);
+      return;
+    }
+  } else {
+    S.weight[i] *= P.fycorr[(size_t)u * 9 + shell];
+  }
   shell = d_coster_kronig(a, &s, shell);
   r = phx_uniform(&s);
   sumz = 0.0;
@@ -1758,6 +1923,7 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
   S.phi[i] = ph;
   S.prev_line[i] = line;
   S.prev_u[i] = u;
+  if (P.brute && (S.cflags[i] & 2)) d_cascade_radiative(P, S, i, u, a, shell, line, &s);
 }
 
 /* one launch serves the three interaction queues: thread q takes entry q of
@@ -1934,6 +2100,184 @@ static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks)
   return vb < 1 ? 1 : vb;
 }
 
+/* ------------------------------------------------------------------------ */
+/* brute-force runs (use_variance_reduction = 0): the analogue step of
+ * src/xmi_main.F90:1274-1407 with the detector tests of every path segment,
+ * one thread per photon slot; a photon that reaches the detector is tallied
+ * by its energy and its last interaction (src/xmi_main.F90:445-535) and its
+ * history ends, as it does when it leaves the sample elsewhere or hits the
+ * collimator */
+
+/* lab -> detector frame: x along the detector axis towards the sample, the
+ * window at x = 0 */
+__device__ __forceinline__ void d_det_frame(const EngineParams &P, const double *x, double *o) {
+  const double d[3] = {x[0] - P.p_det[0], x[1] - P.p_det[1], x[2] - P.p_det[2]};
+  d_matvec(P.det_inv, d, o);
+}
+
+/* xmi_check_detector_intersection (src/xmi_aux_f.F90:1622-1833) of the path
+ * segment b -> e: 0 none, 1 collimator, 2 window, 3 window from behind.
+ * Without a collimator the reference's line direction is norm(e - b), a
+ * scalar assigned to all three components (src/xmi_aux_f.F90:1668): the line
+ * (L, L, L) crosses x = 0 at (0, e_y - e_x, e_z - e_x), always as a crossing
+ * from behind.  Kept as computed there.  A line parallel to the window,
+ * where the reference stops the program, counts as no crossing. */
+__device__ int d_detector_segment(const EngineParams &P, const double *b, const double *e) {
+  double bd[3], ed[3], ip[3];
+  d_det_frame(P, b, bd);
+  d_det_frame(P, e, ed);
+  const double o[3] = {0.0, 0.0, 0.0}, nx[3] = {1.0, 0.0, 0.0};
+  if (!P.coll_present) {
+    if (bd[0] * ed[0] > 0.0) return 0;
+    const double L = d_distance(ed, bd);
+    const double ld[3] = {L, L, L};
+    if (!d_intersect(o, nx, ed, ld, ip)) return 0;
+    ip[0] = 0.0;
+    if (d_norm(ip) <= P.det_radius) return ld[0] >= 0.0 ? 3 : 2;
+    return 0;
+  }
+  const double ld[3] = {ed[0] - bd[0], ed[1] - bd[1], ed[2] - bd[2]};
+  const double t_begin = (bd[0] - ed[0]) / ld[0], t_end = (ed[0] - ed[0]) / ld[0];
+  const double t_hi = t_begin > t_end ? t_begin : t_end, t_lo = t_begin < t_end ? t_begin : t_end;
+  const double c2t = P.det_cos2;
+  const double m[3] = {1.0 - c2t, -c2t, -c2t};
+  const double delta[3] = {ed[0] - P.det_vertex[0], ed[1] - P.det_vertex[1], ed[2] - P.det_vertex[2]};
+  const double dM[3] = {ld[0] * m[0], ld[1] * m[1], ld[2] * m[2]};
+  const double deltaM[3] = {delta[0] * m[0], delta[1] * m[1], delta[2] * m[2]};
+  const double c2 = d_dot(dM, ld), c1 = d_dot(dM, delta), c0 = d_dot(deltaM, delta);
+  const double disc = c1 * c1 - c0 * c2;
+  if (disc < 0.0) return 0;
+  const double t1 = (-c1 + sqrt(disc)) / c2, t2 = (-c1 - sqrt(disc)) / c2;
+  const double x1 = ed[0] + t1 * ld[0], x2 = ed[0] + t2 * ld[0];
+  const int v1 = -(x1 - P.det_vertex[0]) >= 0.0, v2 = -(x2 - P.det_vertex[0]) >= 0.0;
+  const double h = P.coll_height;
+  if (!v1 && !v2) return 0;
+  if (v1 && v2) {
+    if (t1 <= t_hi && t1 >= t_lo && x1 <= h) return 1;
+    if (t2 <= t_hi && t2 >= t_lo && x2 <= h) return 1;
+    return 0;
+  }
+  const double t = v1 ? t1 : t2, xx = v1 ? x1 : x2;
+  if (t <= t_hi && t >= t_lo && xx <= h) return 1;
+  if (!d_intersect(o, nx, ed, ld, ip)) return 0;
+  ip[0] = 0.0;
+  if (d_norm(ip) <= P.det_radius && d_distance(bd, ip) <= d_distance(bd, ed)) return ld[0] >= 0.0 ? 3 : 2;
+  return 0;
+}
+
+/* xmi_check_photon_detector_hit (src/xmi_main.F90:1920-1984): a photon that
+ * left the sample reaches the detector when it moves against the detector
+ * normal and crosses the window (and the collimator's top) within their
+ * radii */
+__device__ int d_escape_hits_detector(const EngineParams &P, const double *coords, const double *dirv) {
+  if (d_dot(dirv, P.n_det) >= 0.0) return 0;
+  double dd[3], cd[3], ip[3];
+  d_matvec(P.det_inv, dirv, dd);
+  d_det_frame(P, coords, cd);
+  const double o[3] = {0.0, 0.0, 0.0}, nx[3] = {1.0, 0.0, 0.0};
+  if (!d_intersect(o, nx, cd, dd, ip)) return 0;
+  if (d_norm(ip) > P.det_radius) return 0;
+  if (!P.coll_present) return 1;
+  const double pc[3] = {P.coll_height, 0.0, 0.0};
+  if (!d_intersect(pc, nx, cd, dd, ip)) return 0;
+  ip[0] = 0.0;
+  return d_norm(ip) <= P.coll_radius;
+}
+
+/* order n of a brute-force run over 2 * count threads when cascades are on
+ * (slot t < count: primary t; otherwise offspring slot off_half + t - count),
+ * count threads otherwise.  A photon alive after n - 1 interactions takes
+ * its analogue step; one that stays in the sample with fewer than n_int
+ * interactions gets its atom and interaction type and joins the queues, the
+ * others end, those that reach the detector tallied */
+__global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, PhotonSoA S, int n, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t < count ? t : P.off_half + (t - count);
+  const bool valid = t < count || (P.off_half && t < 2 * count);
+  int type = -1;
+  long key_ch = -1, key_h = -1;
+  double w_hit = 0.0;
+  if (valid && S.layer[i] > 0) {
+    const size_t cap = (size_t)P.cap;
+    const double E = S.energy[i];
+    if (E < 1.0) {
+      S.layer[i] = 0;
+    } else {
+      phx_stream s;
+      phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
+      const double R = phx_uniform(&s);
+      const double dirv[3] = {S.dx[i], S.dy[i], S.dz[i]};
+      double c[3] = {S.x[i], S.y[i], S.z[i]};
+      const int cur = S.layer[i];
+      const int up = d_dot(dirv, P.n_sample) > 0.0;
+      const int last = up ? P.n_layers : 1, dir = up ? 1 : -1;
+      double max_r = 0.0, min_r = 0.0, blbs = 1.0;
+      int inside = 0, hit = 0, ended = 0;
+      for (int l = cur; up ? l <= last : l >= last; l += dir) {
+        const int l0 = l - 1;
+        const double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]};
+        double ip[3];
+        d_intersect(pp, P.n_sample, c, dirv, ip);
+        const double mu = S.mus[(size_t)l0 * cap + i];
+        double dist = d_distance(c, ip);
+        const double temp_prod = -1.0 * dist * P.layer_density[l0] * mu;
+        const double tempexp = exp(temp_prod);
+        min_r = max_r;
+        max_r = max_r - blbs * expm1(temp_prod);
+        if (R <= max_r) {
+          dist = -1.0 * log1p(-1.0 * (R - min_r) / blbs) / mu / P.layer_density[l0];
+          const double to[3] = {c[0] + dist * dirv[0], c[1] + dist * dirv[1], c[2] + dist * dirv[2]};
+          const int rc = d_detector_segment(P, c, to);
+          c[0] = to[0];
+          c[1] = to[1];
+          c[2] = to[2];
+          if (rc) {
+            ended = 1;
+            hit = rc == 2;
+          } else {
+            inside = l;
+          }
+          break;
+        }
+        const int rc = d_detector_segment(P, c, ip);
+        if (rc) {
+          ended = 1;
+          hit = rc == 2;
+          break;
+        }
+        c[0] = ip[0];
+        c[1] = ip[1];
+        c[2] = ip[2];
+        blbs = blbs * tempexp;
+      }
+      if (!inside && !ended) hit = d_escape_hits_detector(P, c, dirv);
+      if (hit) {
+        /* channels(n_interactions:, channel) and the history row of the last
+         * interaction; the host sums the orders into the cumulative rows */
+        w_hit = S.weight[i];
+        if (E >= 1.0) {
+          const int ch = (int)((E - P.zero) / P.gain);
+          if (ch >= 0 && ch < P.nch) key_ch = (long)(n - 1) * P.nch + ch;
+        }
+        if (n >= 2 && S.prev_line[i] > 0 && S.prev_u[i] >= 0)
+          key_h = ((long)(n - 2) * P.nZ + S.prev_u[i]) * 385 + (S.prev_line[i] - 1);
+      }
+      if (!inside || n > P.n_int) {
+        S.layer[i] = 0;
+      } else {
+        S.x[i] = c[0];
+        S.y[i] = c[1];
+        S.z[i] = c[2];
+        S.layer[i] = inside;
+        type = d_select(P, S, i, E, inside, &s);
+      }
+    }
+  }
+  d_enqueue(P, type, i);
+  d_fx_add_keyed(P.t_bch, key_ch, w_hit, P.tally_shift);
+  d_fx_add_keyed(P.t_bhist, key_h, w_hit, P.tally_shift);
+}
+
 /* photons per lane of k_vr_lines: each line's wavefront sum then serves
  * 64 KP photons; fewer where the per-photon path lengths take more registers.
  * Three rather than four up to four layers: 126 VGPRs give four waves per
@@ -2033,6 +2377,16 @@ hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const Vr
   hipLaunchKernelGGL(k_interact, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
   const int eblocks = (int)((3 * (size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
   hipLaunchKernelGGL(k_escape, dim3(eblocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, *Q);
+  return hipGetLastError();
+}
+
+hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st) {
+  /* primaries and, with cascades, their offspring slots */
+  const int threads = P->off_half ? 2 * count : count;
+  const int blocks = (threads + ENG_BLOCK - 1) / ENG_BLOCK;
+  hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
+  hipLaunchKernelGGL(k_step_brute, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+  if (n <= P->n_int) hipLaunchKernelGGL(k_interact, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
   return hipGetLastError();
 }
 

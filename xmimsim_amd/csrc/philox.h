@@ -14,6 +14,8 @@
  * substream 1: transport of interaction n (step, atom, type, kinematics)
  * substream 2: detection-weight estimator of interaction n
  * substream 3: solid-angle grid (history = cell index)
+ * substream 4: transport of interaction n of the offspring photon that a
+ *              cascade of a brute-force run starts (history = its parent's)
  *
  * Each block yields four 32-bit words = two doubles in [0,1) with 53 random
  * bits each.
@@ -38,6 +40,7 @@
 #define PHX_SUB_TRANSPORT 1u
 #define PHX_SUB_VARRED 2u
 #define PHX_SUB_GRID 3u
+#define PHX_SUB_OFFSPRING 4u
 
 PHX_FN void phx_mulhilo(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo) {
   uint64_t p = (uint64_t)a * (uint64_t)b;

@@ -706,6 +706,16 @@ double xmi_provider_line_data(int Z, int which, int index) {
   }
 }
 
+double xmi_provider_auger(int Z, int trans, int *shells3) {
+  const xrl_atom *a = xrl_atom_get(Z);
+  if (!a || trans < 0 || trans >= XRL_AUGER_N) return -1.0;
+  if (shells3) {
+    shells3[0] = trans < 64 ? XRL_K_SHELL : XRL_L1_SHELL + (trans - 64) / 30;
+    xrl_auger_shells(trans, &shells3[1], &shells3[2]);
+  }
+  return xrl_auger_rate(a, trans);
+}
+
 int xmi_provider_shell_cross_sections(int Z, int cascade_type, double E, double *P9) {
   const xrl_atom *a = xrl_atom_get(Z);
   if (!a) return 0;

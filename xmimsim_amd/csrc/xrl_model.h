@@ -428,6 +428,74 @@ XRL_FN int xrl_line_LM(int i, int j) {                   /* L1M1.., L2M1.., L3M1
 }
 
 /*
+ * Auger transitions of a K or L_i vacancy, the equivalent of xraylib's
+ * AugerRate(Z, trans) over the transition list K_L1L1_AUGER .. L3_M5Q3_AUGER
+ * that src/xmi_main.F90:2413-4594 walks.  Each transition t leaves two
+ * vacancies (xrl_auger_shells); the model fills only the shells whose
+ * vacancy transfer it defines (auger_KL, auger_KM, auger_LM), so its list is
+ *   K   : t = 0 .. 63,            pairs (X, Y), X and Y in L1 L2 L3 M1 .. M5
+ *   L_i : t = 64 + 30 i .. +29,   pairs (X, Y), X in M1 .. M5, Y in M1 .. M5
+ *                                 or N1 (every shell beyond M5)
+ * and its rates are those pairs' probabilities under the model's transfer:
+ * a K decay puts each vacancy independently into L or M (the fractions of
+ * auger_KL and auger_KM, split by occupancy); an L decay puts the first into
+ * M and the second into M (0.6) or beyond (0.4).  The expected vacancies of
+ * every shell per decay are then exactly auger_KL, auger_KM and auger_LM, so
+ * explicit cascades reproduce the cascade-corrected cross sections.
+ */
+#define XRL_N1_SHELL 9
+#define XRL_AUGER_N 154
+
+/* first and last transition of a vacancy in shell; 0 when it has none */
+XRL_FN int xrl_auger_range(int shell, int *first, int *last) {
+  if (shell == XRL_K_SHELL) {
+    *first = 0;
+    *last = 63;
+    return 1;
+  }
+  if (shell >= XRL_L1_SHELL && shell <= XRL_L3_SHELL) {
+    *first = 64 + 30 * (shell - 1);
+    *last = *first + 29;
+    return 1;
+  }
+  return 0;
+}
+
+XRL_FN void xrl_auger_shells(int t, int *s1, int *s2) {
+  if (t < 64) {
+    *s1 = 1 + t / 8;
+    *s2 = 1 + t % 8;
+  } else {
+    const int k = (t - 64) % 30;
+    *s1 = XRL_M1_SHELL + k / 6;
+    *s2 = k % 6 < 5 ? XRL_M1_SHELL + k % 6 : XRL_N1_SHELL;
+  }
+}
+
+XRL_FN double xrl_auger_rate(const xrl_atom *a, int t) {
+  if (t < 0 || t >= XRL_AUGER_N) return 0.0;
+  double nL = a->occupancy[1] + a->occupancy[2] + a->occupancy[3], nM = 0.0;
+  for (int j = 0; j < 5; j++) nM += a->occupancy[4 + j];
+  int s1, s2;
+  xrl_auger_shells(t, &s1, &s2);
+  if (t < 64) {
+    /* q(shell): the chance that one vacancy of a K decay lands there */
+    const double fL = nM > 0.0 ? 0.8 : 1.0, fM = nL > 0.0 ? 0.2 : 1.0;
+    double q[2];
+    const int s[2] = {s1, s2};
+    for (int k = 0; k < 2; k++) {
+      if (s[k] <= XRL_L3_SHELL) q[k] = nL > 0.0 ? fL * a->occupancy[s[k]] / nL : 0.0;
+      else q[k] = nM > 0.0 ? fM * a->occupancy[s[k]] / nM : 0.0;
+    }
+    return q[0] * q[1];
+  }
+  const int i = (t - 64) / 30;
+  if (!(a->edge[1 + i] > 0.0) || !(nM > 0.0)) return 0.0;
+  const double p1 = a->occupancy[s1] / nM;
+  return s2 == XRL_N1_SHELL ? p1 * 0.4 : p1 * 0.6 * a->occupancy[s2] / nM;
+}
+
+/*
  * Cascade-corrected shell ionisation cross sections (P_K .. P_M5) at energy E,
  * the equivalent of the xraylib family P{L1..M5}_{pure,auger_cascade,
  * rad_cascade,full_cascade}_kissel used in src/xmi_variance_reduction.F90:443-572

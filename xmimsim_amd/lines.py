@@ -67,6 +67,8 @@ def _lib():
         lib.xmi_provider_dcs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
         lib.xmi_provider_dcs.restype = C.c_double
         lib.xmi_provider_shell_cross_sections.argtypes = [C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]
+        lib.xmi_provider_auger.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int)]
+        lib.xmi_provider_auger.restype = C.c_double
         lib._lines_ready = True
     return lib
 
@@ -107,6 +109,22 @@ def shell_cross_sections(Z: int, E: float, cascade: int = 4) -> np.ndarray:
     out = (C.c_double * 9)()
     _lib().xmi_provider_shell_cross_sections(Z, cascade, E, out)
     return np.array(out)
+
+
+def auger_transitions(Z: int):
+    """The provider's Auger transitions of element Z as (rate, vacancy
+    shell, first new vacancy, second new vacancy) rows (shell indices 0 K ..
+    8 M5, 9 beyond M5)."""
+    out = []
+    sh = (C.c_int * 3)()
+    t = 0
+    while True:
+        r = _lib().xmi_provider_auger(Z, t, sh)
+        if r < 0:
+            break
+        out.append((r, sh[0], sh[1], sh[2]))
+        t += 1
+    return out
 
 
 def line_energy(Z: int, line: int) -> float:
