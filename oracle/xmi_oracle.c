@@ -1,7 +1,7 @@
 /*
  * xmi_oracle.c -- CPU/OpenMP restatement of the reference photon-history
- * engine (variance-reduction path), used as the correctness oracle and the CPU
- * baseline of the GPU engine.
+ * engine (variance-reduction and brute-force paths), used as the correctness
+ * oracle and the CPU baseline of the GPU engine.
  *
  * TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.
@@ -12,7 +12,17 @@
  *                            reshape and live-time scaling :923-950)
  *   orc_coords_dir           src/xmi_main.F90:957-1136
  *   orc_shift_first_layer    src/xmi_main.F90:1140-1186
- *   orc_simulate_photon      src/xmi_main.F90:1188-1685 (forced step :1414-1518)
+ *   orc_simulate_photon      src/xmi_main.F90:1188-1685 (forced step :1414-1518,
+ *                            orc_forced_step; analogue step :1274-1407,
+ *                            orc_brute_step)
+ *   orc_detector_intersection src/xmi_aux_f.F90:1622-1833
+ *   orc_photon_detector_hit  src/xmi_main.F90:1920-1984
+ *   orc_brute_tally          src/xmi_main.F90:445-535
+ *   orc_cascade_auger        src/xmi_main.F90:2413-4594
+ *   orc_cascade_radiative    src/xmi_main.F90:4596-4783
+ *   orc_yield_check          src/xmi_main.F90:5325-5350
+ *   orc_history              src/xmi_main.F90:435-538 (a primary and its
+ *                            cascade's offspring)
  *   orc_rayleigh             src/xmi_main.F90:1986-2101
  *   orc_compton              src/xmi_main.F90:2103-2229, energy :4985-5069
  *   orc_fluorescence         src/xmi_main.F90:2231-2411, CK :5184-5323,
