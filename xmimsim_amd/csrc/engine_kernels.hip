@@ -1490,12 +1490,17 @@ __device__ __forceinline__ uint32_t d_tr_sub(const EngineParams &P, int i) {
   return (P.off_half && i >= P.off_half) ? PHX_SUB_OFFSPRING : PHX_SUB_TRANSPORT;
 }
 
+/* the interaction routines are compiled twice: BRUTE (brute-force runs)
+ * adds the history rows of Rayleigh and Compton, the analogue yield and the
+ * cascades, which the variance-reduction path does without */
+
+template <bool BRUTE>
 __device__ __forceinline__ void d_rayleigh_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
   phx_stream s;
-  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, BRUTE ? d_tr_sub(P, i) : PHX_SUB_TRANSPORT, S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
   double r = phx_uniform(&s);
@@ -1522,17 +1527,18 @@ __device__ __forceinline__ void d_rayleigh_one(const EngineParams &P, const Phot
   S.theta[i] = th;
   S.phi[i] = ph;
   /* brute force: the history row of the last interaction, and its element */
-  S.prev_line[i] = P.brute ? 384 : 0;
-  if (P.brute) S.prev_u[i] = u;
+  S.prev_line[i] = BRUTE ? 384 : 0;
+  if (BRUTE) S.prev_u[i] = u;
   (void)cap;
 }
 
+template <bool BRUTE>
 __device__ __forceinline__ void d_compton_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
   phx_stream s;
-  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, BRUTE ? d_tr_sub(P, i) : PHX_SUB_TRANSPORT, S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
   double theta_i = d_bilinear(P.comp_icdf + (size_t)u * P.n_e * P.n_r, P.tab_E, P.n_e, P.tab_E0, P.tab_Einv, P.tab_R,
@@ -1641,8 +1647,8 @@ __device__ __forceinline__ void d_compton_one(const EngineParams &P, const Photo
   S.ez[i] = ev[2];
   S.theta[i] = th;
   S.phi[i] = ph;
-  S.prev_line[i] = P.brute ? 385 : 0;
-  if (P.brute) S.prev_u[i] = u;
+  S.prev_line[i] = BRUTE ? 385 : 0;
+  if (BRUTE) S.prev_u[i] = u;
 }
 
 /* Coster-Kronig vacancy shift, src/xmi_main.F90:5184-5323 */
@@ -1848,13 +1854,14 @@ __device__ void d_cascade_radiative(const EngineParams &P, const PhotonSoA &S, i
   d_cascade_emit(P, S, d_offspring(P, S, i), e, ln, u, s);
 }
 
+template <bool BRUTE>
 __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
   const xrl_atom *a = &P.atoms[u];
   phx_stream s;
-  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, BRUTE ? d_tr_sub(P, i) : PHX_SUB_TRANSPORT, S.hid[i]);
   d_stream_seek(&s, 3u);
   double E = S.energy[i];
   xrl_ectx ec = xrl_energy(E);
@@ -1876,7 +1883,7 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
     return;
   }
   const double ry = phx_uniform(&s); /* unused by the reference's yield weighting */
-  if (P.brute) {
+  if (BRUTE) {
     /* the analogue yield, perhaps an Auger cascade (src/xmi_main.F90:2298-2315) */
     if (ry > P.fycorr[(size_t)u * 9 + shell]) {
       S.energy[i] = 0.0;
@@ -1923,7 +1930,7 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
   S.phi[i] = ph;
   S.prev_line[i] = line;
   S.prev_u[i] = u;
-  if (P.brute && (S.cflags[i] & 2)) d_cascade_radiative(P, S, i, u, a, shell, line, &s);
+  if (BRUTE && (S.cflags[i] & 2)) d_cascade_radiative(P, S, i, u, a, shell, line, &s);
 }
 
 /* one launch serves the three interaction queues: thread q takes entry q of
@@ -1933,6 +1940,7 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
  * at two places at most.  Each photon is one thread's, with its own random
  * stream, so results do not depend on the assignment. */
 /* occupancy: five waves per SIMD (96 VGPRs, a few spilled; 6 % faster than four) */
+template <bool BRUTE>
 __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
 k_interact(EngineParams P, PhotonSoA S, int n) {
   constexpr int NS = 3 * ENG_SPREAD;
@@ -1965,9 +1973,9 @@ k_interact(EngineParams P, PhotonSoA S, int n) {
   }
   const int t = lo / ENG_SPREAD;
   const int i = P.queue[(size_t)lo * P.qreg + (q - off[lo])];
-  if (t == 0) d_rayleigh_one(P, S, n, i);
-  else if (t == 1) d_compton_one(P, S, n, i);
-  else d_photo_one(P, S, n, i);
+  if (t == 0) d_rayleigh_one<BRUTE>(P, S, n, i);
+  else if (t == 1) d_compton_one<BRUTE>(P, S, n, i);
+  else d_photo_one<BRUTE>(P, S, n, i);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2374,7 +2382,7 @@ hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const Vr
   }
   hipError_t e = hipMemcpyAsync(V->pesc, S->weight, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_interact, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
+  hipLaunchKernelGGL(k_interact<false>, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
   const int eblocks = (int)((3 * (size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
   hipLaunchKernelGGL(k_escape, dim3(eblocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, *Q);
   return hipGetLastError();
@@ -2386,7 +2394,7 @@ hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int
   const int blocks = (threads + ENG_BLOCK - 1) / ENG_BLOCK;
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
   hipLaunchKernelGGL(k_step_brute, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
-  if (n <= P->n_int) hipLaunchKernelGGL(k_interact, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  if (n <= P->n_int) hipLaunchKernelGGL(k_interact<true>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
   return hipGetLastError();
 }
 
@@ -2410,7 +2418,7 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
   }
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
-  hipLaunchKernelGGL(k_interact, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  hipLaunchKernelGGL(k_interact<false>, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, n);
   if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
