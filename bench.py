@@ -32,7 +32,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="three_layer_kramers",
-                    choices=["three_layer_kramers", "single_layer_steel", "quantify", "batch", "tables", "escape"])
+                    choices=["three_layer_kramers", "single_layer_steel", "quantify", "batch", "tables", "escape",
+                             "brute", "train"])
     ap.add_argument("--batch-samples", type=int, default=1024, help="batch: independent samples")
     ap.add_argument("--streams", type=int, default=4, help="batch: engines (HIP streams) serving the sample queue")
     ap.add_argument("--train-samples", type=int, default=32768, help="quantify: Monte Carlo training samples")
@@ -324,6 +325,132 @@ def escape(args):
         "roofline": None, "cpu_baseline": cpu}))
 
 
+# brute-force step kernel, per photon step: the record it reads (layer,
+# energy, history id, direction, position, the attenuation of one layer) and
+# writes (position, layer, element)
+BRUTE_STEP_BYTES = 4 + 8 + 8 + 24 + 24 + 8 + 24 + 4 + 4
+
+
+def brute(args):
+    """The three-layer Kramers workload without variance reduction (the
+    reference's --disable-variance-reduction): analogue transport, detector
+    hits, Auger and radiative cascades.  histories/s of whole calls; the
+    roofline of the analogue step kernel from its per-step record bytes over
+    its HIP-event time; the oracle's brute-force restatement timed beside it."""
+    from xmimsim_amd import api
+    lib = api.load_library()
+    inp = make_input(argparse.Namespace(**{**vars(args), "workload": "three_layer_kramers"}))
+    opts = api.default_options(use_variance_reduction=0)
+    prep = api.Prepared(inp, options=opts)
+    prof = None
+    for w in range(args.warmup):
+        lib.xmi_msim_set_profiling(1 if w == 0 else 0)
+        _, _, _, st = prep.run(seed=args.seed + 1000 + w)
+        if w == 0:
+            prof = (st.step_kernel_ms, st.interaction_kernel_ms, st.launches, st.photon_steps, st.detector_hits)
+        print(f"[bench] warmup {w}: {st.histories} histories, {st.transport_ms:.1f} ms", file=sys.stderr, flush=True)
+    lib.xmi_msim_set_profiling(0)
+    hist = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        _, _, _, st = prep.run(seed=args.seed + k)
+        hist += st.histories
+        print(f"[bench] step {k}: {st.histories} histories, {st.photon_steps} steps, {st.detector_hits} hits, "
+              f"{st.transport_ms:.1f} ms device, {st.total_ms:.1f} ms wall", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    roofline = None
+    if prof is not None and prof[2] > 0:
+        step_ms, inter_ms, launches, steps, hits = prof
+        sec = step_ms * 1e-3 / launches
+        bytes_launch = steps / launches * BRUTE_STEP_BYTES
+        roofline = {"bound": "hbm", "achieved": round(bytes_launch / sec / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(bytes_launch / sec / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic": _measured_traffic("brute").get("k_step_brute"),
+                    "kernel": "k_step_brute (analogue step, detector tests, hit tallies)",
+                    "avg_launch_ms": round(sec * 1e3, 4), "algorithmic_bytes_per_launch": round(bytes_launch),
+                    "bytes_per_step": BRUTE_STEP_BYTES, "steps_per_launch": round(steps / launches),
+                    "launches": int(launches), "detector_hits": int(hits),
+                    "kernel_ms": {"step": round(step_ms, 2), "interactions": round(inter_ms, 2)}}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or cores)
+        n = 2000
+        for attempt in range(4):
+            sample = make_input(argparse.Namespace(**{**vars(args), "workload": "three_layer_kramers", "photons": n}))
+            p2 = api.Prepared(sample, options=opts)
+            _, _, _, ost = oracle.run(p2, seed=args.seed, rng_mode=oracle.RNG_PER_THREAD, n_threads=cores)
+            p2.close()
+            if ost.seconds >= 0.5 * args.cpu_seconds or attempt == 3:
+                break
+            n = max(n + 1, int(n * min(8.0, args.cpu_seconds / max(ost.seconds, 1e-3))))
+        cpu = {"value": round(ost.histories / ost.seconds, 1), "unit": "histories/s", "cores": cores, "kind": "port",
+               "sample": f"brute force, three_layer_kramers with {n} photons per interval/line "
+                         f"({ost.histories} histories, {ost.seconds:.1f} s)"}
+    prep.close()
+    print(json.dumps({
+        "metric": "photon histories/s on 1 GPU (brute force)", "value": round(hist / elapsed, 1),
+        "unit": "histories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / max(args.steps, 1), 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded sample composition and spectrum)",
+        "config": {"workload": "brute", "sample": "three_layer_kramers", "photons_per_interval_and_line": int(args.photons),
+                   "histories_per_step": int(hist / max(args.steps, 1)), "n_interactions": inp.n_interactions_trajectory,
+                   "cascades": "auger+radiative"},
+        "roofline": roofline, "cpu_baseline": cpu}))
+
+
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def train(args):
+    """Training throughput of the net-line surrogate at its training
+    configuration: an ensemble of 4 MLPs 85 -> 4 x 512 (GELU) -> 256, batch
+    4096, Adam, on device-resident synthetic data (32768 rows).  Reported:
+    samples/s (rows x ensemble members per second) and, as the roofline, the
+    algorithmic matrix flops of forward, input-gradient and weight-gradient
+    products per step over the step time against the BF16 matrix peak."""
+    import numpy as np
+    from xmimsim_amd import surrogate as S
+    dims = S.DEFAULT_DIMS
+    nets, batch, n = 4, 4096, 32768
+    rng = np.random.default_rng(args.seed)
+    X = rng.normal(0, 1, (n, dims[0])).astype(np.float32)
+    T = rng.normal(0, 1, (n, dims[-1])).astype(np.float32)
+    M = (rng.random((n, dims[-1])) > 0.3).astype(np.float32)
+    sg = S.Surrogate(n_nets=nets, dims=dims, max_batch=batch, seed=args.seed)
+    sg.set_data(X, T, M)
+    steps = max(args.steps, 1) * 100
+    lr = np.full(steps, 1e-3, np.float32)
+    for w in range(args.warmup):
+        sg.train_resident(rng.integers(0, n, (20, batch)).astype(np.int32), lr[:20])
+    idx = rng.integers(0, n, (steps, batch)).astype(np.int32)
+    t0 = time.perf_counter()
+    loss = sg.train_resident(idx, lr)
+    dt = time.perf_counter() - t0
+    sg.close()
+    # matrix flops per step: forward B*N*K, weight gradient N*K*B, input
+    # gradient B*K*N (not for the first layer), 2 flops each, per member
+    fl = 0.0
+    for l in range(len(dims) - 1):
+        k, nn = dims[l], dims[l + 1]
+        fl += 2.0 * batch * nn * k * (3 if l > 0 else 2)
+    fl *= nets
+    achieved = fl * steps / dt / 1e12
+    print(json.dumps({
+        "metric": "surrogate training samples/s (ensemble of 4)", "value": round(batch * nets * steps / dt, 1),
+        "unit": "samples/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / steps, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulation and master weights)", "data": "synthetic (random rows, resident)",
+        "config": {"workload": "train", "dims": list(dims), "ensemble": nets, "batch": batch, "rows": n,
+                   "loss_first": float(loss[0].mean()), "loss_last": float(loss[-1].mean())},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None,
+                     "flops_per_step": fl, "kernel": "whole training step (GEMMs, epilogues, loss, Adam)"},
+        "cpu_baseline": None}))
+
+
 def quantify(args):
     """Surrogate-screened vs plain Monte Carlo quantification of single-layer
     samples: training data from the GPU Monte Carlo, training of the BF16
@@ -372,6 +499,10 @@ def main():
         return tables(args)
     if args.workload == "escape":
         return escape(args)
+    if args.workload == "brute":
+        return brute(args)
+    if args.workload == "train":
+        return train(args)
     from xmimsim_amd import api
     inp = make_input(args)
     prep = api.Prepared(inp)

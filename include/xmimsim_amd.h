@@ -297,7 +297,8 @@ typedef struct _xmi_msim_stats {
   int64_t launches;           /* with profiling on: launches of each kernel class */
   double vr_kernel_ms;        /* with profiling on: device time of the detection-weight kernels */
   int64_t vr_events;          /* detection-weight evaluations (photon-interactions) */
-  double step_kernel_ms;      /* with profiling on: forced-step kernel (incl. queue reset) */
+  double step_kernel_ms;      /* with profiling on: forced-step kernel (incl. queue reset); brute force:
+                                 the analogue step with its detector tests and tallies */
   double interaction_kernel_ms; /* with profiling on: Rayleigh + Compton + photo kernels */
   int64_t solid_angle_fallbacks; /* estimator points off the solid-angle grid */
   double vr_geom_ms;          /* with profiling on: the estimator's four kernels */
@@ -305,6 +306,8 @@ typedef struct _xmi_msim_stats {
   double vr_scatter_ms;
   double vr_lines_ms;
   int64_t grid_uploads;       /* solid-angle grid copies to the device (0 when the resident copy was current) */
+  int64_t photon_steps;       /* brute force: analogue steps taken (every order, offspring included) */
+  int64_t detector_hits;      /* brute force: photons that reached the detector */
 } xmi_msim_stats;
 void xmi_msim_get_stats(xmi_msim_stats *out);
 
@@ -448,6 +451,16 @@ int xmi_surrogate_padded_width(const xmi_surrogate *s, int layer);
 /* one Adam step of every net on the batch; loss[n_nets] before the step */
 int xmi_surrogate_train_step(xmi_surrogate *s, const float *X, const float *T, const float *mask, int batch,
                              float lr, float *loss);
+/* Training data kept on the device: n rows of inputs X[n][dims[0]], targets
+ * T[n][dims[n_layers]] and mask[n][dims[n_layers]]; replaces an earlier set. */
+int xmi_surrogate_set_data(xmi_surrogate *s, const float *X, const float *T, const float *mask, long n);
+/* steps Adam steps of every net on mini-batches of the resident data: step k
+ * uses rows idx[k][0..batch) and learning rate lr[k]; loss[steps][n_nets]
+ * receives each step's loss before its update.  The same arithmetic as
+ * xmi_surrogate_train_step on those rows, without a host transfer or a
+ * synchronisation per step. */
+int xmi_surrogate_train_resident(xmi_surrogate *s, const int32_t *idx, int steps, int batch, const float *lr,
+                                 float *loss);
 /* loss and parameter gradients without an update; grad[n_nets][param_count] */
 int xmi_surrogate_gradients(xmi_surrogate *s, const float *X, const float *T, const float *mask, int batch,
                             float *loss, float *grad);

@@ -110,3 +110,26 @@ def test_gradients_at_training_scale():
             flat_ex = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in eg])
             assert np.linalg.norm(g[e] - flat_ex) / np.linalg.norm(flat_ex) < 5e-2
     s.close()
+
+
+def test_resident_training_equals_step_by_step():
+    """Adam steps on device-resident data (rows chosen by index on the GPU,
+    no host round trip per step) give the same parameters and losses, bit
+    for bit, as train_step on the same rows."""
+    rng = np.random.default_rng(23)
+    n, B, steps = 1000, 128, 6
+    X = rng.normal(0, 1, (n, DIMS[0])).astype(np.float32)
+    T = rng.normal(0, 1, (n, DIMS[-1])).astype(np.float32)
+    M = (rng.random((n, DIMS[-1])) > 0.25).astype(np.float32)
+    idx = rng.integers(0, n, (steps, B)).astype(np.int32)
+    lr = np.linspace(2e-3, 5e-4, steps).astype(np.float32)
+    a, b = _net(seed=9), _net(seed=9)
+    la = np.array([a.train_step(X[i], T[i], M[i], float(r)) for i, r in zip(idx, lr)])
+    b.set_data(X, T, M)
+    lb = b.train_resident(idx, lr)
+    assert np.array_equal(la, lb)
+    assert np.array_equal(a.get_params(), b.get_params())
+    with pytest.raises(RuntimeError):
+        b.train_resident(np.full((1, B), n, np.int32), lr[:1])
+    a.close()
+    b.close()

@@ -102,7 +102,7 @@ class xmi_msim_stats(C.Structure):
                 ("step_kernel_ms", C.c_double), ("interaction_kernel_ms", C.c_double),
                 ("solid_angle_fallbacks", C.c_int64), ("vr_geom_ms", C.c_double), ("vr_cone_ms", C.c_double),
                 ("vr_scatter_ms", C.c_double), ("vr_lines_ms", C.c_double),
-                ("grid_uploads", C.c_int64)]
+                ("grid_uploads", C.c_int64), ("photon_steps", C.c_int64), ("detector_hits", C.c_int64)]
 
 
 class xmi_error(C.Structure):

@@ -1022,7 +1022,20 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         if (brute) {
           /* order n_int + 1: the step after the last interaction, in which a
            * photon can still reach the detector */
-          for (int n = 1; n <= n_int + 1; n++) HIP_TRY(eng_launch_order_brute(&P, &S, n, count, st));
+          const int bprof = g_profile && 3 * (n_int + 1) <= ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED;
+          for (int n = 1; n <= n_int + 1; n++)
+            HIP_TRY(eng_launch_order_brute(&P, &S, n, count, st, bprof ? &G.prof_ev[3 * (n - 1)] : nullptr));
+          if (bprof) {
+            HIP_TRY(hipStreamSynchronize(st));
+            for (int n = 0; n <= n_int; n++) {
+              float t0 = 0.f, t1 = 0.f;
+              HIP_TRY(hipEventElapsedTime(&t0, G.prof_ev[3 * n], G.prof_ev[3 * n + 1]));
+              HIP_TRY(hipEventElapsedTime(&t1, G.prof_ev[3 * n + 1], G.prof_ev[3 * n + 2]));
+              g_stats.step_kernel_ms += t0;
+              g_stats.interaction_kernel_ms += t1;
+            }
+            g_stats.launches += n_int + 1; /* step launches */
+          }
           continue;
         }
         int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
@@ -1065,6 +1078,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     g_stats.histories = cnt[ENG_CNT_HISTORIES];
     g_stats.vr_events = (int64_t)cnt[ENG_CNT_VR];
     g_stats.solid_angle_fallbacks = cnt[ENG_CNT_FALLBACKS];
+    g_stats.photon_steps = (int64_t)cnt[ENG_CNT_STEPS];
+    g_stats.detector_hits = (int64_t)cnt[ENG_CNT_HITS];
     for (int l = 0; l < ENG_MAX_LAYERS; l++) g_layer_events[l] = cnt[ENG_CNT_LAYER0 + l];
     if (cnt[ENG_CNT_ERRORS]) {
       xmi_set_error("xmi_main_msim: %llu photons hit an invalid state on the device (source %llu, estimator Doppler %llu, "

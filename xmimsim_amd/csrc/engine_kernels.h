@@ -29,7 +29,9 @@
 #define ENG_CNT_ERR_SOURCE (ENG_CNT_LAYER0 + ENG_MAX_LAYERS)
 #define ENG_CNT_ERR_VR_DOPPLER (ENG_CNT_ERR_SOURCE + 1)
 #define ENG_CNT_ERR_DOPPLER (ENG_CNT_ERR_SOURCE + 2)
-#define ENG_NCOUNTERS (ENG_CNT_ERR_SOURCE + 4)
+#define ENG_CNT_STEPS (ENG_CNT_ERR_SOURCE + 3)    /* brute force: photons that took a step */
+#define ENG_CNT_HITS (ENG_CNT_ERR_SOURCE + 4)     /* brute force: photons that reached the detector */
+#define ENG_NCOUNTERS (ENG_CNT_ERR_SOURCE + 5)
 
 /* events recorded per order when profiling: start, after the step, after
  * each of the four estimator kernels, after the interactions */
@@ -89,7 +91,8 @@ hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const So
                              int count, hipStream_t st);
 /* order n of a brute-force run: the analogue step with its detector tests
  * and tallies, then (n <= n_int) the interactions */
-hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st);
+hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st,
+                                  hipEvent_t *ev /* 3 events (before, between, after) or NULL */);
 hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
                             hipStream_t st, hipEvent_t *ev);
 #ifdef __cplusplus

@@ -2202,7 +2202,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, Photon
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = t < count ? t : P.off_half + (t - count);
   const bool valid = t < count || (P.off_half && t < 2 * count);
-  int type = -1;
+  int type = -1, stepped = 0, reached = 0;
   long key_ch = -1, key_h = -1;
   double w_hit = 0.0;
   if (valid && S.layer[i] > 0) {
@@ -2211,6 +2211,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, Photon
     if (E < 1.0) {
       S.layer[i] = 0;
     } else {
+      stepped = 1;
       phx_stream s;
       phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
       const double R = phx_uniform(&s);
@@ -2262,6 +2263,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, Photon
       if (hit) {
         /* channels(n_interactions:, channel) and the history row of the last
          * interaction; the host sums the orders into the cumulative rows */
+        reached = 1;
         w_hit = S.weight[i];
         if (E >= 1.0) {
           const int ch = (int)((E - P.zero) / P.gain);
@@ -2282,6 +2284,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, Photon
     }
   }
   d_enqueue(P, type, i);
+  d_wave_count(d_ctr(P, ENG_CNT_STEPS), stepped);
+  d_wave_count(d_ctr(P, ENG_CNT_HITS), reached);
   d_fx_add_keyed(P.t_bch, key_ch, w_hit, P.tally_shift);
   d_fx_add_keyed(P.t_bhist, key_h, w_hit, P.tally_shift);
 }
@@ -2388,13 +2392,17 @@ hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const Vr
   return hipGetLastError();
 }
 
-hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st) {
+hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st,
+                                  hipEvent_t *ev) {
   /* primaries and, with cascades, their offspring slots */
   const int threads = P->off_half ? 2 * count : count;
   const int blocks = (threads + ENG_BLOCK - 1) / ENG_BLOCK;
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
+  if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_step_brute, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+  if (ev) (void)hipEventRecord(ev[1], st);
   if (n <= P->n_int) hipLaunchKernelGGL(k_interact<true>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  if (ev) (void)hipEventRecord(ev[2], st);
   return hipGetLastError();
 }
 

@@ -60,7 +60,11 @@ struct Layer {
 struct xmi_surrogate {
   int nets = 0, n_layers = 0, max_batch = 0, in_features = 0, out_features = 0, steps = 0;
   std::vector<Layer> L;
-  Buf Xf, X0b, X0T, T, Mk, Y, partial;
+  Buf Xf, X0b, X0T, T, Mk, Y, partial, colsum;
+  /* resident training data (xmi_surrogate_set_data) and the per-call
+   * buffers of xmi_surrogate_train_resident */
+  Buf Xall, Tall, Mall, idx, counts, losses;
+  long n_data = 0;
   hipStream_t st = nullptr;
 };
 
@@ -133,6 +137,12 @@ xmi_surrogate *xmi_surrogate_create(int n_nets, int n_layers, const int *dims, i
          s->X0T.alloc(B * K0 * 2) == hipSuccess && s->T.alloc(B * No * 4) == hipSuccess &&
          s->Mk.alloc(B * No * 4) == hipSuccess && s->Y.alloc((size_t)n_nets * B * No * 4) == hipSuccess &&
          s->partial.alloc((size_t)n_nets * SG_LOSS_BLOCKS * 4) == hipSuccess;
+    size_t cs = 0;
+    for (int l = 0; l < n_layers; l++) {
+      const size_t c = sg_colsum_scratch(n_nets, max_batch, s->L[l].N);
+      cs = c > cs ? c : cs;
+    }
+    if (ok) ok = s->colsum.alloc(cs * 4) == hipSuccess;
   }
   /* bf16 copies of the initial weights */
   for (int l = 0; l < n_layers && ok; l++) {
@@ -210,18 +220,15 @@ static int forward(xmi_surrogate *s, int batch) {
   return 1;
 }
 
-/* loss, gradients of all parameters into gW/gb; returns per-net loss in *loss */
-static int backward(xmi_surrogate *s, int batch, const float *T, const float *mask, float *loss) {
+/* loss partials (into partial[nets][SG_LOSS_BLOCKS]) and the gradients of all
+ * parameters into gW/gb, for targets and mask already in T/Mk; the gradient
+ * scale is inv, or 1 / *count when count (device) is given.  Launches only. */
+static int backward_launch(xmi_surrogate *s, int batch, float inv, const unsigned int *count, float *partial) {
   const size_t B = (size_t)batch, No = (size_t)s->out_features;
-  SG_TRY(hipMemcpyAsync(s->T.p, T, B * No * 4, hipMemcpyHostToDevice, s->st));
-  SG_TRY(hipMemcpyAsync(s->Mk.p, mask, B * No * 4, hipMemcpyHostToDevice, s->st));
-  double count = 0.0;
-  for (size_t k = 0; k < B * No; k++) count += mask[k] != 0.0f ? 1.0 : 0.0;
-  const float inv = count > 0.0 ? (float)(1.0 / count) : 0.0f;
   Layer &out = s->L[s->n_layers - 1];
-  SG_TRY(sg_launch_loss(batch, (int)No, s->nets, s->Y.as<float>(), s->T.as<float>(), s->Mk.as<float>(), inv,
-                        out.G.as<float>(), out.Gb.as<uint16_t>(), out.GT.as<uint16_t>(), s->partial.as<float>(),
-                        SG_LOSS_BLOCKS, s->st));
+  SG_TRY(sg_launch_loss(batch, (int)No, s->nets, s->Y.as<float>(), s->T.as<float>(), s->Mk.as<float>(), inv, count,
+                        out.G.as<float>(), out.Gb.as<uint16_t>(), out.GT.as<uint16_t>(), partial, SG_LOSS_BLOCKS,
+                        s->st));
   for (int l = s->n_layers - 1; l >= 0; l--) {
     Layer &y = s->L[l];
     /* dW = G^T X: [N x B] . [K x B]^T */
@@ -233,7 +240,8 @@ static int backward(xmi_surrogate *s, int batch, const float *T, const float *ma
     e.s_cf = (long)y.N * y.K;
     SG_TRY(sg_launch_gemm(SG_EPI_DW, s->nets, y.N, y.K, batch, y.GT.as<uint16_t>(), batch, (long)(B * y.N), XT, batch,
                           sxt, &e, s->st));
-    SG_TRY(sg_launch_colsum(s->nets, batch, y.N, y.G.as<float>(), (long)(B * y.N), y.gb.as<float>(), y.N, s->st));
+    SG_TRY(sg_launch_colsum(s->nets, batch, y.N, y.G.as<float>(), (long)(B * y.N), s->colsum.as<float>(),
+                            y.gb.as<float>(), y.N, s->st));
     if (l > 0) {
       /* gradient at the previous pre-activation: (G W) * gelu'(Z_prev): [B x N] . [K x N]^T */
       Layer &p = s->L[l - 1];
@@ -251,6 +259,18 @@ static int backward(xmi_surrogate *s, int batch, const float *T, const float *ma
                             y.WT.as<uint16_t>(), y.N, (long)y.N * y.K, &d, s->st));
     }
   }
+  return 1;
+}
+
+/* loss, gradients of all parameters into gW/gb; returns per-net loss in *loss */
+static int backward(xmi_surrogate *s, int batch, const float *T, const float *mask, float *loss) {
+  const size_t B = (size_t)batch, No = (size_t)s->out_features;
+  SG_TRY(hipMemcpyAsync(s->T.p, T, B * No * 4, hipMemcpyHostToDevice, s->st));
+  SG_TRY(hipMemcpyAsync(s->Mk.p, mask, B * No * 4, hipMemcpyHostToDevice, s->st));
+  double count = 0.0;
+  for (size_t k = 0; k < B * No; k++) count += mask[k] != 0.0f ? 1.0 : 0.0;
+  const float inv = count > 0.0 ? (float)(1.0 / count) : 0.0f;
+  if (!backward_launch(s, batch, inv, nullptr, s->partial.as<float>())) return 0;
   std::vector<float> part((size_t)s->nets * SG_LOSS_BLOCKS);
   SG_TRY(hipMemcpyAsync(part.data(), s->partial.p, part.size() * 4, hipMemcpyDeviceToHost, s->st));
   SG_TRY(hipStreamSynchronize(s->st));
@@ -277,14 +297,8 @@ static int check_batch(const xmi_surrogate *s, int batch) {
   return 1;
 }
 
-extern "C" {
-
-/* one Adam step of every net on the same batch; loss[n_nets] receives the
- * masked mean-square loss before the step */
-int xmi_surrogate_train_step(xmi_surrogate *s, const float *X, const float *T, const float *mask, int batch, float lr,
-                             float *loss) {
-  if (!s || !check_batch(s, batch)) return 0;
-  if (!pack_inputs(s, X, batch) || !forward(s, batch) || !backward(s, batch, T, mask, loss)) return 0;
+/* the Adam update of every parameter after a backward pass */
+static int adam_launch(xmi_surrogate *s, float lr) {
   s->steps++;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float c1 = 1.0f - powf(b1, (float)s->steps), c2 = 1.0f - powf(b2, (float)s->steps);
@@ -295,7 +309,81 @@ int xmi_surrogate_train_step(xmi_surrogate *s, const float *X, const float *T, c
     SG_TRY(sg_launch_adam(nb, 1, 1, y.b.as<float>(), y.gb.as<float>(), y.mb.as<float>(), y.vb.as<float>(), lr, b1, b2,
                           eps, c1, c2, nullptr, nullptr, s->st));
   }
+  return 1;
+}
+
+extern "C" {
+
+/* one Adam step of every net on the same batch; loss[n_nets] receives the
+ * masked mean-square loss before the step */
+int xmi_surrogate_train_step(xmi_surrogate *s, const float *X, const float *T, const float *mask, int batch, float lr,
+                             float *loss) {
+  if (!s || !check_batch(s, batch)) return 0;
+  if (!pack_inputs(s, X, batch) || !forward(s, batch) || !backward(s, batch, T, mask, loss)) return 0;
+  if (!adam_launch(s, lr)) return 0;
   SG_TRY(hipStreamSynchronize(s->st));
+  return 1;
+}
+
+int xmi_surrogate_set_data(xmi_surrogate *s, const float *X, const float *T, const float *mask, long n) {
+  if (!s || !X || !T || !mask || n < 1) {
+    xmi_set_error("surrogate: set_data needs data and n >= 1");
+    return 0;
+  }
+  const size_t ni = (size_t)n * s->in_features * 4, no = (size_t)n * s->out_features * 4;
+  s->n_data = 0;
+  SG_TRY(s->Xall.alloc(ni));
+  SG_TRY(s->Tall.alloc(no));
+  SG_TRY(s->Mall.alloc(no));
+  SG_TRY(hipMemcpy(s->Xall.p, X, ni, hipMemcpyHostToDevice));
+  SG_TRY(hipMemcpy(s->Tall.p, T, no, hipMemcpyHostToDevice));
+  SG_TRY(hipMemcpy(s->Mall.p, mask, no, hipMemcpyHostToDevice));
+  s->n_data = n;
+  return 1;
+}
+
+int xmi_surrogate_train_resident(xmi_surrogate *s, const int32_t *idx, int steps, int batch, const float *lr,
+                                 float *loss) {
+  if (!s || !idx || !lr || !loss || steps < 1 || !check_batch(s, batch)) return 0;
+  if (s->n_data < 1) {
+    xmi_set_error("surrogate: no resident training data (xmi_surrogate_set_data)");
+    return 0;
+  }
+  for (size_t k = 0; k < (size_t)steps * batch; k++)
+    if (idx[k] < 0 || idx[k] >= s->n_data) {
+      xmi_set_error("surrogate: row index %d outside the %ld resident rows", idx[k], s->n_data);
+      return 0;
+    }
+  const size_t ni = (size_t)steps * batch * 4, nl = (size_t)steps * s->nets * SG_LOSS_BLOCKS;
+  if (s->idx.n < ni) SG_TRY(s->idx.alloc(ni));
+  if (s->counts.n < (size_t)steps * 4) SG_TRY(s->counts.alloc((size_t)steps * 4));
+  if (s->losses.n < nl * 4) SG_TRY(s->losses.alloc(nl * 4));
+  SG_TRY(hipMemcpyAsync(s->idx.p, idx, ni, hipMemcpyHostToDevice, s->st));
+  SG_TRY(hipMemsetAsync(s->counts.p, 0, (size_t)steps * 4, s->st));
+  const int ld = s->L[0].K;
+  for (int k = 0; k < steps; k++) {
+    unsigned int *cnt = s->counts.as<unsigned int>() + k;
+    SG_TRY(sg_launch_gather(batch, s->in_features, ld, s->out_features, s->idx.as<int>() + (size_t)k * batch,
+                            s->Xall.as<float>(), s->Tall.as<float>(), s->Mall.as<float>(), s->X0b.as<uint16_t>(),
+                            s->X0T.as<uint16_t>(), s->T.as<float>(), s->Mk.as<float>(), cnt, s->st));
+    if (!forward(s, batch) ||
+        !backward_launch(s, batch, 0.0f, cnt, s->losses.as<float>() + (size_t)k * s->nets * SG_LOSS_BLOCKS) ||
+        !adam_launch(s, lr[k]))
+      return 0;
+  }
+  std::vector<float> part(nl);
+  std::vector<unsigned int> cnt(steps);
+  SG_TRY(hipMemcpyAsync(part.data(), s->losses.p, nl * 4, hipMemcpyDeviceToHost, s->st));
+  SG_TRY(hipMemcpyAsync(cnt.data(), s->counts.p, (size_t)steps * 4, hipMemcpyDeviceToHost, s->st));
+  SG_TRY(hipStreamSynchronize(s->st));
+  for (int k = 0; k < steps; k++) {
+    const float inv = cnt[k] ? (float)(1.0 / (double)cnt[k]) : 0.0f;
+    for (int e = 0; e < s->nets; e++) {
+      double t = 0.0;
+      for (int b = 0; b < SG_LOSS_BLOCKS; b++) t += part[((size_t)k * s->nets + e) * SG_LOSS_BLOCKS + b];
+      loss[(size_t)k * s->nets + e] = (float)(t * inv);
+    }
+  }
   return 1;
 }
 

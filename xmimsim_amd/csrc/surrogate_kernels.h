@@ -36,9 +36,18 @@ extern "C" {
 hipError_t sg_launch_gemm(int epi, int nets, int M, int N, int K, const uint16_t *A, long lda, long s_a,
                           const uint16_t *B, long ldb, long s_b, const SgEpiArgs *a, hipStream_t st);
 hipError_t sg_launch_pack(int rows, int cols, int ld, const float *X, uint16_t *Xb, uint16_t *XbT, hipStream_t st);
+/* inv_count scales the gradient; a non-NULL count (device) overrides it with 1 / *count */
 hipError_t sg_launch_loss(int rows, int N, int nets, const float *Y, const float *T, const float *Mk, float inv_count,
-                          float *G, uint16_t *Gb, uint16_t *GT, float *partial, int blocks, hipStream_t st);
-hipError_t sg_launch_colsum(int nets, int rows, int N, const float *G, long s_g, float *db, long s_db, hipStream_t st);
+                          const unsigned int *count, float *G, uint16_t *Gb, uint16_t *GT, float *partial, int blocks,
+                          hipStream_t st);
+/* column sums (bias gradient); part: sg_colsum_scratch(nets, rows, N) floats of scratch */
+size_t sg_colsum_scratch(int nets, int rows, int N);
+hipError_t sg_launch_colsum(int nets, int rows, int N, const float *G, long s_g, float *part, float *db, long s_db,
+                            hipStream_t st);
+/* mini-batch rows idx[batch] of the resident data into the packed inputs, targets, mask; count += mask entries */
+hipError_t sg_launch_gather(int batch, int in_f, int ld, int out_f, const int *idx, const float *Xall, const float *Tall,
+                            const float *Mall, uint16_t *Xb, uint16_t *XbT, float *T, float *Mk, unsigned int *count,
+                            hipStream_t st);
 hipError_t sg_launch_refresh(size_t n, int N, int K, const float *p, uint16_t *pb, uint16_t *pt, hipStream_t st);
 hipError_t sg_launch_adam(size_t n, int N, int K, float *p, const float *g, float *m, float *v, float lr, float b1,
                           float b2, float eps, float c1, float c2, uint16_t *pb, uint16_t *pt, hipStream_t st);

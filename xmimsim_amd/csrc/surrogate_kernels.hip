@@ -129,8 +129,10 @@ __global__ void k_sg_pack(int rows, int cols, int ld, const float *X, uint16_t *
  * dY = 2 (Y - T) mask / count, stored f32, bf16 and transposed bf16;
  * loss partials per (net, row block) */
 __global__ void k_sg_loss(int rows, int N, int nets, const float *Y, const float *T, const float *Mk, float inv_count,
-                          float *G, uint16_t *Gb, uint16_t *GT, float *partial) {
+                          const unsigned int *count, float *G, uint16_t *Gb, uint16_t *GT, float *partial) {
   const int net = blockIdx.y;
+  /* with a device count (mask entries of the batch), the same rounding as the host's (float)(1.0 / count) */
+  if (count) inv_count = *count ? (float)(1.0 / (double)*count) : 0.0f;
   const size_t base = (size_t)net * rows * N;
   float s = 0.0f;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (size_t)rows * N;
@@ -155,15 +157,59 @@ __global__ void k_sg_loss(int rows, int N, int nets, const float *Y, const float
   if (threadIdx.x == 0) partial[(size_t)net * gridDim.x + blockIdx.x] = red[0];
 }
 
-/* column sums of G [rows x N] per net: the bias gradient (fixed order) */
-__global__ void k_sg_colsum(int rows, int N, const float *G, long s_g, float *db, long s_db) {
-  const int net = blockIdx.y;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+/* column sums of G [rows x N] per net, the bias gradient, in a fixed order:
+ * workgroup (column tile of 64, net, chunk of SG_CS_ROWS rows) sums its
+ * chunk with 4 threads per column and an LDS step into part[net][chunk][N];
+ * k_sg_colsum2 adds the chunks in order */
+#define SG_CS_ROWS 256
+__global__ void __launch_bounds__(256) k_sg_colsum(int rows, int N, const float *G, long s_g, float *part) {
+  __shared__ float red[4][64];
+  const int net = blockIdx.y, chunk = blockIdx.z;
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), t = threadIdx.x >> 6;
   const float *g = G + (size_t)net * s_g;
   float s = 0.0f;
-  for (int r = 0; r < rows; r++) s += g[(size_t)r * N + c];
+  const int r1 = min(rows, (chunk + 1) * SG_CS_ROWS);
+  for (int r = chunk * SG_CS_ROWS + t; r < r1; r += 4) s += g[(size_t)r * N + c];
+  red[t][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (t == 0)
+    part[((size_t)net * gridDim.z + chunk) * N + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                                                      (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+__global__ void k_sg_colsum2(int N, int chunks, const float *part, float *db, long s_db) {
+  const int net = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  float s = 0.0f;
+  for (int k = 0; k < chunks; k++) s += part[((size_t)net * chunks + k) * N + c];
   db[(size_t)net * s_db + c] = s;
+}
+
+/* the rows idx[0..batch) of the resident training data: inputs packed to
+ * bf16 (row-major, zero padded to ld) with their transpose, targets and mask
+ * in f32, and the count of mask entries of the batch (wave-aggregated) */
+__global__ void k_sg_gather(int batch, int in_f, int ld, int out_f, const int *idx, const float *Xall,
+                            const float *Tall, const float *Mall, uint16_t *Xb, uint16_t *XbT, float *T, float *Mk,
+                            unsigned int *count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nx = (size_t)batch * ld, nt = (size_t)batch * out_f;
+  int on = 0;
+  if (i < nx) {
+    const int r = (int)(i / ld), c = (int)(i % ld);
+    const uint16_t h = c < in_f ? sg_f2bf(Xall[(size_t)idx[r] * in_f + c]) : (uint16_t)0;
+    Xb[i] = h;
+    XbT[(size_t)c * batch + r] = h;
+  }
+  if (i < nt) {
+    const int r = (int)(i / out_f), c = (int)(i % out_f);
+    const size_t src = (size_t)idx[r] * out_f + c;
+    T[i] = Tall[src];
+    const float m = Mall[src];
+    Mk[i] = m;
+    on = m != 0.0f;
+  }
+  const unsigned long long b = __ballot(on);
+  if (b && __lane_id() == 0) atomicAdd(count, (unsigned int)__popcll(b));
 }
 
 /* Adam (Kingma & Ba 2015) on f32 master parameters; refreshes the bf16 copy
@@ -241,14 +287,31 @@ hipError_t sg_launch_pack(int rows, int cols, int ld, const float *X, uint16_t *
 }
 
 hipError_t sg_launch_loss(int rows, int N, int nets, const float *Y, const float *T, const float *Mk, float inv_count,
-                          float *G, uint16_t *Gb, uint16_t *GT, float *partial, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_sg_loss, dim3(blocks, nets), dim3(256), 0, st, rows, N, nets, Y, T, Mk, inv_count, G, Gb, GT,
-                     partial);
+                          const unsigned int *count, float *G, uint16_t *Gb, uint16_t *GT, float *partial, int blocks,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_loss, dim3(blocks, nets), dim3(256), 0, st, rows, N, nets, Y, T, Mk, inv_count, count, G, Gb,
+                     GT, partial);
   return hipGetLastError();
 }
 
-hipError_t sg_launch_colsum(int nets, int rows, int N, const float *G, long s_g, float *db, long s_db, hipStream_t st) {
-  hipLaunchKernelGGL(k_sg_colsum, dim3((N + 255) / 256, nets), dim3(256), 0, st, rows, N, G, s_g, db, s_db);
+size_t sg_colsum_scratch(int nets, int rows, int N) {
+  return (size_t)nets * ((rows + SG_CS_ROWS - 1) / SG_CS_ROWS) * N;
+}
+
+hipError_t sg_launch_colsum(int nets, int rows, int N, const float *G, long s_g, float *part, float *db, long s_db,
+                            hipStream_t st) {
+  const int chunks = (rows + SG_CS_ROWS - 1) / SG_CS_ROWS;
+  hipLaunchKernelGGL(k_sg_colsum, dim3(N / 64, nets, chunks), dim3(256), 0, st, rows, N, G, s_g, part);
+  hipLaunchKernelGGL(k_sg_colsum2, dim3((N + 255) / 256, nets), dim3(256), 0, st, N, chunks, part, db, s_db);
+  return hipGetLastError();
+}
+
+hipError_t sg_launch_gather(int batch, int in_f, int ld, int out_f, const int *idx, const float *Xall, const float *Tall,
+                            const float *Mall, uint16_t *Xb, uint16_t *XbT, float *T, float *Mk, unsigned int *count,
+                            hipStream_t st) {
+  const size_t n = (size_t)batch * (ld > out_f ? ld : out_f);
+  hipLaunchKernelGGL(k_sg_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, batch, in_f, ld, out_f, idx, Xall,
+                     Tall, Mall, Xb, XbT, T, Mk, count);
   return hipGetLastError();
 }
 

@@ -133,17 +133,22 @@ def generate(engine: Engine, n: int, photons: int, seed: int, n_el=(2, 30), alph
 def train(sg: S.Surrogate, X, T, M, steps: int, batch: int, lr: float = 1e-3, lr_min: float = 1e-5, seed: int = 0,
           log=None):
     """Adam on random mini-batches with a cosine learning-rate decay from lr
-    to lr_min; returns the per-step mean loss."""
+    to lr_min; returns the per-step mean loss.  The data stay on the GPU and
+    the steps run in chunks without a host round trip per step."""
     rng = np.random.default_rng(seed)
-    hist = []
     n = X.shape[0]
-    for k in range(steps):
-        idx = rng.integers(0, n, batch)
-        lr_k = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * k / max(1, steps - 1)))
-        loss = sg.train_step(X[idx], T[idx], M[idx], lr_k)
-        hist.append(float(loss.mean()))
-        if log and (k % max(1, steps // 10) == 0 or k == steps - 1):
-            log(f"step {k}: loss {hist[-1]:.5f}")
+    sg.set_data(X, T, M)
+    k = np.arange(steps)
+    lrs = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * k / max(1, steps - 1)))
+    hist = []
+    chunk = 250
+    for k0 in range(0, steps, chunk):
+        c = min(chunk, steps - k0)
+        idx = np.stack([rng.integers(0, n, batch) for _ in range(c)])
+        loss = sg.train_resident(idx, lrs[k0:k0 + c])
+        hist.extend(loss.mean(axis=1).tolist())
+        if log:
+            log(f"step {k0 + loss.shape[0] - 1}: loss {hist[-1]:.5f}")
     return np.array(hist)
 
 

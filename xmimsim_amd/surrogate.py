@@ -77,6 +77,10 @@ def _lib():
         lib.xmi_surrogate_padded_width.restype = C.c_int
         lib.xmi_surrogate_train_step.argtypes = [C.c_void_p, fp, fp, fp, C.c_int, C.c_float, fp]
         lib.xmi_surrogate_train_step.restype = C.c_int
+        lib.xmi_surrogate_set_data.argtypes = [C.c_void_p, fp, fp, fp, C.c_long]
+        lib.xmi_surrogate_set_data.restype = C.c_int
+        lib.xmi_surrogate_train_resident.argtypes = [C.c_void_p, P(C.c_int32), C.c_int, C.c_int, fp, fp]
+        lib.xmi_surrogate_train_resident.restype = C.c_int
         lib.xmi_surrogate_gradients.argtypes = [C.c_void_p, fp, fp, fp, C.c_int, fp, fp]
         lib.xmi_surrogate_gradients.restype = C.c_int
         lib.xmi_surrogate_predict.argtypes = [C.c_void_p, fp, C.c_int, fp]
@@ -127,6 +131,23 @@ class Surrogate:
         loss = np.zeros(self.n_nets, np.float32)
         self._check(self.lib.xmi_surrogate_train_step(self.h, _fp(X), _fp(T), _fp(M), X.shape[0], lr, _fp(loss)),
                     "train_step")
+        return loss
+
+    def set_data(self, X: np.ndarray, T: np.ndarray, M: np.ndarray):
+        """Keeps the training rows on the GPU for train_resident."""
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        self._check(self.lib.xmi_surrogate_set_data(self.h, _fp(X), _fp(T), _fp(M), X.shape[0]), "set_data")
+        self.n_data = X.shape[0]
+
+    def train_resident(self, idx: np.ndarray, lr: np.ndarray) -> np.ndarray:
+        """Adam steps on rows idx[steps][batch] of the resident data with
+        learning rates lr[steps]; returns loss[steps][n_nets]."""
+        idx = np.ascontiguousarray(idx, np.int32)
+        lr = np.ascontiguousarray(lr, np.float32)
+        steps, batch = idx.shape
+        loss = np.zeros((steps, self.n_nets), np.float32)
+        self._check(self.lib.xmi_surrogate_train_resident(self.h, idx.ctypes.data_as(C.POINTER(C.c_int32)), steps,
+                                                           batch, _fp(lr), _fp(loss)), "train_resident")
         return loss
 
     def gradients(self, X: np.ndarray, T: np.ndarray, M: np.ndarray):
