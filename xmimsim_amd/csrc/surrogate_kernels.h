@@ -31,8 +31,8 @@ typedef struct SgEpiArgs {
 #ifdef __cplusplus
 extern "C" {
 #endif
-/* C[M x N] = A[M x K] * B[N x K]^T per net (gridDim.z = nets); M, N multiples
- * of 64, K a multiple of 32, lda/ldb multiples of 8 */
+/* C[M x N] = A[M x K] * B[N x K]^T per net (gridDim.z = nets); M, N and K
+ * multiples of 64, lda/ldb multiples of 8 */
 hipError_t sg_launch_gemm(int epi, int nets, int M, int N, int K, const uint16_t *A, long lda, long s_a,
                           const uint16_t *B, long ldb, long s_b, const SgEpiArgs *a, hipStream_t st);
 hipError_t sg_launch_pack(int rows, int cols, int ld, const float *X, uint16_t *Xb, uint16_t *XbT, hipStream_t st);
