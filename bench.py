@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--train-samples", type=int, default=32768, help="quantify: Monte Carlo training samples")
     ap.add_argument("--train-steps", type=int, default=8000, help="quantify: Adam steps")
     ap.add_argument("--quant-samples", type=int, default=64, help="quantify: unknown samples")
+    ap.add_argument("--plain-samples", type=int, default=0,
+                    help="quantify: run the plain loop on the first N unknowns only (0: all)")
     ap.add_argument("--train-photons", type=int, default=100_000, help="quantify: photons per training sample")
     ap.add_argument("--plain", default="pymca", choices=["pymca", "gn"],
                     help="quantify: the plain Monte Carlo loop (the reference's, or Gauss-Newton)")
@@ -459,7 +461,7 @@ def quantify(args):
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
     r = quant.run_pipeline(n_train=args.train_samples, photons_train=args.train_photons, steps=args.train_steps,
                            n_quant=args.quant_samples, photons_quant=1_000_000, seed=args.seed % 100000,
-                           plain=args.plain, log=log)
+                           plain=args.plain, log=log, n_plain=args.plain_samples)
     out = {
         "metric": "quantified samples/s (surrogate-screened vs plain MC)",
         "value": round(r["screened"]["samples_per_s"], 3),
@@ -471,7 +473,7 @@ def quantify(args):
         "data": "synthetic (seeded single-layer samples, 3-12 elements, Z 11-92)",
         "config": {"workload": "quantify", "train_samples": args.train_samples, "train_photons": args.train_photons,
                    "train_steps": args.train_steps, "ensemble": 4, "quant_samples": args.quant_samples,
-                   "quant_photons": 1_000_000, "plain": args.plain,
+                   "quant_photons": 1_000_000, "plain": args.plain, "plain_samples": r["plain_samples"],
                    "train_histories": r["train_histories"]},
         "plain_mc": r["plain"],
         "screened": r["screened"],

@@ -82,6 +82,7 @@ class Engine:
         self.mc_runs = 0
         self.mc_histories = 0
         self.streams, self.chunk = streams, chunk
+        self.log = None  # progress of long runs, one line per chunk
         deep = Sample(Zs[:2], np.full(len(Zs[:2]), 1.0 / len(Zs[:2])), 1.0, 1.0, 40.0)
         self.grid = api.Prepared(make_input(deep, photons=10), tables=self.tables)
         self.grid.compute_solid_angle(hits_per_single=grid_hits, seed=seed)
@@ -111,14 +112,23 @@ class Engine:
                     p.close()
             self.mc_runs += len(preps)
             out.extend(vr for _, vr in res)
+            if self.log and len(samples) > self.chunk:
+                self.log(f"Monte Carlo: {min(c + self.chunk, len(samples))} of {len(samples)} samples")
         return out
 
     def lines(self, samples: Sequence[Sample], Ws: Sequence[np.ndarray], photons: int, seeds: Sequence[int]):
         """log10 net lines [n][N_OUT] and loss masks of the samples at the
         compositions Ws."""
-        vrs = self.run(samples, Ws, photons, seeds)
-        tm = [S.targets(vr, s.Z) for vr, s in zip(vrs, samples)]
-        return np.array([t for t, _ in tm]), np.array([m for _, m in tm])
+        T = np.zeros((len(samples), S.N_OUT), np.float32)
+        M = np.zeros((len(samples), S.N_OUT), np.float32)
+        # a chunk at a time: the histories of all samples at once would not fit
+        for c in range(0, len(samples), self.chunk):
+            e = min(c + self.chunk, len(samples))
+            for k, vr in enumerate(self.run(samples[c:e], Ws[c:e], photons, seeds[c:e])):
+                T[c + k], M[c + k] = S.targets(vr, samples[c + k].Z)
+            if self.log and len(samples) > self.chunk:
+                self.log(f"Monte Carlo: {e} of {len(samples)} samples")
+        return T, M
 
 
 def generate(engine: Engine, n: int, photons: int, seed: int, n_el=(2, 30), alpha=0.5, zrange=(S.Z_MIN, S.Z_MAX)):
@@ -473,16 +483,19 @@ def _summary(W, samples, info, dt, mc_runs):
 
 def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int = 8000, n_quant: int = 64,
                  photons_quant: int = 1_000_000, zrange=(S.Z_MIN, S.Z_MAX), n_el_train=(2, 30), n_el_quant=(3, 12),
-                 n_nets: int = 4, batch: int = 4096, seed: int = 1, plain: str = "pymca", log=None):
+                 n_nets: int = 4, batch: int = 4096, seed: int = 1, plain: str = "pymca", log=None,
+                 n_plain: int = 0):
     """Training data from the GPU Monte Carlo, training of the ensemble, and
     quantification of n_quant unknown samples both ways (plain: "pymca", the
-    reference's loop, or "gn", Gauss-Newton with Monte Carlo Jacobians).
-    Returns a dict of timings, sample rates, Monte Carlo run counts and
-    errors."""
+    reference's loop, or "gn", Gauss-Newton with Monte Carlo Jacobians; on
+    the first n_plain of them when 0 < n_plain < n_quant: its samples/s is a
+    per-sample rate either way).  Returns a dict of timings, sample rates,
+    Monte Carlo run counts and errors."""
     log = log or (lambda *a: None)
     out = {}
     t0 = time.time()
     eng = Engine(Zs=range(zrange[0], zrange[1] + 1))
+    eng.log = log
     out["tables_s"] = time.time() - t0
     t0 = time.time()
     X, T, M = generate(eng, n_train, photons_train, seed, n_el_train, 0.5, zrange)
@@ -503,17 +516,20 @@ def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int 
     rng = np.random.default_rng(seed + 2024)
     samples = [random_sample(rng, n_el_quant, 0.5, zrange) for _ in range(n_quant)]
     meas = measure(eng, samples, photons_quant, seed=5000)
-    plain_fn = ((lambda: quantify_pymca(eng, samples, meas, photons_quant, seed=seed, log=log)) if plain == "pymca"
-                else (lambda: quantify_plain_gn(eng, samples, meas, photons_quant, seed=seed)))
-    for name, fn in (("screened", lambda: quantify_screened(sg, eng, samples, meas, photons_quant, seed=seed)),
-                     ("plain", plain_fn)):
+    npl = n_plain if 0 < n_plain < n_quant else n_quant
+    ps, pm = samples[:npl], meas[:npl]
+    plain_fn = ((lambda: quantify_pymca(eng, ps, pm, photons_quant, seed=seed, log=log)) if plain == "pymca"
+                else (lambda: quantify_plain_gn(eng, ps, pm, photons_quant, seed=seed)))
+    out["plain_samples"] = npl
+    for name, fn, sm in (("screened", lambda: quantify_screened(sg, eng, samples, meas, photons_quant, seed=seed),
+                          samples), ("plain", plain_fn, ps)):
         c0, t0 = eng.mc_runs, time.time()
         W, info = fn()
         dt = time.time() - t0
-        out[name] = _summary(W, samples, info, dt, eng.mc_runs - c0)
+        out[name] = _summary(W, sm, info, dt, eng.mc_runs - c0)
         if name == "screened":
             out[name].update({"gated": info["gated"], "fallback": info["fallback"],
-                              "fallback_fraction": info["fallback"] / n_quant,
+                              "fallback_fraction": info["fallback"] / len(sm),
                               "mc_confirmed_fraction": float(np.mean(info["confirmed"]))})
         elif plain == "pymca":
             out[name].update({"method": "bin/xmimsim-pymca.c:597-806",

@@ -48,19 +48,17 @@ def encode(Z: Sequence[int], w: Sequence[float], thickness: float, density: floa
     return x
 
 
+_OUT_Z = np.array([z for z, _ in OUT_LINES])
+_OUT_ROW = np.array([line - 1 for _, line in OUT_LINES])
+
+
 def targets(var_red_history: np.ndarray, Z: Sequence[int]) -> Tuple[np.ndarray, np.ndarray]:
     """log10 net-line intensities (summed over orders) and the loss mask."""
+    v = var_red_history[_OUT_Z - 1, _OUT_ROW, :].sum(axis=-1)
+    on = np.isin(_OUT_Z, np.asarray(Z)) & (v > 0.0)
     t = np.zeros(N_OUT, np.float32)
-    m = np.zeros(N_OUT, np.float32)
-    present = set(Z)
-    for k, (z, line) in enumerate(OUT_LINES):
-        if z not in present:
-            continue
-        v = float(var_red_history[z - 1, line - 1, :].sum())
-        if v > 0.0:
-            t[k] = np.log10(v)
-            m[k] = 1.0
-    return t, m
+    t[on] = np.log10(v[on])
+    return t, on.astype(np.float32)
 
 
 def _lib():
