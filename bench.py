@@ -547,7 +547,7 @@ def main():
         bpe = vr_bytes_per_event(work)
         fpe = vr_flops_per_event(work)
         kernel_ms = {k: round(prof[k], 2) for k in ("geom", "cone", "scatter", "lines", "step", "interactions")}
-        names = {"scatter": "k_vr_scatter", "lines": "k_vr_lines", "geom": "k_vr_geom", "cone": "k_vr_cone",
+        names = {"scatter": "k_vr_scatter", "lines": "k_vr_lines", "geom": "k_vr_list", "cone": "k_vr_cone",
                  "step": "k_step", "interactions": "k_interact"}
         traffic = _measured_traffic(args.workload)
         # the dominant kernel of the step: the scatter and line kernels are

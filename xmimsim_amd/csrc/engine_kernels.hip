@@ -9,10 +9,11 @@
  *   per interaction order n = 1..n_int:
  *     k_step        forced interaction (src/xmi_main.F90:1414-1518), atom and
  *                   interaction-type selection (:1558-1652); appends the
- *                   photon to one of three queues (Rayleigh, Compton, photo)
- *                   and counts each wave's events per layer
+ *                   photon to one of three queues (Rayleigh, Compton, photo),
+ *                   counts each wave's events per layer, and computes the
+ *                   estimator's geometry of the new point (d_vr_geom)
  *     k_layer_scan, k_layer_base   offsets of the per-layer event lists
- *     k_vr_geom, k_vr_cone, k_vr_scatter, k_vr_lines
+ *     k_vr_list, k_vr_cone, k_vr_scatter, k_vr_lines
  *                   detection-weight estimator of every photon at order n
  *                   (src/xmi_variance_reduction.F90:29-726, 949-1101)
  *     k_interact    one launch over the three interaction queues:
@@ -491,7 +492,7 @@ __device__ __forceinline__ int d_dequeue(const EngineParams &P, int t, unsigned 
 }
 
 /* layer lists: k_step stores each wave's number of events per layer (0 = no
- * event), k_layer_scan turns them into offsets, k_vr_geom writes the slots.
+ * event), k_layer_scan turns them into offsets, k_vr_list writes the slots.
  * Both launches run one thread per slot with the same workgroups, so a slot
  * has the same wave and lane in both.  Call from converged code. */
 __device__ __forceinline__ void d_layer_count(const EngineParams &P, int layer) {
@@ -511,7 +512,7 @@ __device__ __forceinline__ void d_layer_list(const EngineParams &P, int layer, i
       const int pos = P.lay_woff[(size_t)(L - 1) * P.n_waves + w] +
                       P.lay_segbase[(L - 1) * ENG_SCAN_NSEG + (int)(w / ENG_SCAN_SEG)] + __popcll(m & below);
       if ((unsigned int)pos < (unsigned int)P.cap) P.lay_list[(size_t)(L - 1) * P.cap + pos] = idx;
-      else atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull); /* counts of k_step and k_vr_geom disagree */
+      else atomicAdd(d_ctr(P, ENG_CNT_ERRORS), 1ull); /* counts of k_step and k_vr_list disagree */
     }
   }
 }
@@ -561,16 +562,142 @@ __device__ __forceinline__ int d_select(const EngineParams &P, const PhotonSoA &
   return (R3 < pr) ? 0 : ((R3 < pc) ? 1 : 2);
 }
 
+#define VR_ACTIVE 1
+#define VR_UP 2
+#define VR_NC_SHIFT 8
+
+/* the geometry of the detection-weight estimator for the photon of slot i,
+ * at its interaction point coords (lab frame) with direction dirv_l in layer
+ * cur (src/xmi_variance_reduction.F90:29-330): a point of the detector
+ * window, the scattering angle and polarisation azimuth towards it, the path
+ * through the layers and its escape probability, the solid angle from the
+ * grid (or the slot queued for a cone estimate: *need_sa).  Writes the
+ * photon's estimator record; returns 1 when there is an event (*info set),
+ * 0 for a point behind the window plane. */
+template <int NL>
+__device__ __forceinline__ int d_vr_geom(const EngineParams &P, const PhotonSoA &S, const VrSoA &V, int n, int i,
+                                         const double *coords, const double *dirv_l, int cur, int *need_sa,
+                                         int *info) {
+  const size_t cap = (size_t)P.cap;
+  phx_stream s;
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
+  double radius = sqrt(phx_uniform(&s)) * P.det_radius;
+  double th = 2.0 * ENG_PI * phx_uniform(&s);
+  double dp[3] = {0.0, cos(th) * radius, sin(th) * radius};
+  double rel[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
+  double lpnt[3], dirv[3], ld[3];
+  d_matvec(P.det_inv, rel, lpnt);
+  d_matvec(P.det_inv, dirv_l, dirv);
+  for (int k = 0; k < 3; k++) ld[k] = dp[k] - lpnt[k];
+  if (ld[0] >= 0.0) {
+    return 0;
+  } else {
+    double total_distance = d_distance(dp, lpnt);
+    d_normalize(dirv);
+    d_normalize(ld);
+    double nd[3];
+    d_matvec(P.det_new, ld, nd);
+    double c = d_dot(dirv, ld);
+    c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+    const double theta = acos(c);
+    double proj[3], ndd = d_dot(nd, dirv_l);
+    for (int k = 0; k < 3; k++) proj[k] = nd[k] - ndd * dirv_l[k];
+    d_normalize(proj);
+    double ev[3] = {S.ex[i], S.ey[i], S.ez[i]};
+    double en = d_norm(ev);
+    double evn[3] = {ev[0] / en, ev[1] / en, ev[2] / en};
+    c = d_dot(proj, evn);
+    c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+    const double phi = acos(c);
+    const int up = d_dot(nd, P.n_sample) > 0.0;
+    const int dir = up ? 1 : -1;
+    const int ncross = up ? P.n_layers - cur + 1 : cur;
+    /* path to the detector through the layers, in crossing order; the
+     * layer holding the detector point ends the walk */
+    double tc[3] = {coords[0], coords[1], coords[2]};
+    int done = 0, nc = 0;
+    double murhod = 0.0;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+      double dd = 0.0;
+      if (k < ncross && !done) {
+        const int l0 = cur - 1 + k * dir;
+        double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]}, ip[3];
+        d_intersect(pp, P.n_sample, tc, nd, ip);
+        dd = d_distance(tc, ip);
+        if (dd > total_distance) {
+          dd = total_distance;
+          done = 1;
+        } else {
+          tc[0] = ip[0];
+          tc[1] = ip[1];
+          tc[2] = ip[2];
+          total_distance -= dd;
+        }
+        murhod += S.mus[(size_t)l0 * cap + i] * P.layer_density[l0] * dd;
+        nc = k + 1;
+      }
+      V.dist[(size_t)k * cap + i] = dd;
+    }
+    /* solid angle, src/xmi_solid_angle_f.F90:712-801 */
+    double omega = 0.0;
+    double r = d_distance(P.p_det, coords);
+    double dv[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
+    d_normalize(dv);
+    double tt = acos(d_dot(dv, P.n_det));
+    if (tt > ENG_PI / 2.0) tt = ENG_PI - tt;
+    double sth = ENG_PI / 2.0 - tt;
+    if (sth >= P.sa_t[0]) {
+      int p1 = d_findpos_uniform(P.sa_r, P.sa_nr, P.sa_r0, P.sa_rinv, r);
+      int p2 = d_findpos_uniform(P.sa_t, P.sa_nt, P.sa_t0, P.sa_tinv, sth);
+      if (p1 < 0 || p2 < 0) {
+        *need_sa = 1;
+        V.sar[i] = r;
+        V.sat[i] = sth;
+      } else {
+        const double *a1 = P.sa_r, *a2 = P.sa_t;
+        double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
+        double c1 = (a1[p1 + 1] - r) * (a2[p2 + 1] - sth) / denom;
+        double c2 = (r - a1[p1]) * (a2[p2 + 1] - sth) / denom;
+        double c3 = (a1[p1 + 1] - r) * (sth - a2[p2]) / denom;
+        double c4 = (r - a1[p1]) * (sth - a2[p2]) / denom;
+        const double *g0 = P.sa_grid + (size_t)p2 * P.sa_nr + p1;
+        const double *g1 = P.sa_grid + (size_t)(p2 + 1) * P.sa_nr + p1;
+        omega = c1 * g0[0] + c2 * g0[1] + c3 * g1[0] + c4 * g1[1];
+      }
+    }
+    V.theta[i] = theta;
+    V.phi[i] = phi;
+    V.omega[i] = omega;
+    V.pesc[i] = exp(-murhod);
+    V.blk[i] = s.ctr[0]; /* two uniforms drawn: block 1, nothing buffered */
+    *info = VR_ACTIVE | (up ? VR_UP : 0) | (nc << VR_NC_SHIFT);
+  }
+  return 1;
+}
+
+/* the layer lists of an order's estimator events: one thread per slot, as
+ * the step kernel, whose per-wave counts k_layer_scan has turned into
+ * offsets (a photon of layer l with energy above the threshold is listed,
+ * with or without an event) */
+__global__ void __launch_bounds__(ENG_BLOCK) k_vr_list(EngineParams P, PhotonSoA S, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ev_layer = (i < count && S.layer[i] > 0 && S.energy[i] > 1.0) ? S.layer[i] : 0;
+  d_layer_list(P, ev_layer, i);
+}
+
 /* NL bounds the layer count; path lengths and mu*rho are kept in crossing
  * order (k = 0: the photon's layer) so that register arrays are only indexed
  * by compile-time indices */
 /* occupancy: six waves per SIMD up to four layers (80 VGPRs, two spilled;
  * measured 2 % faster than five waves); more layers keep their natural budget */
-template <int NL>
+/* EST: the estimator's geometry of the new interaction point follows in the
+ * same thread (d_vr_geom), while the photon's record is in registers */
+template <int NL, bool EST>
 __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(NL <= 4 ? 6 : 1)))
-k_step(EngineParams P, PhotonSoA S, int n, int count) {
+k_step(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int type = -1, ev_layer = 0;
+  int type = -1, ev_layer = 0, need_sa = 0, info = 0, vr_active = 0;
   const size_t cap = (size_t)P.cap;
   if (i < count && S.layer[i] > 0) {
     double E = S.energy[i];
@@ -634,19 +761,28 @@ k_step(EngineParams P, PhotonSoA S, int n, int count) {
         if (k <= kmy) tsum += (1.0 - (md[k] / mrm)) * dist[k];
       tsum = tsum - 1.0 * log1p(-1.0 * R * Pabs2) / mrm;
       const int my = cur + kmy * dir;
-      S.x[i] = coords[0] + tsum * dirv[0];
-      S.y[i] = coords[1] + tsum * dirv[1];
-      S.z[i] = coords[2] + tsum * dirv[2];
+      const double pt[3] = {coords[0] + tsum * dirv[0], coords[1] + tsum * dirv[1], coords[2] + tsum * dirv[2]};
+      S.x[i] = pt[0];
+      S.y[i] = pt[1];
+      S.z[i] = pt[2];
       S.weight[i] = w;
       S.layer[i] = my;
-      /* an estimator event of this order (the test of k_vr_geom) */
+      /* an estimator event of this order */
       ev_layer = E > 1.0 ? my : 0;
 
       type = d_select(P, S, i, E, my, &s);
+      if (EST && ev_layer) vr_active = d_vr_geom<NL>(P, S, V, n, i, pt, dirv, my, &need_sa, &info);
     }
   }
   d_enqueue(P, type, i);
   d_layer_count(P, ev_layer);
+  if (EST) {
+    if (i < count) V.info[i] = info;
+    d_append(P, ENG_CNT_Q_SA, need_sa, i);
+    d_wave_count(d_ctr(P, ENG_CNT_FALLBACKS), need_sa);
+    d_wave_count(d_ctr(P, ENG_CNT_VR), vr_active);
+    for (int L = 1; L <= P.n_layers; L++) d_wave_count(d_ctr(P, ENG_CNT_LAYER0 + L - 1), vr_active && ev_layer == L);
+  }
 }
 
 /* exclusive scan of the per-wave counts in segments of ENG_SCAN_SEG waves:
@@ -691,11 +827,12 @@ __global__ void k_layer_base(EngineParams P, int nseg) {
 /* ------------------------------------------------------------------------ */
 /* detection-weight estimator of every photon at order n
  * (src/xmi_variance_reduction.F90:29-726, xmi_compton_varred2 :949-1101),
- * as four kernels with their own register and LDS budgets:
+ * after its geometry in the step kernel (d_vr_geom: detector point, angles,
+ * path through the layers to the detector, escape probability, solid angle
+ * from the grid; off-grid points go to a queue), as kernels with their own
+ * register and LDS budgets:
  *
- *   k_vr_geom     one lane per photon: detector point, angles, path through
- *                 the layers to the detector, escape probability, solid angle
- *                 from the grid; off-grid points go to a queue
+ *   k_vr_list     one lane per slot: the per-layer event lists
  *   k_vr_cone     one wavefront per queued photon: the cone estimate of
  *                 src/xmi_solid_angle_f.F90:432-710 with the trials split over
  *                 the lanes (same draws, same hit count as the serial loop)
@@ -778,125 +915,6 @@ __device__ __forceinline__ double d_exp_neg(double x, const double *tab) {
   return ldexp(fma(T, p, T), ni >> 6);
 }
 
-#define VR_ACTIVE 1
-#define VR_UP 2
-#define VR_NC_SHIFT 8
-
-/* occupancy: six waves per SIMD (80 VGPRs, no spills; 8 % faster than five) */
-template <int NL>
-__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(6)))
-k_vr_geom(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t cap = (size_t)P.cap;
-  int active = (i < count) && S.layer[i] > 0 && S.energy[i] > 1.0;
-  const int ev_layer = active ? S.layer[i] : 0;
-  int cur = 0, need_sa = 0, info = 0;
-  if (active) {
-    phx_stream s;
-    phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, PHX_SUB_VARRED, S.hid[i]);
-    cur = S.layer[i];
-    double radius = sqrt(phx_uniform(&s)) * P.det_radius;
-    double th = 2.0 * ENG_PI * phx_uniform(&s);
-    double dp[3] = {0.0, cos(th) * radius, sin(th) * radius};
-    double coords[3] = {S.x[i], S.y[i], S.z[i]};
-    double dirv_l[3] = {S.dx[i], S.dy[i], S.dz[i]};
-    double rel[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
-    double lpnt[3], dirv[3], ld[3];
-    d_matvec(P.det_inv, rel, lpnt);
-    d_matvec(P.det_inv, dirv_l, dirv);
-    for (int k = 0; k < 3; k++) ld[k] = dp[k] - lpnt[k];
-    if (ld[0] >= 0.0) {
-      active = 0;
-    } else {
-      double total_distance = d_distance(dp, lpnt);
-      d_normalize(dirv);
-      d_normalize(ld);
-      double nd[3];
-      d_matvec(P.det_new, ld, nd);
-      double c = d_dot(dirv, ld);
-      c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-      const double theta = acos(c);
-      double proj[3], ndd = d_dot(nd, dirv_l);
-      for (int k = 0; k < 3; k++) proj[k] = nd[k] - ndd * dirv_l[k];
-      d_normalize(proj);
-      double ev[3] = {S.ex[i], S.ey[i], S.ez[i]};
-      double en = d_norm(ev);
-      double evn[3] = {ev[0] / en, ev[1] / en, ev[2] / en};
-      c = d_dot(proj, evn);
-      c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
-      const double phi = acos(c);
-      const int up = d_dot(nd, P.n_sample) > 0.0;
-      const int dir = up ? 1 : -1;
-      const int ncross = up ? P.n_layers - cur + 1 : cur;
-      /* path to the detector through the layers, in crossing order; the
-       * layer holding the detector point ends the walk */
-      double tc[3] = {coords[0], coords[1], coords[2]};
-      int done = 0, nc = 0;
-      double murhod = 0.0;
-#pragma unroll
-      for (int k = 0; k < NL; k++) {
-        double dd = 0.0;
-        if (k < ncross && !done) {
-          const int l0 = cur - 1 + k * dir;
-          double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]}, ip[3];
-          d_intersect(pp, P.n_sample, tc, nd, ip);
-          dd = d_distance(tc, ip);
-          if (dd > total_distance) {
-            dd = total_distance;
-            done = 1;
-          } else {
-            tc[0] = ip[0];
-            tc[1] = ip[1];
-            tc[2] = ip[2];
-            total_distance -= dd;
-          }
-          murhod += S.mus[(size_t)l0 * cap + i] * P.layer_density[l0] * dd;
-          nc = k + 1;
-        }
-        V.dist[(size_t)k * cap + i] = dd;
-      }
-      /* solid angle, src/xmi_solid_angle_f.F90:712-801 */
-      double omega = 0.0;
-      double r = d_distance(P.p_det, coords);
-      double dv[3] = {coords[0] - P.p_det[0], coords[1] - P.p_det[1], coords[2] - P.p_det[2]};
-      d_normalize(dv);
-      double tt = acos(d_dot(dv, P.n_det));
-      if (tt > ENG_PI / 2.0) tt = ENG_PI - tt;
-      double sth = ENG_PI / 2.0 - tt;
-      if (sth >= P.sa_t[0]) {
-        int p1 = d_findpos_uniform(P.sa_r, P.sa_nr, P.sa_r0, P.sa_rinv, r);
-        int p2 = d_findpos_uniform(P.sa_t, P.sa_nt, P.sa_t0, P.sa_tinv, sth);
-        if (p1 < 0 || p2 < 0) {
-          need_sa = 1;
-          V.sar[i] = r;
-          V.sat[i] = sth;
-        } else {
-          const double *a1 = P.sa_r, *a2 = P.sa_t;
-          double denom = (a1[p1 + 1] - a1[p1]) * (a2[p2 + 1] - a2[p2]);
-          double c1 = (a1[p1 + 1] - r) * (a2[p2 + 1] - sth) / denom;
-          double c2 = (r - a1[p1]) * (a2[p2 + 1] - sth) / denom;
-          double c3 = (a1[p1 + 1] - r) * (sth - a2[p2]) / denom;
-          double c4 = (r - a1[p1]) * (sth - a2[p2]) / denom;
-          const double *g0 = P.sa_grid + (size_t)p2 * P.sa_nr + p1;
-          const double *g1 = P.sa_grid + (size_t)(p2 + 1) * P.sa_nr + p1;
-          omega = c1 * g0[0] + c2 * g0[1] + c3 * g1[0] + c4 * g1[1];
-        }
-      }
-      V.theta[i] = theta;
-      V.phi[i] = phi;
-      V.omega[i] = omega;
-      V.pesc[i] = exp(-murhod);
-      V.blk[i] = s.ctr[0]; /* two uniforms drawn: block 1, nothing buffered */
-      info = VR_ACTIVE | (up ? VR_UP : 0) | (nc << VR_NC_SHIFT);
-    }
-  }
-  if (i < count) V.info[i] = info;
-  d_layer_list(P, ev_layer, i);
-  d_append(P, ENG_CNT_Q_SA, need_sa, i);
-  d_wave_count(d_ctr(P, ENG_CNT_FALLBACKS), need_sa);
-  d_wave_count(d_ctr(P, ENG_CNT_VR), active);
-  for (int L = 1; L <= P.n_layers; L++) d_wave_count(d_ctr(P, ENG_CNT_LAYER0 + L - 1), active && cur == L);
-}
 
 /* persistent: the off-grid queue's photons (entries flattened over the
  * ENG_SPREAD sub-queues) times ENG_CONE_SEGS(hits) segments of
@@ -1532,7 +1550,7 @@ __device__ __forceinline__ void d_rayleigh_one(const EngineParams &P, const Phot
   (void)cap;
 }
 
-template <bool BRUTE>
+template <bool BRUTE, bool ADV>
 __device__ __forceinline__ void d_compton_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
@@ -1552,7 +1570,7 @@ __device__ __forceinline__ void d_compton_one(const EngineParams &P, const Photo
   double th = S.theta[i], ph = S.phi[i];
   double ev[3] = {S.ex[i], S.ey[i], S.ez[i]};
   double phi0 = d_phi0(th, ph, ev);
-  if (P.adv_compton) {
+  if (ADV) {
     /* xmi_update_photon_energy_compton: the subshell by count * CDF(Qimax),
      * Q from its profile below Qimax, the energy from Q; 0 ends the photon */
     const xrl_atom *a = &P.atoms[u];
@@ -1939,8 +1957,10 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
  * photons that interact at this order, and a wavefront straddles two queues
  * at two places at most.  Each photon is one thread's, with its own random
  * stream, so results do not depend on the assignment. */
-/* occupancy: five waves per SIMD (96 VGPRs, a few spilled; 6 % faster than four) */
-template <bool BRUTE>
+/* occupancy: five waves per SIMD (96 VGPRs, a few spilled; 6 % faster than four).
+ * ADV: the advanced Compton model, compiled apart so that the default
+ * instance spills less (10 VGPRs instead of 28) */
+template <bool BRUTE, bool ADV>
 __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
 k_interact(EngineParams P, PhotonSoA S, int n) {
   constexpr int NS = 3 * ENG_SPREAD;
@@ -1974,7 +1994,7 @@ k_interact(EngineParams P, PhotonSoA S, int n) {
   const int t = lo / ENG_SPREAD;
   const int i = P.queue[(size_t)lo * P.qreg + (q - off[lo])];
   if (t == 0) d_rayleigh_one<BRUTE>(P, S, n, i);
-  else if (t == 1) d_compton_one<BRUTE>(P, S, n, i);
+  else if (t == 1) d_compton_one<BRUTE, ADV>(P, S, n, i);
   else d_photo_one<BRUTE>(P, S, n, i);
 }
 
@@ -2290,6 +2310,14 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, Photon
   d_fx_add_keyed(P.t_bhist, key_h, w_hit, P.tally_shift);
 }
 
+template <bool BRUTE>
+static void launch_interact(const EngineParams *P, const PhotonSoA *S, int n, int blocks, hipStream_t st) {
+  if (P->adv_compton)
+    hipLaunchKernelGGL((k_interact<BRUTE, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  else
+    hipLaunchKernelGGL((k_interact<BRUTE, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+}
+
 /* photons per lane of k_vr_lines: each line's wavefront sum then serves
  * 64 KP photons; fewer where the per-photon path lengths take more registers.
  * Three rather than four up to four layers: 126 VGPRs give four waves per
@@ -2323,7 +2351,7 @@ static void launch_scatter(const EngineParams *P, const PhotonSoA *S, const VrSo
 template <int NL>
 static void launch_estimator_nl(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count,
                                 int n_cu, int blocks, hipStream_t st, hipEvent_t *ev) {
-  hipLaunchKernelGGL((k_vr_geom<NL>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+  hipLaunchKernelGGL(k_vr_list, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, count);
   if (ev) (void)hipEventRecord(ev[0], st);
   /* one resident round: the trial segments are of equal size, so a second,
    * partial round of workgroups would only add a tail */
@@ -2378,15 +2406,15 @@ hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const Vr
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
   {
     const int nl = P->n_layers;
-    if (nl <= 1) hipLaunchKernelGGL((k_step<1>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
-    else if (nl <= 2) hipLaunchKernelGGL((k_step<2>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
-    else if (nl <= 4) hipLaunchKernelGGL((k_step<4>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
-    else if (nl <= 8) hipLaunchKernelGGL((k_step<8>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
-    else hipLaunchKernelGGL((k_step<16>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, 1, count);
+    if (nl <= 1) hipLaunchKernelGGL((k_step<1, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, 1, count);
+    else if (nl <= 2) hipLaunchKernelGGL((k_step<2, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, 1, count);
+    else if (nl <= 4) hipLaunchKernelGGL((k_step<4, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, 1, count);
+    else if (nl <= 8) hipLaunchKernelGGL((k_step<8, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, 1, count);
+    else hipLaunchKernelGGL((k_step<16, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, 1, count);
   }
   hipError_t e = hipMemcpyAsync(V->pesc, S->weight, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_interact<false>, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, 1);
+  launch_interact<false>(P, S, 1, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
   const int eblocks = (int)((3 * (size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
   hipLaunchKernelGGL(k_escape, dim3(eblocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, *Q);
   return hipGetLastError();
@@ -2401,7 +2429,7 @@ hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_step_brute, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
   if (ev) (void)hipEventRecord(ev[1], st);
-  if (n <= P->n_int) hipLaunchKernelGGL(k_interact<true>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  if (n <= P->n_int) launch_interact<true>(P, S, n, blocks, st);
   if (ev) (void)hipEventRecord(ev[2], st);
   return hipGetLastError();
 }
@@ -2413,11 +2441,11 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
   {
     const int nl = P->n_layers;
-    if (nl <= 1) hipLaunchKernelGGL((k_step<1>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
-    else if (nl <= 2) hipLaunchKernelGGL((k_step<2>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
-    else if (nl <= 4) hipLaunchKernelGGL((k_step<4>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
-    else if (nl <= 8) hipLaunchKernelGGL((k_step<8>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
-    else hipLaunchKernelGGL((k_step<16>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
+    if (nl <= 1) hipLaunchKernelGGL((k_step<1, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+    else if (nl <= 2) hipLaunchKernelGGL((k_step<2, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+    else if (nl <= 4) hipLaunchKernelGGL((k_step<4, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+    else if (nl <= 8) hipLaunchKernelGGL((k_step<8, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
+    else hipLaunchKernelGGL((k_step<16, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n, count);
   }
   {
     const int nw = blocks * (ENG_BLOCK / 64), nseg = (nw + ENG_SCAN_SEG - 1) / ENG_SCAN_SEG;
@@ -2426,7 +2454,7 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
   }
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
-  hipLaunchKernelGGL(k_interact<false>, dim3((count + ENG_BLOCK - 1) / ENG_BLOCK), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  launch_interact<false>(P, S, n, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
   if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
