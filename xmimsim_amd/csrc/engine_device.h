@@ -184,7 +184,19 @@ struct EngineParams {
   int tally_shift;              /* fixed point: value * 2^tally_shift */
   unsigned long long *counters; /* [slot ENG_CNT_*][ENG_SPREAD]: queue sizes, events, fallbacks, errors */
   int *queue;                   /* [ENG_NQUEUES][ENG_SPREAD][qreg] */
+  unsigned long long *qcount;   /* sizes of queue's sub-queues [ENG_NQUEUES][ENG_SPREAD]: counters' first slots,
+                                   or (brute force) those of the queue set of this order */
   int qreg;                     /* entries per sub-queue */
+  /* brute force: the photons alive at the start of order n are those of
+   * order n - 1's interaction queues (pqueue, pqcount: the other queue set)
+   * and the offspring that order's cascades started (olist_in, ocount_in);
+   * this order's offspring go to olist_out / ocount_out */
+  const int *pqueue;
+  const unsigned long long *pqcount;
+  const int *olist_in;
+  const unsigned int *ocount_in;
+  int *olist_out;
+  unsigned int *ocount_out;
   /* layer lists of an order's estimator events: the slots of the photons
    * whose interaction lies in layer l, in slot order, so that every
    * wavefront of the scatter and line kernels holds photons of one layer */

@@ -148,6 +148,7 @@ struct Engine {
   DevBuf lay_rc, lay_edge, lay_pb, lay_kstart, lay_cell;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, lay_segbase, cone_acc, esc_tally;
+  DevBuf queue2, qcount2, olist;
 };
 
 static Engine g_engine;
@@ -797,12 +798,33 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(G.state.ensure(state_bytes));
     HIP_TRY(G.vr.ensure(vr_bytes));
     HIP_TRY(G.tallies.ensure(tally_words * sizeof(unsigned long long)));
+    int dev = 0, n_cu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
     /* sub-queue size: k_step runs ceil(cap / ENG_BLOCK) workgroups, and each
      * sub-queue takes the photons of every ENG_SPREAD-th of their waves */
     const int step_waves = ((slots + ENG_BLOCK - 1) / ENG_BLOCK) * (ENG_BLOCK / 64);
     P.qreg = ((step_waves + ENG_SPREAD - 1) / ENG_SPREAD) * 64;
+    /* the brute-force grid of bblocks workgroups strides over at most `slots`
+     * live photons: each of its waves takes at most ceil(slots / (64 W))
+     * trips of 64 entries, and a sub-queue serves ceil(W / ENG_SPREAD) of
+     * its W waves */
+    const int bblocks = brute ? eng_brute_blocks(n_cu) : 0;
+    if (brute) {
+      const long W = (long)bblocks * (ENG_BLOCK / 64);
+      const long trips = ((long)slots + 64 * W - 1) / (64 * W);
+      const long need = ((W + ENG_SPREAD - 1) / ENG_SPREAD) * 64 * trips;
+      if (need > P.qreg) P.qreg = (int)need;
+    }
     HIP_TRY(G.counters.ensure((size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long)));
     HIP_TRY(G.queue.ensure((size_t)ENG_NQUEUES * ENG_SPREAD * P.qreg * sizeof(int)));
+    /* brute force: a second queue set (the interactions of order n - 1 while
+     * order n's step fills the first), and two offspring lists */
+    if (brute) {
+      HIP_TRY(G.queue2.ensure((size_t)ENG_NQUEUES * ENG_SPREAD * P.qreg * sizeof(int)));
+      HIP_TRY(G.qcount2.ensure((size_t)ENG_NQUEUES * ENG_SPREAD * sizeof(unsigned long long) + 2 * sizeof(unsigned int)));
+      HIP_TRY(G.olist.ensure((size_t)2 * (P.off_half ? P.off_half : 1) * sizeof(int)));
+    }
     /* layer lists: a row of cap slots per layer; per-wave offsets of the
      * k_step launch (ceil(cap / ENG_BLOCK) workgroups of ENG_BLOCK / 64 waves) */
     P.n_waves = step_waves;
@@ -950,6 +972,13 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.t_bhist = brute ? P.t_bch + n_bch2 : nullptr;
     P.counters = G.counters.as<unsigned long long>();
     P.queue = G.queue.as<int>();
+    P.qcount = P.counters;
+    P.pqueue = nullptr;
+    P.pqcount = nullptr;
+    P.olist_in = nullptr;
+    P.ocount_in = nullptr;
+    P.olist_out = nullptr;
+    P.ocount_out = nullptr;
     P.lay_list = G.lay_list.as<int>();
     P.lay_woff = G.lay_woff.as<int>();
     P.lay_total = G.lay_total.as<int>();
@@ -999,10 +1028,6 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     ST.n_exc = in->n_exc_layers;
     ST.exc_atoms = G.exc_atoms.as<xrl_atom>();
 
-    int dev = 0, n_cu = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-
 
     HIP_TRY(hipEventCreate(&evs[0]));
     HIP_TRY(hipEventCreate(&evs[1]));
@@ -1023,8 +1048,25 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
           /* order n_int + 1: the step after the last interaction, in which a
            * photon can still reach the detector */
           const int bprof = g_profile && 3 * (n_int + 1) <= ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED;
-          for (int n = 1; n <= n_int + 1; n++)
-            HIP_TRY(eng_launch_order_brute(&P, &S, n, count, st, bprof ? &G.prof_ev[3 * (n - 1)] : nullptr));
+          /* order n fills queue set n & 1 and offspring list n & 1, and
+           * reads those of order n - 1 */
+          int *qs[2] = {G.queue.as<int>(), G.queue2.as<int>()};
+          unsigned long long *qc[2] = {G.counters.as<unsigned long long>(), G.qcount2.as<unsigned long long>()};
+          unsigned int *oc = (unsigned int *)(qc[1] + ENG_NQUEUES * ENG_SPREAD);
+          int *ol[2] = {G.olist.as<int>(), G.olist.as<int>() + (P.off_half ? P.off_half : 1)};
+          for (int n = 1; n <= n_int + 1; n++) {
+            EngineParams Pn = P;
+            const int c = n & 1, p = c ^ 1;
+            Pn.queue = qs[c];
+            Pn.qcount = qc[c];
+            Pn.pqueue = qs[p];
+            Pn.pqcount = qc[p];
+            Pn.olist_in = ol[p];
+            Pn.ocount_in = oc + p;
+            Pn.olist_out = ol[c];
+            Pn.ocount_out = oc + c;
+            HIP_TRY(eng_launch_order_brute(&Pn, &S, n, count, bblocks, st, bprof ? &G.prof_ev[3 * (n - 1)] : nullptr));
+          }
           if (bprof) {
             HIP_TRY(hipStreamSynchronize(st));
             for (int n = 0; n <= n_int; n++) {

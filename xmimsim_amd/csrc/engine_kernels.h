@@ -90,9 +90,12 @@ hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv
 hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,
                              int count, hipStream_t st);
 /* order n of a brute-force run: the analogue step with its detector tests
- * and tallies, then (n <= n_int) the interactions */
-hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st,
-                                  hipEvent_t *ev /* 3 events (before, between, after) or NULL */);
+ * and tallies over a fixed grid of `blocks` workgroups (eng_brute_blocks)
+ * that strides over the live photons, then (n <= n_int) the interactions */
+hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, int blocks,
+                                  hipStream_t st, hipEvent_t *ev /* 3 events (before, between, after) or NULL */);
+/* workgroups of the brute-force grid: one resident round of the step kernel */
+int eng_brute_blocks(int n_cu);
 hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
                             hipStream_t st, hipEvent_t *ev);
 #ifdef __cplusplus

@@ -455,7 +455,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA 
 /* queues */
 
 __global__ void k_reset_queues(EngineParams P) {
-  if (threadIdx.x < ENG_NQUEUES * ENG_SPREAD) P.counters[threadIdx.x] = 0ull;
+  if (threadIdx.x < ENG_NQUEUES * ENG_SPREAD) P.qcount[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0 && P.ocount_out) *P.ocount_out = 0u;
 }
 
 /* wavefront-aggregated append of the lanes with pred to queue t, in this
@@ -471,7 +472,7 @@ __device__ __forceinline__ void d_append(const EngineParams &P, int t, int pred,
   int leader = __ffsll((long long)mask) - 1;
   unsigned int base = 0;
   if (lane == leader)
-    base = (unsigned int)atomicAdd(&P.counters[(size_t)t * ENG_SPREAD + j], (unsigned long long)__popcll(mask));
+    base = (unsigned int)atomicAdd(&P.qcount[(size_t)t * ENG_SPREAD + j], (unsigned long long)__popcll(mask));
   base = __shfl(base, leader);
   if (pred) {
     unsigned long long below = mask & ((1ull << lane) - 1ull);
@@ -487,7 +488,7 @@ __device__ __forceinline__ void d_enqueue(const EngineParams &P, int type, int i
 /* photon of queue entry q (0 <= q < ENG_SPREAD * qreg) of type t, or -1 */
 __device__ __forceinline__ int d_dequeue(const EngineParams &P, int t, unsigned int q) {
   const unsigned int j = q / (unsigned int)P.qreg, r = q - j * (unsigned int)P.qreg;
-  if (j >= ENG_SPREAD || r >= P.counters[(size_t)t * ENG_SPREAD + j]) return -1;
+  if (j >= ENG_SPREAD || r >= P.qcount[(size_t)t * ENG_SPREAD + j]) return -1;
   return P.queue[((size_t)t * ENG_SPREAD + j) * P.qreg + r];
 }
 
@@ -932,7 +933,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA
   const unsigned int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const unsigned int waves = (gridDim.x * blockDim.x) >> 6;
   /* sub-queue sizes, one per lane, and their inclusive prefix sums */
-  const unsigned long long sz = P.counters[(size_t)ENG_CNT_Q_SA * ENG_SPREAD + lane];
+  const unsigned long long sz = P.qcount[(size_t)ENG_CNT_Q_SA * ENG_SPREAD + lane];
   unsigned long long incl = sz;
   for (int off = 1; off < 64; off <<= 1) {
     const unsigned long long v = __shfl_up(incl, off);
@@ -1818,10 +1819,10 @@ __device__ __forceinline__ int d_offspring(const EngineParams &P, const PhotonSo
  * vacancy's Auger transition (the provider's rates) leaves two vacancies;
  * the first may fluoresce as the parent photon, the second as the
  * offspring.  M vacancies count with use_M_lines only. */
-__device__ void d_cascade_auger(const EngineParams &P, const PhotonSoA &S, int i, int u, const xrl_atom *a,
-                                int shell, phx_stream *s) {
-  int first, last;
-  if (!xrl_auger_range(shell, &first, &last)) return;
+__device__ int d_cascade_auger(const EngineParams &P, const PhotonSoA &S, int i, int u, const xrl_atom *a,
+                               int shell, phx_stream *s) {
+  int first, last, spawned = 0;
+  if (!xrl_auger_range(shell, &first, &last)) return 0;
   const double r = phx_uniform(s);
   double sumz = 0.0;
   int t = -1;
@@ -1832,7 +1833,7 @@ __device__ void d_cascade_auger(const EngineParams &P, const PhotonSoA &S, int i
       break;
     }
   }
-  if (t < 0) return;
+  if (t < 0) return 0;
   int sh[2];
   xrl_auger_shells(t, &sh[0], &sh[1]);
   for (int k = 0; k < 2; k++) {
@@ -1843,15 +1844,17 @@ __device__ void d_cascade_auger(const EngineParams &P, const PhotonSoA &S, int i
     const double e = d_line_check(P, a, u, v, s, &line);
     if (e <= 1.0) continue;
     d_cascade_emit(P, S, k == 0 ? i : d_offspring(P, S, i), e, line, u, s);
+    spawned = k == 1;
   }
+  return spawned;
 }
 
 /* xmi_simulate_photon_cascade_radiative (src/xmi_main.F90:4596-4783): the
  * line the parent emitted leaves a vacancy in its other shell (K -> L or M;
  * L -> M with use_M_lines), which may fluoresce as the offspring; then the
  * parent has no cascades left either */
-__device__ void d_cascade_radiative(const EngineParams &P, const PhotonSoA &S, int i, int u, const xrl_atom *a,
-                                    int shell, int line, phx_stream *s) {
+__device__ int d_cascade_radiative(const EngineParams &P, const PhotonSoA &S, int i, int u, const xrl_atom *a,
+                                   int shell, int line, phx_stream *s) {
   int sn = -1;
   if (shell == XRL_K_SHELL) {
     if (line >= xrl_line_KL(0) && line <= xrl_line_KM(4)) sn = line; /* KL1..KL3, KM1..KM5 -> L1..M5 */
@@ -1859,21 +1862,23 @@ __device__ void d_cascade_radiative(const EngineParams &P, const PhotonSoA &S, i
     const int base = xrl_line_LM(shell - 1, 0);
     if (line >= base && line <= base + 4) sn = XRL_M1_SHELL + (line - base);
   } else {
-    return;
+    return 0;
   }
-  if (sn < 0) return;
-  if (sn >= XRL_M1_SHELL && sn <= XRL_M5_SHELL && !P.use_M_lines) return;
-  if (!d_yield_check(P, u, sn, s)) return;
+  if (sn < 0) return 0;
+  if (sn >= XRL_M1_SHELL && sn <= XRL_M5_SHELL && !P.use_M_lines) return 0;
+  if (!d_yield_check(P, u, sn, s)) return 0;
   sn = d_coster_kronig(a, s, sn);
   int ln;
   const double e = d_line_check(P, a, u, sn, s, &ln);
-  if (e <= 1.0) return;
+  if (e <= 1.0) return 0;
   S.cflags[i] = 0;
   d_cascade_emit(P, S, d_offspring(P, S, i), e, ln, u, s);
+  return 1;
 }
 
+/* returns 1 when a cascade started an offspring photon (brute force) */
 template <bool BRUTE>
-__device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
+__device__ __forceinline__ int d_photo_one(const EngineParams &P, const PhotonSoA &S, int n, int i) {
   const size_t cap = (size_t)P.cap;
   int L = S.layer[i];
   int u = P.el_u[P.layer_off[L - 1] + S.elem[i]];
@@ -1898,15 +1903,14 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
   S.prev_line[i] = 0;
   if (shell < 0) {
     S.energy[i] = 0.0;
-    return;
+    return 0;
   }
   const double ry = phx_uniform(&s); /* unused by the reference's yield weighting */
   if (BRUTE) {
     /* the analogue yield, perhaps an Auger cascade (src/xmi_main.F90:2298-2315) */
     if (ry > P.fycorr[(size_t)u * 9 + shell]) {
       S.energy[i] = 0.0;
-      if (S.cflags[i] & 1) d_cascade_auger(P, S, i, u, a, shell, &s);
-      return;
+      return (S.cflags[i] & 1) ? d_cascade_auger(P, S, i, u, a, shell, &s) : 0;
     }
   } else {
     S.weight[i] *= P.fycorr[(size_t)u * 9 + shell];
@@ -1925,7 +1929,7 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
   }
   if (line == 0) {
     S.energy[i] = 0.0;
-    return;
+    return 0;
   }
   E = P.line_E[(size_t)u * XMI_NLINES_VR_DEV + (line - 1)];
   S.energy[i] = E;
@@ -1948,29 +1952,19 @@ __device__ __forceinline__ void d_photo_one(const EngineParams &P, const PhotonS
   S.phi[i] = ph;
   S.prev_line[i] = line;
   S.prev_u[i] = u;
-  if (BRUTE && (S.cflags[i] & 2)) d_cascade_radiative(P, S, i, u, a, shell, line, &s);
+  if (BRUTE && (S.cflags[i] & 2)) return d_cascade_radiative(P, S, i, u, a, shell, line, &s);
+  return 0;
 }
 
-/* one launch serves the three interaction queues: thread q takes entry q of
- * their concatenation (Rayleigh, Compton, photo; each queue's ENG_SPREAD
- * sub-queues in order), so the grid's empty waves are only those beyond the
- * photons that interact at this order, and a wavefront straddles two queues
- * at two places at most.  Each photon is one thread's, with its own random
- * stream, so results do not depend on the assignment. */
-/* occupancy: five waves per SIMD (96 VGPRs, a few spilled; 6 % faster than four).
- * ADV: the advanced Compton model, compiled apart so that the default
- * instance spills less (10 VGPRs instead of 28) */
-template <bool BRUTE, bool ADV>
-__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
-k_interact(EngineParams P, PhotonSoA S, int n) {
-  constexpr int NS = 3 * ENG_SPREAD;
-  __shared__ unsigned int off[NS + 1];
+/* exclusive prefix sums of the 3 x ENG_SPREAD sub-queue lengths of the
+ * interaction queues into off[0..3 ENG_SPREAD], off[3 ENG_SPREAD] = total
+ * (first wave of the workgroup; the caller synchronises) */
+__device__ __forceinline__ void d_queue_offsets(const unsigned long long *qc, unsigned int *off) {
   if (threadIdx.x < 64) {
-    /* exclusive prefix sums of the 3 x 64 sub-queue lengths */
     const int lane = threadIdx.x;
     unsigned int base = 0;
     for (int t = 0; t < 3; t++) {
-      const unsigned int c = (unsigned int)P.counters[(size_t)t * ENG_SPREAD + lane];
+      const unsigned int c = (unsigned int)qc[(size_t)t * ENG_SPREAD + lane];
       unsigned int x = c;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
@@ -1980,11 +1974,29 @@ k_interact(EngineParams P, PhotonSoA S, int n) {
       off[t * ENG_SPREAD + lane] = base + x - c;
       base += __shfl(x, 63);
     }
-    if (lane == 0) off[NS] = base;
+    if (lane == 0) off[3 * ENG_SPREAD] = base;
   }
-  __syncthreads();
-  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= off[NS]) return;
+}
+
+/* wavefront-aggregated append of the lanes with pred to a list (one atomic
+ * per wave; for rare entries such as a cascade's offspring) */
+__device__ __forceinline__ void d_list_append(int *list, unsigned int *cnt, int pred, int idx) {
+  const unsigned long long m = __ballot(pred);
+  if (!m) return;
+  const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
+  unsigned int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (unsigned int)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = idx;
+}
+
+/* entry q of the concatenated interaction queues (Rayleigh, Compton, photo;
+ * each queue's ENG_SPREAD sub-queues in order; off: their prefix sums):
+ * the photon's interaction.  Returns 1 when a cascade started an offspring */
+template <bool BRUTE, bool ADV>
+__device__ __forceinline__ int d_interact_entry(const EngineParams &P, const PhotonSoA &S, int n,
+                                                const unsigned int *off, unsigned int q, int *slot) {
+  constexpr int NS = 3 * ENG_SPREAD;
   int lo = 0, hi = NS;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -1993,9 +2005,50 @@ k_interact(EngineParams P, PhotonSoA S, int n) {
   }
   const int t = lo / ENG_SPREAD;
   const int i = P.queue[(size_t)lo * P.qreg + (q - off[lo])];
+  *slot = i;
   if (t == 0) d_rayleigh_one<BRUTE>(P, S, n, i);
   else if (t == 1) d_compton_one<BRUTE, ADV>(P, S, n, i);
-  else d_photo_one<BRUTE>(P, S, n, i);
+  else return d_photo_one<BRUTE>(P, S, n, i);
+  return 0;
+}
+
+/* one launch serves the three interaction queues: thread q takes entry q of
+ * their concatenation, so the grid's empty waves are only those beyond the
+ * photons that interact at this order, and a wavefront straddles two queues
+ * at two places at most.  Each photon is one thread's, with its own random
+ * stream, so results do not depend on the assignment. */
+/* occupancy: five waves per SIMD (96 VGPRs, a few spilled; 6 % faster than four).
+ * ADV: the advanced Compton model, compiled apart so that the default
+ * instance spills less (10 VGPRs instead of 28) */
+template <bool ADV>
+__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
+k_interact(EngineParams P, PhotonSoA S, int n) {
+  __shared__ unsigned int off[3 * ENG_SPREAD + 1];
+  d_queue_offsets(P.qcount, off);
+  __syncthreads();
+  const unsigned int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= off[3 * ENG_SPREAD]) return;
+  int i;
+  (void)d_interact_entry<false, ADV>(P, S, n, off, q, &i);
+}
+
+/* the brute-force interactions (analogue yields, cascades), one thread per
+ * queue entry as in k_interact; the offspring that cascades start join the
+ * offspring list.  (A fixed grid striding over the entries would keep the
+ * kernel's many table pointers live across the loop: at five waves per SIMD
+ * that spills 200 VGPRs instead of about 50.)  Four waves per SIMD run 3 %
+ * faster than five here. */
+template <bool ADV>
+__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_interact_brute(EngineParams P, PhotonSoA S, int n) {
+  __shared__ unsigned int off[3 * ENG_SPREAD + 1];
+  d_queue_offsets(P.qcount, off);
+  __syncthreads();
+  const unsigned int total = off[3 * ENG_SPREAD], q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x * blockDim.x >= total) return;
+  int i = -1, spawned = 0;
+  if (q < total) spawned = d_interact_entry<true, ADV>(P, S, n, off, q, &i);
+  d_list_append(P.olist_out, P.ocount_out, spawned, i + P.off_half);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2212,110 +2265,144 @@ __device__ int d_escape_hits_detector(const EngineParams &P, const double *coord
   return d_norm(ip) <= P.coll_radius;
 }
 
-/* order n of a brute-force run over 2 * count threads when cascades are on
- * (slot t < count: primary t; otherwise offspring slot off_half + t - count),
- * count threads otherwise.  A photon alive after n - 1 interactions takes
- * its analogue step; one that stays in the sample with fewer than n_int
- * interactions gets its atom and interaction type and joins the queues, the
- * others end, those that reach the detector tallied */
-__global__ void __launch_bounds__(ENG_BLOCK) k_step_brute(EngineParams P, PhotonSoA S, int n, int count) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t < count ? t : P.off_half + (t - count);
-  const bool valid = t < count || (P.off_half && t < 2 * count);
-  int type = -1, stepped = 0, reached = 0;
-  long key_ch = -1, key_h = -1;
-  double w_hit = 0.0;
-  if (valid && S.layer[i] > 0) {
-    const size_t cap = (size_t)P.cap;
-    const double E = S.energy[i];
-    if (E < 1.0) {
-      S.layer[i] = 0;
-    } else {
-      stepped = 1;
-      phx_stream s;
-      phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
-      const double R = phx_uniform(&s);
-      const double dirv[3] = {S.dx[i], S.dy[i], S.dz[i]};
-      double c[3] = {S.x[i], S.y[i], S.z[i]};
-      const int cur = S.layer[i];
-      const int up = d_dot(dirv, P.n_sample) > 0.0;
-      const int last = up ? P.n_layers : 1, dir = up ? 1 : -1;
-      double max_r = 0.0, min_r = 0.0, blbs = 1.0;
-      int inside = 0, hit = 0, ended = 0;
-      for (int l = cur; up ? l <= last : l >= last; l += dir) {
-        const int l0 = l - 1;
-        const double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]};
-        double ip[3];
-        d_intersect(pp, P.n_sample, c, dirv, ip);
-        const double mu = S.mus[(size_t)l0 * cap + i];
-        double dist = d_distance(c, ip);
-        const double temp_prod = -1.0 * dist * P.layer_density[l0] * mu;
-        const double tempexp = exp(temp_prod);
-        min_r = max_r;
-        max_r = max_r - blbs * expm1(temp_prod);
-        if (R <= max_r) {
-          dist = -1.0 * log1p(-1.0 * (R - min_r) / blbs) / mu / P.layer_density[l0];
-          const double to[3] = {c[0] + dist * dirv[0], c[1] + dist * dirv[1], c[2] + dist * dirv[2]};
-          const int rc = d_detector_segment(P, c, to);
-          c[0] = to[0];
-          c[1] = to[1];
-          c[2] = to[2];
-          if (rc) {
-            ended = 1;
-            hit = rc == 2;
-          } else {
-            inside = l;
-          }
-          break;
-        }
-        const int rc = d_detector_segment(P, c, ip);
-        if (rc) {
-          ended = 1;
-          hit = rc == 2;
-          break;
-        }
-        c[0] = ip[0];
-        c[1] = ip[1];
-        c[2] = ip[2];
-        blbs = blbs * tempexp;
-      }
-      if (!inside && !ended) hit = d_escape_hits_detector(P, c, dirv);
-      if (hit) {
-        /* channels(n_interactions:, channel) and the history row of the last
-         * interaction; the host sums the orders into the cumulative rows */
-        reached = 1;
-        w_hit = S.weight[i];
-        if (E >= 1.0) {
-          const int ch = (int)((E - P.zero) / P.gain);
-          if (ch >= 0 && ch < P.nch) key_ch = (long)(n - 1) * P.nch + ch;
-        }
-        if (n >= 2 && S.prev_line[i] > 0 && S.prev_u[i] >= 0)
-          key_h = ((long)(n - 2) * P.nZ + S.prev_u[i]) * 385 + (S.prev_line[i] - 1);
-      }
-      if (!inside || n > P.n_int) {
-        S.layer[i] = 0;
-      } else {
-        S.x[i] = c[0];
-        S.y[i] = c[1];
-        S.z[i] = c[2];
-        S.layer[i] = inside;
-        type = d_select(P, S, i, E, inside, &s);
-      }
-    }
+/* the analogue step of photon i at order n: a photon alive after n - 1
+ * interactions takes its step; one that stays in the sample with fewer than
+ * n_int interactions gets its atom and interaction type (*type), the others
+ * end, those that reach the detector tallied (*key_ch, *key_h, *w_hit) */
+__device__ __forceinline__ void d_brute_step(const EngineParams &P, const PhotonSoA &S, int n, int i, int *type,
+                                             int *stepped, int *reached, long *key_ch, long *key_h, double *w_hit) {
+  const size_t cap = (size_t)P.cap;
+  const double E = S.energy[i];
+  if (E < 1.0) {
+    S.layer[i] = 0;
+    return;
   }
-  d_enqueue(P, type, i);
-  d_wave_count(d_ctr(P, ENG_CNT_STEPS), stepped);
-  d_wave_count(d_ctr(P, ENG_CNT_HITS), reached);
-  d_fx_add_keyed(P.t_bch, key_ch, w_hit, P.tally_shift);
-  d_fx_add_keyed(P.t_bhist, key_h, w_hit, P.tally_shift);
+  *stepped = 1;
+  phx_stream s;
+  phx_stream_init(&s, P.seed, P.sample, (uint32_t)n, d_tr_sub(P, i), S.hid[i]);
+  const double R = phx_uniform(&s);
+  const double dirv[3] = {S.dx[i], S.dy[i], S.dz[i]};
+  double c[3] = {S.x[i], S.y[i], S.z[i]};
+  const int cur = S.layer[i];
+  const int up = d_dot(dirv, P.n_sample) > 0.0;
+  const int last = up ? P.n_layers : 1, dir = up ? 1 : -1;
+  double max_r = 0.0, min_r = 0.0, blbs = 1.0;
+  int inside = 0, hit = 0, ended = 0;
+  for (int l = cur; up ? l <= last : l >= last; l += dir) {
+    const int l0 = l - 1;
+    const double pp[3] = {0.0, 0.0, up ? P.layer_ze[l0] : P.layer_zb[l0]};
+    double ip[3];
+    d_intersect(pp, P.n_sample, c, dirv, ip);
+    const double mu = S.mus[(size_t)l0 * cap + i];
+    double dist = d_distance(c, ip);
+    const double temp_prod = -1.0 * dist * P.layer_density[l0] * mu;
+    const double tempexp = exp(temp_prod);
+    min_r = max_r;
+    max_r = max_r - blbs * expm1(temp_prod);
+    if (R <= max_r) {
+      dist = -1.0 * log1p(-1.0 * (R - min_r) / blbs) / mu / P.layer_density[l0];
+      const double to[3] = {c[0] + dist * dirv[0], c[1] + dist * dirv[1], c[2] + dist * dirv[2]};
+      const int rc = d_detector_segment(P, c, to);
+      c[0] = to[0];
+      c[1] = to[1];
+      c[2] = to[2];
+      if (rc) {
+        ended = 1;
+        hit = rc == 2;
+      } else {
+        inside = l;
+      }
+      break;
+    }
+    const int rc = d_detector_segment(P, c, ip);
+    if (rc) {
+      ended = 1;
+      hit = rc == 2;
+      break;
+    }
+    c[0] = ip[0];
+    c[1] = ip[1];
+    c[2] = ip[2];
+    blbs = blbs * tempexp;
+  }
+  if (!inside && !ended) hit = d_escape_hits_detector(P, c, dirv);
+  if (hit) {
+    /* channels(n_interactions:, channel) and the history row of the last
+     * interaction; the host sums the orders into the cumulative rows */
+    *reached = 1;
+    *w_hit = S.weight[i];
+    const int ch = (int)((E - P.zero) / P.gain);
+    if (ch >= 0 && ch < P.nch) *key_ch = (long)(n - 1) * P.nch + ch;
+    if (n >= 2 && S.prev_line[i] > 0 && S.prev_u[i] >= 0)
+      *key_h = ((long)(n - 2) * P.nZ + S.prev_u[i]) * 385 + (S.prev_line[i] - 1);
+  }
+  if (!inside || n > P.n_int) {
+    S.layer[i] = 0;
+  } else {
+    S.x[i] = c[0];
+    S.y[i] = c[1];
+    S.z[i] = c[2];
+    S.layer[i] = inside;
+    *type = d_select(P, S, i, E, inside, &s);
+  }
 }
 
-template <bool BRUTE>
+/* order n of a brute-force run.  The photons alive at its start are the
+ * batch's primaries at n = 1; afterwards the entries of order n - 1's
+ * interaction queues (those that interacted) followed by the offspring that
+ * its cascades started.  A fixed grid strides over them, every lane of a
+ * wave taking the same trips, so that the queue appends, counters and
+ * tallies see converged waves.  Three waves per SIMD (166 VGPRs, 8 spilled)
+ * run 17 % faster than the two that 183 unconstrained VGPRs allow. */
+__global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+k_step_brute(EngineParams P, PhotonSoA S, int n, int count) {
+  constexpr int NS = 3 * ENG_SPREAD;
+  __shared__ unsigned int off[NS + 1];
+  unsigned int nq = 0, total = (unsigned int)count;
+  if (n >= 2) {
+    d_queue_offsets(P.pqcount, off);
+    __syncthreads();
+    nq = off[NS];
+    total = nq + (P.off_half ? *P.ocount_in : 0u);
+  }
+  const unsigned int stride = gridDim.x * blockDim.x;
+  for (unsigned int base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    const unsigned int q = base + threadIdx.x;
+    int type = -1, stepped = 0, reached = 0, i = -1;
+    long key_ch = -1, key_h = -1;
+    double w_hit = 0.0;
+    if (q < total) {
+      if (n == 1) {
+        i = (int)q;
+      } else if (q < nq) {
+        int lo = 0, hi = NS;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (off[mid] <= q) lo = mid;
+          else hi = mid;
+        }
+        i = P.pqueue[(size_t)lo * P.qreg + (q - off[lo])];
+      } else {
+        i = P.olist_in[q - nq];
+      }
+      if (S.layer[i] > 0) d_brute_step(P, S, n, i, &type, &stepped, &reached, &key_ch, &key_h, &w_hit);
+    }
+    d_enqueue(P, type, i);
+    d_wave_count(d_ctr(P, ENG_CNT_STEPS), stepped);
+    d_wave_count(d_ctr(P, ENG_CNT_HITS), reached);
+    d_fx_add_keyed(P.t_bch, key_ch, w_hit, P.tally_shift);
+    d_fx_add_keyed(P.t_bhist, key_h, w_hit, P.tally_shift);
+  }
+}
+
 static void launch_interact(const EngineParams *P, const PhotonSoA *S, int n, int blocks, hipStream_t st) {
-  if (P->adv_compton)
-    hipLaunchKernelGGL((k_interact<BRUTE, true>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
-  else
-    hipLaunchKernelGGL((k_interact<BRUTE, false>), dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  if (P->adv_compton) hipLaunchKernelGGL(k_interact<true>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  else hipLaunchKernelGGL(k_interact<false>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+}
+
+static void launch_interact_brute(const EngineParams *P, const PhotonSoA *S, int n, int blocks, hipStream_t st) {
+  if (P->adv_compton) hipLaunchKernelGGL(k_interact_brute<true>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
+  else hipLaunchKernelGGL(k_interact_brute<false>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
 }
 
 /* photons per lane of k_vr_lines: each line's wavefront sum then serves
@@ -2414,24 +2501,31 @@ hipError_t eng_launch_escape(const EngineParams *P, const PhotonSoA *S, const Vr
   }
   hipError_t e = hipMemcpyAsync(V->pesc, S->weight, sizeof(double) * (size_t)count, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
-  launch_interact<false>(P, S, 1, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
+  launch_interact(P, S, 1, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
   const int eblocks = (int)((3 * (size_t)ENG_SPREAD * P->qreg + ENG_BLOCK - 1) / ENG_BLOCK);
   hipLaunchKernelGGL(k_escape, dim3(eblocks), dim3(ENG_BLOCK), 0, st, *P, *S, *V, *Q);
   return hipGetLastError();
 }
 
-hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, hipStream_t st,
-                                  hipEvent_t *ev) {
-  /* primaries and, with cascades, their offspring slots */
-  const int threads = P->off_half ? 2 * count : count;
-  const int blocks = (threads + ENG_BLOCK - 1) / ENG_BLOCK;
+hipError_t eng_launch_order_brute(const EngineParams *P, const PhotonSoA *S, int n, int count, int blocks,
+                                  hipStream_t st, hipEvent_t *ev) {
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(ENG_NQUEUES * ENG_SPREAD), 0, st, *P);
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_step_brute, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n, count);
   if (ev) (void)hipEventRecord(ev[1], st);
-  if (n <= P->n_int) launch_interact<true>(P, S, n, blocks, st);
+  /* the interactions: one thread per photon that may be alive */
+  const int alive = (n == 1 || !P->off_half) ? count : 2 * count;
+  if (n <= P->n_int) launch_interact_brute(P, S, n, (alive + ENG_BLOCK - 1) / ENG_BLOCK, st);
   if (ev) (void)hipEventRecord(ev[2], st);
   return hipGetLastError();
+}
+
+int eng_brute_blocks(int n_cu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_step_brute, ENG_BLOCK, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return n_cu * per_cu;
 }
 
 hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int count, int n_cu,
@@ -2454,7 +2548,7 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
   }
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
-  launch_interact<false>(P, S, n, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
+  launch_interact(P, S, n, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
   if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
