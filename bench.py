@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--grid-hits", type=int, default=5000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default="", metavar="K=V,...",
+                    help="launch shape (xmi_msim_set_launch): batch_log2, use_graph, lines_kp, wg_per_cu")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch shape with xmimsim_amd.autotune on a sample of the workload first")
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (a reference kernel of known traffic for the "
@@ -178,6 +182,24 @@ def _measured_traffic(workload):
         return {k: v["bytes_per_launch"] for k, v in json.load(f)["kernels"].items()}
 
 
+def apply_launch(args, prep=None):
+    """Sets the launch shape of --launch, or the autotuner's for prep's
+    workload with --autotune (outside the timed region); returns the shape
+    and the tuning record."""
+    from xmimsim_amd import api, autotune
+    shape, tuned = dict(autotune.DEFAULT), None
+    for kv in filter(None, args.launch.split(",")):
+        k, v = kv.split("=")
+        shape[k.strip()] = int(v)
+    if args.autotune and prep is not None:
+        t = time.time()
+        shape, table = autotune.tune(prep, cache=None, seed=args.seed)
+        tuned = {"seconds": round(time.time() - t, 2), "timed": table}
+        print(f"[bench] autotune: {shape} in {tuned['seconds']} s", file=sys.stderr, flush=True)
+    api.set_launch(**shape)
+    return shape, tuned
+
+
 def batch(args):
     """Independent samples (1-5 layers, 2-30 elements, budgets 1e5-1e7, one
     line each) through xmi_main_msim_batch: aggregate histories/s and the
@@ -204,6 +226,8 @@ def batch(args):
         preps.append(p)
     hist = sum(inp.n_photons_line for inp in inps)
     api.run_batch(preps[:8], tables, seed=args.seed, n_streams=args.streams)  # warm-up: engines, table upload
+    # one shape for every sample: the autotuner's for the largest budget
+    launch, tuned = apply_launch(args, max(preps, key=lambda p: p.inp.n_photons_line))
     runs = {}
     for ns in (1, args.streams):
         t = time.perf_counter()
@@ -232,7 +256,8 @@ def batch(args):
         "dtype": "f64",
         "data": "synthetic (seeded independent samples)",
         "config": {"workload": "batch", "samples": len(inps), "histories": int(hist), "streams": args.streams,
-                   "elements": len(Zs), "solid_angle_grid": f"shared, 1024x1024x{args.grid_hits}"},
+                   "elements": len(Zs), "solid_angle_grid": f"shared, 1024x1024x{args.grid_hits}",
+                   "launch": launch, "autotune": tuned},
         "per_sample_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"]},
         "one_stream": runs[1],
     }
@@ -344,6 +369,7 @@ def brute(args):
     inp = make_input(argparse.Namespace(**{**vars(args), "workload": "three_layer_kramers"}))
     opts = api.default_options(use_variance_reduction=0)
     prep = api.Prepared(inp, options=opts)
+    launch, tuned = apply_launch(args, prep)
     prof = None
     for w in range(args.warmup):
         lib.xmi_msim_set_profiling(1 if w == 0 else 0)
@@ -399,7 +425,7 @@ def brute(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded sample composition and spectrum)",
         "config": {"workload": "brute", "sample": "three_layer_kramers", "photons_per_interval_and_line": int(args.photons),
                    "histories_per_step": int(hist / max(args.steps, 1)), "n_interactions": inp.n_interactions_trajectory,
-                   "cascades": "auger+radiative"},
+                   "cascades": "auger+radiative", "launch": launch, "autotune": tuned},
         "roofline": roofline, "cpu_baseline": cpu}))
 
 
@@ -513,6 +539,7 @@ def main():
     prep.compute_solid_angle(hits_per_single=args.grid_hits, seed=args.seed)
     grid_s = time.time() - t
     print(f"[bench] solid-angle grid {grid_s * 1e3:.1f} ms ({args.grid_hits} trials/cell)", file=sys.stderr, flush=True)
+    launch, tuned = apply_launch(args, prep)
 
     prof = None
     for w in range(args.warmup):
@@ -598,7 +625,8 @@ def main():
         "config": {"workload": args.workload, "photons_per_interval_and_line": int(args.photons),
                    "histories_per_step": int(hist / max(args.steps, 1)), "n_interactions": inp.n_interactions_trajectory,
                    "channels": inp.nchannels, "solid_angle_grid_ms": round(grid_s * 1e3, 1),
-                   "device_ms_per_step": round(dev_ms / max(args.steps, 1), 2)},
+                   "device_ms_per_step": round(dev_ms / max(args.steps, 1), 2), "launch": launch,
+                   "autotune": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -308,12 +308,32 @@ typedef struct _xmi_msim_stats {
   int64_t grid_uploads;       /* solid-angle grid copies to the device (0 when the resident copy was current) */
   int64_t photon_steps;       /* brute force: analogue steps taken (every order, offspring included) */
   int64_t detector_hits;      /* brute force: photons that reached the detector */
+  int64_t batches;            /* batches of the run order simulated */
+  int64_t graph_replays;      /* of them, replays of the run's captured graph (0: launched one by one) */
 } xmi_msim_stats;
 void xmi_msim_get_stats(xmi_msim_stats *out);
 
 /* Times every kernel class with HIP events (synchronising once per batch)
  * when on; used by the benchmark to measure the detection-weight kernel. */
 void xmi_msim_set_profiling(int on);
+
+/* Launch shape of the transport (no counterpart in the reference, whose
+ * OpenMP loop has no launch shape; for callers that tune the engine to a
+ * workload, such as xmimsim_amd.autotune).  Results do not depend on
+ * batch_log2 or use_graph beyond the last bits of the estimator's sums;
+ * lines_kp and wg_per_cu change only the order of those sums.  Zero fields
+ * take the defaults.  Applies to the calls that start after it; not to be
+ * changed while a call runs. */
+typedef struct _xmi_msim_launch {
+  int batch_log2; /* histories per batch 2^batch_log2, 12..21 (default 21) */
+  int use_graph;  /* 1: capture one batch's launches in a HIP graph, replayed per batch; each replay pops its
+                     batch from a device batch queue */
+  int lines_kp;   /* photons per lane of the estimator's line kernel: 2..4 up to 4 layers (default 3),
+                     1..2 up to 8 (default 2), 1 beyond */
+  int wg_per_cu;  /* resident workgroups per CU of the estimator's grid-stride kernels (default: occupancy) */
+} xmi_msim_launch;
+int xmi_msim_set_launch(const xmi_msim_launch *launch); /* 0 and an error message for a shape out of range */
+void xmi_msim_get_launch(xmi_msim_launch *launch);
 
 /* Detection-weight evaluations per layer of the last xmi_main_msim call on
  * this thread; writes min(n, 16) values, returns how many. */

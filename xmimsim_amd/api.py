@@ -102,7 +102,12 @@ class xmi_msim_stats(C.Structure):
                 ("step_kernel_ms", C.c_double), ("interaction_kernel_ms", C.c_double),
                 ("solid_angle_fallbacks", C.c_int64), ("vr_geom_ms", C.c_double), ("vr_cone_ms", C.c_double),
                 ("vr_scatter_ms", C.c_double), ("vr_lines_ms", C.c_double),
-                ("grid_uploads", C.c_int64), ("photon_steps", C.c_int64), ("detector_hits", C.c_int64)]
+                ("grid_uploads", C.c_int64), ("photon_steps", C.c_int64), ("detector_hits", C.c_int64),
+                ("batches", C.c_int64), ("graph_replays", C.c_int64)]
+
+
+class xmi_msim_launch(C.Structure):
+    _fields_ = [("batch_log2", C.c_int), ("use_graph", C.c_int), ("lines_kp", C.c_int), ("wg_per_cu", C.c_int)]
 
 
 class xmi_error(C.Structure):
@@ -140,7 +145,7 @@ EXPORTED_SYMBOLS = [
     "xmi_init_from_hdf5", "xmi_init_from_provider", "xmi_update_input_from_hdf5", "xmi_free_hdf5_F",
     "xmi_solid_angle_calculation_cl", "xmi_free_solid_angle", "xmi_hits_per_single", "xmi_main_msim",
     "xmi_free", "xmi_set_seed", "xmi_msim_get_stats", "xmi_msim_last_error", "xmi_msim_set_profiling",
-    "xmi_msim_get_layer_events", "xmi_provider_active_lines", "xmi_detector_convolute_all",
+    "xmi_msim_get_layer_events", "xmi_msim_set_launch", "xmi_msim_get_launch", "xmi_provider_active_lines", "xmi_detector_convolute_all",
     "xmi_detector_convolute_all_custom", "xmi_surrogate_create", "xmi_surrogate_train_step", "xmi_main_msim_batch",
     "xmi_tables_set_device", "xmi_tables_copy", "xmi_escape_ratios_calculation", "xmi_free_escape_ratios",
     "xmi_get_default_escape_ratios_options", "xmi_input_read_from_xml_file", "xmi_input_read_from_xml_string",
@@ -202,6 +207,10 @@ def load_library(path: str = LIB_PATH):
     lib.xmi_msim_set_profiling.argtypes = [C.c_int]
     lib.xmi_msim_get_layer_events.argtypes = [P(C.c_int64), C.c_int]
     lib.xmi_msim_get_layer_events.restype = C.c_int
+    lib.xmi_msim_set_launch.argtypes = [P(xmi_msim_launch)]
+    lib.xmi_msim_set_launch.restype = C.c_int
+    lib.xmi_msim_get_launch.argtypes = [P(xmi_msim_launch)]
+    lib.xmi_msim_get_launch.restype = None
     lib.xmi_provider_active_lines.argtypes = [C.c_int, C.c_int]
     lib.xmi_provider_active_lines.restype = C.c_int
     lib.xmi_provider_dcs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
@@ -250,6 +259,23 @@ def get_stats() -> "xmi_msim_stats":
     st = xmi_msim_stats()
     lib.xmi_msim_get_stats(C.byref(st))
     return st
+
+
+def set_launch(batch_log2: int = 0, use_graph: int = 0, lines_kp: int = 0, wg_per_cu: int = 0):
+    """Launch shape of the following engine calls (xmi_msim_set_launch);
+    zeros take the defaults.  Raises ValueError for a shape out of range."""
+    lib = load_library()
+    L = xmi_msim_launch(int(batch_log2), int(use_graph), int(lines_kp), int(wg_per_cu))
+    if not lib.xmi_msim_set_launch(C.byref(L)):
+        raise ValueError(last_error())
+
+
+def get_launch() -> dict:
+    """The launch shape in force, as keyword arguments of set_launch."""
+    lib = load_library()
+    L = xmi_msim_launch()
+    lib.xmi_msim_get_launch(C.byref(L))
+    return {f: getattr(L, f) for f, _ in xmi_msim_launch._fields_}
 
 
 def last_error() -> str:

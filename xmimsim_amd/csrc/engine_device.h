@@ -186,6 +186,8 @@ struct EngineParams {
   int *queue;                   /* [ENG_NQUEUES][ENG_SPREAD][qreg] */
   unsigned long long *qcount;   /* sizes of queue's sub-queues [ENG_NQUEUES][ENG_SPREAD]: counters' first slots,
                                    or (brute force) those of the queue set of this order */
+  int lines_kp;                 /* launch shape (host only): photons per lane of the lines kernel, 0 = default */
+  int wg_per_cu;                /* launch shape (host only): estimator workgroups per CU, 0 = occupancy */
   int qreg;                     /* entries per sub-queue */
   /* brute force: the photons alive at the start of order n are those of
    * order n - 1's interaction queues (pqueue, pqcount: the other queue set)

@@ -37,6 +37,31 @@ static int g_seed_set = 0;
 static int g_profile = 0;
 
 extern "C" void xmi_msim_set_profiling(int on) { g_profile = on; }
+
+static xmi_msim_launch g_launch = {0, 0, 0, 0};
+
+extern "C" int xmi_msim_set_launch(const xmi_msim_launch *l) {
+  if (!l) {
+    xmi_set_error("xmi_msim_set_launch: NULL argument");
+    return 0;
+  }
+  if (l->batch_log2 && (l->batch_log2 < 12 || l->batch_log2 > 21)) {
+    xmi_set_error("xmi_msim_set_launch: batch_log2 %d outside 12..21", l->batch_log2);
+    return 0;
+  }
+  if (l->lines_kp < 0 || l->lines_kp > 4 || l->wg_per_cu < 0 || l->wg_per_cu > 32 || l->use_graph < 0 ||
+      l->use_graph > 1) {
+    xmi_set_error("xmi_msim_set_launch: lines_kp %d, wg_per_cu %d or use_graph %d out of range", l->lines_kp,
+                  l->wg_per_cu, l->use_graph);
+    return 0;
+  }
+  g_launch = *l;
+  return 1;
+}
+
+extern "C" void xmi_msim_get_launch(xmi_msim_launch *l) {
+  if (l) *l = g_launch;
+}
 static thread_local xmi_msim_stats g_stats;
 static thread_local int64_t g_layer_events[ENG_MAX_LAYERS];
 
@@ -78,6 +103,11 @@ extern "C" void xmi_msim_get_stats(xmi_msim_stats *out) {
 /* ------------------------------------------------------------------------ */
 /* device allocations */
 
+/* held while a stream captures a graph and around every device allocation
+ * and release of the engines: HIP refuses those (they synchronise the
+ * device) while another thread's stream is capturing */
+static std::mutex g_capture_lock;
+
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
@@ -86,6 +116,7 @@ struct DevBuf {
   }
   hipError_t ensure(size_t bytes) {
     if (bytes <= n && p) return hipSuccess;
+    std::lock_guard<std::mutex> g(g_capture_lock);
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
@@ -99,12 +130,14 @@ struct DevBuf {
   }
 };
 
+/* a copy on the caller's stream, complete on return (not on the legacy
+ * stream, which would wait for streams that may be capturing) */
 template <typename T>
-static hipError_t upload(DevBuf &b, const T *src, size_t count) {
+static hipError_t upload(DevBuf &b, const T *src, size_t count, hipStream_t st) {
   hipError_t e = b.ensure(sizeof(T) * count);
-  if (e != hipSuccess) return e;
-  if (count) return hipMemcpy(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice);
-  return hipSuccess;
+  if (e != hipSuccess || !count) return e;
+  e = hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, st);
+  return e == hipSuccess ? hipStreamSynchronize(st) : e;
 }
 
 /* tables of one xmi_hdf5F handle on the device */
@@ -149,6 +182,7 @@ struct Engine {
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, lay_segbase, cone_acc, esc_tally;
   DevBuf queue2, qcount2, olist;
+  DevBuf batch_queue;
 };
 
 static Engine g_engine;
@@ -171,7 +205,7 @@ static int engine_init() { return engine_init(g_engine); }
 
 static double uniform_inv(const double *a, int n) { return (double)(n - 1) / (a[n - 1] - a[0]); }
 
-static DeviceTables *get_tables(const xmi_hdf5F *h) {
+static DeviceTables *get_tables(const xmi_hdf5F *h, hipStream_t st) {
   std::lock_guard<std::mutex> g(g_tables_lock);
   for (DeviceTables *t : g_tables)
     if (t->owner == h) return t;
@@ -247,26 +281,26 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
   t->phi_R0 = h->RandomNumbers[0];
   t->phi_Rinv = uniform_inv(h->RandomNumbers, h->phi_n_r);
   hipError_t e = hipSuccess;
-  e = upload(t->atoms, atoms.data(), atoms.size());
-  if (e == hipSuccess) e = upload(t->ray, ray.data(), ray.size());
-  if (e == hipSuccess) e = upload(t->comp, comp.data(), comp.size());
-  if (e == hipSuccess) e = upload(t->tabE, z0->Energies, (size_t)t->n_e);
-  if (e == hipSuccess) e = upload(t->tabR, z0->RandomNumbers, (size_t)t->n_r);
-  if (e == hipSuccess) e = upload(t->fycorr, fy.data(), fy.size());
-  if (e == hipSuccess) e = upload(t->ipE, ipE.data(), ipE.size());
-  if (e == hipSuccess) e = upload(t->ipr, ipr.data(), ipr.size());
-  if (e == hipSuccess) e = upload(t->iprc, iprc.data(), iprc.size());
-  if (e == hipSuccess) e = upload(t->ipoff, ipoff.data(), ipoff.size());
-  if (e == hipSuccess) e = upload(t->ipacc, ipacc.data(), ipacc.size());
-  if (e == hipSuccess) e = upload(t->cp_icdf, cp.data(), cp.size());
-  if (e == hipSuccess) e = upload(t->cp_rn, z0->cp_random_numbers, (size_t)t->cp_len);
-  if (e == hipSuccess) e = upload(t->cp_cdfg, cdfg.data(), cdfg.size());
-  if (e == hipSuccess) e = upload(t->cp_gscale, gscale.data(), gscale.size());
-  if (e == hipSuccess) e = upload(t->precalc_xrf, pxrf.data(), pxrf.size());
-  if (e == hipSuccess) e = upload(t->lineE, lineE.data(), lineE.size());
-  if (e == hipSuccess) e = upload(t->phi, h->Phi_ICDF, (size_t)h->phi_n_theta * h->phi_n_r);
-  if (e == hipSuccess) e = upload(t->phiT, h->Thetas, (size_t)h->phi_n_theta);
-  if (e == hipSuccess) e = upload(t->phiR, h->RandomNumbers, (size_t)h->phi_n_r);
+  e = upload(t->atoms, atoms.data(), atoms.size(), st);
+  if (e == hipSuccess) e = upload(t->ray, ray.data(), ray.size(), st);
+  if (e == hipSuccess) e = upload(t->comp, comp.data(), comp.size(), st);
+  if (e == hipSuccess) e = upload(t->tabE, z0->Energies, (size_t)t->n_e, st);
+  if (e == hipSuccess) e = upload(t->tabR, z0->RandomNumbers, (size_t)t->n_r, st);
+  if (e == hipSuccess) e = upload(t->fycorr, fy.data(), fy.size(), st);
+  if (e == hipSuccess) e = upload(t->ipE, ipE.data(), ipE.size(), st);
+  if (e == hipSuccess) e = upload(t->ipr, ipr.data(), ipr.size(), st);
+  if (e == hipSuccess) e = upload(t->iprc, iprc.data(), iprc.size(), st);
+  if (e == hipSuccess) e = upload(t->ipoff, ipoff.data(), ipoff.size(), st);
+  if (e == hipSuccess) e = upload(t->ipacc, ipacc.data(), ipacc.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_icdf, cp.data(), cp.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_rn, z0->cp_random_numbers, (size_t)t->cp_len, st);
+  if (e == hipSuccess) e = upload(t->cp_cdfg, cdfg.data(), cdfg.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_gscale, gscale.data(), gscale.size(), st);
+  if (e == hipSuccess) e = upload(t->precalc_xrf, pxrf.data(), pxrf.size(), st);
+  if (e == hipSuccess) e = upload(t->lineE, lineE.data(), lineE.size(), st);
+  if (e == hipSuccess) e = upload(t->phi, h->Phi_ICDF, (size_t)h->phi_n_theta * h->phi_n_r, st);
+  if (e == hipSuccess) e = upload(t->phiT, h->Thetas, (size_t)h->phi_n_theta, st);
+  if (e == hipSuccess) e = upload(t->phiR, h->RandomNumbers, (size_t)h->phi_n_r, st);
   if (e != hipSuccess) {
     xmi_set_error("table upload failed: %s", hipGetErrorString(e));
     delete t;
@@ -278,7 +312,7 @@ static DeviceTables *get_tables(const xmi_hdf5F *h) {
 
 /* the partial Compton profiles of every element, flattened over the
  * occupied subshells (advanced Compton model only) */
-static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h) {
+static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h, hipStream_t st) {
   std::lock_guard<std::mutex> g(g_tables_lock);
   if (t->adv_ready) return 1;
   const int nZ = h->n_Z, N = t->cp_len;
@@ -296,12 +330,12 @@ static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h) {
     qinv.insert(qinv.end(), z->cp_Qs_inv, z->cp_Qs_inv + (size_t)z->cp_nshells * N);
   }
   off[nZ] = (int)idx.size();
-  hipError_t e = upload(t->cp_shoff, off.data(), off.size());
-  if (e == hipSuccess) e = upload(t->cp_shidx, idx.data(), idx.size());
-  if (e == hipSuccess) e = upload(t->cp_pcdf, pcdf.data(), pcdf.size());
-  if (e == hipSuccess) e = upload(t->cp_qinv, qinv.data(), qinv.size());
-  if (e == hipSuccess) e = upload(t->cp_Qs, h->Zs[0].cp_Qs, (size_t)N);
-  if (e == hipSuccess) e = upload(t->cp_inv, h->Zs[0].cp_partial_cdf_inv, (size_t)N);
+  hipError_t e = upload(t->cp_shoff, off.data(), off.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_shidx, idx.data(), idx.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_pcdf, pcdf.data(), pcdf.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_qinv, qinv.data(), qinv.size(), st);
+  if (e == hipSuccess) e = upload(t->cp_Qs, h->Zs[0].cp_Qs, (size_t)N, st);
+  if (e == hipSuccess) e = upload(t->cp_inv, h->Zs[0].cp_partial_cdf_inv, (size_t)N, st);
   if (e != hipSuccess) {
     xmi_set_error("advanced Compton table upload failed: %s", hipGetErrorString(e));
     return 0;
@@ -353,9 +387,9 @@ static int ensure_grid(Engine &E, const xmi_solid_angle *sa) {
   uint64_t hsh = grid_hash(sa);
   if (E.sa_key == sa && E.sa_hash == hsh) return 1;
   size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
-  if (upload(E.sa_grid, sa->solid_angles, tot) != hipSuccess ||
-      upload(E.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n) != hipSuccess ||
-      upload(E.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n) != hipSuccess) {
+  if (upload(E.sa_grid, sa->solid_angles, tot, E.stream) != hipSuccess ||
+      upload(E.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n, E.stream) != hipSuccess ||
+      upload(E.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n, E.stream) != hipSuccess) {
     xmi_set_error("solid-angle grid upload failed");
     return 0;
   }
@@ -458,9 +492,9 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   }
   if (n_mpi_hosts < 1) n_mpi_hosts = 1;
   if (!engine_init(E)) return 0;
-  DeviceTables *T = get_tables(h);
+  DeviceTables *T = get_tables(h, E.stream);
   if (!T) return 0;
-  if (options->use_advanced_compton && !ensure_adv_tables(T, h)) return 0;
+  if (options->use_advanced_compton && !ensure_adv_tables(T, h, E.stream)) return 0;
   if (!esc && !brute && !T->has_precalc) {
     xmi_set_error("xmi_main_msim: the tables were read without the cross sections of the variance reduction "
                   "(xmi_init_from_hdf5 with use_variance_reduction=0)");
@@ -475,6 +509,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   const int NL = in->n_layers;
   double *ch_out = nullptr, *vr_out = nullptr, *bf_out = nullptr;
   hipEvent_t evs[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   int ok = 0;
 
   EngineParams P;
@@ -769,6 +805,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   if (shift > 900) shift = 900;
   if (shift < -900) shift = -900;
   P.tally_shift = shift;
+  P.lines_kp = g_launch.lines_kp;
+  P.wg_per_cu = g_launch.wg_per_cu;
 
   {
     Engine &G = E;
@@ -779,7 +817,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       d.run0 = (uint64_t)total_hist;
       total_hist += (d.valid && d.count > 0) ? d.count : 0;
     }
-    const int cap_max = 1 << 21;
+    const int cap_max = 1 << (g_launch.batch_log2 ? g_launch.batch_log2 : 21);
     int cap = (int)(total_hist < cap_max ? (total_hist > 0 ? total_hist : 1) : cap_max);
     /* a brute-force run with cascades keeps a second slot per primary for
      * the offspring photon of its cascade */
@@ -844,30 +882,30 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       HIP_TRY(G.cone_acc.ensure(cone_bytes));
       if (fresh) HIP_TRY(hipMemsetAsync(G.cone_acc.p, 0, cone_bytes, st));
     }
-    HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size()));
-    HIP_TRY(upload(G.el_w, elW.data(), elW.size()));
-    HIP_TRY(upload(G.el_u, elU.data(), elU.size()));
-    HIP_TRY(upload(G.precalc_mu, pmu.data(), pmu.size()));
-    HIP_TRY(upload(G.act_n, actn.data(), actn.size()));
-    HIP_TRY(upload(G.act_line, actl.data(), actl.size()));
-    HIP_TRY(upload(G.act_shell, acts.data(), acts.size()));
-    HIP_TRY(upload(G.act_factor, actf.data(), actf.size()));
-    HIP_TRY(upload(G.act_E, actE.data(), actE.size()));
-    HIP_TRY(upload(G.act_ch, actc.data(), actc.size()));
-    HIP_TRY(upload(G.act_murho, actm.data(), actm.size()));
-    HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size()));
-    HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size()));
-    HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size()));
-    HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size()));
-    HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size()));
-    HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size()));
-    HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size()));
-    HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size()));
-    HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size()));
-    HIP_TRY(upload(G.src_desc, src.data(), src.size()));
-    HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size()));
-    HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size()));
-    HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size()));
+    HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size(), st));
+    HIP_TRY(upload(G.el_w, elW.data(), elW.size(), st));
+    HIP_TRY(upload(G.el_u, elU.data(), elU.size(), st));
+    HIP_TRY(upload(G.precalc_mu, pmu.data(), pmu.size(), st));
+    HIP_TRY(upload(G.act_n, actn.data(), actn.size(), st));
+    HIP_TRY(upload(G.act_line, actl.data(), actl.size(), st));
+    HIP_TRY(upload(G.act_shell, acts.data(), acts.size(), st));
+    HIP_TRY(upload(G.act_factor, actf.data(), actf.size(), st));
+    HIP_TRY(upload(G.act_E, actE.data(), actE.size(), st));
+    HIP_TRY(upload(G.act_ch, actc.data(), actc.size(), st));
+    HIP_TRY(upload(G.act_murho, actm.data(), actm.size(), st));
+    HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size(), st));
+    HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size(), st));
+    HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size(), st));
+    HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size(), st));
+    HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size(), st));
+    HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size(), st));
+    HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size(), st));
+    HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size(), st));
+    HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size(), st));
+    HIP_TRY(upload(G.src_desc, src.data(), src.size(), st));
+    HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size(), st));
+    HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size(), st));
+    HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size(), st));
     HIP_TRY(hipMemsetAsync(G.tallies.p, 0, tally_words * sizeof(unsigned long long), st));
     HIP_TRY(hipMemsetAsync(G.counters.p, 0, (size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long), st));
 
@@ -1037,24 +1075,24 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(hipEventRecord(evs[0], st));
     /* batches of `cap` consecutive positions of the run order */
     {
-      for (int64_t done = 0; done < total_hist; done += cap) {
-        int count = (int)((total_hist - done) < cap ? (total_hist - done) : cap);
-        HIP_TRY(eng_launch_source(&P, &S, &ST, (uint64_t)done, count, st));
-        if (esc) {
-          HIP_TRY(eng_launch_escape(&P, &S, &V, &Q, count, st));
-          continue;
-        }
+      const int64_t nb = (total_hist + cap - 1) / cap;
+      g_stats.batches = nb;
+      /* the launches of one batch after its source kernel (count: the
+       * batch's primaries, or the capacity in a captured graph, whose
+       * source kernel marks the slots beyond the batch empty) */
+      int *qs[2] = {G.queue.as<int>(), G.queue2.as<int>()};
+      unsigned long long *qc[2] = {G.counters.as<unsigned long long>(), G.qcount2.as<unsigned long long>()};
+      unsigned int *oc = brute ? (unsigned int *)(qc[1] + ENG_NQUEUES * ENG_SPREAD) : nullptr;
+      int *ol[2] = {G.olist.as<int>(), brute ? G.olist.as<int>() + (P.off_half ? P.off_half : 1) : nullptr};
+      auto orders = [&](int count) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (esc) return eng_launch_escape(&P, &S, &V, &Q, count, st);
         if (brute) {
           /* order n_int + 1: the step after the last interaction, in which a
-           * photon can still reach the detector */
+           * photon can still reach the detector.  Order n fills queue set
+           * n & 1 and offspring list n & 1, and reads those of order n - 1 */
           const int bprof = g_profile && 3 * (n_int + 1) <= ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED;
-          /* order n fills queue set n & 1 and offspring list n & 1, and
-           * reads those of order n - 1 */
-          int *qs[2] = {G.queue.as<int>(), G.queue2.as<int>()};
-          unsigned long long *qc[2] = {G.counters.as<unsigned long long>(), G.qcount2.as<unsigned long long>()};
-          unsigned int *oc = (unsigned int *)(qc[1] + ENG_NQUEUES * ENG_SPREAD);
-          int *ol[2] = {G.olist.as<int>(), G.olist.as<int>() + (P.off_half ? P.off_half : 1)};
-          for (int n = 1; n <= n_int + 1; n++) {
+          for (int n = 1; n <= n_int + 1 && e == hipSuccess; n++) {
             EngineParams Pn = P;
             const int c = n & 1, p = c ^ 1;
             Pn.queue = qs[c];
@@ -1065,31 +1103,31 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
             Pn.ocount_in = oc + p;
             Pn.olist_out = ol[c];
             Pn.ocount_out = oc + c;
-            HIP_TRY(eng_launch_order_brute(&Pn, &S, n, count, bblocks, st, bprof ? &G.prof_ev[3 * (n - 1)] : nullptr));
+            e = eng_launch_order_brute(&Pn, &S, n, count, bblocks, st, bprof ? &G.prof_ev[3 * (n - 1)] : nullptr);
           }
-          if (bprof) {
-            HIP_TRY(hipStreamSynchronize(st));
+          if (bprof && e == hipSuccess) {
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
             for (int n = 0; n <= n_int; n++) {
               float t0 = 0.f, t1 = 0.f;
-              HIP_TRY(hipEventElapsedTime(&t0, G.prof_ev[3 * n], G.prof_ev[3 * n + 1]));
-              HIP_TRY(hipEventElapsedTime(&t1, G.prof_ev[3 * n + 1], G.prof_ev[3 * n + 2]));
+              if ((e = hipEventElapsedTime(&t0, G.prof_ev[3 * n], G.prof_ev[3 * n + 1])) != hipSuccess) return e;
+              if ((e = hipEventElapsedTime(&t1, G.prof_ev[3 * n + 1], G.prof_ev[3 * n + 2])) != hipSuccess) return e;
               g_stats.step_kernel_ms += t0;
               g_stats.interaction_kernel_ms += t1;
             }
             g_stats.launches += n_int + 1; /* step launches */
           }
-          continue;
+          return e;
         }
-        int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
-        for (int n = 1; n <= n_int; n++)
-          HIP_TRY(eng_launch_order(&P, &S, &V, n, count, n_cu, st,
-                                   prof ? &G.prof_ev[ENG_ORDER_EVENTS * (n - 1)] : nullptr));
-        if (prof) {
-          HIP_TRY(hipStreamSynchronize(st));
+        const int prof = g_profile && n_int <= ENG_MAX_ORDERS_PROFILED;
+        for (int n = 1; n <= n_int && e == hipSuccess; n++)
+          e = eng_launch_order(&P, &S, &V, n, count, n_cu, st, prof ? &G.prof_ev[ENG_ORDER_EVENTS * (n - 1)] : nullptr);
+        if (prof && e == hipSuccess) {
+          if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
           for (int n = 0; n < n_int; n++) {
             float t[ENG_ORDER_EVENTS - 1];
-            const hipEvent_t *e = &G.prof_ev[ENG_ORDER_EVENTS * n];
-            for (int k = 0; k + 1 < ENG_ORDER_EVENTS; k++) HIP_TRY(hipEventElapsedTime(&t[k], e[k], e[k + 1]));
+            const hipEvent_t *ev = &G.prof_ev[ENG_ORDER_EVENTS * n];
+            for (int k = 0; k + 1 < ENG_ORDER_EVENTS; k++)
+              if ((e = hipEventElapsedTime(&t[k], ev[k], ev[k + 1])) != hipSuccess) return e;
             g_stats.step_kernel_ms += t[0];
             g_stats.vr_geom_ms += t[1];
             g_stats.vr_cone_ms += t[2];
@@ -1099,6 +1137,39 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
             g_stats.interaction_kernel_ms += t[5];
           }
           g_stats.launches += n_int; /* launches of each kernel class */
+        }
+        return e;
+      };
+      if (g_launch.use_graph && !esc && !g_profile) {
+        /* one captured graph per run: the pop of the next batch from the
+         * device batch queue, the source kernel over `cap` slots and the
+         * orders; replayed once per batch */
+        HIP_TRY(G.batch_queue.ensure(sizeof(BatchQueue)));
+        BatchQueue *bq = G.batch_queue.as<BatchQueue>();
+        HIP_TRY(eng_launch_batch_init(bq, (unsigned long long)total_hist, st));
+        hipError_t ce, ee, ie = hipSuccess;
+        {
+          std::lock_guard<std::mutex> g(g_capture_lock);
+          ce = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+          if (ce == hipSuccess) {
+            ce = eng_launch_source_queued(&P, &S, &ST, bq, cap, st);
+            if (ce == hipSuccess) ce = orders(cap);
+            ee = hipStreamEndCapture(st, &graph);
+            if (ce == hipSuccess && ee == hipSuccess) ie = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+          } else {
+            ee = hipSuccess;
+          }
+        }
+        HIP_TRY(ce);
+        HIP_TRY(ee);
+        HIP_TRY(ie);
+        for (int64_t b = 0; b < nb; b++) HIP_TRY(hipGraphLaunch(graph_exec, st));
+        g_stats.graph_replays = nb;
+      } else {
+        for (int64_t done = 0; done < total_hist; done += cap) {
+          const int count = (int)((total_hist - done) < cap ? (total_hist - done) : cap);
+          HIP_TRY(eng_launch_source(&P, &S, &ST, (uint64_t)done, count, st));
+          HIP_TRY(orders(count));
         }
       }
     }
@@ -1134,7 +1205,9 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       /* ratios: escaped weight over the weight of the photons that
        * interacted, per input energy (src/xmi_main.F90:5773-5776) */
       std::vector<unsigned long long> et(esc_words);
-      HIP_TRY(hipMemcpy(et.data(), G.esc_tally.p, esc_words * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpyAsync(et.data(), G.esc_tally.p, esc_words * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             st));
+      HIP_TRY(hipStreamSynchronize(st));
       const size_t nf = (size_t)esc->nE * ENG_ESC_LINES * nZ, nc = (size_t)esc->nOut * esc->nE;
       const unsigned long long *tf = et.data(), *tc = tf + 2 * nf, *ti = tc + 2 * nc;
       esc->fluo.assign(nf, 0.0);
@@ -1227,6 +1300,12 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     ok = 1;
   }
 fail:
+  if (graph_exec || graph) {
+    /* replays may still run when a later step failed */
+    (void)hipStreamSynchronize(E.stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
   for (int k = 0; k < 2; k++)
     if (evs[k]) (void)hipEventDestroy(evs[k]);
   if (!ok) {
@@ -1550,8 +1629,8 @@ extern "C" int xmi_solid_angle_calculation_cl(xmi_inputFPtr inputFPtr, xmi_solid
   }
   hipStream_t st = g_engine.stream;
   int ok = 0;
-  if (upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)n) == hipSuccess &&
-      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)n) == hipSuccess &&
+  if (upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)n, g_engine.stream) == hipSuccess &&
+      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)n, g_engine.stream) == hipSuccess &&
       g_engine.sa_grid.ensure(sizeof(double) * n * n) == hipSuccess &&
       eng_launch_grid(&P, g_engine.sa_grid.as<double>(), g_engine.sa_r.as<double>(), g_engine.sa_t.as<double>(), (int)n,
                       (int)n, xmi_hits_per_single, seed, st) == hipSuccess &&

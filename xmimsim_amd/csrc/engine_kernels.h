@@ -68,6 +68,14 @@ struct SourceTable {
   const xrl_atom *exc_atoms;  /* [95] indexed by Z */
 };
 
+/* device batch queue of a run whose batches replay one captured graph: the
+ * batches are the run order's positions in steps of the batch capacity;
+ * start and count describe the batch being simulated */
+struct BatchQueue {
+  unsigned long long next, total, start;
+  int count, pad;
+};
+
 /* escape-ratio runs (src/xmi_main.F90:5473-5801): fixed-point tallies of
  * the photons that leave the crystal after their one forced interaction */
 #define ENG_ESC_LINES 109 /* K and L lines: KL1 (1) .. L3P3 (109) */
@@ -89,6 +97,11 @@ hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv
                            long hits, uint64_t seed, hipStream_t st);
 hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,
                              int count, hipStream_t st);
+/* the batch queue: set up for `total` positions; then per batch the pop and
+ * the source kernel over `cap` slots (those beyond the batch marked empty) */
+hipError_t eng_launch_batch_init(BatchQueue *Q, unsigned long long total, hipStream_t st);
+hipError_t eng_launch_source_queued(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, BatchQueue *Q,
+                                    int cap, hipStream_t st);
 /* order n of a brute-force run: the analogue step with its detector tests
  * and tallies over a fixed grid of `blocks` workgroups (eng_brute_blocks)
  * that strides over the live photons, then (n <= n_int) the interactions */

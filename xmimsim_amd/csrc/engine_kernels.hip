@@ -444,11 +444,43 @@ __device__ int d_source_one(const EngineParams &P, const PhotonSoA &S, const Sou
   return 1;
 }
 
+/* the batch's primaries: positions batch_start .. batch_start + count - 1
+ * of the run order, or, with a batch queue Q, those of the batch that
+ * k_batch_next popped; then slots count .. limit - 1 (limit: the launch's
+ * count argument) are marked empty */
 __global__ void __launch_bounds__(ENG_BLOCK) k_source(EngineParams P, PhotonSoA S, SourceTable T,
-                                                       uint64_t batch_start, int count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int alive = (i < count) ? d_source_one(P, S, T, batch_start, i) : 0;
+                                                       uint64_t batch_start, int count, const BatchQueue *Q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, limit = count;
+  if (Q) {
+    batch_start = Q->start;
+    count = Q->count;
+  }
+  int alive = 0;
+  if (i < count) {
+    alive = d_source_one(P, S, T, batch_start, i);
+  } else if (i < limit) {
+    S.layer[i] = 0;
+    if (P.off_half) S.layer[i + P.off_half] = 0;
+  }
   d_wave_count(d_ctr(P, ENG_CNT_HISTORIES), alive);
+}
+
+/* the device batch queue: k_batch_init before a run's first replay, then
+ * k_batch_next, the first node of every replay, pops the next batch of at
+ * most cap positions (one thread each) */
+__global__ void k_batch_init(BatchQueue *Q, unsigned long long total) {
+  Q->next = 0ull;
+  Q->total = total;
+  Q->start = 0ull;
+  Q->count = 0;
+}
+
+__global__ void k_batch_next(BatchQueue *Q, int cap) {
+  const unsigned long long s = Q->next, left = Q->total > s ? Q->total - s : 0ull;
+  const int c = left < (unsigned long long)cap ? (int)left : cap;
+  Q->start = s;
+  Q->count = c;
+  Q->next = s + (unsigned long long)c;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2164,15 +2196,9 @@ static size_t lines_lds_bytes(const EngineParams *P) { return lds_if_fits((size_
 
 /* Workgroups of a grid-stride kernel: as many as can be resident at once,
  * since every workgroup ends with one flush of its LDS tallies to global
- * memory.  XMIMSIM_AMD_VR_WG_PER_CU overrides the resident count per CU
- * (launch-shape sweeps). */
-static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks) {
-  static int env = -1;
-  if (env < 0) {
-    const char *e = getenv("XMIMSIM_AMD_VR_WG_PER_CU");
-    env = e ? atoi(e) : 0;
-  }
-  int per_cu = env;
+ * memory.  per_cu > 0 (the run's launch configuration) overrides the
+ * resident count per CU. */
+static int resident_blocks(const void *kernel, size_t lds, int n_cu, int blocks, int per_cu = 0) {
   if (per_cu <= 0 &&
       (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, ENG_BLOCK, lds) != hipSuccess || per_cu < 1))
     per_cu = 1;
@@ -2405,21 +2431,37 @@ static void launch_interact_brute(const EngineParams *P, const PhotonSoA *S, int
   else hipLaunchKernelGGL(k_interact_brute<false>, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, n);
 }
 
-/* photons per lane of k_vr_lines: each line's wavefront sum then serves
- * 64 KP photons; fewer where the per-photon path lengths take more registers.
- * Three rather than four up to four layers: 126 VGPRs give four waves per
- * SIMD instead of three (3 % faster) */
+template <int NL, int KP>
+static void launch_lines_kp(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
+                            hipStream_t st) {
+  const size_t ll = lines_lds_bytes(P);
+  if (ll) {
+    int vb = resident_blocks((const void *)k_vr_lines<NL, KP, true>, ll, n_cu, blocks, P->wg_per_cu);
+    hipLaunchKernelGGL((k_vr_lines<NL, KP, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
+  } else {
+    int vb = resident_blocks((const void *)k_vr_lines<NL, KP, false>, 0, n_cu, blocks, P->wg_per_cu);
+    hipLaunchKernelGGL((k_vr_lines<NL, KP, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+  }
+}
+
+/* photons per lane of k_vr_lines (P->lines_kp, 0: the default): each
+ * line's wavefront sum then serves 64 KP photons; fewer where the
+ * per-photon path lengths take more registers.  Three rather than four up
+ * to four layers by default: 126 VGPRs give four waves per SIMD instead of
+ * three (3 % faster) */
 template <int NL>
 static void launch_lines(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
                          hipStream_t st) {
-  constexpr int KP = NL <= 4 ? 3 : (NL <= 8 ? 2 : 1);
-  const size_t ll = lines_lds_bytes(P);
-  if (ll) {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, KP, true>, ll, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, KP, true>), dim3(vb), dim3(ENG_BLOCK), ll, st, *P, *S, *V, n);
+  const int kp = P->lines_kp;
+  if constexpr (NL <= 4) {
+    if (kp == 2) launch_lines_kp<NL, 2>(P, S, V, n, n_cu, blocks, st);
+    else if (kp == 4) launch_lines_kp<NL, 4>(P, S, V, n, n_cu, blocks, st);
+    else launch_lines_kp<NL, 3>(P, S, V, n, n_cu, blocks, st);
+  } else if constexpr (NL <= 8) {
+    if (kp == 1) launch_lines_kp<NL, 1>(P, S, V, n, n_cu, blocks, st);
+    else launch_lines_kp<NL, 2>(P, S, V, n, n_cu, blocks, st);
   } else {
-    int vb = resident_blocks((const void *)k_vr_lines<NL, KP, false>, 0, n_cu, blocks);
-    hipLaunchKernelGGL((k_vr_lines<NL, KP, false>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
+    launch_lines_kp<NL, 1>(P, S, V, n, n_cu, blocks, st);
   }
 }
 
@@ -2427,10 +2469,10 @@ template <int NL, bool ADV>
 static void launch_scatter(const EngineParams *P, const PhotonSoA *S, const VrSoA *V, int n, int n_cu, int blocks,
                            size_t ls, hipStream_t st) {
   if (ls) {
-    int vb = resident_blocks((const void *)k_vr_scatter<NL, true, ADV>, ls, n_cu, blocks);
+    int vb = resident_blocks((const void *)k_vr_scatter<NL, true, ADV>, ls, n_cu, blocks, P->wg_per_cu);
     hipLaunchKernelGGL((k_vr_scatter<NL, true, ADV>), dim3(vb), dim3(ENG_BLOCK), ls, st, *P, *S, *V, n);
   } else {
-    int vb = resident_blocks((const void *)k_vr_scatter<NL, false, ADV>, 0, n_cu, blocks);
+    int vb = resident_blocks((const void *)k_vr_scatter<NL, false, ADV>, 0, n_cu, blocks, P->wg_per_cu);
     hipLaunchKernelGGL((k_vr_scatter<NL, false, ADV>), dim3(vb), dim3(ENG_BLOCK), 0, st, *P, *S, *V, n);
   }
 }
@@ -2480,7 +2522,22 @@ hipError_t eng_launch_grid(const EngineParams *P, double *grid, const double *rv
 hipError_t eng_launch_source(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, uint64_t start,
                              int count, hipStream_t st) {
   int blocks = (count + ENG_BLOCK - 1) / ENG_BLOCK;
-  hipLaunchKernelGGL(k_source, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *T, start, count);
+  hipLaunchKernelGGL(k_source, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *T, start, count,
+                     (const BatchQueue *)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t eng_launch_batch_init(BatchQueue *Q, unsigned long long total, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(1), 0, st, Q, total);
+  return hipGetLastError();
+}
+
+hipError_t eng_launch_source_queued(const EngineParams *P, const PhotonSoA *S, const SourceTable *T, BatchQueue *Q,
+                                    int cap, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_next, dim3(1), dim3(1), 0, st, Q, cap);
+  const int blocks = (cap + ENG_BLOCK - 1) / ENG_BLOCK;
+  hipLaunchKernelGGL(k_source, dim3(blocks), dim3(ENG_BLOCK), 0, st, *P, *S, *T, (uint64_t)0, cap,
+                     (const BatchQueue *)Q);
   return hipGetLastError();
 }
 
