@@ -130,14 +130,58 @@ struct DevBuf {
   }
 };
 
-/* a copy on the caller's stream, complete on return (not on the legacy
- * stream, which would wait for streams that may be capturing) */
+/* page-locked staging memory of one engine's host <-> device copies, so
+ * that no copy takes the runtime's path for pageable memory */
+struct Staging {
+  void *p = nullptr;
+  size_t n = 0;
+  ~Staging() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    std::lock_guard<std::mutex> g(g_capture_lock);
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+};
+
+static const size_t STAGE_CHUNK = (size_t)16 << 20;
+
+/* copies through the staging memory on the caller's stream, complete on
+ * return (not on the legacy stream, which would wait for streams that may
+ * be capturing) */
+static hipError_t copy_h2d(void *dst, const void *src, size_t bytes, hipStream_t st, Staging &sg) {
+  hipError_t e = sg.ensure(bytes < STAGE_CHUNK ? bytes : STAGE_CHUNK);
+  for (size_t off = 0; e == hipSuccess && off < bytes; off += STAGE_CHUNK) {
+    const size_t c = bytes - off < STAGE_CHUNK ? bytes - off : STAGE_CHUNK;
+    memcpy(sg.p, (const char *)src + off, c);
+    e = hipMemcpyAsync((char *)dst + off, sg.p, c, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
+  return e;
+}
+
+static hipError_t copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t st, Staging &sg) {
+  hipError_t e = sg.ensure(bytes < STAGE_CHUNK ? bytes : STAGE_CHUNK);
+  for (size_t off = 0; e == hipSuccess && off < bytes; off += STAGE_CHUNK) {
+    const size_t c = bytes - off < STAGE_CHUNK ? bytes - off : STAGE_CHUNK;
+    e = hipMemcpyAsync(sg.p, (const char *)src + off, c, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) memcpy((char *)dst + off, sg.p, c);
+  }
+  return e;
+}
+
 template <typename T>
-static hipError_t upload(DevBuf &b, const T *src, size_t count, hipStream_t st) {
+static hipError_t upload(DevBuf &b, const T *src, size_t count, hipStream_t st, Staging &sg) {
   hipError_t e = b.ensure(sizeof(T) * count);
   if (e != hipSuccess || !count) return e;
-  e = hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, st);
-  return e == hipSuccess ? hipStreamSynchronize(st) : e;
+  return copy_h2d(b.p, src, sizeof(T) * count, st, sg);
 }
 
 /* tables of one xmi_hdf5F handle on the device */
@@ -169,6 +213,7 @@ struct Engine {
   std::mutex lock;
   bool ready = false;
   hipStream_t stream = nullptr;
+  Staging stage;
   hipEvent_t prof_ev[ENG_ORDER_EVENTS * ENG_MAX_ORDERS_PROFILED] = {};
   /* solid-angle grid cache */
   const void *sa_key = nullptr;
@@ -205,7 +250,7 @@ static int engine_init() { return engine_init(g_engine); }
 
 static double uniform_inv(const double *a, int n) { return (double)(n - 1) / (a[n - 1] - a[0]); }
 
-static DeviceTables *get_tables(const xmi_hdf5F *h, hipStream_t st) {
+static DeviceTables *get_tables(const xmi_hdf5F *h, hipStream_t st, Staging &sg) {
   std::lock_guard<std::mutex> g(g_tables_lock);
   for (DeviceTables *t : g_tables)
     if (t->owner == h) return t;
@@ -281,26 +326,26 @@ static DeviceTables *get_tables(const xmi_hdf5F *h, hipStream_t st) {
   t->phi_R0 = h->RandomNumbers[0];
   t->phi_Rinv = uniform_inv(h->RandomNumbers, h->phi_n_r);
   hipError_t e = hipSuccess;
-  e = upload(t->atoms, atoms.data(), atoms.size(), st);
-  if (e == hipSuccess) e = upload(t->ray, ray.data(), ray.size(), st);
-  if (e == hipSuccess) e = upload(t->comp, comp.data(), comp.size(), st);
-  if (e == hipSuccess) e = upload(t->tabE, z0->Energies, (size_t)t->n_e, st);
-  if (e == hipSuccess) e = upload(t->tabR, z0->RandomNumbers, (size_t)t->n_r, st);
-  if (e == hipSuccess) e = upload(t->fycorr, fy.data(), fy.size(), st);
-  if (e == hipSuccess) e = upload(t->ipE, ipE.data(), ipE.size(), st);
-  if (e == hipSuccess) e = upload(t->ipr, ipr.data(), ipr.size(), st);
-  if (e == hipSuccess) e = upload(t->iprc, iprc.data(), iprc.size(), st);
-  if (e == hipSuccess) e = upload(t->ipoff, ipoff.data(), ipoff.size(), st);
-  if (e == hipSuccess) e = upload(t->ipacc, ipacc.data(), ipacc.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_icdf, cp.data(), cp.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_rn, z0->cp_random_numbers, (size_t)t->cp_len, st);
-  if (e == hipSuccess) e = upload(t->cp_cdfg, cdfg.data(), cdfg.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_gscale, gscale.data(), gscale.size(), st);
-  if (e == hipSuccess) e = upload(t->precalc_xrf, pxrf.data(), pxrf.size(), st);
-  if (e == hipSuccess) e = upload(t->lineE, lineE.data(), lineE.size(), st);
-  if (e == hipSuccess) e = upload(t->phi, h->Phi_ICDF, (size_t)h->phi_n_theta * h->phi_n_r, st);
-  if (e == hipSuccess) e = upload(t->phiT, h->Thetas, (size_t)h->phi_n_theta, st);
-  if (e == hipSuccess) e = upload(t->phiR, h->RandomNumbers, (size_t)h->phi_n_r, st);
+  e = upload(t->atoms, atoms.data(), atoms.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->ray, ray.data(), ray.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->comp, comp.data(), comp.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->tabE, z0->Energies, (size_t)t->n_e, st, sg);
+  if (e == hipSuccess) e = upload(t->tabR, z0->RandomNumbers, (size_t)t->n_r, st, sg);
+  if (e == hipSuccess) e = upload(t->fycorr, fy.data(), fy.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->ipE, ipE.data(), ipE.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->ipr, ipr.data(), ipr.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->iprc, iprc.data(), iprc.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->ipoff, ipoff.data(), ipoff.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->ipacc, ipacc.data(), ipacc.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_icdf, cp.data(), cp.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_rn, z0->cp_random_numbers, (size_t)t->cp_len, st, sg);
+  if (e == hipSuccess) e = upload(t->cp_cdfg, cdfg.data(), cdfg.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_gscale, gscale.data(), gscale.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->precalc_xrf, pxrf.data(), pxrf.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->lineE, lineE.data(), lineE.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->phi, h->Phi_ICDF, (size_t)h->phi_n_theta * h->phi_n_r, st, sg);
+  if (e == hipSuccess) e = upload(t->phiT, h->Thetas, (size_t)h->phi_n_theta, st, sg);
+  if (e == hipSuccess) e = upload(t->phiR, h->RandomNumbers, (size_t)h->phi_n_r, st, sg);
   if (e != hipSuccess) {
     xmi_set_error("table upload failed: %s", hipGetErrorString(e));
     delete t;
@@ -312,7 +357,7 @@ static DeviceTables *get_tables(const xmi_hdf5F *h, hipStream_t st) {
 
 /* the partial Compton profiles of every element, flattened over the
  * occupied subshells (advanced Compton model only) */
-static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h, hipStream_t st) {
+static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h, hipStream_t st, Staging &sg) {
   std::lock_guard<std::mutex> g(g_tables_lock);
   if (t->adv_ready) return 1;
   const int nZ = h->n_Z, N = t->cp_len;
@@ -330,12 +375,12 @@ static int ensure_adv_tables(DeviceTables *t, const xmi_hdf5F *h, hipStream_t st
     qinv.insert(qinv.end(), z->cp_Qs_inv, z->cp_Qs_inv + (size_t)z->cp_nshells * N);
   }
   off[nZ] = (int)idx.size();
-  hipError_t e = upload(t->cp_shoff, off.data(), off.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_shidx, idx.data(), idx.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_pcdf, pcdf.data(), pcdf.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_qinv, qinv.data(), qinv.size(), st);
-  if (e == hipSuccess) e = upload(t->cp_Qs, h->Zs[0].cp_Qs, (size_t)N, st);
-  if (e == hipSuccess) e = upload(t->cp_inv, h->Zs[0].cp_partial_cdf_inv, (size_t)N, st);
+  hipError_t e = upload(t->cp_shoff, off.data(), off.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_shidx, idx.data(), idx.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_pcdf, pcdf.data(), pcdf.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_qinv, qinv.data(), qinv.size(), st, sg);
+  if (e == hipSuccess) e = upload(t->cp_Qs, h->Zs[0].cp_Qs, (size_t)N, st, sg);
+  if (e == hipSuccess) e = upload(t->cp_inv, h->Zs[0].cp_partial_cdf_inv, (size_t)N, st, sg);
   if (e != hipSuccess) {
     xmi_set_error("advanced Compton table upload failed: %s", hipGetErrorString(e));
     return 0;
@@ -387,9 +432,9 @@ static int ensure_grid(Engine &E, const xmi_solid_angle *sa) {
   uint64_t hsh = grid_hash(sa);
   if (E.sa_key == sa && E.sa_hash == hsh) return 1;
   size_t tot = (size_t)sa->grid_dims_r_n * sa->grid_dims_theta_n;
-  if (upload(E.sa_grid, sa->solid_angles, tot, E.stream) != hipSuccess ||
-      upload(E.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n, E.stream) != hipSuccess ||
-      upload(E.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n, E.stream) != hipSuccess) {
+  if (upload(E.sa_grid, sa->solid_angles, tot, E.stream, E.stage) != hipSuccess ||
+      upload(E.sa_r, sa->grid_dims_r_vals, (size_t)sa->grid_dims_r_n, E.stream, E.stage) != hipSuccess ||
+      upload(E.sa_t, sa->grid_dims_theta_vals, (size_t)sa->grid_dims_theta_n, E.stream, E.stage) != hipSuccess) {
     xmi_set_error("solid-angle grid upload failed");
     return 0;
   }
@@ -492,9 +537,9 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   }
   if (n_mpi_hosts < 1) n_mpi_hosts = 1;
   if (!engine_init(E)) return 0;
-  DeviceTables *T = get_tables(h, E.stream);
+  DeviceTables *T = get_tables(h, E.stream, E.stage);
   if (!T) return 0;
-  if (options->use_advanced_compton && !ensure_adv_tables(T, h, E.stream)) return 0;
+  if (options->use_advanced_compton && !ensure_adv_tables(T, h, E.stream, E.stage)) return 0;
   if (!esc && !brute && !T->has_precalc) {
     xmi_set_error("xmi_main_msim: the tables were read without the cross sections of the variance reduction "
                   "(xmi_init_from_hdf5 with use_variance_reduction=0)");
@@ -882,30 +927,30 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       HIP_TRY(G.cone_acc.ensure(cone_bytes));
       if (fresh) HIP_TRY(hipMemsetAsync(G.cone_acc.p, 0, cone_bytes, st));
     }
-    HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size(), st));
-    HIP_TRY(upload(G.el_w, elW.data(), elW.size(), st));
-    HIP_TRY(upload(G.el_u, elU.data(), elU.size(), st));
-    HIP_TRY(upload(G.precalc_mu, pmu.data(), pmu.size(), st));
-    HIP_TRY(upload(G.act_n, actn.data(), actn.size(), st));
-    HIP_TRY(upload(G.act_line, actl.data(), actl.size(), st));
-    HIP_TRY(upload(G.act_shell, acts.data(), acts.size(), st));
-    HIP_TRY(upload(G.act_factor, actf.data(), actf.size(), st));
-    HIP_TRY(upload(G.act_E, actE.data(), actE.size(), st));
-    HIP_TRY(upload(G.act_ch, actc.data(), actc.size(), st));
-    HIP_TRY(upload(G.act_murho, actm.data(), actm.size(), st));
-    HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size(), st));
-    HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size(), st));
-    HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size(), st));
-    HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size(), st));
-    HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size(), st));
-    HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size(), st));
-    HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size(), st));
-    HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size(), st));
-    HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size(), st));
-    HIP_TRY(upload(G.src_desc, src.data(), src.size(), st));
-    HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size(), st));
-    HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size(), st));
-    HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size(), st));
+    HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size(), st, G.stage));
+    HIP_TRY(upload(G.el_w, elW.data(), elW.size(), st, G.stage));
+    HIP_TRY(upload(G.el_u, elU.data(), elU.size(), st, G.stage));
+    HIP_TRY(upload(G.precalc_mu, pmu.data(), pmu.size(), st, G.stage));
+    HIP_TRY(upload(G.act_n, actn.data(), actn.size(), st, G.stage));
+    HIP_TRY(upload(G.act_line, actl.data(), actl.size(), st, G.stage));
+    HIP_TRY(upload(G.act_shell, acts.data(), acts.size(), st, G.stage));
+    HIP_TRY(upload(G.act_factor, actf.data(), actf.size(), st, G.stage));
+    HIP_TRY(upload(G.act_E, actE.data(), actE.size(), st, G.stage));
+    HIP_TRY(upload(G.act_ch, actc.data(), actc.size(), st, G.stage));
+    HIP_TRY(upload(G.act_murho, actm.data(), actm.size(), st, G.stage));
+    HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size(), st, G.stage));
+    HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size(), st, G.stage));
+    HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size(), st, G.stage));
+    HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size(), st, G.stage));
+    HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size(), st, G.stage));
+    HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size(), st, G.stage));
+    HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size(), st, G.stage));
+    HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size(), st, G.stage));
+    HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size(), st, G.stage));
+    HIP_TRY(upload(G.src_desc, src.data(), src.size(), st, G.stage));
+    HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size(), st, G.stage));
+    HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size(), st, G.stage));
+    HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size(), st, G.stage));
     HIP_TRY(hipMemsetAsync(G.tallies.p, 0, tally_words * sizeof(unsigned long long), st));
     HIP_TRY(hipMemsetAsync(G.counters.p, 0, (size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long), st));
 
@@ -1177,10 +1222,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     std::vector<unsigned long long> th(tally_words);
     std::vector<unsigned long long> cnt_sub((size_t)ENG_NCOUNTERS * ENG_SPREAD);
     unsigned long long cnt[ENG_NCOUNTERS];
-    HIP_TRY(hipMemcpyAsync(th.data(), G.tallies.p, tally_words * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(cnt_sub.data(), G.counters.p, cnt_sub.size() * sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(copy_d2h(th.data(), G.tallies.p, tally_words * sizeof(unsigned long long), st, G.stage));
+    HIP_TRY(copy_d2h(cnt_sub.data(), G.counters.p, cnt_sub.size() * sizeof(unsigned long long), st, G.stage));
     for (int c = 0; c < ENG_NCOUNTERS; c++) {
       cnt[c] = 0ull;
       for (int j = 0; j < ENG_SPREAD; j++) cnt[c] += cnt_sub[(size_t)c * ENG_SPREAD + j];
@@ -1205,9 +1248,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       /* ratios: escaped weight over the weight of the photons that
        * interacted, per input energy (src/xmi_main.F90:5773-5776) */
       std::vector<unsigned long long> et(esc_words);
-      HIP_TRY(hipMemcpyAsync(et.data(), G.esc_tally.p, esc_words * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                             st));
-      HIP_TRY(hipStreamSynchronize(st));
+      HIP_TRY(copy_d2h(et.data(), G.esc_tally.p, esc_words * sizeof(unsigned long long), st, G.stage));
       const size_t nf = (size_t)esc->nE * ENG_ESC_LINES * nZ, nc = (size_t)esc->nOut * esc->nE;
       const unsigned long long *tf = et.data(), *tc = tf + 2 * nf, *ti = tc + 2 * nc;
       esc->fluo.assign(nf, 0.0);
@@ -1629,14 +1670,12 @@ extern "C" int xmi_solid_angle_calculation_cl(xmi_inputFPtr inputFPtr, xmi_solid
   }
   hipStream_t st = g_engine.stream;
   int ok = 0;
-  if (upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)n, g_engine.stream) == hipSuccess &&
-      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)n, g_engine.stream) == hipSuccess &&
+  if (upload(g_engine.sa_r, sa->grid_dims_r_vals, (size_t)n, g_engine.stream, g_engine.stage) == hipSuccess &&
+      upload(g_engine.sa_t, sa->grid_dims_theta_vals, (size_t)n, g_engine.stream, g_engine.stage) == hipSuccess &&
       g_engine.sa_grid.ensure(sizeof(double) * n * n) == hipSuccess &&
       eng_launch_grid(&P, g_engine.sa_grid.as<double>(), g_engine.sa_r.as<double>(), g_engine.sa_t.as<double>(), (int)n,
                       (int)n, xmi_hits_per_single, seed, st) == hipSuccess &&
-      hipMemcpyAsync(sa->solid_angles, g_engine.sa_grid.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, st) ==
-          hipSuccess &&
-      hipStreamSynchronize(st) == hipSuccess)
+      copy_d2h(sa->solid_angles, g_engine.sa_grid.p, sizeof(double) * n * n, st, g_engine.stage) == hipSuccess)
     ok = 1;
   if (!ok) {
     xmi_set_error("xmi_solid_angle_calculation_cl: device failure");
