@@ -588,7 +588,10 @@ def main():
         hbm = {}
         for k in sorted(kernel_ms, key=lambda k: -prof[k])[:4]:
             b = traffic.get(names[k])
-            t = prof[k] * 1e-3 / launches
+            # the last order launches no interactions
+            n_k = launches * (inp.n_interactions_trajectory - 1) / inp.n_interactions_trajectory \
+                if k == "interactions" else launches
+            t = prof[k] * 1e-3 / max(n_k, 1)
             hbm[names[k]] = {"avg_launch_ms": round(t * 1e3, 4), "bytes_per_launch": b,
                              "GBps": round(b / t / 1e9, 1) if b else None,
                              "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 4) if b else None}

@@ -2605,7 +2605,9 @@ hipError_t eng_launch_order(const EngineParams *P, const PhotonSoA *S, const VrS
   }
   if (ev) (void)hipEventRecord(ev[1], st);
   launch_estimator(P, S, V, n, count, n_cu, blocks, st, ev ? ev + 2 : nullptr);
-  launch_interact(P, S, n, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
+  /* the interactions of the last order change only photon states that no
+   * later step reads (the estimator has scored this order): not launched */
+  if (n < P->n_int) launch_interact(P, S, n, (count + ENG_BLOCK - 1) / ENG_BLOCK, st);
   if (ev) (void)hipEventRecord(ev[6], st);
   return hipGetLastError();
 }
