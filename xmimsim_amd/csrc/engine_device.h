@@ -116,6 +116,8 @@ struct EngineParams {
   const double *lay_pb;         /* [lay_eoff[l] + l .. lay_eoff[l+1] + l] */
   const int *lay_kstart;        /* [n_layers][XRL_NCS] */
   const struct LayCell *lay_cell; /* [n_layers][XRL_NCS] */
+  const struct LayCell *el_cell;  /* [elements of the layers, layer_off order][XRL_NCS]: each element's own
+                                     total cross section (K counts its own edges) */
   int lay_eoff[ENG_MAX_LAYERS + 1];
   /* per element */
   const xrl_atom *atoms;        /* [nZ] */

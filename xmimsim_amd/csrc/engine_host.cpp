@@ -223,7 +223,7 @@ struct Engine {
   /* per-call buffers (grow only) */
   DevBuf el_Z, el_w, el_u, precalc_mu, act_n, act_line, act_shell, act_factor, act_E, act_ch, act_murho;
   DevBuf fl_info, fl_wf, fl_mr, lbin_g;
-  DevBuf lay_rc, lay_edge, lay_pb, lay_kstart, lay_cell;
+  DevBuf lay_rc, lay_edge, lay_pb, lay_kstart, lay_cell, el_cell;
   DevBuf src_desc, src_line_mus, exc_layers, exc_atoms;
   DevBuf state, vr, tallies, counters, queue, lay_list, lay_woff, lay_total, lay_segbase, cone_acc, esc_tally;
   DevBuf queue2, qcount2, olist;
@@ -654,6 +654,33 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       }
     }
   }
+  /* per element of every layer, the same records of its own total cross
+   * section (the atom selection of the step: one record load per element
+   * instead of the atom's edges, photo sum and table, in a chain) */
+  std::vector<LayCell> el_cell(elZ.size() * (size_t)XRL_NCS);
+  {
+    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
+    for (size_t j = 0; j < elZ.size(); j++) {
+      const xrl_atom *a = xrl_atom_get(elZ[j]);
+      std::vector<double> ed;
+      for (int sh = 0; sh < 9; sh++)
+        if (a->edge[sh] > 0.0) ed.push_back(a->edge[sh]);
+      std::sort(ed.begin(), ed.end(), [](double x, double y) { return x > y; });
+      const int ne = (int)ed.size();
+      for (int c = 0; c < XRL_NCS; c++) {
+        LayCell &r = el_cell[j * XRL_NCS + c];
+        r.T0 = a->cs_rc[c];
+        r.dT = c + 1 < XRL_NCS ? a->cs_rc[c + 1] - a->cs_rc[c] : 0.0;
+        const double e_hi = exp(XRL_CS_LNE0 + (double)(c + 1) * h) * (1.0 + 1e-9);
+        const double e_lo = exp(XRL_CS_LNE0 + (double)c * h) * (1.0 - 1e-9);
+        int K = 0;
+        while (K < ne && ed[K] > e_hi) K++;
+        r.K = K;
+        r.edge_free = c > 0 && c < XRL_NCS - 2 && (K >= ne || ed[K] < e_lo);
+        r.pb = a->photo_Bsum[K];
+      }
+    }
+  }
   /* the same lookup on the host (as d_layer_mu_tab on the device) */
   auto layer_mu_tab = [&](int l, double E) -> double {
     const xrl_ectx c = xrl_energy(E);
@@ -947,6 +974,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size(), st, G.stage));
     HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size(), st, G.stage));
     HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size(), st, G.stage));
+    HIP_TRY(upload(G.el_cell, el_cell.data(), el_cell.size(), st, G.stage));
     HIP_TRY(upload(G.src_desc, src.data(), src.size(), st, G.stage));
     HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size(), st, G.stage));
     HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size(), st, G.stage));
@@ -1009,6 +1037,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
     P.lay_pb = G.lay_pb.as<double>();
     P.lay_kstart = G.lay_kstart.as<int>();
     P.lay_cell = G.lay_cell.as<LayCell>();
+    P.el_cell = G.el_cell.as<LayCell>();
     P.phi_icdf = T->phi.as<double>();
     P.phi_T = T->phiT.as<double>();
     P.phi_R = T->phiR.as<double>();

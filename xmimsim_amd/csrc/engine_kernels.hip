@@ -553,6 +553,21 @@ __device__ __forceinline__ void d_layer_list(const EngineParams &P, int layer, i
 /* ------------------------------------------------------------------------ */
 /* forced interaction + atom and interaction-type selection */
 
+/* CS_Total_Kissel of element slot j of the layers (xrl_cs_total_c, the same
+ * operations on the same values), from its cell record: the photo sum of the
+ * cell's edge count where no edge of the element lies in the cell, else
+ * counted from the atom's edges */
+__device__ __forceinline__ double d_elem_cs_total(const EngineParams &P, int j, const xrl_ectx *c) {
+  if (!(c->E > 0.0)) return 0.0;
+  const LayCell r = P.el_cell[(size_t)j * XRL_NCS + c->ci];
+  double pb = r.pb;
+  if (!r.edge_free) {
+    const xrl_atom *a = &P.atoms[P.el_u[j]];
+    pb = a->photo_Bsum[xrl_edges_above(a, c->E)];
+  }
+  return c->pe * pb + (r.T0 + c->cf * r.dT);
+}
+
 /* atom selection and interaction type of the photon in slot i (energy E,
  * now in layer my), from its transport stream after the step's number:
  * S.elem set, the type returned (0 Rayleigh, 1 Compton, 2 photo-absorption) */
@@ -568,7 +583,7 @@ __device__ __forceinline__ int d_select(const EngineParams &P, const PhotonSoA &
   int sel = nel - 1;
   xrl_ectx ec = xrl_energy(E);
   for (int k = 0; k < nel; k++) {
-    thr += P.el_w[off + k] * xrl_cs_total_c(&P.atoms[P.el_u[off + k]], &ec) / mu_cur;
+    thr += P.el_w[off + k] * d_elem_cs_total(P, off + k, &ec) / mu_cur;
     if (R2 < thr) {
       sel = k;
       break;
@@ -803,7 +818,9 @@ k_step(EngineParams P, PhotonSoA S, VrSoA V, int n, int count) {
       /* an estimator event of this order */
       ev_layer = E > 1.0 ? my : 0;
 
-      type = d_select(P, S, i, E, my, &s);
+      /* atom and type serve the interaction, which the last order of the
+       * estimator path does not launch */
+      if (!EST || n < P.n_int) type = d_select(P, S, i, E, my, &s);
       if (EST && ev_layer) vr_active = d_vr_geom<NL>(P, S, V, n, i, pt, dirv, my, &need_sa, &info);
     }
   }
