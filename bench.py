@@ -606,6 +606,10 @@ def main():
                     "events": events, "events_per_launch": round(ev_launch), "launches": int(launches),
                     "work_per_event": {k: round(v, 4) for k, v in work.items()},
                     "hbm_measured": hbm,
+                    # both estimator kernels against the FP64 peak, by their counted flops
+                    "fp64_frac_estimator": {names[k]: round(fpe[k] * ev_launch / (prof[k] * 1e-3 / launches)
+                                                            / 1e12 / FP64_PEAK_TFLOPS, 4)
+                                            for k in ("scatter", "lines")},
                     "solid_angle_fallbacks": int(prof["fallbacks"]), "kernel_ms": kernel_ms}
 
     cpu = None

@@ -29,7 +29,7 @@
  * stream keeps the order in 8 bits (philox.h) */
 #define ENG_MAX_INTERACTIONS 255
 #define ENG_MAX_ACT 96          /* active fluorescence lines per element */
-#define ENG_IP_ACC 4096         /* interaction-probability accelerator cells */
+#define ENG_IP_ACC 32768        /* interaction-probability accelerator cells (about one per table point) */
 #define ENG_PI 3.14159265358979323846
 
 struct PhotonSoA {
