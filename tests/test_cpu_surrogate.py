@@ -72,3 +72,42 @@ def test_reference_loop_step_limits():
     # weights above 1 in total: the largest and those before it absorb the excess
     w = quant._pymca_composition(np.array([0.2, 0.7, 0.3]))
     assert w.sum() == pytest.approx(1.0) and w[2] == 0.3
+
+
+def test_line_baseline_limits(lib):
+    """The first-order estimate from the encoded inputs against the same
+    physics evaluated directly at the sample's energy: thin layers (x << 1)
+    go as w P_s omega R rho d, thick ones as w P_s omega R / chi, an absent
+    element and an unexcitable line give 0."""
+    import math
+    from xmimsim_amd import lines as L, surrogate as S
+    B = S.LineBaseline()
+    comp = ([26, 29], [0.6, 0.4])
+    E0 = 17.4  # a grid node: no interpolation error
+    j_fe = S.OUT_LINES.index((26, S.KL3))
+    j_cu = S.OUT_LINES.index((29, S.KL3))
+    j_ni = S.OUT_LINES.index((28, S.KL3))
+
+    def direct(j, w, d, rho):
+        z, line = S.OUT_LINES[j]
+        e_line = L.line_energy(z, line)
+        mu0 = sum(wi * L.cs_total(zi, E0) for zi, wi in zip(*comp))
+        mul = sum(wi * L.cs_total(zi, e_line) for zi, wi in zip(*comp))
+        chi = math.sqrt(2.0) * (mu0 + mul)
+        P = L.shell_cross_sections(z, E0, 4)[0]
+        wr = lib.xmi_provider_line_data(z, 0, 0) * lib.xmi_provider_line_data(z, 1, line)
+        return math.log10(w * P * wr * -math.expm1(-chi * rho * d) / chi), math.log10(w * P * wr * rho * d), \
+            math.log10(w * P * wr / chi)
+
+    for d, limit in ((1e-7, 1), (1.0, 2)):
+        X = S.encode(comp[0], comp[1], d, 8.0, E0)[None]
+        b = B(X)[0]
+        for j, w in ((j_fe, 0.6), (j_cu, 0.4)):
+            exact, thin, thick = direct(j, w, d, 8.0)
+            assert b[j] == pytest.approx(exact, abs=2e-5)  # weights pass through float32 inputs
+            assert b[j] == pytest.approx((thin, thick)[limit - 1], abs=1e-3)
+        assert b[j_ni] == 0.0
+    # Ag at 20 keV: below its K edge (25.5 keV) no K line, its L lines excited
+    X = S.encode([47], [1.0], 1e-3, 10.5, 20.0)[None]
+    b = B(X)[0]
+    assert b[S.OUT_LINES.index((47, S.KL3))] == 0.0 and b[S.OUT_LINES.index((47, S.L3M5))] != 0.0

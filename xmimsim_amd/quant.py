@@ -140,13 +140,19 @@ def generate(engine: Engine, n: int, photons: int, seed: int, n_el=(2, 30), alph
     return X, T.astype(np.float32), M.astype(np.float32)
 
 
-def train(sg: S.Surrogate, X, T, M, steps: int, batch: int, lr: float = 1e-3, lr_min: float = 1e-5, seed: int = 0,
+def train(sg, X, T, M, steps: int, batch: int, lr: float = 1e-3, lr_min: float = 1e-5, seed: int = 0,
           log=None):
     """Adam on random mini-batches with a cosine learning-rate decay from lr
     to lr_min; returns the per-step mean loss.  The data stay on the GPU and
-    the steps run in chunks without a host round trip per step."""
+    the steps run in chunks without a host round trip per step.  sg: a
+    Surrogate, or a NetLineModel whose network learns the targets' residual
+    from its baseline."""
     rng = np.random.default_rng(seed)
     n = X.shape[0]
+    if isinstance(sg, S.NetLineModel):
+        # the network learns the correction to the first-order estimate
+        T = T - sg.baseline(X)
+        sg = sg.net
     sg.set_data(X, T, M)
     k = np.arange(steps)
     lrs = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * k / max(1, steps - 1)))
@@ -502,7 +508,9 @@ def run_pipeline(n_train: int = 32768, photons_train: int = 100_000, steps: int 
     out["generate_s"] = time.time() - t0
     out["train_histories"] = eng.mc_histories
     log(f"generated {n_train} samples ({eng.mc_histories:.3g} histories) in {out['generate_s']:.1f} s")
-    sg = S.Surrogate(n_nets=n_nets, max_batch=batch, seed=seed)
+    t0 = time.time()
+    sg = S.NetLineModel(S.Surrogate(n_nets=n_nets, max_batch=batch, seed=seed))
+    out["baseline_s"] = time.time() - t0
     t0 = time.time()
     b = min(batch, max(64, n_train // 64 * 64))
     hist = train(sg, X, T, M, steps, b, log=log)

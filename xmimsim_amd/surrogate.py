@@ -61,6 +61,79 @@ def targets(var_red_history: np.ndarray, Z: Sequence[int]) -> Tuple[np.ndarray, 
     return t, on.astype(np.float32)
 
 
+# shell of each output line's initial vacancy (0 K .. 8 M5)
+_LINE_SHELL = {KL3: 0, KM3: 0, L3M5: 3, M5N7: 8}
+
+
+class LineBaseline:
+    """First-order fundamental-parameter estimate of every output line from
+    the encoded inputs: log10 of w_Z * P_s(E0) * omega_s * R_line * (1 -
+    exp(-chi rho d)) / chi with chi = sqrt(2) (mu(E0) + mu(E_line)), the
+    primary fluorescence of a homogeneous layer in the 45/45 degree geometry
+    (P_s: the provider's cascade-enhanced shell cross section; mu: the
+    mixture's mass attenuation).  The network learns the rest (secondary
+    fluorescence, scattering, detector and geometry factors) as a correction
+    to it; 0 where the line cannot be excited.  Tables on an excitation grid of
+    DE keV, interpolated linearly."""
+    E_LO, E_HI, DE = 9.5, 40.5, 0.005
+
+    def __init__(self):
+        from . import lines as L
+        lib = api.load_library()
+        self.E = np.arange(self.E_LO, self.E_HI + self.DE / 2, self.DE)
+        Zs = range(Z_MIN, Z_MAX + 1)
+        self.sig = np.array([[lib.xmi_provider_cs(0, z, e) for e in self.E] for z in Zs])
+        e_line = np.array([L.line_energy(z, line) for z, line in OUT_LINES])
+        self.sig_line = np.array([[lib.xmi_provider_cs(0, z, e) if e > 0.0 else 0.0 for e in e_line] for z in Zs])
+        self.wr = np.array([lib.xmi_provider_line_data(z, 0, _LINE_SHELL[line]) * lib.xmi_provider_line_data(z, 1, line)
+                            for z, line in OUT_LINES])
+        ps = {}
+        for z in sorted({z for z, _ in OUT_LINES}):
+            ps[z] = np.array([L.shell_cross_sections(z, e, 4) for e in self.E])  # [nE][9]
+        self.ps = np.array([ps[z][:, _LINE_SHELL[line]] for z, line in OUT_LINES])  # [256][nE]
+        self.ok = (e_line > 0.0) & (self.wr > 0.0)
+
+    def __call__(self, X: np.ndarray, chunk: int = 16384) -> np.ndarray:
+        X = np.asarray(X)
+        out = np.zeros((X.shape[0], N_OUT), np.float32)
+        for s in range(0, X.shape[0], chunk):
+            out[s:s + chunk] = self._block(np.asarray(X[s:s + chunk], np.float64))
+        return out
+
+    def _block(self, X: np.ndarray) -> np.ndarray:
+        xw = X[:, :Z_MAX - Z_MIN + 1]
+        w = np.where(xw > 0.0, 10.0 ** (4.0 * (xw - 1.0)), 0.0)
+        d, rho, E0 = 10.0 ** X[:, -3], 20.0 * X[:, -2], 40.0 * X[:, -1]
+        t = np.clip((E0 - self.E_LO) / self.DE, 0.0, len(self.E) - 1.000001)
+        i0 = t.astype(np.int64)
+        f = t - i0
+        mu0 = np.einsum("nz,zn->n", w, self.sig[:, i0] * (1.0 - f) + self.sig[:, i0 + 1] * f)
+        chi = np.sqrt(2.0) * (mu0[:, None] + w @ self.sig_line)
+        P = (self.ps[:, i0] * (1.0 - f) + self.ps[:, i0 + 1] * f).T
+        x = chi * (rho * d)[:, None]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            A = np.where(chi > 0.0, -np.expm1(-x) / chi, 0.0)
+            v = w[:, _OUT_Z - Z_MIN] * P * self.wr[None] * A
+            return np.where((v > 0.0) & self.ok[None], np.log10(v), 0.0)
+
+
+class NetLineModel:
+    """The surrogate of the quantification: the MLP ensemble's output is a
+    correction added to LineBaseline, so that predict(X) gives log10 net-line
+    intensities; train on targets(...) minus baseline(X) (quant.train does)."""
+
+    def __init__(self, net: "Surrogate", baseline: "LineBaseline" = None):
+        self.net = net
+        self.baseline = baseline or LineBaseline()
+        self.n_nets = net.n_nets
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        return self.net.predict(X) + self.baseline(X)[None]
+
+    def close(self):
+        self.net.close()
+
+
 def _lib():
     lib = api.load_library()
     if not getattr(lib, "_sg_ready", False):
