@@ -70,6 +70,11 @@ static inline double orc_uniform(orc_rng *r) {
   return orc_word_to_double(c[0], c[1]);
 }
 
+/* a uniform on (0, 1) at the midpoint of one 32-bit word of orc_word (the
+ * solid-angle cone trials: two words per trial, two trials per block) */
+static inline uint32_t orc_word(orc_rng *r);
+static inline double orc_u32(orc_rng *r) { return ((double)orc_word(r) + 0.5) * (1.0 / 4294967296.0); }
+
 /* 32-bit words from the following blocks, four per block in order (the
  * estimator's Doppler trials draw one word each: 31 bits of random number and
  * a sign bit); a pending double of orc_uniform is dropped */

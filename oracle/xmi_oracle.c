@@ -302,6 +302,8 @@ typedef struct {
   double detector_radius, collimator_radius, collimator_height;
 } orc_det;
 
+/* the cone trials draw their two uniforms from one 32-bit word each
+ * (orc_u32): trials 2b and 2b + 1 share the stream's block b */
 static double orc_single_solid_angle(const orc_det *d, double r1, double theta1, long hits, orc_rng *rng) {
   double r, theta, full_cone_base_radius;
   int outside_collimator = 0;
@@ -349,8 +351,8 @@ static double orc_single_solid_angle(const orc_det *d, double r1, double theta1,
   double line_point[3] = {0.0, r1 * cos(theta1), r1 * sin(theta1)};
   long hitsc = 0;
   for (long i = 0; i < hits; i++) {
-    double theta_rng = acos(1.0 - orc_uniform(rng) * (1.0 - cos_apex));
-    double phi_rng = orc_uniform(rng) * 2.0 * ORC_PI;
+    double theta_rng = acos(1.0 - orc_u32(rng) * (1.0 - cos_apex));
+    double phi_rng = orc_u32(rng) * 2.0 * ORC_PI;
     double cone[3] = {sin(theta_rng) * cos(phi_rng), sin(theta_rng) * sin(phi_rng), cos(theta_rng)};
     double dv[3];
     matvec3(R, cone, dv);

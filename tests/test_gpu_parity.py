@@ -28,14 +28,14 @@ def _prep(inp, hits=64, seed=3, options=None):
     return prep
 
 
-def _assert_same(gpu, orc, label):
+def _assert_same(gpu, orc, label, rel=REL):
     ch, vr = gpu[0], gpu[1]
     och, ovr = orc[0], orc[1]
     scale = max(ovr.max(), 1e-300)
-    bad = np.abs(vr - ovr) > REL * np.maximum(np.abs(ovr), 1e-6 * scale)
+    bad = np.abs(vr - ovr) > rel * np.maximum(np.abs(ovr), 1e-6 * scale)
     assert not bad.any(), (label, np.argwhere(bad)[:5], vr[bad][:5], ovr[bad][:5])
     cs = max(och.max(), 1e-300)
-    badc = np.abs(ch - och) > REL * np.maximum(np.abs(och), 1e-6 * cs)
+    badc = np.abs(ch - och) > rel * np.maximum(np.abs(och), 1e-6 * cs)
     assert not badc.any(), (label, np.argwhere(badc)[:5])
 
 
@@ -62,11 +62,15 @@ def test_same_streams_single_layer(orc):
 
 
 def test_same_streams_three_layers_continuum(orc):
+    """2e-9: in this run the second-order Compton rows of the air elements
+    are sensitive to the last bits of single operations; the oracle built
+    with and without fused multiply-add differs from itself there by 1.23e-9
+    (the GPU contracts), everywhere else by less than 3e-10."""
     prep = _prep(workloads.three_layer_kramers(n_photons_interval=40, n_photons_line=2000))
     gpu = prep.run(seed=7)
     ref = orc.run(prep, seed=7, rng_mode=orc.RNG_PER_HISTORY)
     assert gpu[3].histories == ref[3].histories
-    _assert_same(gpu, ref, "kramers")
+    _assert_same(gpu, ref, "kramers", rel=2e-9)
 
 
 @pytest.mark.parametrize("which", ["steel", "kramers", "random"])

@@ -216,17 +216,26 @@ __device__ __forceinline__ int d_sa_trial(const SaCone &c, double u1, double u2)
   return sqrt(ix * ix + iy * iy + 0.0) <= c.R;
 }
 
-/* sequential estimate (one thread per grid cell) */
+/* cone trials 2 b and 2 b + 1 from block b of a stream (its counter
+ * word 0 advanced by b): words 0, 1 and 2, 3, one uniform each */
+__device__ __forceinline__ int d_sa_trial_pair(const SaCone &c, const uint32_t *ctr, const uint32_t *key, uint32_t b,
+                                               int both) {
+  uint32_t o[4];
+  const uint32_t cc[4] = {ctr[0] + b, ctr[1], ctr[2], ctr[3]};
+  phx_4x32_10(cc, key, o);
+  int n = d_sa_trial(c, phx_u32(o[0]), phx_u32(o[1]));
+  if (both) n += d_sa_trial(c, phx_u32(o[2]), phx_u32(o[3]));
+  return n;
+}
+
+/* sequential estimate (one thread per grid cell): hits trials from blocks
+ * 0 .. ceil(hits / 2) - 1 of the cell's stream */
 __device__ __forceinline__ double d_single_solid_angle_impl(const SaGeom g, double r1, double theta1, long hits,
                                                            phx_stream *rng) {
   SaCone c;
   if (!d_sa_setup(g, r1, theta1, &c)) return 0.0;
   long nhit = 0;
-  for (long i = 0; i < hits; i++) {
-    double u1 = phx_uniform(rng);
-    double u2 = phx_uniform(rng);
-    nhit += d_sa_trial(c, u1, u2);
-  }
+  for (long i = 0; i < hits; i += 2) nhit += d_sa_trial_pair(c, rng->ctr, rng->key, (uint32_t)(i >> 1), i + 1 < hits);
   return c.cone_sa * (double)nhit / (double)hits;
 }
 
@@ -981,8 +990,9 @@ __device__ __forceinline__ double d_exp_neg(double x, const double *tab) {
 /* persistent: the off-grid queue's photons (entries flattened over the
  * ENG_SPREAD sub-queues) times ENG_CONE_SEGS(hits) segments of
  * ENG_CONE_SEG trials are the work items; a wavefront takes items w, w + W,
- * ...  Trial t of a photon uses block ctr0 + t of its estimator stream, as
- * the serial loop does, and each segment adds its integer hit count and one
+ * ...  Trials 2 b and 2 b + 1 of a photon use block ctr0 + b of its
+ * estimator stream (one 32-bit word per uniform), as the serial loop does,
+ * and each segment adds its integer hit count and one
  * completion to the photon's packed accumulator (completions in bits 40 and
  * up); the wavefront that completes the photon writes its estimate and clears
  * the accumulator for the next launch.  Integer sums: the estimate equals the
@@ -1017,13 +1027,12 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA
     const int open = d_sa_setup(g, V.sar[i], V.sat[i], &cone);
     unsigned long long nhit = 0;
     if (open) {
+      /* trial pairs (2 b, 2 b + 1) from block ctr0 + b; segments start at
+       * even trials (ENG_CONE_SEG is even) */
       const long t1 = min((long)((sg + 1) * ENG_CONE_SEG), hits);
-      for (long t = (long)(sg * ENG_CONE_SEG) + lane; t < t1; t += 64) {
-        uint32_t o[4];
-        const uint32_t cc[4] = {ctr0 + (uint32_t)t, s.ctr[1], s.ctr[2], s.ctr[3]};
-        phx_4x32_10(cc, s.key, o);
-        nhit += d_sa_trial(cone, phx_u01(o[0], o[1]), phx_u01(o[2], o[3]));
-      }
+      const uint32_t cs[4] = {ctr0, s.ctr[1], s.ctr[2], s.ctr[3]};
+      for (long t = (long)(sg * ENG_CONE_SEG) + 2 * lane; t < t1; t += 128)
+        nhit += d_sa_trial_pair(cone, cs, s.key, (uint32_t)(t >> 1), t + 1 < t1);
     }
     for (int off = 32; off > 0; off >>= 1) nhit += __shfl_xor(nhit, off);
     if (lane == 0) {
@@ -1032,7 +1041,7 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_cone(EngineParams P, PhotonSoA
       if ((old >> ENG_CONE_DONE_SHIFT) + 1ull == nseg) {
         const unsigned long long all = (old & ((1ull << ENG_CONE_DONE_SHIFT) - 1ull)) + nhit;
         V.omega[i] = open ? cone.cone_sa * (double)all / (double)hits : 0.0;
-        if (open) V.blk[i] = ctr0 + (uint32_t)hits;
+        if (open) V.blk[i] = ctr0 + (uint32_t)((hits + 1) / 2);
         *acc = 0ull;
       }
     }

@@ -84,6 +84,10 @@ PHX_FN double phx_u01(uint32_t a, uint32_t b) {
   return (double)u * (1.0 / 9007199254740992.0);
 }
 
+/* a uniform on (0, 1) from one 32-bit word, at the word's midpoint: for the
+ * solid-angle cone trials, two per trial, so that one block serves two trials */
+PHX_FN double phx_u32(uint32_t a) { return ((double)a + 0.5) * (1.0 / 4294967296.0); }
+
 /* a sequential stream: regenerates blocks on demand */
 typedef struct phx_stream {
   uint32_t key[2];
