@@ -130,6 +130,20 @@ struct DevBuf {
   }
 };
 
+/* the energies bounding each cell of the provider's ln(E) grid, with the
+ * margins of the cell records' edge tests: lo[c] below cell c, hi[c] above it */
+struct CellBounds {
+  double lo[XRL_NCS], hi[XRL_NCS];
+  CellBounds() {
+    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
+    for (int c = 0; c < XRL_NCS; c++) {
+      lo[c] = exp(XRL_CS_LNE0 + (double)c * h) * (1.0 - 1e-9);
+      hi[c] = exp(XRL_CS_LNE0 + (double)(c + 1) * h) * (1.0 + 1e-9);
+    }
+  }
+};
+static const CellBounds g_cells;
+
 /* page-locked staging memory of one engine's host <-> device copies, so
  * that no copy takes the runtime's path for pageable memory */
 struct Staging {
@@ -624,9 +638,8 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         above[edges[K].second]++;
       }
     }
-    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
     for (int c = 0; c < XRL_NCS; c++) {
-      const double e_hi = exp(XRL_CS_LNE0 + (double)(c + 1) * h) * (1.0 + 1e-9);
+      const double e_hi = g_cells.hi[c];
       int K = 0;
       while (K < (int)edges.size() && edges[K].first > e_hi) K++;
       lay_kstart[(size_t)l * XRL_NCS + c] = K;
@@ -636,7 +649,6 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
   /* the same tables per cell, for one record load per lookup */
   std::vector<LayCell> lay_cell((size_t)NL * XRL_NCS);
   {
-    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
     for (int l = 0; l < NL; l++) {
       const int eo = P.lay_eoff[l], ne = P.lay_eoff[l + 1] - eo;
       for (int c = 0; c < XRL_NCS; c++) {
@@ -648,7 +660,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         /* every energy of the cell lies above the next edge (with a margin
          * far beyond the rounding of the cell index); the end cells also
          * take the energies beyond the grid, clamped, and keep the walk */
-        const double e_lo = exp(XRL_CS_LNE0 + (double)c * h) * (1.0 - 1e-9);
+        const double e_lo = g_cells.lo[c];
         r.edge_free = c > 0 && c < XRL_NCS - 2 && (r.K >= ne || lay_edge[eo + r.K] < e_lo);
         r.pb = lay_pb[eo + l + r.K];
       }
@@ -659,7 +671,6 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
    * instead of the atom's edges, photo sum and table, in a chain) */
   std::vector<LayCell> el_cell(elZ.size() * (size_t)XRL_NCS);
   {
-    const double h = (XRL_CS_LNE1 - XRL_CS_LNE0) / (double)(XRL_NCS - 1);
     for (size_t j = 0; j < elZ.size(); j++) {
       const xrl_atom *a = xrl_atom_get(elZ[j]);
       std::vector<double> ed;
@@ -671,8 +682,7 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
         LayCell &r = el_cell[j * XRL_NCS + c];
         r.T0 = a->cs_rc[c];
         r.dT = c + 1 < XRL_NCS ? a->cs_rc[c + 1] - a->cs_rc[c] : 0.0;
-        const double e_hi = exp(XRL_CS_LNE0 + (double)(c + 1) * h) * (1.0 + 1e-9);
-        const double e_lo = exp(XRL_CS_LNE0 + (double)c * h) * (1.0 - 1e-9);
+        const double e_hi = g_cells.hi[c], e_lo = g_cells.lo[c];
         int K = 0;
         while (K < ne && ed[K] > e_hi) K++;
         r.K = K;
