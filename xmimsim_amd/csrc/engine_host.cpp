@@ -191,6 +191,40 @@ static hipError_t copy_d2h(void *dst, const void *src, size_t bytes, hipStream_t
   return e;
 }
 
+/* many uploads through one staging fill: every array copied into its own
+ * 256-byte aligned part of the staging memory, one asynchronous copy each,
+ * and one synchronisation at the end */
+struct UploadSet {
+  struct Item {
+    DevBuf *b;
+    const void *src;
+    size_t bytes;
+  };
+  std::vector<Item> items;
+  template <typename T>
+  void add(DevBuf &b, const T *src, size_t count) {
+    items.push_back({&b, src, sizeof(T) * count});
+  }
+  hipError_t run(hipStream_t st, Staging &sg) {
+    size_t total = 0;
+    for (const Item &it : items) total += (it.bytes + 255) & ~(size_t)255;
+    hipError_t e = sg.ensure(total);
+    size_t off = 0;
+    for (const Item &it : items) {
+      if (e == hipSuccess) e = it.b->ensure(it.bytes);
+      if (e != hipSuccess) break;
+      if (it.bytes) {
+        memcpy((char *)sg.p + off, it.src, it.bytes);
+        e = hipMemcpyAsync(it.b->p, (char *)sg.p + off, it.bytes, hipMemcpyHostToDevice, st);
+        off += (it.bytes + 255) & ~(size_t)255;
+      }
+    }
+    /* the staging memory is reused after this */
+    hipError_t e2 = hipStreamSynchronize(st);
+    return e != hipSuccess ? e : e2;
+  }
+};
+
 template <typename T>
 static hipError_t upload(DevBuf &b, const T *src, size_t count, hipStream_t st, Staging &sg) {
   hipError_t e = b.ensure(sizeof(T) * count);
@@ -964,31 +998,35 @@ static int msim_run(Engine &E, xmi_inputFPtr inputFPtr, xmi_hdf5FPtr hdf5FPtr, i
       HIP_TRY(G.cone_acc.ensure(cone_bytes));
       if (fresh) HIP_TRY(hipMemsetAsync(G.cone_acc.p, 0, cone_bytes, st));
     }
-    HIP_TRY(upload(G.el_Z, elZ.data(), elZ.size(), st, G.stage));
-    HIP_TRY(upload(G.el_w, elW.data(), elW.size(), st, G.stage));
-    HIP_TRY(upload(G.el_u, elU.data(), elU.size(), st, G.stage));
-    HIP_TRY(upload(G.precalc_mu, pmu.data(), pmu.size(), st, G.stage));
-    HIP_TRY(upload(G.act_n, actn.data(), actn.size(), st, G.stage));
-    HIP_TRY(upload(G.act_line, actl.data(), actl.size(), st, G.stage));
-    HIP_TRY(upload(G.act_shell, acts.data(), acts.size(), st, G.stage));
-    HIP_TRY(upload(G.act_factor, actf.data(), actf.size(), st, G.stage));
-    HIP_TRY(upload(G.act_E, actE.data(), actE.size(), st, G.stage));
-    HIP_TRY(upload(G.act_ch, actc.data(), actc.size(), st, G.stage));
-    HIP_TRY(upload(G.act_murho, actm.data(), actm.size(), st, G.stage));
-    HIP_TRY(upload(G.fl_info, fl_info.data(), fl_info.size(), st, G.stage));
-    HIP_TRY(upload(G.fl_wf, fl_wf.data(), fl_wf.size(), st, G.stage));
-    HIP_TRY(upload(G.fl_mr, fl_mr.data(), fl_mr.size(), st, G.stage));
-    HIP_TRY(upload(G.lbin_g, lbin_g.data(), lbin_g.size(), st, G.stage));
-    HIP_TRY(upload(G.lay_rc, lay_rc.data(), lay_rc.size(), st, G.stage));
-    HIP_TRY(upload(G.lay_edge, lay_edge.data(), lay_edge.size(), st, G.stage));
-    HIP_TRY(upload(G.lay_pb, lay_pb.data(), lay_pb.size(), st, G.stage));
-    HIP_TRY(upload(G.lay_kstart, lay_kstart.data(), lay_kstart.size(), st, G.stage));
-    HIP_TRY(upload(G.lay_cell, lay_cell.data(), lay_cell.size(), st, G.stage));
-    HIP_TRY(upload(G.el_cell, el_cell.data(), el_cell.size(), st, G.stage));
-    HIP_TRY(upload(G.src_desc, src.data(), src.size(), st, G.stage));
-    HIP_TRY(upload(G.src_line_mus, src_mus.data(), src_mus.size(), st, G.stage));
-    HIP_TRY(upload(G.exc_layers, exc_flat.data(), exc_flat.size(), st, G.stage));
-    HIP_TRY(upload(G.exc_atoms, exc_atoms.data(), exc_atoms.size(), st, G.stage));
+    {
+      UploadSet up;
+      up.add(G.el_Z, elZ.data(), elZ.size());
+      up.add(G.el_w, elW.data(), elW.size());
+      up.add(G.el_u, elU.data(), elU.size());
+      up.add(G.precalc_mu, pmu.data(), pmu.size());
+      up.add(G.act_n, actn.data(), actn.size());
+      up.add(G.act_line, actl.data(), actl.size());
+      up.add(G.act_shell, acts.data(), acts.size());
+      up.add(G.act_factor, actf.data(), actf.size());
+      up.add(G.act_E, actE.data(), actE.size());
+      up.add(G.act_ch, actc.data(), actc.size());
+      up.add(G.act_murho, actm.data(), actm.size());
+      up.add(G.fl_info, fl_info.data(), fl_info.size());
+      up.add(G.fl_wf, fl_wf.data(), fl_wf.size());
+      up.add(G.fl_mr, fl_mr.data(), fl_mr.size());
+      up.add(G.lbin_g, lbin_g.data(), lbin_g.size());
+      up.add(G.lay_rc, lay_rc.data(), lay_rc.size());
+      up.add(G.lay_edge, lay_edge.data(), lay_edge.size());
+      up.add(G.lay_pb, lay_pb.data(), lay_pb.size());
+      up.add(G.lay_kstart, lay_kstart.data(), lay_kstart.size());
+      up.add(G.lay_cell, lay_cell.data(), lay_cell.size());
+      up.add(G.el_cell, el_cell.data(), el_cell.size());
+      up.add(G.src_desc, src.data(), src.size());
+      up.add(G.src_line_mus, src_mus.data(), src_mus.size());
+      up.add(G.exc_layers, exc_flat.data(), exc_flat.size());
+      up.add(G.exc_atoms, exc_atoms.data(), exc_atoms.size());
+      HIP_TRY(up.run(st, G.stage));
+    }
     HIP_TRY(hipMemsetAsync(G.tallies.p, 0, tally_words * sizeof(unsigned long long), st));
     HIP_TRY(hipMemsetAsync(G.counters.p, 0, (size_t)ENG_NCOUNTERS * ENG_SPREAD * sizeof(unsigned long long), st));
 
