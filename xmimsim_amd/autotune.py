@@ -116,9 +116,10 @@ def tune(prep: "api.Prepared", sample_histories: int = 1 << 23, seed: int = 1,
     shapes ({"launch": shape, "ms": device time}); (shape, []) when the
     cache holds the signature.  Candidates are tried besides the defaults
     (2^21 histories per batch, kernel by kernel, the default photons per lane
-    and the occupancy's workgroups).  The engine's launch shape is left at
-    the default; apply the result with api.set_launch(**shape).  timer(shape)
-    -> ms replaces the timed runs (tests of the search)."""
+    and the occupancy's workgroups).  The engine's launch shape in force
+    before the call is restored; apply the result with
+    api.set_launch(**shape).  timer(shape) -> ms replaces the timed runs
+    (tests of the search)."""
     key = f"{signature(prep)};sample={sample_histories}"
     data = _load_cache(cache)
     if key in data:
@@ -133,6 +134,8 @@ def tune(prep: "api.Prepared", sample_histories: int = 1 << 23, seed: int = 1,
             sample.solid_angle = prep.solid_angle
             sample._sa_owner = "borrowed"
 
+        previous = api.get_launch()
+
         def timer(shape):
             best = float("inf")
             try:
@@ -140,7 +143,7 @@ def tune(prep: "api.Prepared", sample_histories: int = 1 << 23, seed: int = 1,
                 for _ in range(repeats):
                     best = min(best, sample.run(seed=seed)[3].transport_ms)
             finally:
-                api.set_launch()
+                api.set_launch(**previous)
             return best
 
     def measure(shape: Dict[str, int]) -> float:
