@@ -1,6 +1,7 @@
 """CPU tests of the surrogate's reference arithmetic (oracle/surrogate_ref.py)
 and of the XRF encoding: the reference backward pass equals central finite
-differences of its own loss, and bf16 rounding is round-to-nearest-even."""
+differences of its own loss, and bf16 rounding is round-to-nearest-even; the
+first-order line estimate; the array forms of the screened quantification."""
 import numpy as np
 import pytest
 
@@ -111,3 +112,88 @@ def test_line_baseline_limits(lib):
     X = S.encode([47], [1.0], 1e-3, 10.5, 20.0)[None]
     b = B(X)[0]
     assert b[S.OUT_LINES.index((47, S.KL3))] == 0.0 and b[S.OUT_LINES.index((47, S.L3M5))] != 0.0
+
+
+def test_screening_arrays_equal_the_per_sample_forms():
+    """The screened quantification's array forms: grouped candidates and
+    encodings equal surrogate.encode of each composition bit for bit, the
+    grouped Gauss-Newton step equals the per-sample step, and the Jacobians
+    of a linear stand-in network are its own coefficients."""
+    from xmimsim_amd import quant as Q, surrogate as S
+    rng = np.random.default_rng(4)
+    samples = [Q.random_sample(rng, (2, 12)) for _ in range(60)]
+    W = [rng.dirichlet(np.ones(len(s.Z))) for s in samples]
+    groups = Q._Groups(samples)
+    assert sorted(i for idx, _, _ in groups.groups for i in idx) == list(range(60))
+    h = 0.2
+    for idx, Z, geo in groups.groups:
+        C = Q._candidates(np.array([W[i] for i in idx]), h)
+        X = Q._encode_group(Z, C, geo)
+        for k, i in enumerate(idx):
+            s = samples[i]
+            assert np.array_equal(X[k, 0], S.encode(s.Z, W[i], s.thickness, s.density, s.energy))
+            for j in range(len(s.Z)):
+                wp = W[i].copy()
+                wp[j] *= np.exp(h)
+                assert np.array_equal(X[k, 1 + j], S.encode(s.Z, wp / wp.sum(), s.thickness, s.density, s.energy))
+    # Gauss-Newton: grouped against one sample at a time, over masked lines
+    for n in (3, 7):
+        g = 5
+        Wg = rng.dirichlet(np.ones(n), size=g)
+        J = rng.normal(size=(g, S.N_OUT, n))
+        r = rng.normal(size=(g, S.N_OUT))
+        m = rng.random((g, S.N_OUT)) < 0.1
+        m[0] = False  # a sample without lines keeps its weights
+        out = Q._gn_update_many(Wg, J, np.where(m, r, 0.0), m, 1e-3)
+        for k in range(g):
+            ref = Q._gn_update(Wg[k], J[k][m[k]], r[k][m[k]], 1e-3)
+            assert np.allclose(out[k], ref, rtol=1e-10, atol=0)
+        assert np.allclose(out[0], Wg[0], rtol=1e-12)
+
+    # a linear network in the encoded inputs: d log10 I / d log w_j = A[z_j] / (4 ln 10) up to the renormalisation
+    A = rng.normal(size=(S.N_IN, S.N_OUT))
+
+    class Linear:
+        n_nets = 2
+
+        def predict(self, X):
+            y = X.astype(np.float64) @ A
+            return np.stack([y, y])
+
+    jac = Q._surrogate_jacobians(Linear(), samples, W, h=h)
+    for i in (0, 17, 59):
+        s = samples[i]
+        x0 = S.encode(s.Z, W[i], s.thickness, s.density, s.energy).astype(np.float64)
+        assert np.allclose(jac[i][0], x0 @ A, rtol=1e-12, atol=1e-12) and jac[i][2].shape == (2, S.N_OUT)
+        for j in range(len(s.Z)):
+            wp = W[i].copy()
+            wp[j] *= np.exp(h)
+            x1 = S.encode(s.Z, wp / wp.sum(), s.thickness, s.density, s.energy).astype(np.float64)
+            assert np.allclose(jac[i][1][:, j], (x1 - x0) @ A / h, rtol=1e-9, atol=1e-9)
+
+
+def test_line_baseline_sparse_form_equals_dense(lib):
+    """The estimate is evaluated only at a row's own elements and lines; a
+    dense evaluation of the same expression over every (row, line) gives the
+    same values, and zeros elsewhere."""
+    from xmimsim_amd import quant as Q, surrogate as S
+    B = S.LineBaseline()
+    rng = np.random.default_rng(8)
+    samples = [Q.random_sample(rng, (2, 30)) for _ in range(200)]
+    X = np.stack([S.encode(s.Z, s.w, s.thickness, s.density, s.energy) for s in samples]).astype(np.float64)
+    xw = X[:, :S.Z_MAX - S.Z_MIN + 1]
+    w = np.where(xw > 0.0, 10.0 ** (4.0 * (xw - 1.0)), 0.0)
+    d, rho, E0 = 10.0 ** X[:, -3], 20.0 * X[:, -2], 40.0 * X[:, -1]
+    t = np.clip((E0 - B.E_LO) / B.DE, 0.0, len(B.E) - 1.000001)
+    i0 = t.astype(np.int64)
+    f = t - i0
+    mu0 = np.einsum("nz,zn->n", w, B.sig[:, i0] * (1.0 - f) + B.sig[:, i0 + 1] * f)
+    chi = np.sqrt(2.0) * (mu0[:, None] + w @ B.sig_line)
+    P = (B.ps[:, i0] * (1.0 - f) + B.ps[:, i0 + 1] * f).T
+    with np.errstate(divide="ignore", invalid="ignore"):
+        A = np.where(chi > 0.0, -np.expm1(-chi * (rho * d)[:, None]) / chi, 0.0)
+        v = w[:, S._OUT_Z - S.Z_MIN] * P * B.wr[None] * A
+        dense = np.where((v > 0.0) & B.ok[None], np.log10(v), 0.0)
+    got = B(X)
+    assert np.allclose(got, dense.astype(np.float32), rtol=1e-6, atol=0)
+    assert np.array_equal(got != 0.0, dense != 0.0) and (got != 0.0).sum() > 1000

@@ -329,10 +329,6 @@ def quantify_pymca(engine: Engine, samples: List[Sample], meas: List[Measurement
 # ---------------------------------------------------------------- screened loop
 
 
-def _encode_batch(s: Sample, W: np.ndarray) -> np.ndarray:
-    return np.stack([S.encode(s.Z, w, s.thickness, s.density, s.energy) for w in W])
-
-
 def _gn_update(w, J, r, lam):
     """Damped Gauss-Newton step on log weights, renormalised."""
     A = J.T @ J + lam * np.eye(J.shape[1]) * max(1e-12, float(np.trace(J.T @ J)) / J.shape[1])
@@ -341,26 +337,89 @@ def _gn_update(w, J, r, lam):
     return w_new / w_new.sum()
 
 
+def _gn_update_many(W, J, r, m, lam):
+    """_gn_update of g samples with n elements each at once: W [g][n], J
+    [g][N_OUT][n], r and the line masks m [g][N_OUT] (rows outside the mask
+    take no part)."""
+    Jm = J * m[:, :, None]
+    A = np.swapaxes(Jm, 1, 2) @ J
+    n = W.shape[1]
+    tr = np.trace(A, axis1=1, axis2=2)
+    A = A + (lam * np.maximum(1e-12, tr / n))[:, None, None] * np.eye(n)
+    step = np.linalg.solve(A, -(np.swapaxes(Jm, 1, 2) @ r[:, :, None]))[:, :, 0]
+    w = W * np.exp(np.clip(step, -1.0, 1.0))
+    return w / w.sum(axis=1, keepdims=True)
+
+
+class _Groups:
+    """The samples grouped by element count, so that the candidates,
+    encodings and Gauss-Newton steps of a group are array operations:
+    idx [g] sample numbers, Z [g][n], thickness, density, energy [g]."""
+
+    def __init__(self, samples: Sequence[Sample], idx: Optional[Sequence[int]] = None):
+        by = {}
+        for i in (range(len(samples)) if idx is None else idx):
+            by.setdefault(len(samples[i].Z), []).append(i)
+        self.groups = []
+        for n in sorted(by):
+            ii = np.array(by[n])
+            self.groups.append((ii, np.array([samples[i].Z for i in ii]),
+                                np.array([[samples[i].thickness, samples[i].density, samples[i].energy]
+                                          for i in ii])))
+
+
+def _candidates(W: np.ndarray, h: float) -> np.ndarray:
+    """[g][n + 1][n]: each composition, then for every element j the
+    composition with w_j scaled by e^h and renormalised."""
+    g, n = W.shape
+    C = np.repeat(W[:, None, :], n + 1, axis=1)
+    j = np.arange(n)
+    C[:, 1 + j, j] *= np.exp(h)
+    C[:, 1:] /= C[:, 1:].sum(axis=2, keepdims=True)
+    return C
+
+
+def _encode_group(Z: np.ndarray, C: np.ndarray, geo: np.ndarray) -> np.ndarray:
+    """surrogate.encode of every composition C [g][r][n] of the samples with
+    elements Z [g][n] and (thickness, density, energy) geo [g][3]."""
+    g, r, n = C.shape
+    X = np.zeros((g, r, S.N_IN), np.float32)
+    X[np.arange(g)[:, None, None], np.arange(r)[None, :, None], (Z - S.Z_MIN)[:, None, :]] = \
+        1.0 + np.log10(np.maximum(C, 1e-4)) / 4.0
+    X[:, :, -3] = np.log10(geo[:, 0])[:, None]
+    X[:, :, -2] = (geo[:, 1] / 20.0)[:, None]
+    X[:, :, -1] = (geo[:, 2] / 40.0)[:, None]
+    return X
+
+
+def _group_jacobians(sg: S.Surrogate, groups: _Groups, W, h=0.2):
+    """Per group: ensemble-mean log lines Y [g][N_OUT], finite-difference
+    Jacobians J [g][N_OUT][n] (d log10 I / d log w_j) and the ensemble's
+    lines Ye [n_nets][g][N_OUT] at the compositions W (list by sample), from
+    one batched forward pass over all groups.  The step moves the encoded
+    input by h / (4 ln 10) = 0.022, well above the bf16 resolution of the
+    network's inputs (2^-8)."""
+    blocks = [_encode_group(Z, _candidates(np.array([W[i] for i in idx]), h), geo)
+              for idx, Z, geo in groups.groups]
+    Ye = sg.predict(np.concatenate([b.reshape(-1, S.N_IN) for b in blocks]))
+    out, row = [], 0
+    for b in blocks:
+        g, r = b.shape[:2]
+        y = Ye[:, row:row + g * r].reshape(Ye.shape[0], g, r, -1)
+        row += g * r
+        Y = y.mean(axis=0)
+        out.append((Y[:, 0], np.swapaxes(Y[:, 1:] - Y[:, :1], 1, 2) / h, y[:, :, 0]))
+    return out
+
+
 def _surrogate_jacobians(sg: S.Surrogate, samples, W, h=0.2):
-    """Ensemble-mean log lines and finite-difference Jacobians (d log10 I /
-    d log w_j) of every sample at W, from one batched forward pass.  The step
-    moves the encoded input by h / (4 ln 10) = 0.022, well above the bf16
-    resolution of the network's inputs (2^-8)."""
-    rows, where = [], []
-    for i, s in enumerate(samples):
-        cand = [W[i]]
-        for j in range(len(s.Z)):
-            wp = W[i].copy()
-            wp[j] *= np.exp(h)
-            cand.append(wp / wp.sum())
-        where.append((len(rows), len(cand)))
-        rows.extend(_encode_batch(s, np.array(cand)))
-    Ye = sg.predict(np.array(rows))
-    Y = Ye.mean(axis=0)
-    out = []
-    for i in range(len(samples)):
-        o, c = where[i]
-        out.append((Y[o], (Y[o + 1:o + c] - Y[o]).T / h, Ye[:, o]))
+    """(Y [N_OUT], J [N_OUT][n], Ye [n_nets][N_OUT]) of every sample, as
+    _group_jacobians gives them, in the order of samples."""
+    groups = _Groups(samples)
+    out = [None] * len(samples)
+    for (idx, _, _), (Y, J, Ye) in zip(groups.groups, _group_jacobians(sg, groups, W, h)):
+        for k, i in enumerate(idx):
+            out[i] = (Y[k], J[k], Ye[:, k])
     return out
 
 
@@ -400,8 +459,9 @@ def _mc_gn_batch(engine: Engine, samples, meas, idx, W, photons, seed, tol, iter
 
 def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], meas: List[Measurement], photons: int,
                       iters: int = 15, gate: float = 0.02, tol: float = 0.02, hybrid_iters: int = 4,
-                      fallback_iters: int = 3, seed: int = 0):
-    """Gauss-Newton on the surrogate for all samples at once; then Monte Carlo
+                      fallback_iters: int = 3, seed: int = 0, step_tol: float = 1e-4):
+    """Gauss-Newton on the surrogate for all samples at once (a sample stops
+    once no log weight moves by more than step_tol); then Monte Carlo
     confirmation of every answer, refined while needed by Gauss-Newton steps
     that take the residual from the Monte Carlo run and the Jacobian from the
     surrogate, Broyden-corrected by the Monte Carlo secants (one seed per
@@ -412,18 +472,28 @@ def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], me
     (w_hat list, info dict)."""
     n = len(samples)
     W = [np.full(len(s.Z), 1.0 / len(s.Z)) for s in samples]
+    moving = range(n)
     for it in range(iters):
-        jac = _surrogate_jacobians(sg, samples, W)
-        for i in range(n):
-            mk = meas[i].m > 0
-            f0, J, _ = jac[i]
-            W[i] = _gn_update(W[i], J[mk], f0[mk] - meas[i].t[mk], 1e-3)
-    jac = _surrogate_jacobians(sg, samples, W)
+        # a sample whose last step moved no log weight by more than step_tol stays where it is
+        groups, still = _Groups(samples, moving), []
+        for (idx, _, _), (Y, J, _) in zip(groups.groups, _group_jacobians(sg, groups, W)):
+            t, m = np.array([meas[i].t for i in idx]), np.array([meas[i].m > 0 for i in idx])
+            W0 = np.array([W[i] for i in idx])
+            Wn = _gn_update_many(W0, J, np.where(m, Y - t, 0.0), m, 1e-3)
+            moved = np.abs(np.log(Wn / W0)).max(axis=1) > step_tol
+            for k, i in enumerate(idx):
+                W[i] = Wn[k]
+            still.extend(idx[moved])
+        moving = sorted(still)
+        if not moving:
+            break
+    groups = _Groups(samples)
     spread = np.zeros(n)
-    for i in range(n):
-        mk = meas[i].m > 0
-        lin = 10.0 ** jac[i][2][:, mk]
-        spread[i] = float(np.max(lin.std(axis=0) / lin.mean(axis=0))) if mk.any() else 0.0
+    for (idx, _, _), (_, _, Ye) in zip(groups.groups, _group_jacobians(sg, groups, W)):
+        m = np.array([meas[i].m > 0 for i in idx])
+        lin = 10.0 ** Ye
+        rel = np.where(m, lin.std(axis=0) / lin.mean(axis=0), 0.0)
+        spread[idx] = rel.max(axis=1)
     gated = spread > gate
     n_ref = np.where(gated, 2 * hybrid_iters, hybrid_iters)
     line_err = np.full(n, np.inf)
@@ -439,17 +509,23 @@ def quantify_screened(sg: S.Surrogate, engine: Engine, samples: List[Sample], me
                             [seed + 7919 * i for i in active])
         still = []
         for k, i in enumerate(active):
-            mk = meas[i].m > 0
-            t = T[k]
             mc_calls[i] += 1
             confirmed[i] = True
-            line_err[i] = _line_err(t, meas[i].t, mk)
+            line_err[i] = _line_err(T[k], meas[i].t, meas[i].m > 0)
+        # the surrogate Jacobians of the samples refined for the first time, in one forward pass
+        fresh = [i for i in active if J[i] is None and line_err[i] > tol and r != n_ref[i]]
+        if fresh:
+            for i, (_, Jf, _) in zip(fresh, _surrogate_jacobians(sg, [samples[i] for i in fresh],
+                                                                 [W[i] for i in fresh])):
+                J[i] = Jf[meas[i].m > 0]
+        fresh = set(fresh)
+        for k, i in enumerate(active):
+            mk = meas[i].m > 0
+            t = T[k]
             if line_err[i] <= tol or r == n_ref[i]:
                 continue
             u = np.log(W[i])
-            if J[i] is None:
-                J[i] = _surrogate_jacobians(sg, [samples[i]], [W[i]])[0][1][mk]
-            else:
+            if i not in fresh:
                 t_prev, u_prev = prev[i]
                 du, df = u - u_prev, t[mk] - t_prev
                 J[i] = J[i] + np.outer(df - J[i] @ du, du) / max(float(du @ du), 1e-30)

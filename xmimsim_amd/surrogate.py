@@ -92,6 +92,11 @@ class LineBaseline:
             ps[z] = np.array([L.shell_cross_sections(z, e, 4) for e in self.E])  # [nE][9]
         self.ps = np.array([ps[z][:, _LINE_SHELL[line]] for z, line in OUT_LINES])  # [256][nE]
         self.ok = (e_line > 0.0) & (self.wr > 0.0)
+        # output lines of each element that the estimate covers, padded with -1
+        cols = [np.flatnonzero((_OUT_Z == z) & self.ok) for z in Zs]
+        self.cols = np.full((len(cols), max(len(c) for c in cols)), -1)
+        for k, c in enumerate(cols):
+            self.cols[k, :len(c)] = c
 
     def __call__(self, X: np.ndarray, chunk: int = 16384) -> np.ndarray:
         X = np.asarray(X)
@@ -101,20 +106,29 @@ class LineBaseline:
         return out
 
     def _block(self, X: np.ndarray) -> np.ndarray:
-        xw = X[:, :Z_MAX - Z_MIN + 1]
-        w = np.where(xw > 0.0, 10.0 ** (4.0 * (xw - 1.0)), 0.0)
+        # sparse over the rows' own elements (a dozen of 82) and their lines
+        n_, z_ = np.nonzero(X[:, :Z_MAX - Z_MIN + 1] > 0.0)
+        wz = 10.0 ** (4.0 * (X[n_, z_] - 1.0))
         d, rho, E0 = 10.0 ** X[:, -3], 20.0 * X[:, -2], 40.0 * X[:, -1]
         t = np.clip((E0 - self.E_LO) / self.DE, 0.0, len(self.E) - 1.000001)
         i0 = t.astype(np.int64)
         f = t - i0
-        mu0 = np.einsum("nz,zn->n", w, self.sig[:, i0] * (1.0 - f) + self.sig[:, i0 + 1] * f)
-        chi = np.sqrt(2.0) * (mu0[:, None] + w @ self.sig_line)
-        P = (self.ps[:, i0] * (1.0 - f) + self.ps[:, i0 + 1] * f).T
-        x = chi * (rho * d)[:, None]
+        mu0 = np.bincount(n_, wz * (self.sig[z_, i0[n_]] * (1.0 - f[n_]) + self.sig[z_, i0[n_] + 1] * f[n_]),
+                          minlength=X.shape[0])
+        w = np.zeros((X.shape[0], Z_MAX - Z_MIN + 1))
+        w[n_, z_] = wz
+        cz = self.cols[z_]
+        e, k = np.nonzero(cz >= 0)
+        r, c = n_[e], cz[e, k]
+        chi = np.sqrt(2.0) * (mu0[r] + (w @ self.sig_line)[r, c])
+        P = self.ps[c, i0[r]] * (1.0 - f[r]) + self.ps[c, i0[r] + 1] * f[r]
+        x = chi * (rho * d)[r]
+        out = np.zeros((X.shape[0], N_OUT))
         with np.errstate(divide="ignore", invalid="ignore"):
             A = np.where(chi > 0.0, -np.expm1(-x) / chi, 0.0)
-            v = w[:, _OUT_Z - Z_MIN] * P * self.wr[None] * A
-            return np.where((v > 0.0) & self.ok[None], np.log10(v), 0.0)
+            v = wz[e] * P * self.wr[c] * A
+            out[r, c] = np.where(v > 0.0, np.log10(v), 0.0)
+        return out
 
 
 class NetLineModel:
