@@ -1105,15 +1105,30 @@ __device__ __forceinline__ uint32_t d_ww_word(WaveWords &w) {
   return x;
 }
 
-/* the Compton profile's CDF at momentum pz >= 0 from its grid (the
+/* the Compton profile's CDF at momentum pz from its grid (the
  * xmi_profile_cdf_grid table of the element; gs grid points per unit
- * momentum), clamped to the grid's last point */
-__device__ __forceinline__ double d_profile_cdf(const double *g, double gs, double pz) {
+ * momentum), clamped to the grid's last point, 0 for pz <= 0; in two parts,
+ * so that the grid values of one element can be loaded while the previous
+ * element is scored: the two grid values the lookup interpolates between,
+ * then the linear interpolation */
+__device__ __forceinline__ void d_profile_cdf_load(const double *g, double gs, double pz, double v[2]) {
+  v[0] = v[1] = 0.0;
+  if (!(pz > 0.0)) return;
+  const double x = pz * gs;
+  if (x >= (double)(XMI_CP_GRID - 1)) {
+    v[0] = v[1] = g[XMI_CP_GRID - 1];
+    return;
+  }
+  const int j = (int)x;
+  v[0] = g[j];
+  v[1] = g[j + 1];
+}
+
+__device__ __forceinline__ double d_profile_cdf_eval(const double v[2], double gs, double pz) {
   if (!(pz > 0.0)) return 0.0;
   const double x = pz * gs;
-  if (x >= (double)(XMI_CP_GRID - 1)) return g[XMI_CP_GRID - 1];
-  const int j = (int)x;
-  return g[j] + (g[j + 1] - g[j]) * (x - (double)j);
+  if (x >= (double)(XMI_CP_GRID - 1)) return v[0];
+  return v[0] + (v[1] - v[0]) * (x - (double)(int)x);
 }
 
 /* ---- the advanced Compton model (use_advanced_compton) ------------------
@@ -1284,10 +1299,31 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
     const int nel = __builtin_amdgcn_readfirstlane(P.layer_nel[L - 1]);
     double twr_sum = 0.0; /* Rayleigh keeps the energy: one channel per photon */
     double rsum = 0.0;    /* lane 2j + kind: row total of element k0 + j of this group */
+    /* the accepted momenta's bounds of the Doppler draw (per photon), and
+     * the profile-CDF values at them of the next element, loaded one
+     * element ahead */
+    const double pz_max = A1 > 0.0 ? A0 / A1 : 0.0, pz_neg = A1 > 0.0 ? (c_lamb0 + A0) / A1 : 0.0;
+    double cdf_max[2] = {0.0, 0.0}, cdf_neg[2] = {0.0, 0.0};
+    if (!ADV && mine && A1 > 0.0 && nel > 0) {
+      const int u0 = d_const(P.el_u)[off];
+      d_profile_cdf_load(P.cp_cdfg + (size_t)u0 * XMI_CP_GRID, P.cp_gscale[u0], pz_max, cdf_max);
+      d_profile_cdf_load(P.cp_cdfg + (size_t)u0 * XMI_CP_GRID, P.cp_gscale[u0], pz_neg, cdf_neg);
+    }
     for (int k = 0; k < nel; k++) {
       const int u = d_const(P.el_u)[off + k];
       const xrl_atom *a = &P.atoms[u];
       const double Pconv = d_const(P.el_w)[off + k] * inv_mu;
+      double f_max = 0.0, f_neg = 0.0;
+      if (!ADV && mine && A1 > 0.0) {
+        const double gs = P.cp_gscale[u];
+        f_max = d_profile_cdf_eval(cdf_max, gs, pz_max);
+        f_neg = d_profile_cdf_eval(cdf_neg, gs, pz_neg);
+        if (k + 1 < nel) {
+          const int u1 = d_const(P.el_u)[off + k + 1];
+          d_profile_cdf_load(P.cp_cdfg + (size_t)u1 * XMI_CP_GRID, P.cp_gscale[u1], pz_max, cdf_max);
+          d_profile_cdf_load(P.cp_cdfg + (size_t)u1 * XMI_CP_GRID, P.cp_gscale[u1], pz_neg, cdf_neg);
+        }
+      }
       /* Rayleigh */
       double twr = 0.0;
       if (mine) {
@@ -1333,14 +1369,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
          * table without its last interval */
         const double *rn = P.cp_rn;
         const double *icdf = P.cp_icdf + (size_t)u * P.cp_len;
-        const double *g = P.cp_cdfg + (size_t)u * XMI_CP_GRID;
-        const double gs = P.cp_gscale[u];
         const double r_end = rn[P.cp_len - 2];
-        double f_max = r_end, f_neg = r_end;
-        if (A1 > 0.0) {
-          f_max = d_profile_cdf(g, gs, A0 / A1);
-          f_neg = d_profile_cdf(g, gs, (c_lamb0 + A0) / A1);
-        }
+        if (!(A1 > 0.0)) f_max = f_neg = r_end; /* the profile CDFs at A0 / A1 were loaded ahead */
         const double m_pos = f_max + (r_end - f_neg);
         const uint32_t wd = d_ww_word(s);
         const double uu = (double)wd * 0x1.0p-32 * (r_end + m_pos);
