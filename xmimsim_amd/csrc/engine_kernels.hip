@@ -1441,14 +1441,20 @@ __global__ void __launch_bounds__(ENG_BLOCK) __attribute__((amdgpu_waves_per_eu(
  * interaction (or a photon not coming from fluorescence) uses
  * E^-p * shell_C[cascade][edges above E][shell], a fluorescence photon the
  * precalculated cascade table (src/xmi_variance_reduction.F90:540-580).  The
- * terms of a line are summed over each lane's photons, then over the lanes
- * in a fixed DPP order (deterministic), and added once to the line's
- * fixed-point bin: one wavefront sum per line and 64 KP photons. */
+ * terms of a line are summed over each lane's photons, then over the lanes in
+ * a fixed order (deterministic), and added once to the line's fixed-point
+ * bin: one wavefront sum per line and 64 KP photons.  The lane sums go
+ * through LDS eight lines at a time: the lanes store their terms of eight
+ * lines, then eight lanes per line add its 64 terms (eight each, then three
+ * DPP steps), instead of a six-step DPP reduction per line. */
+#define ENG_LRED_STRIDE 72 /* doubles per line row of the staging: 8 mod 32, conflict-free reads */
 template <int NL, int KP, bool use_lds>
 __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSoA S, VrSoA V, int n) {
   /* LDS tallies (when they fit): compact line bins [n_lbins][2] */
   extern __shared__ unsigned long long lds[];
   __shared__ double exp_tab[64];
+  /* per wavefront: the lanes' terms of 8 lines, row j = line j of the group */
+  __shared__ double lred[ENG_BLOCK / 64][8 * ENG_LRED_STRIDE];
   const size_t n_line2 = (size_t)P.n_lbins * 2;
   const size_t cap = (size_t)P.cap;
   unsigned long long *gline = P.t_line + (size_t)(n - 1) * P.nZ * ENG_MAX_ACT * 2;
@@ -1509,9 +1515,8 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
     if (__ballot(any_act) == 0ull) continue;
     const int t0 = __builtin_amdgcn_readfirstlane(P.fl_off[L - 1]), t1 = __builtin_amdgcn_readfirstlane(P.fl_off[L]);
     int u_cur = -1, z_cur = -1, any = 0;
-    /* lane j holds the chunk's total of line t0 + 64 g + j of the current
-     * group g of 64 lines; the group is added to the tallies in one pass */
-    double lsum = 0.0;
+    double *wred = lred[threadIdx.x >> 6];
+    const int lane = __lane_id();
     const auto *fl_info = d_const((const int *)P.fl_info);
     const auto *fl_wf = d_const(P.fl_wf);
     const auto *fl_mr = d_const(P.fl_mr);
@@ -1566,10 +1571,11 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
         }
         any = __ballot(nz) != 0ull;
       }
-      const int j = (t - t0) & 63;
+      const int j = (t - t0) & 7;
+      double part = 0.0;
       if (any) {
         /* attenuation rows are padded to NL entries (zeros past the stack);
-         * the lane's photons are summed before the one wavefront sum */
+         * the lane's photons are summed before the wavefront sum */
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < KP; k++) {
@@ -1578,18 +1584,31 @@ __global__ void __launch_bounds__(ENG_BLOCK) k_vr_lines(EngineParams P, PhotonSo
           for (int l = 0; l < NL; l++) m2 += mr[l] * dl[k][l];
           acc = fma(x[k], d_exp_neg(-m2, exp_tab), acc);
         }
-        const double term = d_wave_total(acc * wf);
-        if (__lane_id() == j) lsum = term;
+        part = acc * wf;
       }
-      if (j == 63 || t == t1 - 1) {
-        const int tl = t - j + __lane_id();
-        if (tl <= t && lsum > 0.0) {
+      wred[j * ENG_LRED_STRIDE + lane] = part;
+      if (j == 7 || t == t1 - 1) {
+        /* line m of the group (m <= j): lane 8m + q adds the terms of lanes
+         * 8r + q (r = 0..7), then the eight lanes of m are summed (xor 1,
+         * xor 2 in quads, then the half-row mirror joins the two quads); the
+         * staging of the next group is written after these reads (one
+         * wavefront's LDS accesses complete in order) */
+        __builtin_amdgcn_wave_barrier();
+        const int m = lane >> 3, q = lane & 7;
+        double sum = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) sum += wred[m * ENG_LRED_STRIDE + 8 * r + q];
+        sum += d_dpp0<0xB1, 0xf>(sum);  /* quad_perm [1, 0, 3, 2] */
+        sum += d_dpp0<0x4E, 0xf>(sum);  /* quad_perm [2, 3, 0, 1] */
+        sum += d_dpp0<0x141, 0xf>(sum); /* row_half_mirror */
+        __builtin_amdgcn_wave_barrier();
+        const int tl = t - j + m;
+        if (q == 0 && tl <= t && sum > 0.0) {
           const int4 il = P.fl_info[tl];
           unsigned long long *bin =
               use_lds ? &tline[(size_t)il.w * 2] : &tline[((size_t)il.x * ENG_MAX_ACT + il.y) * 2];
-          fx_add_global(bin, lsum, shift);
+          fx_add_global(bin, sum, shift);
         }
-        lsum = 0.0;
       }
     }
   }
